@@ -1,0 +1,96 @@
+"""ctypes mirror of include/hgx.h (structs, constants, signatures)."""
+from __future__ import annotations
+
+import ctypes as C
+
+HGX_OK = 0
+HGX_ERR_INVALID = 1
+HGX_ERR_NO_DEVICE = 2
+HGX_ERR_DEVICE = 3
+HGX_ERR_CAPACITY = 4
+HGX_ERR_NOT_READY = 5
+HGX_ERR_UNSUPPORTED = 6
+
+HGX_REC_CAUSAL = 0x1
+HGX_REC_BRANCHIAL = 0x2
+HGX_REC_RELATIONS = 0x4
+HGX_REC_STATES = 0x8
+HGX_OPT_TR = 0x100
+HGX_REC_DEFAULT = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL
+HGX_MAX_STEPS = 63
+
+
+class RunDesc(C.Structure):
+    _fields_ = [
+        ("rules", C.POINTER(C.c_uint8)),
+        ("rules_len", C.c_uint32),
+        ("n_rules", C.c_uint32),
+        ("edge_offsets", C.POINTER(C.c_uint32)),
+        ("vertices", C.POINTER(C.c_uint32)),
+        ("n_edges", C.c_uint32),
+        ("root_offsets", C.POINTER(C.c_uint32)),
+        ("n_roots", C.c_uint32),
+        ("steps", C.c_uint32),
+        ("state_cap", C.c_uint32),
+        ("flags", C.c_uint32),
+    ]
+
+
+class Counts(C.Structure):
+    _fields_ = [
+        ("raw_states", C.c_uint64),
+        ("events", C.c_uint64),
+        ("canonical_states", C.c_uint64),
+        ("causal_triples", C.c_uint64),
+        ("causal_pairs", C.c_uint64),
+        ("causal_pairs_tr", C.c_uint64),
+        ("causal_triples_tr", C.c_uint64),
+        ("branchial_pairs", C.c_uint64),
+        ("fingerprint", C.c_uint64),
+        ("steps_done", C.c_uint32),
+        ("truncated", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Event(C.Structure):
+    _fields_ = [("input_state", C.c_uint32), ("output_state", C.c_uint32),
+                ("rule", C.c_uint32), ("step", C.c_uint32)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("a", C.c_uint32), ("b", C.c_uint32)]
+
+
+def _ptr(arr, ctype):
+    """Pointer into a contiguous numpy array (kept alive by the caller)."""
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_run_desc(req, keep: list) -> RunDesc:
+    """Fill an hgx_run_desc from a rules.Request; `keep` holds the backing buffers."""
+    import numpy as np
+
+    enc = req.encoded()
+    prog = np.frombuffer(enc["prog"], dtype=np.uint8).copy()
+    offs, verts, roots = enc["offs"], enc["verts"], enc["roots"]
+    if verts.size == 0:
+        verts = np.zeros(1, dtype=np.uint32)
+    keep.extend([prog, offs, verts, roots])
+    d = RunDesc()
+    d.rules = _ptr(prog, C.c_uint8)
+    d.rules_len = len(enc["prog"])
+    d.n_rules = len(req.rules)
+    d.edge_offsets = _ptr(offs, C.c_uint32)
+    d.vertices = _ptr(verts, C.c_uint32)
+    d.n_edges = len(offs) - 1
+    d.root_offsets = _ptr(roots, C.c_uint32) if roots is not None else None
+    d.n_roots = (len(roots) - 1) if roots is not None else 1
+    d.steps = int(req.steps)
+    d.state_cap = int(req.state_cap)
+    d.flags = int(req.flags)
+    return d

@@ -1,0 +1,221 @@
+/*
+ * hgx.h -- C ABI of the MI355X multiway-evolution backend.
+ *
+ * Plain C: PODs, pointers and sizes, integer status codes, no exceptions and no
+ * framework types cross this boundary. Host C++ (include/hgx_engine.hpp), Python
+ * (ctypes) and a paclet FFI all bind to exactly these symbols.
+ *
+ * WHAT ONE REQUEST IS. One multiway evolution, with the semantics of the reference
+ * CPU engine in StateCanonicalizationMode::None + EVENT_SIG_NONE: every raw state is
+ * expanded, every match is an event, every event creates its own raw state, and
+ * each raw state carries its WL canonical hash
+ *   (hypergraph/src/parallel_evolution.cpp:74-180, hypergraph/src/rewriter.cpp:11-205,
+ *    hypergraph/src/hypergraph.cpp:154-254, common/include/hgcommon/wl_core.hpp:67-202).
+ * NumStates is the number of distinct WL hashes over the raw states; the raw count is
+ * reported beside it.
+ *
+ * WHAT THIS REPLACES.
+ *   hgx_evolve_batch / hgx_request_*  stand in for hg_gpu::PersistentEvolver::run
+ *       (gpu/include/hg_gpu/evolve.hpp:466-497) as called by run_gpu_evolution
+ *       (paclet_source/hg_gpu_backend.cpp:174-183), and sit under the C++ adapter
+ *       hgx::ParallelEvolutionEngine that mirrors ParallelEvolutionEngine::add_rule /
+ *       evolve (hypergraph/include/hypergraph/parallel_evolution.hpp:741-1041).
+ *   hgx_wl_hash_batch               stands in for the per-state WL call
+ *       Hypergraph::compute_wl_hash (hypergraph/src/hypergraph.cpp:740-755).
+ *   hgx_match_count                 stands in for the all-matches join of one state
+ *       (common/include/hgcommon/join_core.hpp:143-208, gpu/src/match.cu:343-398).
+ */
+#ifndef HGX_H
+#define HGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Structural limits, the reference's (common/include/hgcommon/core.hpp:42-44). */
+#define HGX_MAX_ARITY          16
+#define HGX_MAX_PATTERN_EDGES  16
+#define HGX_MAX_VARS           32
+/* Causal lineage masks are 64-bit, one bit per generation. */
+#define HGX_MAX_STEPS          63
+
+/* Status codes. Every entry point returns one; hgx_last_error() gives the text. */
+enum {
+    HGX_OK = 0,
+    HGX_ERR_INVALID = 1,      /* malformed rule / state / argument                    */
+    HGX_ERR_NO_DEVICE = 2,    /* no HIP device or the kernels are missing for it       */
+    HGX_ERR_DEVICE = 3,       /* a HIP runtime call failed                             */
+    HGX_ERR_CAPACITY = 4,     /* the request does not fit the context's HBM budget     */
+    HGX_ERR_NOT_READY = 5,    /* hgx_request_poll: still running                        */
+    HGX_ERR_UNSUPPORTED = 6   /* a size beyond what this build handles                  */
+};
+
+/* Request flags (hgx_run_desc.flags). Recording flags mirror hgcommon::RecordSet
+ * (common/include/hgcommon/core.hpp:128-155); the TR flag mirrors
+ * ParallelEvolutionEngine::set_transitive_reduction (parallel_evolution.hpp:781). */
+#define HGX_REC_CAUSAL      0x1u   /* count causal triples / pairs                   */
+#define HGX_REC_BRANCHIAL   0x2u   /* count branchial pairs                          */
+#define HGX_REC_RELATIONS   0x4u   /* also materialise the pair lists for readback   */
+#define HGX_REC_STATES      0x8u   /* keep every state's edge list for readback      */
+#define HGX_OPT_TR          0x100u /* causal pairs: keep the transitive-reduction tag */
+#define HGX_REC_DEFAULT     (HGX_REC_CAUSAL | HGX_REC_BRANCHIAL)
+
+/*
+ * RULE PROGRAM. A rule set is a byte string, rule after rule:
+ *   n_lhs, n_rhs,
+ *   n_lhs x ( arity, var[arity] ),
+ *   n_rhs x ( arity, var[arity] )
+ * Variables are small integers < HGX_MAX_VARS. A variable that occurs only on the RHS
+ * is fresh: each application mints a new vertex for it, ascending by variable index
+ * (common/include/hgcommon/rewrite_core.hpp:53-62). Rule i is reported as rule index i.
+ * Limits: 1 <= n_lhs <= 16, 0 <= n_rhs <= 16, 1 <= arity <= 16.
+ */
+
+/* One evolution request. Vertex ids of the initial state(s) are arbitrary uint32; each
+ * root is renumbered densely by first appearance, as the FFI does
+ * (paclet_source/hypergraph_ffi.cpp:599-636). */
+typedef struct hgx_run_desc {
+    const uint8_t*  rules;         /* rule program                                    */
+    uint32_t        rules_len;     /* bytes in the rule program                       */
+    uint32_t        n_rules;
+    const uint32_t* edge_offsets;  /* [n_edges+1] offsets of each edge in `vertices`   */
+    const uint32_t* vertices;
+    uint32_t        n_edges;
+    const uint32_t* root_offsets;  /* [n_roots+1] edge-index ranges, or NULL: one root */
+    uint32_t        n_roots;
+    uint32_t        steps;         /* generations; 0 gives the roots only             */
+    uint32_t        state_cap;     /* 0 = none. Otherwise the run stops after the
+                                      largest step s <= steps whose cumulative raw
+                                      state count is <= state_cap.                    */
+    uint32_t        flags;         /* HGX_REC_* | HGX_OPT_TR                          */
+} hgx_run_desc;
+
+/* Per-request totals. Field meanings follow the reference's counters:
+ *   raw_states        Hypergraph::num_states()
+ *   events            Hypergraph::num_events()
+ *   canonical_states  distinct WL hashes over the raw states
+ *   causal_triples    CausalGraph::num_causal_edges() with TR off (one per consumed
+ *                     edge that has a producer event)
+ *   causal_pairs      CausalGraph::num_causal_event_pairs() with TR off
+ *   causal_pairs_tr   the same relation after transitive reduction
+ *   causal_triples_tr triples whose pair survives the reduction (num_causal_edges, TR on)
+ *   branchial_pairs   CausalGraph::num_branchial_edges()
+ *   fingerprint       sum of mix64(h) over distinct WL hashes
+ *                     (reference/golden_matrix.hpp:101-107)
+ * (hypergraph/src/causal_graph.cpp:187-234, :324-351, :356-386). */
+typedef struct hgx_counts {
+    uint64_t raw_states;
+    uint64_t events;
+    uint64_t canonical_states;
+    uint64_t causal_triples;
+    uint64_t causal_pairs;
+    uint64_t causal_pairs_tr;
+    uint64_t causal_triples_tr;
+    uint64_t branchial_pairs;
+    uint64_t fingerprint;
+    uint32_t steps_done;       /* generations actually evolved                        */
+    uint32_t truncated;        /* 1 when state_cap stopped the run before `steps`      */
+} hgx_counts;
+
+/* Event record. States are numbered roots first (0..n_roots-1), then the child of event
+ * e is state n_roots + e. Events are numbered generation by generation; inside a
+ * generation by input state, then rule, then match. */
+typedef struct hgx_event {
+    uint32_t input_state;
+    uint32_t output_state;
+    uint32_t rule;
+    uint32_t step;             /* the output state's step                               */
+} hgx_event;
+
+typedef struct hgx_pair {
+    uint32_t a;                /* causal: producer event; branchial: smaller event id   */
+    uint32_t b;                /* causal: consumer event; branchial: larger event id    */
+} hgx_pair;
+
+typedef struct hgx_ctx hgx_ctx;
+typedef struct hgx_result hgx_result;
+
+/* ---- context ------------------------------------------------------------------------ */
+
+/* budget_bytes = 0 takes 80% of the device's free HBM. */
+int  hgx_ctx_create(int device, uint64_t budget_bytes, hgx_ctx** out);
+void hgx_ctx_destroy(hgx_ctx* ctx);
+/* Text of the last failure on this context (or of the last failed hgx_ctx_create when
+ * ctx is NULL). Returns the full length; writes at most len-1 bytes plus NUL. */
+size_t hgx_last_error(const hgx_ctx* ctx, char* buf, size_t len);
+/* Kernel/device description string (arch, CU count, HBM), for reports. */
+size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
+
+/* ---- synchronous batch: the plain and the batched path -------------------------------- */
+
+/* Evolve n independent requests together on the device, each with its own rules, roots,
+ * steps and cap. results[i] receives a handle per request (release each). Results do not
+ * depend on which other requests share the batch. */
+int hgx_evolve_batch(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_result** results);
+
+/* ---- served path: continuous batching ------------------------------------------------- */
+
+/* Queue a request. It is admitted when its HBM footprint estimate fits what running
+ * requests leave of the budget, joins the next generation launch, and completes on its
+ * own. Submission is thread-safe. */
+int hgx_request_submit(hgx_ctx* ctx, const hgx_run_desc* run, uint64_t* request_id);
+/* Drive the device by one generation of every admitted request (called by the serving
+ * loop; hgx_request_wait calls it too). Returns the number of requests still live. */
+int hgx_serve_step(hgx_ctx* ctx, uint32_t* live);
+/* HGX_OK and *out set when done; HGX_ERR_NOT_READY while running. */
+int hgx_request_poll(hgx_ctx* ctx, uint64_t request_id, hgx_result** out);
+int hgx_request_wait(hgx_ctx* ctx, uint64_t request_id, hgx_result** out);
+/* Footprint estimate (bytes of HBM) the admission control charges for a request. */
+uint64_t hgx_request_footprint(const hgx_run_desc* run);
+
+/* ---- results ------------------------------------------------------------------------- */
+
+int hgx_result_counts(const hgx_result* r, hgx_counts* out);
+/* Per raw state: WL hash, step, and the event that created it (0xFFFFFFFF for roots).
+ * Any pointer may be NULL. cap is the capacity in elements; returns HGX_ERR_INVALID if
+ * cap < raw_states. */
+int hgx_result_states(const hgx_result* r, uint64_t* hash, uint32_t* step,
+                      uint32_t* creator_event, uint64_t cap);
+int hgx_result_events(const hgx_result* r, hgx_event* out, uint64_t cap);
+/* Edge lists of every raw state (needs HGX_REC_STATES): state_offsets[raw_states+1] in
+ * edges, edge_offsets[n_edges_total+1] in vertices, vertices[]. Call with NULL arrays to
+ * learn the totals. Vertex labels are per-lineage: each root renumbered from 0, fresh
+ * vertices continue from the largest label on the lineage. */
+int hgx_result_state_edges(const hgx_result* r, uint64_t* state_offsets, uint32_t* edge_offsets,
+                           uint32_t* vertices, uint64_t* n_edges_total, uint64_t* n_vertices_total);
+/* Causal pairs (needs HGX_REC_CAUSAL|HGX_REC_RELATIONS), deduplicated; in_tr[i] = 1 if the
+ * pair survives transitive reduction. */
+int hgx_result_causal(const hgx_result* r, hgx_pair* pairs, uint8_t* in_tr, uint64_t cap);
+/* Branchial pairs (needs HGX_REC_BRANCHIAL|HGX_REC_RELATIONS), (smaller, larger) event ids. */
+int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64_t cap);
+/* Device time spent on this request's generations, and its latency
+ * (completion - submission) on the served path, in nanoseconds. */
+int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint64_t* latency_ns);
+void hgx_result_release(hgx_result* r);
+
+/* ---- building blocks, exposed for tests and the match-join workload -------------------- */
+
+/* WL canonical hash of n states given as edge lists (state_offsets in edges,
+ * edge_offsets in vertices), computed on the device. Bit-identical to
+ * hgcommon::wl_canonical_hash with the caller sizing of wl_hash.hpp:109-239; an empty
+ * state hashes to EMPTY_STATE_CANONICAL_HASH (hypergraph/src/hypergraph.cpp:743-745). */
+int hgx_wl_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
+                      const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash);
+
+/* Number of matches of rule `rule` (index into the program) in one state, computed on the
+ * device with the large-state path (CSR incidence index + seeded join). If
+ * match_edges is non-NULL it receives up to cap matches, n_lhs edge indices each in
+ * pattern order. */
+int hgx_match_count(hgx_ctx* ctx, const uint8_t* rules, uint32_t rules_len, uint32_t n_rules,
+                    uint32_t rule, const uint32_t* edge_offsets, const uint32_t* vertices,
+                    uint32_t n_edges, uint64_t* count, uint32_t* match_edges, uint64_t cap,
+                    float* device_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGX_H */

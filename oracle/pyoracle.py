@@ -1,0 +1,230 @@
+"""TEST INFRASTRUCTURE: Python bindings of the CPU checker.
+
+* ``evolve(req)``       -- the C restatement (liboracle.so, hgx_oracle.c)
+* ``ref_evolve(req)``   -- the reference CPU engine itself (oracle/_ref/hgref), when built
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from hypergraphrewritingengine_amd import abi
+from hypergraphrewritingengine_amd.rules import Request, encode_rules, encode_state
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+        L = C.CDLL(path)
+        L.hgxo_evolve.argtypes = [C.POINTER(abi.RunDesc), C.POINTER(C.c_void_p)]
+        L.hgxo_free.argtypes = [C.c_void_p]
+        L.hgxo_counts.argtypes = [C.c_void_p, C.POINTER(abi.Counts)]
+        L.hgxo_states.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hgxo_events.argtypes = [C.c_void_p, C.c_void_p]
+        L.hgxo_num_causal_pairs.argtypes = [C.c_void_p]
+        L.hgxo_num_causal_pairs.restype = C.c_uint64
+        L.hgxo_causal.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hgxo_num_branchial_pairs.argtypes = [C.c_void_p]
+        L.hgxo_num_branchial_pairs.restype = C.c_uint64
+        L.hgxo_branchial.argtypes = [C.c_void_p, C.c_void_p]
+        L.hgxo_state_edges.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.hgxo_wl_hash.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.hgxo_wl_hash.restype = C.c_uint64
+        L.hgxo_match.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]
+        L.hgxo_match.restype = C.c_int64
+        L.hgxo_match_order.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+class OracleResult:
+    """Everything the restatement computed for one request, as numpy arrays."""
+
+    def __init__(self, counts, hashes, steps, creators, events, causal, causal_tr, branchial,
+                 state_edges=None):
+        self.counts = counts
+        self.state_hash = hashes
+        self.state_step = steps
+        self.state_creator = creators
+        self.events = events          # (n, 4) uint32: input, output, rule, step
+        self.causal = causal          # (m, 2) uint32: producer, consumer
+        self.causal_tr = causal_tr    # (m,) uint8
+        self.branchial = branchial    # (k, 2) uint32
+        self.state_edges = state_edges
+
+
+def evolve(req: Request, with_edges: bool = False) -> OracleResult:
+    L = lib()
+    keep: list = []
+    d = abi.make_run_desc(req, keep)
+    h = C.c_void_p()
+    rc = L.hgxo_evolve(C.byref(d), C.byref(h))
+    if rc != abi.HGX_OK:
+        raise ValueError(f"oracle rejected the request (status {rc})")
+    try:
+        cnt = abi.Counts()
+        L.hgxo_counts(h, C.byref(cnt))
+        ns, ne = cnt.raw_states, cnt.events
+        hashes = np.zeros(ns, np.uint64)
+        steps = np.zeros(ns, np.uint32)
+        creators = np.zeros(ns, np.uint32)
+        L.hgxo_states(h, hashes.ctypes.data, steps.ctypes.data, creators.ctypes.data)
+        events = np.zeros((ne, 4), np.uint32)
+        if ne:
+            L.hgxo_events(h, events.ctypes.data)
+        nc = L.hgxo_num_causal_pairs(h)
+        causal = np.zeros((nc, 2), np.uint32)
+        ctr = np.zeros(nc, np.uint8)
+        if nc:
+            L.hgxo_causal(h, causal.ctypes.data, ctr.ctypes.data)
+        nb = L.hgxo_num_branchial_pairs(h)
+        br = np.zeros((nb, 2), np.uint32)
+        if nb:
+            L.hgxo_branchial(h, br.ctypes.data)
+        edges = None
+        if with_edges:
+            tot_e, tot_v = C.c_uint64(), C.c_uint64()
+            L.hgxo_state_edges(h, None, None, None, C.byref(tot_e), C.byref(tot_v))
+            so = np.zeros(ns + 1, np.uint64)
+            eo = np.zeros(tot_e.value + 1, np.uint32)
+            vv = np.zeros(max(tot_v.value, 1), np.uint32)
+            L.hgxo_state_edges(h, so.ctypes.data, eo.ctypes.data, vv.ctypes.data, None, None)
+            edges = (so, eo, vv)
+        return OracleResult(cnt.as_dict(), hashes, steps, creators, events, causal, ctr, br, edges)
+    finally:
+        L.hgxo_free(h)
+
+
+def wl_hash(edges) -> int:
+    offs, verts = encode_state(edges)
+    v = verts if verts.size else np.zeros(1, np.uint32)
+    return int(lib().hgxo_wl_hash(offs.ctypes.data, v.ctypes.data, len(edges)))
+
+
+def match(rules, rule_index: int, edges, want_edges: bool = False, cap: int = 0):
+    prog = np.frombuffer(encode_rules(rules), np.uint8).copy()
+    offs, verts = encode_state(edges)
+    v = verts if verts.size else np.zeros(1, np.uint32)
+    k = len(rules[rule_index].lhs)
+    out = np.zeros(max(cap, 1) * k, np.uint32) if want_edges else None
+    n = lib().hgxo_match(prog.ctypes.data, len(prog), len(rules), rule_index, offs.ctypes.data,
+                         v.ctypes.data, len(edges), out.ctypes.data if want_edges else None,
+                         cap if want_edges else 0)
+    if n < 0:
+        raise ValueError("oracle rejected the rule")
+    if want_edges:
+        return n, out[: min(n, cap) * k].reshape(-1, k)
+    return n
+
+
+def match_order(rules, rule_index: int):
+    prog = np.frombuffer(encode_rules(rules), np.uint8).copy()
+    out = np.zeros(16, np.uint8)
+    lib().hgxo_match_order(prog.ctypes.data, len(prog), len(rules), rule_index, out.ctypes.data)
+    return [int(x) for x in out[: len(rules[rule_index].lhs)]]
+
+
+# ---- the reference engine (compiled from the reference sources by oracle/Makefile) ----
+
+REF_BIN = os.path.join(_HERE, "_ref", "hgref")
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def ref_evolve(req: Request, threads: int = 1, reps: int = 1, tr: bool = False,
+               dump: bool = False, timeout: float = 600.0):
+    """Run the reference CPU engine; returns (counts dict, dump dict or None)."""
+    with tempfile.TemporaryDirectory() as td:
+        job = os.path.join(td, "job.txt")
+        with open(job, "w") as f:
+            f.write(req.job_text(tr=tr))
+        cmd = [REF_BIN, job, "--threads", str(threads), "--reps", str(reps)]
+        out = os.path.join(td, "dump.txt")
+        if dump:
+            cmd += ["--dump", out]
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"hgref failed: {p.stderr}")
+        counts = json.loads(p.stdout.strip().splitlines()[-1])
+        d = None
+        if dump:
+            S, E, Cz, B = {}, [], [], []
+            with open(out) as f:
+                for line in f:
+                    t = line.split()
+                    if t[0] == "S":
+                        S[int(t[1])] = (int(t[2]), int(t[3]))
+                    elif t[0] == "E":
+                        E.append(tuple(int(x) for x in t[1:]))
+                    elif t[0] == "C":
+                        Cz.append((int(t[1]), int(t[2])))
+                    elif t[0] == "B":
+                        B.append((int(t[1]), int(t[2])))
+            d = {"states": S, "events": E, "causal": Cz, "branchial": B}
+        return counts, d
+
+
+# ---- normalisation: gpu/tests/test_gpu_vs_cpu_differential.cpp:122-160 ----
+
+_FNV_OFF = 14695981039346656037
+_FNV_P = 1099511628211
+_M = (1 << 64) - 1
+
+
+def _mix(h, x):
+    return ((h ^ x) * _FNV_P) & _M
+
+
+def event_key(ih, oh, rule, step):
+    h = _FNV_OFF
+    for x in (ih, oh, rule, step):
+        h = _mix(h, int(x))
+    return h
+
+
+def pair_key(a, b):
+    lo, hi = (a, b) if a < b else (b, a)
+    return _mix(_mix(_FNV_OFF, lo), hi)
+
+
+def causal_key(p, c):
+    return _mix(_mix(_FNV_OFF, p), c)
+
+
+def normalize(state_hash, events, causal_pairs, branchial_pairs, causal_triples=None):
+    """(set of state hashes, multiset of event keys, multiset of causal keys, multiset of
+    branchial keys) -- the cross-engine comparison of the reference's differential suite.
+    state_hash: {state id: hash}; events: iterable of (eid, in, out, rule, step)."""
+    from collections import Counter
+
+    hs = set(int(h) for h in state_hash.values())
+    ek = {}
+    for (eid, i, o, r, s) in events:
+        ek[eid] = event_key(state_hash[i], state_hash[o], r, s)
+    evk = Counter(ek.values())
+    ck = Counter(causal_key(ek[p], ek[c]) for (p, c) in causal_pairs)
+    bk = Counter(pair_key(ek[a], ek[b]) for (a, b) in branchial_pairs)
+    return hs, evk, ck, bk
+
+
+def normalize_oracle(res: OracleResult, triples: bool = False):
+    sh = {i: int(h) for i, h in enumerate(res.state_hash)}
+    ev = [(e, *map(int, res.events[e])) for e in range(len(res.events))]
+    return normalize(sh, ev, [tuple(map(int, p)) for p in res.causal],
+                     [tuple(map(int, p)) for p in res.branchial])
