@@ -1,0 +1,226 @@
+"""hypergraphrewritingengine_amd -- MI355X-native multiway hypergraph evolution.
+
+Python face of the C ABI in include/hgx.h (libhgx.so, HIP kernels for gfx950). The C++ face
+is include/hgx_engine.hpp. Nothing here falls back to a CPU path: without the built library
+or without a gfx950 device every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .rules import (Request, Rule, RuleBuilder, encode_roots, encode_rules, encode_state,
+                    make_rule, rule_from_lists)
+
+__all__ = ["Context", "Result", "Request", "Rule", "RuleBuilder", "make_rule", "rule_from_lists",
+           "encode_rules", "encode_state", "encode_roots", "library_path", "load_library",
+           "HGXError", "EXPORTED_SYMBOLS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# Every symbol include/hgx.h declares.
+EXPORTED_SYMBOLS = [
+    "hgx_ctx_create", "hgx_ctx_destroy", "hgx_last_error", "hgx_device_info",
+    "hgx_evolve_batch", "hgx_request_submit", "hgx_serve_step", "hgx_request_poll",
+    "hgx_request_wait", "hgx_request_footprint", "hgx_result_counts", "hgx_result_states",
+    "hgx_result_events", "hgx_result_state_edges", "hgx_result_causal", "hgx_result_branchial",
+    "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
+]
+
+
+class HGXError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hgx error {code}: {msg}")
+        self.code = code
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libhgx.so")
+
+
+def load_library():
+    """Load libhgx.so (built in-tree by __graft_entry__.build()). Raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise HGXError(abi.HGX_ERR_NO_DEVICE,
+                       f"{path} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.hgx_ctx_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.hgx_ctx_destroy.argtypes = [vp]
+    L.hgx_ctx_destroy.restype = None
+    L.hgx_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.hgx_last_error.restype = C.c_size_t
+    L.hgx_device_info.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.hgx_device_info.restype = C.c_size_t
+    L.hgx_evolve_batch.argtypes = [vp, C.POINTER(abi.RunDesc), C.c_uint32, C.POINTER(vp)]
+    L.hgx_request_submit.argtypes = [vp, C.POINTER(abi.RunDesc), C.POINTER(C.c_uint64)]
+    L.hgx_serve_step.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.hgx_request_poll.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.hgx_request_wait.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.hgx_request_footprint.argtypes = [C.POINTER(abi.RunDesc)]
+    L.hgx_request_footprint.restype = C.c_uint64
+    L.hgx_result_counts.argtypes = [vp, C.POINTER(abi.Counts)]
+    L.hgx_result_states.argtypes = [vp, vp, vp, vp, C.c_uint64]
+    L.hgx_result_events.argtypes = [vp, vp, C.c_uint64]
+    L.hgx_result_state_edges.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hgx_result_causal.argtypes = [vp, vp, vp, C.c_uint64]
+    L.hgx_result_branchial.argtypes = [vp, vp, C.c_uint64]
+    L.hgx_result_timing.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.hgx_result_release.argtypes = [vp]
+    L.hgx_result_release.restype = None
+    L.hgx_wl_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
+    L.hgx_match_count.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
+                                  C.POINTER(C.c_uint64), vp, C.c_uint64, C.POINTER(C.c_float)]
+    _LIB = L
+    return L
+
+
+class Result:
+    """One request's result (hgx_result*). Arrays are read back lazily."""
+
+    def __init__(self, lib, handle, ctx):
+        self._lib = lib
+        self._h = handle
+        self._ctx = ctx           # keep the context alive while results exist
+        c = abi.Counts()
+        self._check(lib.hgx_result_counts(handle, C.byref(c)))
+        self.counts = c.as_dict()
+
+    def _check(self, rc):
+        if rc != abi.HGX_OK:
+            raise HGXError(rc, "result readback failed")
+
+    def states(self):
+        n = self.counts["raw_states"]
+        h = np.zeros(n, np.uint64)
+        s = np.zeros(n, np.uint32)
+        c = np.zeros(n, np.uint32)
+        self._check(self._lib.hgx_result_states(self._h, h.ctypes.data, s.ctypes.data, c.ctypes.data, n))
+        return h, s, c
+
+    def events(self):
+        n = self.counts["events"]
+        ev = np.zeros((n, 4), np.uint32)
+        if n:
+            self._check(self._lib.hgx_result_events(self._h, ev.ctypes.data, n))
+        return ev
+
+    def causal(self):
+        n = self.counts["causal_pairs"]
+        p = np.zeros((n, 2), np.uint32)
+        t = np.zeros(n, np.uint8)
+        self._check(self._lib.hgx_result_causal(self._h, p.ctypes.data if n else None,
+                                                t.ctypes.data if n else None, n))
+        return p, t
+
+    def branchial(self):
+        n = self.counts["branchial_pairs"]
+        p = np.zeros((n, 2), np.uint32)
+        if n:
+            self._check(self._lib.hgx_result_branchial(self._h, p.ctypes.data, n))
+        return p
+
+    def state_edges(self):
+        te, tv = C.c_uint64(), C.c_uint64()
+        self._check(self._lib.hgx_result_state_edges(self._h, None, None, None, C.byref(te), C.byref(tv)))
+        n = self.counts["raw_states"]
+        so = np.zeros(n + 1, np.uint64)
+        eo = np.zeros(te.value + 1, np.uint32)
+        vv = np.zeros(max(tv.value, 1), np.uint32)
+        self._check(self._lib.hgx_result_state_edges(self._h, so.ctypes.data, eo.ctypes.data,
+                                                     vv.ctypes.data, None, None))
+        return so, eo, vv
+
+    def timing(self):
+        d, l = C.c_uint64(), C.c_uint64()
+        self._check(self._lib.hgx_result_timing(self._h, C.byref(d), C.byref(l)))
+        return d.value, l.value
+
+    def close(self):
+        if self._h:
+            self._lib.hgx_result_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """An hgx_ctx on one device: the HBM budget, the stream, the served-request queue."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        rc = self._lib.hgx_ctx_create(device, budget_bytes, C.byref(h))
+        if rc != abi.HGX_OK:
+            buf = C.create_string_buffer(512)
+            self._lib.hgx_last_error(None, buf, 512)
+            raise HGXError(rc, buf.value.decode())
+        self._h = h
+
+    def _err(self, rc):
+        buf = C.create_string_buffer(1024)
+        self._lib.hgx_last_error(self._h, buf, 1024)
+        return HGXError(rc, buf.value.decode())
+
+    @property
+    def info(self) -> str:
+        buf = C.create_string_buffer(512)
+        self._lib.hgx_device_info(self._h, buf, 512)
+        return buf.value.decode()
+
+    def evolve_batch(self, requests):
+        """Evolve requests together on the device; returns one Result per request."""
+        keep: list = []
+        n = len(requests)
+        descs = (abi.RunDesc * max(n, 1))()
+        for i, r in enumerate(requests):
+            descs[i] = abi.make_run_desc(r, keep)
+        outs = (C.c_void_p * max(n, 1))()
+        rc = self._lib.hgx_evolve_batch(self._h, descs, n, outs)
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return [Result(self._lib, C.c_void_p(outs[i]), self) for i in range(n)]
+
+    def evolve(self, request):
+        return self.evolve_batch([request])[0]
+
+    def wl_hash(self, states):
+        """WL hashes of a list of states (each a list of edges)."""
+        offs = [0]
+        edges: list = []
+        for st in states:
+            edges.extend(st)
+            offs.append(len(edges))
+        eo, vv = encode_state(edges) if edges else (np.zeros(1, np.uint32), np.zeros(1, np.uint32))
+        so = np.asarray(offs, np.uint64)
+        if vv.size == 0:
+            vv = np.zeros(1, np.uint32)
+        out = np.zeros(len(states), np.uint64)
+        rc = self._lib.hgx_wl_hash_batch(self._h, so.ctypes.data, len(states), eo.ctypes.data,
+                                         vv.ctypes.data, out.ctypes.data)
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.hgx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

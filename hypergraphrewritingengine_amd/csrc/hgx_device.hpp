@@ -1,0 +1,148 @@
+// hgx_device.hpp -- shared device-side definitions for the gfx950 kernels: integer hash
+// primitives, the device rule/run/generation views, and wave64 primitives.
+//
+// The hash primitives are the reference's (common/include/hgcommon/core.hpp:47-87). They are
+// restated here because the whole WL value must be bit-identical to the reference's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hgx {
+
+constexpr uint64_t FNV_OFFSET = 0xcbf29ce484222325ULL;   // core.hpp:47
+constexpr uint64_t FNV_PRIME = 0x100000001b3ULL;         // core.hpp:48
+constexpr uint64_t EMPTY_STATE_HASH = 0x9E3779B97F4A7C15ULL;  // core.hpp:169
+constexpr uint32_t WL_MAX_ITERS = 100;                   // wl_core.hpp:52
+constexpr uint32_t INV = 0xFFFFFFFFu;
+constexpr int WAVE = 64;
+
+__host__ __device__ inline uint64_t fnv(uint64_t h, uint64_t v) { return (h ^ v) * FNV_PRIME; }
+
+__host__ __device__ inline uint64_t mix64(uint64_t x) {   // core.hpp:64-71
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33; return x;
+}
+
+__host__ __device__ inline uint64_t splitmix(uint64_t z) {   // core.hpp:82-87
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// A rule as the kernels read it. LHS order is the reference's join order
+// (hypergraph/src/pattern.cpp:43-75), computed on the host; var_src_* name the first LHS
+// (edge, position) of each LHS variable so a rewrite resolves a bound vertex in O(1).
+struct DRule {
+    uint8_t n_lhs, n_rhs, n_new, pad0;
+    uint8_t la[16];
+    uint8_t lv[16][16];
+    uint8_t ra[16];
+    uint8_t rv[16][16];
+    uint8_t order[16];
+    uint8_t var_src_edge[32];
+    uint8_t var_src_pos[32];
+    uint32_t lhs_mask, rhs_mask, new_mask, pad1;
+};
+
+// Per request (run) state the kernels read and update.
+struct DRun {
+    uint32_t rule_begin, n_rules;
+    uint32_t n_roots, flags;
+    uint64_t* map_keys;      // open-addressing WL-hash set, first writer wins
+    uint32_t* map_vals;
+    uint32_t map_mask;       // capacity - 1 (power of two)
+    uint32_t zero_seen;      // the hash 0 is kept beside the table (0 is EMPTY)
+    uint32_t ev_base;        // events before the current generation
+    uint32_t child_begin;    // first child index of this run in the current generation
+};
+
+// Per-run counters, accumulated on the device.
+enum {
+    CTR_EVENTS = 0, CTR_CANON, CTR_TRIPLES, CTR_PAIRS, CTR_PAIRS_TR, CTR_TRIPLES_TR,
+    CTR_BRANCHIAL, CTR_FINGERPRINT, CTR_N
+};
+
+// One generation of states (all runs mixed; each run's states contiguous).
+struct GenView {
+    uint32_t n;               // states
+    uint32_t step;            // generation index of these states
+    uint32_t A;               // vertex stride per edge (batch max arity)
+    uint32_t* run;            // [n]
+    uint64_t* eoff;           // [n] first edge in the pool
+    uint32_t* ne;             // [n]
+    uint32_t* nextv;          // [n] next fresh vertex label on the lineage
+    uint32_t* sid;            // [n] state id inside its run
+    uint64_t* hash;           // [n] WL hash
+    uint64_t* lanc;           // [n * step] causal-ancestor masks of the lineage's events
+    uint32_t* ev;             // edge pool: vertices [edges * A]
+    uint8_t* ar;              // edge pool: arity
+    uint8_t* ps;              // edge pool: generation of the producing event (0 = none)
+};
+
+// The matches of one generation (one per event / child).
+struct MatchView {
+    uint32_t n;
+    uint32_t K;               // positions stored per match (batch max n_lhs)
+    uint32_t* parent;         // [n] parent index in its generation
+    uint32_t* rule;           // [n] global rule index
+    uint16_t* q;              // [n * K] matched edge positions in the parent, pattern order
+    uint32_t* cne;            // [n] child edge count
+};
+
+// Per-event outputs of one generation.
+struct EventView {
+    uint32_t* in_state;       // [n] state id of the input (in run)
+    uint32_t* rule;           // [n] rule index inside the run
+    uint64_t* prod;           // [n] generations of the distinct producer events
+    uint64_t* tr;             // [n] the subset kept by transitive reduction
+};
+
+// ---------------- wave64 primitives ----------------
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) { uint32_t w = __shfl_xor(v, o, WAVE); v = w > v ? w : v; }
+    return v;
+}
+
+__device__ inline uint64_t wave_sum_u64(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)v, o, WAVE);
+        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), o, WAVE);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Inclusive prefix sum across the wave.
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
+    const uint32_t l = lane_id();
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t w = __shfl_up(v, o, WAVE);
+        if (l >= (uint32_t)o) v += w;
+    }
+    return v;
+}
+
+}  // namespace hgx

@@ -1,0 +1,580 @@
+// hgx_kernels.hip -- gfx950 kernels of one multiway generation.
+//
+//   k_match<false>   SCAN -> EXPAND -> SINK join, counting matches per state
+//   k_match<true>    the same join, writing match records in the deterministic order
+//   k_expand         rewrite + WL hash + state-map dedup + event + causal emission, one wave
+//                    per child state
+//   k_branchial      sibling-event overlap pairs, one wave per parent state
+//   k_wl_states      WL hash of given states (roots, and the hgx_wl_hash_batch entry point)
+//   small helpers    per-run totals, count masking, map rehash
+//
+// Every kernel is wave64-native: one wave owns one state (or one child), stages that state's
+// edges in LDS and cooperates through ballots, mbcnt prefix sums and LDS atomics. There is no
+// MFMA: nothing here is a dense contraction.
+#include <hip/hip_runtime.h>
+
+#include "hgx_device.hpp"
+#include "hgx_kernels.hpp"
+#include "hgx_wl.hpp"
+
+namespace hgx {
+
+// Logical block index with consecutive indices kept on one XCD (blocks are dealt to the 8
+// XCDs round-robin), so neighbouring children -- which share their parent's edges -- hit the
+// same L2. gridDim.x is a multiple of 8 for every launch that uses it.
+__device__ inline uint32_t xcd_block() {
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    return (b % 8u) * (g / 8u) + b / 8u;
+}
+
+// ------------------------------------------------------------------------------------------
+// The join. A wave owns one parent state, whose edges are staged in LDS. For each rule of the
+// state's run:
+//   SCAN    lanes test every edge against the first position of the rule's join order
+//           (arity + the repetitions the pattern demands, match_core.hpp:32-49) and the seed
+//           list is compacted in edge order with a ballot + mbcnt prefix.
+//   EXPAND  each lane extends one seed depth-first over the remaining positions in join order,
+//           trying candidates in ascending edge order (join_core.hpp:143-180): edge-injective,
+//           vertex-non-injective.
+//   SINK    per-lane match counts are prefix-summed across the wave so every match lands at
+//           its place in the order (state, rule, seed, depth-first) -- the order the oracle
+//           enumerates, so event numbering is identical across engines and runs.
+// ------------------------------------------------------------------------------------------
+
+struct JoinLane {
+    uint32_t matched[16];   // edge index bound at each depth
+    uint32_t smask[17];     // bound-variable mask on entry to each depth
+    uint32_t cidx[16];      // next candidate at each depth
+    uint32_t bind[32];
+};
+
+__device__ inline bool bind_edge(const uint32_t* ev, uint32_t ea, const uint8_t* pv, uint32_t pa,
+                                 uint32_t* bind, uint32_t& mask) {   // match_core.hpp:32-49
+    if (ea != pa) return false;
+    for (uint32_t i = 0; i < ea; ++i) {
+        const uint32_t var = pv[i], bit = 1u << var;
+        if (mask & bit) { if (bind[var] != ev[i]) return false; }
+        else { bind[var] = ev[i]; mask |= bit; }
+    }
+    return true;
+}
+
+// Depth-first completion of one seed; returns the number of matches. With out != nullptr
+// every match is written at out_base + k (k-th match of this seed).
+__device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const uint8_t* sa,
+                                     uint32_t ne, uint32_t A, uint32_t seed, JoinLane& L,
+                                     uint16_t* q_out, uint32_t K, uint32_t out_base,
+                                     uint32_t* par_out, uint32_t* rule_out, uint32_t* cne_out,
+                                     uint32_t parent, uint32_t rule_id, uint32_t child_ne) {
+    const uint32_t n = R.n_lhs;
+    uint32_t mask = 0;
+    if (!bind_edge(sv + seed * A, sa[seed], R.lv[R.order[0]], R.la[R.order[0]], L.bind, mask))
+        return 0;
+    uint32_t found = 0;
+    L.matched[0] = seed;
+    if (n == 1) {
+        if (q_out) {
+            q_out[(size_t)out_base * K + R.order[0]] = (uint16_t)seed;
+            par_out[out_base] = parent; rule_out[out_base] = rule_id; cne_out[out_base] = child_ne;
+        }
+        return 1;
+    }
+    L.smask[1] = mask;
+    L.cidx[1] = 0;
+    uint32_t d = 1;
+    while (d >= 1) {
+        const uint32_t p = R.order[d];
+        bool advanced = false;
+        while (L.cidx[d] < ne) {
+            const uint32_t e = L.cidx[d]++;
+            bool taken = false;
+            for (uint32_t t = 0; t < d; ++t) taken |= (L.matched[t] == e);
+            if (taken) continue;
+            uint32_t m = L.smask[d];
+            if (!bind_edge(sv + e * A, sa[e], R.lv[p], R.la[p], L.bind, m)) continue;
+            L.matched[d] = e;
+            if (d + 1 == n) {
+                if (q_out) {
+                    const size_t o = (size_t)(out_base + found);
+                    for (uint32_t t = 0; t < n; ++t) q_out[o * K + R.order[t]] = (uint16_t)L.matched[t];
+                    par_out[o] = parent; rule_out[o] = rule_id; cne_out[o] = child_ne;
+                }
+                ++found;
+                continue;
+            }
+            L.smask[d + 1] = m;
+            L.cidx[d + 1] = 0;
+            ++d;
+            advanced = true;
+            break;
+        }
+        if (!advanced) --d;
+    }
+    return found;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_match(MatchArgs a) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    uint8_t* my = lds + (size_t)w * a.lds_per_wave;
+    const uint32_t A = a.P.A;
+    uint32_t* sv = (uint32_t*)my;                              // [ne * A] vertices
+    uint8_t* sa = my + (size_t)a.max_ne * A * 4;               // [ne] arity
+    uint16_t* seeds = (uint16_t*)(sa + ((a.max_ne + 7u) & ~7u));   // [ne]
+    JoinLane L;
+
+    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
+        const uint32_t run = a.P.run[s];
+        if (!a.active[run]) { if (!EMIT && l == 0) a.cnt[s] = 0; continue; }
+        const uint32_t ne = a.P.ne[s];
+        const uint64_t eo = a.P.eoff[s];
+        for (uint32_t i = l; i < ne * A; i += WAVE) sv[i] = a.P.ev[eo * A + i];
+        for (uint32_t i = l; i < ne; i += WAVE) sa[i] = a.P.ar[eo + i];
+        wave_sync();
+        const DRun& RN = a.runs[run];
+        uint32_t total = 0;
+        uint32_t out = EMIT ? (uint32_t)a.moff[s] : 0u;
+        for (uint32_t r = RN.rule_begin; r < RN.rule_begin + RN.n_rules; ++r) {
+            const DRule& R = a.rules[r];
+            const uint32_t p0 = R.order[0];
+            const uint32_t child_ne = ne - R.n_lhs + R.n_rhs;
+            // SCAN: compact the seeds of position order[0] in edge order.
+            uint32_t nseed = 0;
+            for (uint32_t base = 0; base < ne; base += WAVE) {
+                const uint32_t e = base + l;
+                bool ok = false;
+                if (e < ne && sa[e] == R.la[p0]) {
+                    uint32_t m = 0;
+                    ok = bind_edge(sv + e * A, sa[e], R.lv[p0], R.la[p0], L.bind, m);
+                }
+                const uint64_t bal = __ballot(ok);
+                if (ok) seeds[nseed + lanes_below(bal)] = (uint16_t)e;
+                nseed += __popcll(bal);
+            }
+            wave_sync();
+            // EXPAND + SINK, 64 seeds per round.
+            for (uint32_t base = 0; base < nseed; base += WAVE) {
+                const uint32_t i = base + l;
+                uint32_t c = 0;
+                if (i < nseed)
+                    c = join_seed(R, sv, sa, ne, A, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
+                                  nullptr, 0, 0, 0);
+                if (EMIT) {
+                    const uint32_t inc = wave_incl_scan_u32(c);
+                    if (i < nseed && c)
+                        join_seed(R, sv, sa, ne, A, seeds[i], L, a.M.q, a.M.K, out + inc - c,
+                                  a.M.parent, a.M.rule, a.M.cne, s, r, child_ne);
+                    out += __shfl(inc, WAVE - 1, WAVE);
+                } else {
+                    total += wave_sum_u32(c);
+                }
+            }
+            wave_sync();
+        }
+        if (!EMIT && l == 0) a.cnt[s] = total;
+    }
+}
+
+template __global__ void k_match<false>(MatchArgs);
+template __global__ void k_match<true>(MatchArgs);
+
+// ------------------------------------------------------------------------------------------
+// Rewrite + hash + dedup + event + causal, one wave per child (Rewriter::apply,
+// hypergraph/src/rewriter.cpp:11-205, with hgcommon rewrite semantics rewrite_core.hpp:53-87).
+// ------------------------------------------------------------------------------------------
+
+__device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
+    if (h == 0) return atomicExch(&run->zero_seen, 1u) == 0u;
+    const uint32_t mask = run->map_mask;
+    uint32_t i = (uint32_t)mix64(h) & mask;
+    while (true) {
+        const uint64_t old = atomicCAS((unsigned long long*)&run->map_keys[i], 0ull,
+                                       (unsigned long long)h);
+        if (old == 0) { run->map_vals[i] = sid; return true; }
+        if (old == h) return false;
+        i = (i + 1) & mask;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    uint8_t* my = lds + (size_t)w * a.lds_per_wave;
+    const uint32_t A = a.P.A;
+    const WLSizes z = WLSizes::make(a.max_child_ne, a.max_nv, A);
+    WLSpace s;
+    uint8_t* p = my;
+    wl_bind(s, p, z, A);
+    uint32_t* gv = (uint32_t*)carve(p, z.NE * A * 4);     // child vertices (labels)
+    uint32_t* vmap = (uint32_t*)carve(p, (a.max_nextv + 1) * 4);
+
+    // Contiguous chunk of children per wave; consecutive chunks on one XCD.
+    const uint32_t wpb = blockDim.x / WAVE;
+    const uint32_t gw = xcd_block() * wpb + w, nw = gridDim.x * wpb;
+    const uint32_t per = (a.M.n + nw - 1) / nw;
+    const uint32_t c0 = gw * per, c1 = min(a.M.n, c0 + per);
+
+    uint32_t acc_run = INV;
+    uint64_t acc[CTR_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto flush = [&]() {
+        if (l == 0 && acc_run != INV)
+            for (int k = 0; k < CTR_N; ++k)
+                if (acc[k]) atomicAdd((unsigned long long*)&a.ctr[(size_t)acc_run * CTR_N + k],
+                                      (unsigned long long)acc[k]);
+        for (int k = 0; k < CTR_N; ++k) acc[k] = 0;
+    };
+
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t par = a.M.parent[c];
+        const uint32_t rid = a.M.rule[c];
+        const DRule& R = a.rules[rid];
+        const uint32_t run = a.P.run[par];
+        if (run != acc_run) { flush(); acc_run = run; }
+        DRun* RN = &a.runs[run];
+        const uint32_t nl = R.n_lhs, nr = R.n_rhs;
+        const uint32_t pne = a.P.ne[par];
+        const uint64_t peo = a.P.eoff[par];
+        const uint64_t ceo = a.C.eoff[c];
+        const uint32_t cne = pne - nl + nr;
+        const uint32_t step = a.C.step;                      // generation of the child
+        const uint16_t* q = a.M.q + (size_t)c * a.M.K;
+
+        // Parent edges minus the consumed ones, order kept (SparseBitset::derive).
+        uint32_t kept = 0;
+        for (uint32_t base = 0; base < pne; base += WAVE) {
+            const uint32_t i = base + l;
+            bool keep = i < pne;
+            if (keep) for (uint32_t t = 0; t < nl; ++t) keep &= (q[t] != i);
+            const uint64_t bal = __ballot(keep);
+            if (keep) {
+                const uint32_t pos = kept + lanes_below(bal);
+                const uint8_t ar = a.P.ar[peo + i];
+                for (uint32_t k = 0; k < A; ++k) {
+                    const uint32_t v = a.P.ev[(peo + i) * A + k];
+                    a.C.ev[(ceo + pos) * A + k] = v;
+                    gv[pos * A + k] = v;
+                }
+                a.C.ar[ceo + pos] = ar;
+                a.C.ps[ceo + pos] = a.P.ps[peo + i];
+                s.e_ar[pos] = ar;
+            }
+            kept += __popcll(bal);
+        }
+        // Produced edges in RHS order; fresh vertices ascending by variable
+        // (assign_fresh_consecutive / resolve_rhs_vertices, rewrite_core.hpp:53-87).
+        const uint32_t fresh_base = a.P.nextv[par];
+        if (l < nr) {
+            const uint32_t pos = kept + l;
+            const uint32_t ra = R.ra[l];
+            for (uint32_t k = 0; k < A; ++k) {
+                uint32_t v = 0;
+                if (k < ra) {
+                    const uint32_t var = R.rv[l][k];
+                    if (R.lhs_mask & (1u << var)) {
+                        const uint32_t se = q[R.var_src_edge[var]];
+                        v = a.P.ev[(peo + se) * A + R.var_src_pos[var]];
+                    } else {
+                        v = fresh_base + __popc(R.new_mask & ((1u << var) - 1u));
+                    }
+                }
+                a.C.ev[(ceo + pos) * A + k] = v;
+                gv[pos * A + k] = v;
+            }
+            a.C.ar[ceo + pos] = (uint8_t)ra;
+            a.C.ps[ceo + pos] = (uint8_t)step;
+            s.e_ar[pos] = (uint8_t)ra;
+        }
+        wave_sync();
+        // Local vertex numbering: ascending label, as the reference's sorted vertex list.
+        const uint32_t nextv = fresh_base + R.n_new;
+        for (uint32_t v = l; v <= nextv; v += WAVE) vmap[v] = 0;
+        wave_sync();
+        for (uint32_t e = l; e < cne; e += WAVE)
+            for (uint32_t k = 0; k < s.e_ar[e]; ++k) vmap[gv[e * A + k]] = 1;
+        wave_sync();
+        const uint32_t nv = wave_excl_scan_lds(vmap, nextv + 1);
+        for (uint32_t e = l; e < cne; e += WAVE)
+            for (uint32_t k = 0; k < A; ++k) s.e_lv[e * A + k] = (uint16_t)(k < s.e_ar[e] ? vmap[gv[e * A + k]] : 0);
+        s.nE = cne;
+        s.nV = nv;
+        wave_sync();
+        const uint64_t h = wl_hash_wave(s);
+
+        // Identity and bookkeeping (lane 0): event id, map insert, causal relation.
+        const uint32_t ev_id = RN->ev_base + (c - RN->child_begin);
+        const uint32_t child_sid = RN->n_roots + ev_id;
+        uint64_t anc_c = 0;
+        if (l == 0) {
+            a.C.run[c] = run;
+            a.C.ne[c] = cne;
+            a.C.nextv[c] = nextv;
+            a.C.sid[c] = child_sid;
+            a.C.hash[c] = h;
+            a.E.in_state[c] = a.P.sid[par];
+            a.E.rule[c] = rid - RN->rule_begin;
+            acc[CTR_EVENTS] += 1;
+            if (map_insert(RN, h, child_sid)) { acc[CTR_CANON] += 1; acc[CTR_FINGERPRINT] += mix64(h); }
+            // Causal: one triple per consumed edge with a producer; the producers are events of
+            // this lineage, named by their generation (rewriter.cpp:121-191,
+            // causal_graph.cpp:187-234). Transitive reduction: (P, C) is redundant exactly when
+            // P is a causal ancestor of another producer of C.
+            uint64_t prod = 0;
+            uint32_t triples = 0;
+            uint8_t psq[16];
+            for (uint32_t t = 0; t < nl; ++t) {
+                psq[t] = a.P.ps[peo + q[t]];
+                if (psq[t]) { ++triples; prod |= 1ull << psq[t]; }
+            }
+            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.step;   // steps 1..P.step
+            uint64_t tr = prod;
+            if (RN->flags & 0x1u) {
+                for (uint64_t m = prod; m; m &= m - 1) {
+                    const uint32_t sp = __builtin_ctzll(m);
+                    anc_c |= (1ull << sp) | plan[sp - 1];
+                }
+                if (RN->flags & 0x100u) {
+                    for (uint64_t m = prod; m; m &= m - 1) {
+                        const uint32_t sp = __builtin_ctzll(m);
+                        uint64_t others = 0;
+                        for (uint64_t m2 = prod; m2; m2 &= m2 - 1) {
+                            const uint32_t sy = __builtin_ctzll(m2);
+                            if (sy != sp) others |= plan[sy - 1];
+                        }
+                        if (others & (1ull << sp)) tr &= ~(1ull << sp);
+                    }
+                }
+                uint32_t triples_tr = 0;
+                for (uint32_t t = 0; t < nl; ++t) triples_tr += (psq[t] && ((tr >> psq[t]) & 1ull)) ? 1u : 0u;
+                acc[CTR_TRIPLES] += triples;
+                acc[CTR_PAIRS] += __popcll(prod);
+                acc[CTR_PAIRS_TR] += __popcll(tr);
+                acc[CTR_TRIPLES_TR] += triples_tr;
+            } else {
+                prod = 0; tr = 0;
+            }
+            a.E.prod[c] = prod;
+            a.E.tr[c] = tr;
+        }
+        // Lineage table of the child: the parent's, then this event's ancestor mask.
+        anc_c = __shfl(anc_c, 0, WAVE);
+        {
+            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.step;
+            uint64_t* clan = a.C.lanc + (size_t)c * step;
+            for (uint32_t t = l; t < step; t += WAVE) clan[t] = (t + 1 == step) ? anc_c : plan[t];
+        }
+        wave_sync();
+    }
+    flush();
+}
+
+// ------------------------------------------------------------------------------------------
+// Branchial pairs: two events of the SAME input state whose consumed edge sets intersect,
+// one pair each (causal_graph.cpp:324-351). One wave per parent; consumed sets as 64-bit
+// position masks when the parent has at most 64 edges, position lists otherwise.
+// ------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    uint64_t* masks = (uint64_t*)(lds + (size_t)w * a.lds_per_wave);
+    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
+        const uint32_t cnt = a.cnt[s];
+        if (cnt < 2) { if (a.emit == 0 && a.pcnt && l == 0) a.pcnt[s] = 0; continue; }
+        const uint32_t run = a.P.run[s];
+        const uint32_t m0 = (uint32_t)a.moff[s];
+        const uint32_t ne = a.P.ne[s];
+        const bool small = ne <= 64 && cnt <= a.mask_cap;
+        if (small) {
+            for (uint32_t i = l; i < cnt; i += WAVE) {
+                const uint32_t nl = a.rules[a.M.rule[m0 + i]].n_lhs;
+                uint64_t m = 0;
+                for (uint32_t t = 0; t < nl; ++t) m |= 1ull << a.M.q[(size_t)(m0 + i) * a.M.K + t];
+                masks[i] = m;
+            }
+            wave_sync();
+        }
+        auto overlap = [&](uint32_t i, uint32_t j) -> bool {
+            if (small) return (masks[i] & masks[j]) != 0;
+            const uint16_t* qi = a.M.q + (size_t)(m0 + i) * a.M.K;
+            const uint16_t* qj = a.M.q + (size_t)(m0 + j) * a.M.K;
+            const uint32_t ni = a.rules[a.M.rule[m0 + i]].n_lhs, nj = a.rules[a.M.rule[m0 + j]].n_lhs;
+            for (uint32_t x = 0; x < ni; ++x)
+                for (uint32_t y = 0; y < nj; ++y)
+                    if (qi[x] == qj[y]) return true;
+            return false;
+        };
+        const DRun& RN = a.runs[run];
+        if (a.emit == 0) {
+            uint32_t c = 0;
+            for (uint32_t i = l; i < cnt; i += WAVE)
+                for (uint32_t j = i + 1; j < cnt; ++j) c += overlap(i, j) ? 1u : 0u;
+            const uint32_t tot = wave_sum_u32(c);
+            if (l == 0) {
+                if (tot) atomicAdd((unsigned long long*)&a.ctr[(size_t)run * CTR_N + CTR_BRANCHIAL],
+                                   (unsigned long long)tot);
+                if (a.pcnt) a.pcnt[s] = tot;
+            }
+        } else {
+            // Materialise: rows in lane order, each row's pairs in column order.
+            uint64_t out = a.poff[s];
+            const uint32_t ev0 = RN.ev_base + (m0 - RN.child_begin);
+            for (uint32_t base = 0; base < cnt; base += WAVE) {
+                const uint32_t i = base + l;
+                uint32_t c = 0;
+                if (i < cnt)
+                    for (uint32_t j = i + 1; j < cnt; ++j) c += overlap(i, j) ? 1u : 0u;
+                const uint32_t inc = wave_incl_scan_u32(c);
+                uint64_t o = out + inc - c;
+                if (i < cnt && c)
+                    for (uint32_t j = i + 1; j < cnt; ++j)
+                        if (overlap(i, j)) { a.pairs[2 * o] = ev0 + i; a.pairs[2 * o + 1] = ev0 + j; ++o; }
+                out += __shfl(inc, WAVE - 1, WAVE);
+            }
+        }
+        wave_sync();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// WL hash of explicitly given states (dense local vertex ids per state). Optionally inserts
+// each into its run's state map (roots).
+// ------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    uint8_t* p = lds + (size_t)w * a.lds_per_wave;
+    const uint32_t A = a.A;
+    const WLSizes z = WLSizes::make(a.max_ne, a.max_nv, A);
+    WLSpace s;
+    wl_bind(s, p, z, A);
+    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + w; i < a.n; i += waves) {
+        const uint64_t e0 = a.state_off[i];
+        const uint32_t ne = (uint32_t)(a.state_off[i + 1] - e0);
+        for (uint32_t e = l; e < ne; e += WAVE) {
+            const uint8_t ar = a.ar[e0 + e];
+            s.e_ar[e] = ar;
+            for (uint32_t k = 0; k < A; ++k) s.e_lv[e * A + k] = (uint16_t)(k < ar ? a.ev[(e0 + e) * A + k] : 0u);
+        }
+        s.nE = ne;
+        s.nV = a.nv[i];
+        wave_sync();
+        const uint64_t h = wl_hash_wave(s);
+        if (l == 0) {
+            a.out[i] = h;
+            if (a.run_of) {
+                DRun* RN = &a.runs[a.run_of[i]];
+                if (map_insert(RN, h, a.sid_of[i])) {
+                    atomicAdd((unsigned long long*)&a.ctr[(size_t)a.run_of[i] * CTR_N + CTR_CANON], 1ull);
+                    atomicAdd((unsigned long long*)&a.ctr[(size_t)a.run_of[i] * CTR_N + CTR_FINGERPRINT],
+                              (unsigned long long)mix64(h));
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Per-run match totals of a generation: runs' states are contiguous ranges.
+__global__ void k_run_totals(const uint64_t* moff, const uint32_t* run_sbeg, const uint32_t* run_send,
+                             uint32_t nruns, uint64_t* tot) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nruns) tot[r] = moff[run_send[r]] - moff[run_sbeg[r]];
+}
+
+// Zero the counts of states whose run does not advance this generation.
+__global__ void k_mask_counts(const uint32_t* run, const uint8_t* active, uint32_t n, uint32_t* cnt) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && !active[run[s]]) cnt[s] = 0;
+}
+
+// Move one run's state map into a larger table.
+__global__ void k_rehash(const uint64_t* old_keys, const uint32_t* old_vals, uint32_t old_cap,
+                         uint64_t* keys, uint32_t* vals, uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= old_cap) return;
+    const uint64_t h = old_keys[i];
+    if (h == 0) return;
+    uint32_t j = (uint32_t)mix64(h) & mask;
+    while (true) {
+        const uint64_t old = atomicCAS((unsigned long long*)&keys[j], 0ull, (unsigned long long)h);
+        if (old == 0) { vals[j] = old_vals[i]; return; }
+        j = (j + 1) & mask;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------
+
+static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
+    uint32_t g = (items + per_block - 1) / per_block;
+    if (g > cap) g = cap;
+    if (g < 8) g = 8;
+    return (g + 7u) & ~7u;
+}
+
+hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
+    const uint32_t grid = grid_for(a.P.n, wpb, grid_cap);
+    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    if (emit) hipLaunchKernelGGL(k_match<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
+    else hipLaunchKernelGGL(k_match<false>, dim3(grid), dim3(64 * wpb), lds, st, a);
+    return hipGetLastError();
+}
+
+uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A) {
+    const WLSizes z = WLSizes::make(max_child_ne, max_nv, A);
+    return ((z.bytes(A) + z.NE * A * 4 + (max_nextv + 1) * 4 + 64u) + 15u) & ~15u;
+}
+
+hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
+    uint32_t grid = grid_for(a.M.n, wpb, grid_cap);
+    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(64 * wpb), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_branchial(const BranchialArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
+    const uint32_t grid = grid_for(a.P.n, wpb, grid_cap);
+    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    hipLaunchKernelGGL(k_branchial, dim3(grid), dim3(64 * wpb), lds, st, a);
+    return hipGetLastError();
+}
+
+uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A) {
+    const WLSizes z = WLSizes::make(max_ne, max_nv, A);
+    return (z.bytes(A) + 64u + 15u) & ~15u;
+}
+
+hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
+    const uint32_t grid = grid_for(a.n, wpb, grid_cap);
+    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    hipLaunchKernelGGL(k_wl_states, dim3(grid), dim3(64 * wpb), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_run_totals(const uint64_t* moff, const uint32_t* b, const uint32_t* e, uint32_t nruns,
+                             uint64_t* tot, hipStream_t st) {
+    hipLaunchKernelGGL(k_run_totals, dim3((nruns + 255) / 256), dim3(256), 0, st, moff, b, e, nruns, tot);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32_t n, uint32_t* cnt,
+                              hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mask_counts, dim3((n + 255) / 256), dim3(256), 0, st, run, active, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_rehash(const uint64_t* ok, const uint32_t* ov, uint32_t old_cap, uint64_t* keys,
+                         uint32_t* vals, uint32_t mask, hipStream_t st) {
+    hipLaunchKernelGGL(k_rehash, dim3((old_cap + 255) / 256), dim3(256), 0, st, ok, ov, old_cap, keys,
+                       vals, mask);
+    return hipGetLastError();
+}
+
+}  // namespace hgx

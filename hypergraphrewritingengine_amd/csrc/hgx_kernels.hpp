@@ -1,0 +1,82 @@
+// hgx_kernels.hpp -- kernel argument blocks and launchers (host <-> device contract).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hgx_device.hpp"
+
+namespace hgx {
+
+struct MatchArgs {
+    GenView P;
+    const DRule* rules;
+    const DRun* runs;
+    const uint8_t* active;     // [runs] does this run advance this generation
+    uint32_t* cnt;             // count pass: [P.n] matches per state
+    const uint64_t* moff;      // emit pass: [P.n + 1] exclusive offsets
+    MatchView M;               // emit pass: outputs
+    uint32_t max_ne;           // largest state in P (LDS staging)
+    uint32_t lds_per_wave;
+};
+
+struct ExpandArgs {
+    GenView P;                 // parents
+    GenView C;                 // children (eoff filled; the rest written here)
+    MatchView M;
+    EventView E;
+    const DRule* rules;
+    DRun* runs;
+    uint64_t* ctr;             // [runs * CTR_N]
+    uint32_t max_child_ne;
+    uint32_t max_nv;           // bound on distinct vertices of a child
+    uint32_t max_nextv;        // bound on vertex labels of a child
+    uint32_t lds_per_wave;
+};
+
+struct BranchialArgs {
+    GenView P;
+    MatchView M;
+    const DRule* rules;
+    const DRun* runs;
+    const uint32_t* cnt;       // [P.n] matches per parent
+    const uint64_t* moff;      // [P.n + 1]
+    uint64_t* ctr;
+    uint32_t emit;             // 0: count (into ctr and pcnt), 1: write pairs at poff
+    uint32_t* pcnt;            // [P.n] pairs per parent (may be null in count mode)
+    const uint64_t* poff;      // [P.n + 1] exclusive offsets (emit)
+    uint32_t* pairs;           // [2 * total] event ids within the run
+    uint32_t mask_cap;         // masks that fit the per-wave LDS
+    uint32_t lds_per_wave;
+};
+
+struct WLStatesArgs {
+    uint32_t n;
+    uint32_t A;
+    const uint64_t* state_off; // [n + 1] in edges
+    const uint8_t* ar;
+    const uint32_t* ev;        // dense local vertex ids, stride A
+    const uint32_t* nv;        // [n] distinct vertices per state
+    uint64_t* out;             // [n]
+    const uint32_t* run_of;    // optional: insert into run maps
+    const uint32_t* sid_of;
+    DRun* runs;
+    uint64_t* ctr;
+    uint32_t max_ne, max_nv;
+    uint32_t lds_per_wave;
+};
+
+hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A);
+hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+hipError_t launch_branchial(const BranchialArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A);
+hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+hipError_t launch_run_totals(const uint64_t* moff, const uint32_t* b, const uint32_t* e, uint32_t nruns,
+                             uint64_t* tot, hipStream_t st);
+hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32_t n, uint32_t* cnt,
+                              hipStream_t st);
+hipError_t launch_rehash(const uint64_t* ok, const uint32_t* ov, uint32_t old_cap, uint64_t* keys,
+                         uint32_t* vals, uint32_t mask, hipStream_t st);
+
+}  // namespace hgx

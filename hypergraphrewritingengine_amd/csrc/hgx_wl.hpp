@@ -1,0 +1,290 @@
+// hgx_wl.hpp -- the WL canonical hash of one state, computed by one wave64 over LDS.
+//
+// Bit-identical to hgcommon::wl_canonical_hash (common/include/hgcommon/wl_core.hpp:67-202)
+// called the way WLHash::compute_state_hash_impl calls it
+// (hypergraph/include/hypergraph/wl_hash.hpp:109-239). The reference runs the algorithm on
+// one thread; here the 64 lanes share it:
+//   occurrence CSR       counting sort with LDS atomics, then every vertex bucket is put in
+//                        (edge, position) order so the CSR is the reference's exactly;
+//   initial colours      one lane per vertex (wl_core.hpp:93-107), nbr_cap truncation kept;
+//   distinct count       LDS open-addressing set with 64-bit CAS, exact;
+//   refinement           one lane per vertex, neighbour multiset sorted in an LDS segment
+//                        and folded in order (wl_core.hpp:119-141);
+//   components           min-label propagation with pointer jumping; the per-component fold
+//                        only needs the partition, which is what union-find gives;
+//   folds                wave-wide sums (commutative, wrap-around).
+// The result does not depend on local vertex numbering (every fold is order-independent and
+// the truncated initial-colour prefix follows edge order), so the caller may use any dense
+// numbering; the kernels use ascending vertex label, the reference's own choice.
+#pragma once
+
+#include "hgx_device.hpp"
+
+namespace hgx {
+
+// LDS workspace of one WL computation. Edges use a fixed vertex stride A.
+struct WLSpace {
+    uint32_t nE, nV, A;
+    uint8_t* e_ar;       // [NE]
+    uint16_t* e_lv;      // [NE * A] local vertex index
+    uint32_t* occ_off;   // [NV + 1]
+    uint32_t* occ_ep;    // [OCC] (edge << 8) | pos, in (edge, pos) order per vertex
+    uint32_t* cursor;    // [NV + 1] scratch (write cursors, then segment offsets)
+    uint32_t* comp;      // [NV]
+    uint64_t* cur;       // [NV]
+    uint64_t* nxt;       // [NV]
+    uint64_t* nbr;       // [NN] per-vertex neighbour segments
+    uint64_t* hset;      // [HT]
+    uint32_t ht_mask;
+};
+
+// Sizes a caller must provision for states with at most ne edges, nv vertices, arity <= A.
+struct WLSizes {
+    uint32_t NE, NV, OCC, NN, HT;
+    __host__ __device__ static WLSizes make(uint32_t ne, uint32_t nv, uint32_t A) {
+        WLSizes s;
+        s.NE = ne ? ne : 1;
+        s.OCC = s.NE * A;
+        s.NV = nv ? nv : 1;
+        if (s.NV > s.OCC) s.NV = s.OCC;
+        s.NN = s.OCC * (A > 1 ? A : 2);
+        uint32_t ht = 64;
+        while (ht < 2 * s.NV) ht <<= 1;
+        s.HT = ht;
+        return s;
+    }
+    // bytes, each array 8-byte aligned
+    __host__ __device__ uint32_t bytes(uint32_t A) const {
+        auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
+        return al(NE) + al(NE * A * 2) + al((NV + 1) * 4) + al(OCC * 4) + al((NV + 1) * 4) +
+               al(NV * 4) + 8 * (NV + NV + NN + HT);
+    }
+};
+
+__device__ inline uint8_t* carve(uint8_t*& p, uint32_t bytes) {
+    uint8_t* r = p;
+    p += (bytes + 7u) & ~7u;
+    return r;
+}
+
+__device__ inline void wl_bind(WLSpace& s, uint8_t*& p, const WLSizes& z, uint32_t A) {
+    s.A = A;
+    s.e_ar = carve(p, z.NE);
+    s.e_lv = (uint16_t*)carve(p, z.NE * A * 2);
+    s.occ_off = (uint32_t*)carve(p, (z.NV + 1) * 4);
+    s.occ_ep = (uint32_t*)carve(p, z.OCC * 4);
+    s.cursor = (uint32_t*)carve(p, (z.NV + 1) * 4);
+    s.comp = (uint32_t*)carve(p, z.NV * 4);
+    s.cur = (uint64_t*)carve(p, z.NV * 8);
+    s.nxt = (uint64_t*)carve(p, z.NV * 8);
+    s.nbr = (uint64_t*)carve(p, z.NN * 8);
+    s.hset = (uint64_t*)carve(p, z.HT * 8);
+    s.ht_mask = z.HT - 1;
+}
+
+// Exclusive prefix sum of a[0..n) in LDS, in place; returns the total.
+__device__ inline uint32_t wave_excl_scan_lds(uint32_t* a, uint32_t n) {
+    const uint32_t l = lane_id();
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += WAVE) {
+        const uint32_t i = base + l;
+        const uint32_t v = i < n ? a[i] : 0u;
+        const uint32_t inc = wave_incl_scan_u32(v);
+        if (i < n) a[i] = carry + inc - v;
+        carry += __shfl(inc, WAVE - 1, WAVE);
+    }
+    wave_sync();
+    return carry;
+}
+
+__device__ inline void isort_lds(uint64_t* a, uint32_t n) {   // core.hpp:54-61
+    for (uint32_t i = 1; i < n; ++i) {
+        const uint64_t key = a[i];
+        uint32_t j = i;
+        while (j > 0 && a[j - 1] > key) { a[j] = a[j - 1]; --j; }
+        a[j] = key;
+    }
+}
+
+// Exact number of distinct values in cur[0..nV).
+__device__ inline uint32_t wl_ndistinct(WLSpace& s, const uint64_t* cur) {
+    const uint32_t l = lane_id();
+    for (uint32_t i = l; i <= s.ht_mask; i += WAVE) s.hset[i] = 0;
+    wave_sync();
+    uint32_t cnt = 0, zero = 0;
+    for (uint32_t v = l; v < s.nV; v += WAVE) {
+        const uint64_t x = cur[v];
+        if (x == 0) { zero = 1; continue; }
+        uint32_t slot = (uint32_t)((x * 0x9E3779B97F4A7C15ULL) >> 40) & s.ht_mask;
+        while (true) {
+            const uint64_t old = atomicCAS((unsigned long long*)&s.hset[slot], 0ull,
+                                           (unsigned long long)x);
+            if (old == 0) { ++cnt; break; }
+            if (old == x) break;
+            slot = (slot + 1) & s.ht_mask;
+        }
+    }
+    const uint32_t total = wave_sum_u32(cnt) + (__ballot(zero) != 0 ? 1u : 0u);
+    wave_sync();
+    return total;
+}
+
+// The hash. Inputs in s: nE, nV, A, e_ar[nE], e_lv[nE*A]. All 64 lanes must call it.
+__device__ inline uint64_t wl_hash_wave(WLSpace& s) {
+    const uint32_t l = lane_id();
+    const uint32_t nE = s.nE, nV = s.nV, A = s.A;
+    if (nE == 0) return EMPTY_STATE_HASH;     // hypergraph.cpp:743-745
+    if (nV == 0) return 0;                    // wl_core.hpp:85
+
+    // ---- occurrence CSR (wl_core.hpp:87-97) ----
+    for (uint32_t v = l; v <= nV; v += WAVE) s.occ_off[v] = 0;
+    wave_sync();
+    for (uint32_t e = l; e < nE; e += WAVE)
+        for (uint32_t p = 0; p < s.e_ar[e]; ++p) atomicAdd(&s.occ_off[s.e_lv[e * A + p]], 1u);
+    wave_sync();
+    wave_excl_scan_lds(s.occ_off, nV + 1);
+    for (uint32_t v = l; v <= nV; v += WAVE) s.cursor[v] = s.occ_off[v];
+    wave_sync();
+    for (uint32_t e = l; e < nE; e += WAVE)
+        for (uint32_t p = 0; p < s.e_ar[e]; ++p) {
+            const uint32_t w = atomicAdd(&s.cursor[s.e_lv[e * A + p]], 1u);
+            s.occ_ep[w] = (e << 8) | p;
+        }
+    wave_sync();
+    // Each bucket into (edge, pos) order -- the order the reference's counting sort leaves.
+    uint32_t nn_max = 0;
+    for (uint32_t v = l; v < nV; v += WAVE) {
+        const uint32_t b = s.occ_off[v], n = s.occ_off[v + 1] - b;
+        for (uint32_t i = 1; i < n; ++i) {
+            const uint32_t key = s.occ_ep[b + i];
+            uint32_t j = i;
+            while (j > 0 && s.occ_ep[b + j - 1] > key) { s.occ_ep[b + j] = s.occ_ep[b + j - 1]; --j; }
+            s.occ_ep[b + j] = key;
+        }
+        uint32_t nn = 0;   // wl_hash.hpp:201-210: sum of (arity - 1) over the occurrences
+        for (uint32_t i = 0; i < n; ++i) nn += s.e_ar[s.occ_ep[b + i] >> 8] - 1u;
+        s.cursor[v] = nn;
+        nn_max = nn > nn_max ? nn : nn_max;
+    }
+    uint32_t nbr_cap = wave_max_u32(nn_max);
+    if (nbr_cap < 1) nbr_cap = 1;
+    wave_sync();
+    // Segment sizes: room for the refinement list (nn) and the initial list (min(deg, cap)).
+    for (uint32_t v = l; v < nV; v += WAVE) {
+        const uint32_t deg = s.occ_off[v + 1] - s.occ_off[v];
+        const uint32_t init = deg < nbr_cap ? deg : nbr_cap;
+        const uint32_t nn = s.cursor[v];
+        s.cursor[v] = nn > init ? nn : init;
+    }
+    wave_sync();
+    wave_excl_scan_lds(s.cursor, nV + 1);   // cursor[v] = segment offset
+
+    // ---- initial colours (wl_core.hpp:93-107) ----
+    for (uint32_t v = l; v < nV; v += WAVE) {
+        const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
+        uint64_t* seg = s.nbr + s.cursor[v];
+        uint64_t h = fnv(FNV_OFFSET, deg);
+        uint32_t cnt = 0;
+        for (uint32_t j = 0; j < deg && cnt < nbr_cap; ++j) {
+            const uint32_t ep = s.occ_ep[b + j];
+            seg[cnt++] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+        }
+        isort_lds(seg, cnt);
+        for (uint32_t i = 0; i < cnt; ++i) { h = fnv(h, seg[i] >> 8); h = fnv(h, seg[i] & 0xFFu); }
+        s.cur[v] = h;
+    }
+    wave_sync();
+    uint64_t* cur = s.cur;
+    uint64_t* nxt = s.nxt;
+    uint32_t prev_d = wl_ndistinct(s, cur);
+
+    // ---- refinement (wl_core.hpp:119-141) ----
+    for (uint32_t it = 0; it < WL_MAX_ITERS; ++it) {
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
+            uint64_t* seg = s.nbr + s.cursor[v];
+            uint32_t nn = 0;
+            for (uint32_t j = 0; j < deg; ++j) {
+                const uint32_t ep = s.occ_ep[b + j];
+                const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
+                const uint32_t a = s.e_ar[e];
+                for (uint32_t k = 0; k < a; ++k) {
+                    if (k == mypos) continue;
+                    if (nn < nbr_cap) seg[nn++] = fnv(cur[s.e_lv[e * A + k]], k);
+                }
+            }
+            isort_lds(seg, nn);
+            uint64_t h = cur[v];
+            for (uint32_t i = 0; i < nn; ++i) h = fnv(h, seg[i]);
+            nxt[v] = h;
+        }
+        wave_sync();
+        uint64_t* t = cur; cur = nxt; nxt = t;
+        const uint32_t d = wl_ndistinct(s, cur);
+        if (d == prev_d) break;
+        prev_d = d;
+    }
+
+    // ---- connected components (wl_core.hpp:143-168) ----
+    for (uint32_t v = l; v < nV; v += WAVE) s.comp[v] = v;
+    wave_sync();
+    while (true) {
+        uint32_t changed = 0;
+        for (uint32_t e = l; e < nE; e += WAVE) {
+            const uint32_t a = s.e_ar[e];
+            uint32_t m = 0xFFFFFFFFu;
+            for (uint32_t p = 0; p < a; ++p) { const uint32_t c = s.comp[s.e_lv[e * A + p]]; m = c < m ? c : m; }
+            for (uint32_t p = 0; p < a; ++p) {
+                const uint32_t x = s.e_lv[e * A + p];
+                if (s.comp[x] > m) { atomicMin(&s.comp[x], m); changed = 1; }
+            }
+        }
+        wave_sync();
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t c = s.comp[v], cc = s.comp[c];
+            if (cc < c) { atomicMin(&s.comp[v], cc); changed = 1; }
+        }
+        wave_sync();
+        if (__ballot(changed) == 0) break;
+    }
+
+    // ---- folds (wl_core.hpp:170-201) ----
+    uint64_t* acc_col = nxt;          // refinement double buffer, finished with
+    uint64_t* acc_cnt = s.hset;       // distinct-count set, finished with (HT >= NV)
+    for (uint32_t v = l; v < nV; v += WAVE) { acc_col[v] = 0; acc_cnt[v] = 0; }
+    wave_sync();
+    uint64_t vsum = 0, esum = 0, csum = 0;
+    for (uint32_t v = l; v < nV; v += WAVE) {
+        const uint64_t sm = splitmix(cur[v]);
+        vsum += sm;
+        atomicAdd((unsigned long long*)&acc_col[s.comp[v]], (unsigned long long)sm);
+        atomicAdd((unsigned long long*)&acc_cnt[s.comp[v]], 1ull);
+    }
+    for (uint32_t e = l; e < nE; e += WAVE) {
+        const uint32_t a = s.e_ar[e];
+        uint64_t eh = fnv(FNV_OFFSET, a);
+        for (uint32_t p = 0; p < a; ++p) eh = fnv(eh, cur[s.e_lv[e * A + p]]);
+        esum += splitmix(eh);
+        atomicAdd((unsigned long long*)&acc_cnt[s.comp[s.e_lv[e * A]]], 1ull << 32);
+    }
+    wave_sync();
+    for (uint32_t v = l; v < nV; v += WAVE) {
+        if (s.comp[v] != v) continue;
+        uint64_t ch = fnv(FNV_OFFSET, acc_cnt[v]);
+        ch = fnv(ch, acc_col[v]);
+        csum += splitmix(ch);
+    }
+    vsum = wave_sum_u64(vsum);
+    esum = wave_sum_u64(esum);
+    csum = wave_sum_u64(csum);
+    wave_sync();
+    uint64_t hash = FNV_OFFSET;
+    hash = fnv(hash, nV);
+    hash = fnv(hash, nE);
+    hash = fnv(hash, vsum);
+    hash = fnv(hash, esum);
+    hash = fnv(hash, csum);
+    return hash;
+}
+
+}  // namespace hgx

@@ -763,8 +763,8 @@ int hgxo_evolve(const hgx_run_desc* d, hgxo_result** out) {
             }
         }
         free(cc.m.p); free(par_begin.p);
+        if (R->s_ne.n > next_begin) steps_done = step;   /* last generation with an event */
         lvl_begin = next_begin; lvl_end = R->s_ne.n;
-        steps_done = step;
     }
 
     /* counts */
