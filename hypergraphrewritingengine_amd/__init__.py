@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = [
     "hgx_request_wait", "hgx_request_footprint", "hgx_result_counts", "hgx_result_states",
     "hgx_result_events", "hgx_result_state_edges", "hgx_result_causal", "hgx_result_branchial",
     "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
+    "hgx_profile_enable", "hgx_profile_read",
 ]
 
 
@@ -79,6 +80,8 @@ def load_library():
     L.hgx_wl_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
     L.hgx_match_count.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
                                   C.POINTER(C.c_uint64), vp, C.c_uint64, C.POINTER(C.c_float)]
+    L.hgx_profile_enable.argtypes = [vp, C.c_int]
+    L.hgx_profile_read.argtypes = [vp, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)]
     _LIB = L
     return L
 
@@ -212,6 +215,26 @@ class Context:
                                          vv.ctypes.data, out.ctypes.data)
         if rc != abi.HGX_OK:
             raise self._err(rc)
+        return out
+
+    def profile(self, on: bool = True):
+        """Event-time every generation kernel launch (see hgx_profile_enable)."""
+        rc = self._lib.hgx_profile_enable(self._h, 1 if on else 0)
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def profile_read(self) -> dict:
+        """{kernel: {launches, total_ms, alg_bytes, items}}; clears the accumulators."""
+        n = C.c_uint32()
+        buf = (abi.KernelStat * 32)()
+        rc = self._lib.hgx_profile_read(self._h, buf, 32, C.byref(n))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        out = {}
+        for i in range(min(n.value, 32)):
+            s = buf[i]
+            out[s.name.decode()] = {"launches": int(s.launches), "total_ms": float(s.total_ms),
+                                    "alg_bytes": int(s.alg_bytes), "items": int(s.items)}
         return out
 
     def close(self):

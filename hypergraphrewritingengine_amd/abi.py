@@ -94,3 +94,8 @@ def make_run_desc(req, keep: list) -> RunDesc:
     d.state_cap = int(req.state_cap)
     d.flags = int(req.flags)
     return d
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double),
+                ("alg_bytes", C.c_uint64), ("items", C.c_uint64)]

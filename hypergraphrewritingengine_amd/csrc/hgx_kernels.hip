@@ -173,7 +173,10 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             }
             wave_sync();
         }
-        if (!EMIT && l == 0) a.cnt[s] = total;
+        if (!EMIT && l == 0) {
+            a.cnt[s] = total;
+            if (total && a.parent_edges) atomicAdd(a.parent_edges, (unsigned long long)ne);
+        }
     }
 }
 

@@ -18,6 +18,7 @@ struct MatchArgs {
     MatchView M;               // emit pass: outputs
     uint32_t max_ne;           // largest state in P (LDS staging)
     uint32_t lds_per_wave;
+    unsigned long long* parent_edges;   // count pass: += edges of each state with a match
 };
 
 struct ExpandArgs {

@@ -241,6 +241,7 @@ static RunHost parse_run(const hgx_run_desc* d) {
 
 struct Gen {
     GenView v{};
+    uint64_t n_edges = 0;
     uint32_t max_ne = 0, max_nextv = 0;
     // per run: state range in this generation
     std::vector<uint32_t> sbeg, send;
@@ -279,7 +280,21 @@ struct hgx_result {
     uint64_t device_ns = 0, latency_ns = 0;
 };
 
+namespace hgx {
+struct ProfPending {
+    std::string name;
+    hipEvent_t a, b;
+    uint64_t bytes, items;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<ProfPending> pend;
+    std::map<std::string, hgx_kernel_stat> acc;
+};
+}  // namespace hgx
+
 struct hgx_ctx {
+    hgx::Profiler prof;
     int device = 0;
     hipStream_t stream = nullptr;
     std::shared_ptr<hgx::ChunkPool> pool;
@@ -327,6 +342,36 @@ static uint32_t waves_per_block(uint32_t lds_per_wave, uint32_t max_lds) {
         throw Error{HGX_ERR_UNSUPPORTED, "a state is too large for the LDS-resident path (" +
                                              std::to_string(lds_per_wave) + " bytes per wave)"};
     return w;
+}
+
+// Launch with optional event timing; `bytes` are the launch's algorithmic HBM bytes.
+template <class F>
+static void timed(hgx_ctx* ctx, hipStream_t st, const char* name, uint64_t bytes, uint64_t items, F&& launch) {
+    if (!ctx->prof.on) { HIP_TRY(launch()); return; }
+    ProfPending p{name, nullptr, nullptr, bytes, items};
+    HIP_TRY(hipEventCreate(&p.a));
+    HIP_TRY(hipEventCreate(&p.b));
+    HIP_TRY(hipEventRecord(p.a, st));
+    HIP_TRY(launch());
+    HIP_TRY(hipEventRecord(p.b, st));
+    ctx->prof.pend.push_back(p);
+}
+
+// After a stream synchronisation: fold finished launches into the per-kernel totals.
+static void prof_resolve(hgx_ctx* ctx) {
+    for (auto& p : ctx->prof.pend) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+        hgx_kernel_stat& s = ctx->prof.acc[p.name];
+        std::snprintf(s.name, sizeof(s.name), "%s", p.name.c_str());
+        s.launches += 1;
+        s.total_ms += ms;
+        s.alg_bytes += p.bytes;
+        s.items += p.items;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    ctx->prof.pend.clear();
 }
 
 // Evolve a set of requests to completion. Returns the store; counts filled.
@@ -452,6 +497,7 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
         if (root_edges) HIP_TRY(hipMemsetAsync(g.v.ps, 0, root_edges, st));
         g.max_ne = max_root_ne;
         g.max_nextv = max_root_nv;
+        g.n_edges = root_edges;
         if (total_roots) {
             WLStatesArgs w{};
             w.n = total_roots; w.A = A; w.state_off = g.v.eoff; w.ar = g.v.ar; w.ev = g.v.ev;
@@ -459,7 +505,8 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
             w.runs = d_runs; w.ctr = d_ctr; w.max_ne = max_root_ne; w.max_nv = std::max(1u, max_root_nv);
             w.lds_per_wave = wl_states_lds_per_wave(w.max_ne, w.max_nv, A);
             const uint32_t wpb = waves_per_block(w.lds_per_wave, ctx->max_lds);
-            HIP_TRY(launch_wl_states(w, ctx->num_cu * 8, wpb, st));
+            timed(ctx, st, "k_wl_states", root_edges * (4ull * A + 1) + total_roots * 36ull, total_roots,
+                  [&] { return launch_wl_states(w, ctx->num_cu * 8, wpb, st); });
         }
         HIP_TRY(hipStreamSynchronize(st));   // host staging vectors end with this scope
         S->gens.push_back(std::move(g));
@@ -475,8 +522,8 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
 
     uint32_t* d_sbeg = dalloc<uint32_t>(ar, R);
     uint32_t* d_send = dalloc<uint32_t>(ar, R);
-    uint64_t* d_tot = dalloc<uint64_t>(ar, R);
-    std::vector<uint64_t> tot(R);
+    uint64_t* d_tot = dalloc<uint64_t>(ar, R + 1);   // [R] = edges of parents with a match
+    std::vector<uint64_t> tot(R + 1);
 
     for (uint32_t L = 1; L <= max_steps; ++L) {
         Gen& P = S->gens[L - 1];
@@ -498,12 +545,18 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
         ma.P = P.v; ma.rules = d_rules; ma.runs = d_runs; ma.active = d_active; ma.cnt = cnt;
         ma.moff = moff; ma.max_ne = std::max(1u, P.max_ne);
         ma.lds_per_wave = (ma.max_ne * A * 4 + ((ma.max_ne + 7u) & ~7u) + ma.max_ne * 2 + 64 + 15) & ~15u;
+        ma.parent_edges = (unsigned long long*)(d_tot + R);
+        HIP_TRY(hipMemsetAsync(d_tot + R, 0, sizeof(uint64_t), st));
         const uint32_t wpb_m = waves_per_block(ma.lds_per_wave, ctx->max_lds);
-        HIP_TRY(launch_match(ma, false, ctx->num_cu * 8, wpb_m, st));
+        // algorithmic bytes of a join pass: every parent edge read once, one record per state
+        const uint64_t match_bytes = P.n_edges * (4ull * A + 1) + (uint64_t)P.v.n * 24;
+        timed(ctx, st, "k_match_count", match_bytes + P.v.n * 4ull, P.v.n,
+              [&] { return launch_match(ma, false, ctx->num_cu * 8, wpb_m, st); });
         scan_u32_to_u64(ar, cnt, moff, P.v.n, st);
         HIP_TRY(launch_run_totals(moff, d_sbeg, d_send, R, d_tot, st));
-        d2h(tot.data(), d_tot, R, st);
+        d2h(tot.data(), d_tot, R + 1, st);
         HIP_TRY(hipStreamSynchronize(st));
+        const uint64_t parent_edges = tot[R];
 
         // deterministic state cap: stop before a generation that would exceed it
         bool masked = false;
@@ -555,7 +608,9 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
         mv.cne = dalloc<uint32_t>(ar, M + 1);
         HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
         ma.M = mv;
-        HIP_TRY(launch_match(ma, true, ctx->num_cu * 8, wpb_m, st));
+        ma.parent_edges = nullptr;
+        timed(ctx, st, "k_match_emit", match_bytes + (uint64_t)M * (12ull + 2ull * K), P.v.n,
+              [&] { return launch_match(ma, true, ctx->num_cu * 8, wpb_m, st); });
 
         // child generation
         Gen C;
@@ -591,6 +646,7 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
         C.ev.tr = dalloc<uint64_t>(ar, M);
         C.max_ne = P.max_ne + max_rhs_gain;
         C.max_nextv = P.max_nextv + max_new;
+        C.n_edges = EC;
 
         ExpandArgs ea{};
         ea.P = P.v; ea.C = C.v; ea.M = mv; ea.E = C.ev; ea.rules = d_rules; ea.runs = d_runs;
@@ -599,7 +655,12 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
         ea.max_nextv = C.max_nextv;
         ea.lds_per_wave = expand_lds_per_wave(ea.max_child_ne, ea.max_nv, ea.max_nextv, A);
         const uint32_t wpb_e = waves_per_block(ea.lds_per_wave, ctx->max_lds);
-        HIP_TRY(launch_expand(ea, ctx->num_cu * 8, wpb_e, st));
+        // algorithmic bytes: match records and the edges of every parent with a child read once;
+        // child edges, child state record (incl. lineage), event record and map slot written once
+        const uint64_t expand_bytes = (uint64_t)M * (8ull + 2ull * K) + parent_edges * (4ull * A + 2) +
+                                      EC * (4ull * A + 2) + (uint64_t)M * (24ull + 8ull * L) +
+                                      (uint64_t)M * 24ull + (uint64_t)M * 12ull;
+        timed(ctx, st, "k_expand", expand_bytes, M, [&] { return launch_expand(ea, ctx->num_cu * 8, wpb_e, st); });
 
         // branchial
         if (any_branch) {
@@ -626,7 +687,8 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
                 HIP_TRY(hipStreamSynchronize(st));
             }
             ba.cnt = bcnt;
-            HIP_TRY(launch_branchial(ba, ctx->num_cu * 8, 4, st));
+            timed(ctx, st, "k_branchial", (uint64_t)M * (6ull + 2ull * K) + (uint64_t)P.v.n * 12ull, P.v.n,
+                  [&] { return launch_branchial(ba, ctx->num_cu * 8, 4, st); });
             if (any_rel) {
                 uint64_t* poff = dalloc<uint64_t>(ar, P.v.n + 1);
                 scan_u32_to_u64(ar, pcnt, poff, P.v.n, st);
@@ -664,6 +726,7 @@ std::shared_ptr<BatchStore> run_batch(hgx_ctx* ctx, std::vector<RunHost>&& runs)
     HIP_TRY(hipStreamSynchronize(st));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+    prof_resolve(ctx);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     S->device_ms = ms;
@@ -1099,4 +1162,24 @@ extern "C" int hgx_match_count(hgx_ctx* ctx, const uint8_t*, uint32_t, uint32_t,
                                const uint32_t*, uint32_t, uint64_t*, uint32_t*, uint64_t, float*) {
     if (ctx) ctx->err = "large-state join not built in this library";
     return HGX_ERR_UNSUPPORTED;
+}
+
+extern "C" int hgx_profile_enable(hgx_ctx* ctx, int on) {
+    if (!ctx) return HGX_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->prof.on = on != 0;
+    return HGX_OK;
+}
+
+extern "C" int hgx_profile_read(hgx_ctx* ctx, hgx_kernel_stat* out, uint32_t cap, uint32_t* n) {
+    if (!ctx || !n) return HGX_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    uint32_t k = 0;
+    for (auto& kv : ctx->prof.acc) {
+        if (out && k < cap) out[k] = kv.second;
+        ++k;
+    }
+    *n = k;
+    ctx->prof.acc.clear();
+    return HGX_OK;
 }

@@ -214,6 +214,23 @@ int hgx_match_count(hgx_ctx* ctx, const uint8_t* rules, uint32_t rules_len, uint
                     uint32_t n_edges, uint64_t* count, uint32_t* match_edges, uint64_t cap,
                     float* device_ms);
 
+/* ---- kernel-level measurement ----------------------------------------------------------- */
+
+/* With profiling on, every launch of the generation kernels is bracketed by HIP events on the
+ * context's stream and its algorithmic HBM bytes (the minimum the launch must move: inputs read
+ * once, outputs written once) are recorded. No counterpart in the reference; it feeds the
+ * roofline report. */
+typedef struct hgx_kernel_stat {
+    char     name[32];
+    uint64_t launches;
+    double   total_ms;        /* sum of event-timed launch durations                     */
+    uint64_t alg_bytes;       /* sum of algorithmic bytes over those launches            */
+    uint64_t items;           /* states / children processed                              */
+} hgx_kernel_stat;
+int hgx_profile_enable(hgx_ctx* ctx, int on);
+/* Copies up to cap entries, sets *n, and clears the accumulators. */
+int hgx_profile_read(hgx_ctx* ctx, hgx_kernel_stat* out, uint32_t cap, uint32_t* n);
+
 #ifdef __cplusplus
 }
 #endif
