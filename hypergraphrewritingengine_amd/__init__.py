@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = [
     "hgx_request_wait", "hgx_request_footprint", "hgx_result_counts", "hgx_result_states",
     "hgx_result_events", "hgx_result_state_edges", "hgx_result_causal", "hgx_result_branchial",
     "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
-    "hgx_profile_enable", "hgx_profile_read",
+    "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
 ]
 
 
@@ -80,6 +80,8 @@ def load_library():
     L.hgx_wl_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
     L.hgx_match_count.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
                                   C.POINTER(C.c_uint64), vp, C.c_uint64, C.POINTER(C.c_float)]
+    L.hgx_result_counts_many.argtypes = [vp, C.c_uint32, C.POINTER(abi.Counts)]
+    L.hgx_serve_drain.argtypes = [vp, C.POINTER(C.c_uint64), vp, C.c_uint32, C.POINTER(C.c_uint32)]
     L.hgx_profile_enable.argtypes = [vp, C.c_int]
     L.hgx_profile_read.argtypes = [vp, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)]
     _LIB = L
@@ -89,13 +91,15 @@ def load_library():
 class Result:
     """One request's result (hgx_result*). Arrays are read back lazily."""
 
-    def __init__(self, lib, handle, ctx):
+    def __init__(self, lib, handle, ctx, counts=None):
         self._lib = lib
         self._h = handle
         self._ctx = ctx           # keep the context alive while results exist
-        c = abi.Counts()
-        self._check(lib.hgx_result_counts(handle, C.byref(c)))
-        self.counts = c.as_dict()
+        if counts is None:
+            c = abi.Counts()
+            self._check(lib.hgx_result_counts(handle, C.byref(c)))
+            counts = c.as_dict()
+        self.counts = counts
 
     def _check(self, rc):
         if rc != abi.HGX_OK:
@@ -194,7 +198,15 @@ class Context:
         rc = self._lib.hgx_evolve_batch(self._h, descs, n, outs)
         if rc != abi.HGX_OK:
             raise self._err(rc)
-        return [Result(self._lib, C.c_void_p(outs[i]), self) for i in range(n)]
+        cnts = (abi.Counts * max(n, 1))()
+        rc = self._lib.hgx_result_counts_many(outs, n, cnts)
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        arr = np.frombuffer(cnts, dtype=np.dtype([(k, np.uint64 if t is C.c_uint64 else np.uint32)
+                                                   for k, t in abi.Counts._fields_]), count=n)
+        names = arr.dtype.names
+        return [Result(self._lib, C.c_void_p(outs[i]), self,
+                       dict(zip(names, map(int, arr[i])))) for i in range(n)]
 
     def evolve(self, request):
         return self.evolve_batch([request])[0]
@@ -216,6 +228,52 @@ class Context:
         if rc != abi.HGX_OK:
             raise self._err(rc)
         return out
+
+    # ---- served path: continuous batching ----
+
+    def submit(self, request) -> int:
+        """Queue a request; it joins the next generation launch that has room for it."""
+        keep: list = []
+        d = abi.make_run_desc(request, keep)
+        rid = C.c_uint64()
+        rc = self._lib.hgx_request_submit(self._h, C.byref(d), C.byref(rid))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return rid.value
+
+    def serve_step(self) -> int:
+        """Advance every admitted request by one generation; returns live + queued requests."""
+        live = C.c_uint32()
+        rc = self._lib.hgx_serve_step(self._h, C.byref(live))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return live.value
+
+    def drain(self, cap: int = 4096):
+        """{request id: Result} of requests that finished since the last drain."""
+        ids = (C.c_uint64 * cap)()
+        outs = (C.c_void_p * cap)()
+        n = C.c_uint32()
+        rc = self._lib.hgx_serve_drain(self._h, ids, outs, cap, C.byref(n))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return {int(ids[i]): Result(self._lib, C.c_void_p(outs[i]), self) for i in range(n.value)}
+
+    def poll(self, request_id: int):
+        out = C.c_void_p()
+        rc = self._lib.hgx_request_poll(self._h, request_id, C.byref(out))
+        if rc == abi.HGX_ERR_NOT_READY:
+            return None
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return Result(self._lib, out, self)
+
+    def wait(self, request_id: int):
+        out = C.c_void_p()
+        rc = self._lib.hgx_request_wait(self._h, request_id, C.byref(out))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return Result(self._lib, out, self)
 
     def profile(self, on: bool = True):
         """Event-time every generation kernel launch (see hgx_profile_enable)."""

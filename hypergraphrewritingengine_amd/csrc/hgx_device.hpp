@@ -44,13 +44,15 @@ struct DRule {
     uint8_t order[16];
     uint8_t var_src_edge[32];
     uint8_t var_src_pos[32];
+    uint8_t piv[16];          // per join depth: a position sharing a bound variable, or 0xFF
     uint32_t lhs_mask, rhs_mask, new_mask, pad1;
 };
 
 // Per request (run) state the kernels read and update.
 struct DRun {
-    uint32_t rule_begin, n_rules;
-    uint32_t n_roots, flags;
+    const DRule* rules;      // this request's rules (device copy)
+    uint32_t n_rules;
+    uint32_t n_roots, flags, pad0;
     uint64_t* map_keys;      // open-addressing WL-hash set, first writer wins
     uint32_t* map_vals;
     uint32_t map_mask;       // capacity - 1 (power of two)
@@ -68,15 +70,16 @@ enum {
 // One generation of states (all runs mixed; each run's states contiguous).
 struct GenView {
     uint32_t n;               // states
-    uint32_t step;            // generation index of these states
-    uint32_t A;               // vertex stride per edge (batch max arity)
+    uint32_t lstride;         // lineage-table entries per state (>= every depth held)
+    uint32_t A;               // vertex stride per edge (max arity among these states)
+    uint8_t* depth;           // [n] generation of each state inside its own run
     uint32_t* run;            // [n]
     uint64_t* eoff;           // [n] first edge in the pool
     uint32_t* ne;             // [n]
     uint32_t* nextv;          // [n] next fresh vertex label on the lineage
     uint32_t* sid;            // [n] state id inside its run
     uint64_t* hash;           // [n] WL hash
-    uint64_t* lanc;           // [n * step] causal-ancestor masks of the lineage's events
+    uint64_t* lanc;           // [n * lstride] causal-ancestor masks of the lineage's events
     uint32_t* ev;             // edge pool: vertices [edges * A]
     uint8_t* ar;              // edge pool: arity
     uint8_t* ps;              // edge pool: generation of the producing event (0 = none)
@@ -87,7 +90,7 @@ struct MatchView {
     uint32_t n;
     uint32_t K;               // positions stored per match (batch max n_lhs)
     uint32_t* parent;         // [n] parent index in its generation
-    uint32_t* rule;           // [n] global rule index
+    uint32_t* rule;           // [n] rule index inside the run
     uint16_t* q;              // [n * K] matched edge positions in the parent, pattern order
     uint32_t* cne;            // [n] child edge count
 };

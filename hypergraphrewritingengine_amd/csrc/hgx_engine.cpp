@@ -1,0 +1,829 @@
+// hgx_engine.cpp -- the generation engine (see hgx_engine.hpp), the batch and served entry
+// points, and result readback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+
+#include "hgx_engine.hpp"
+#include "hgx_kernels.hpp"
+
+namespace hgx {
+
+Engine::Engine(hgx_ctx* ctx) : ctx_(ctx), st_(ctx->stream), pool_(ctx->pool), slab_(ctx->pool.get()) {
+    grow_slots(256);
+}
+
+Engine::~Engine() {
+    (void)hipStreamSynchronize(st_);
+}
+
+void engine_destroy(std::unique_ptr<Engine>& e) { e.reset(); }
+
+void Engine::grow_slots(uint32_t need) {
+    if (need <= cap_slots_) return;
+    uint32_t cap = std::max<uint32_t>(256, cap_slots_);
+    while (cap < need) cap *= 2;
+    auto na = std::make_unique<Arena>(pool_.get());
+    DRun* runs = dalloc<DRun>(*na, cap);
+    uint64_t* ctr = dalloc<uint64_t>(*na, (size_t)cap * CTR_N);
+    uint8_t* active = dalloc<uint8_t>(*na, cap);
+    uint32_t* budget = dalloc<uint32_t>(*na, cap);
+    uint64_t* acc = dalloc<uint64_t>(*na, cap);
+    uint32_t* sb = dalloc<uint32_t>(*na, cap);
+    uint32_t* se = dalloc<uint32_t>(*na, cap);
+    uint64_t* tot = dalloc<uint64_t>(*na, cap + 1);
+    HIP_TRY(hipMemsetAsync(ctr, 0, sizeof(uint64_t) * cap * CTR_N, st_));
+    if (cap_slots_)
+        HIP_TRY(hipMemcpyAsync(ctr, d_ctr_, sizeof(uint64_t) * cap_slots_ * CTR_N, hipMemcpyDeviceToDevice, st_));
+    HIP_TRY(hipStreamSynchronize(st_));
+    h_runs_.resize(cap);
+    for (uint32_t s = cap; s-- > cap_slots_;) {
+        std::memset(&h_runs_[s], 0, sizeof(DRun));
+        free_slots_.push_back(s);
+    }
+    slot_arena_ = std::move(na);
+    d_runs_ = runs; d_ctr_ = ctr; d_active_ = active; d_budget_ = budget; d_run_acc_ = acc;
+    d_sbeg_ = sb; d_send_ = se; d_tot_ = tot;
+    cap_slots_ = cap;
+}
+
+uint32_t Engine::take_slot() {
+    if (free_slots_.empty()) grow_slots(cap_slots_ * 2);
+    const uint32_t s = free_slots_.back();
+    free_slots_.pop_back();
+    return s;
+}
+
+uint64_t Engine::submit(RunHost&& h, uint64_t id) {
+    auto r = std::make_shared<RunRec>();
+    r->id = id;
+    r->h = std::move(h);
+    r->footprint = footprint_bytes(r->h);
+    r->st = st_;
+    r->t_submit = Clock::now();
+    pending_.push_back(r);
+    return id;
+}
+
+bool Engine::known(uint64_t id) const {
+    if (finished.count(id) || live_.count(id)) return true;
+    for (auto& r : pending_) if (r->id == id) return true;
+    return false;
+}
+
+std::shared_ptr<RunRec> Engine::take(uint64_t id) {
+    auto it = finished.find(id);
+    if (it == finished.end()) return nullptr;
+    auto r = it->second;
+    finished.erase(it);
+    return r;
+}
+
+void Engine::admit(std::shared_ptr<RunRec>& r) {
+    const uint32_t slot = take_slot();
+    r->slot = slot;
+    r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->h.rules.size());
+    h2d(r->d_rules, r->h.rules.data(), r->h.rules.size(), st_);
+    uint64_t want = r->h.cap ? (uint64_t)r->h.cap + r->h.roots.size() : std::max<uint64_t>(512, r->h.roots.size());
+    uint64_t mc = 64;
+    while (mc < 2 * want) mc <<= 1;
+    r->map_cap = (uint32_t)mc;
+    r->map_keys = (uint64_t*)slab_.alloc(8 * mc);
+    r->map_vals = (uint32_t*)slab_.alloc(4 * mc);
+    HIP_TRY(hipMemsetAsync(r->map_keys, 0, 8 * mc, st_));
+    DRun& d = h_runs_[slot];
+    std::memset(&d, 0, sizeof(d));
+    d.rules = r->d_rules;
+    d.n_rules = (uint32_t)r->h.rules.size();
+    d.n_roots = (uint32_t)r->h.roots.size();
+    d.flags = r->h.flags;
+    d.map_keys = r->map_keys;
+    d.map_vals = r->map_vals;
+    d.map_mask = (uint32_t)(mc - 1);
+    HIP_TRY(hipMemsetAsync(d_ctr_ + (size_t)slot * CTR_N, 0, sizeof(uint64_t) * CTR_N, st_));
+    r->admitted = true;
+    r->t_admit = Clock::now();
+    live_[r->id] = r;
+    live_footprint_ += r->footprint;
+}
+
+void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
+    if (runs.empty()) return;
+    std::vector<uint64_t> ctr((size_t)cap_slots_ * CTR_N);
+    d2h(ctr.data(), d_ctr_, ctr.size(), st_);
+    HIP_TRY(hipStreamSynchronize(st_));
+    const auto now = Clock::now();
+    for (auto& r : runs) {
+        const uint64_t* x = &ctr[(size_t)r->slot * CTR_N];
+        hgx_counts& c = r->counts;
+        std::memset(&c, 0, sizeof(c));
+        c.raw_states = r->states;
+        c.events = r->events;
+        c.canonical_states = x[CTR_CANON];
+        c.fingerprint = x[CTR_FINGERPRINT];
+        const bool rec_causal = r->h.flags & HGX_REC_CAUSAL;
+        c.causal_triples = rec_causal ? x[CTR_TRIPLES] : 0;
+        c.causal_pairs = rec_causal ? x[CTR_PAIRS] : 0;
+        const bool tr = r->h.flags & HGX_OPT_TR;
+        c.causal_pairs_tr = tr ? x[CTR_PAIRS_TR] : c.causal_pairs;
+        c.causal_triples_tr = tr ? x[CTR_TRIPLES_TR] : c.causal_triples;
+        c.branchial_pairs = (r->h.flags & HGX_REC_BRANCHIAL) ? x[CTR_BRANCHIAL] : 0;
+        c.steps_done = r->steps_done;
+        c.truncated = r->truncated ? 1u : 0u;
+        r->t_done = now;
+        r->device_ms = std::chrono::duration<double, std::milli>(now - r->t_admit).count();
+        finished[r->id] = r;
+        live_.erase(r->id);
+        live_footprint_ -= r->footprint;
+        release_next_.push_back(r->slot);
+        slab_.free(r->d_rules);
+        slab_.free(r->map_keys);
+        slab_.free(r->map_vals);
+        r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
+    }
+}
+
+uint32_t Engine::step() {
+    hipStream_t st = st_;
+    Arena tmp(pool_.get(), (size_t)4 << 20);
+    const uint32_t n = F_ ? F_->v.n : 0;
+    const uint32_t nF = (uint32_t)F_runs_.size();
+    std::vector<uint64_t> tot(nF, 0);
+    uint64_t M = 0, parent_edges = 0;
+    uint32_t K = 1;
+    MatchView mv{};
+    uint32_t* cnt = nullptr;
+    uint64_t* moff = nullptr;
+    std::vector<uint8_t> h_active(cap_slots_, 0);
+
+    // ---- 1. the join over the frontier, with the state-cap decision ----
+    if (n) {
+        std::vector<uint32_t> h_budget(cap_slots_, 0), h_b(cap_slots_, 0), h_e(cap_slots_, 0);
+        bool any = false;
+        for (uint32_t i = 0; i < nF; ++i) {
+            RunRec& r = *F_runs_[i];
+            h_b[r.slot] = F_b_[i];
+            h_e[r.slot] = F_e_[i];
+            if (r.done || r.depth >= r.h.steps) { r.done = true; continue; }
+            h_active[r.slot] = 1;
+            h_budget[r.slot] = !r.h.cap ? 0xFFFFFFFFu : (r.states >= r.h.cap ? 0u : (uint32_t)(r.h.cap - r.states));
+            any = true;
+        }
+        if (any) {
+            h2d(d_active_, h_active.data(), cap_slots_, st);
+            h2d(d_budget_, h_budget.data(), cap_slots_, st);
+            h2d(d_sbeg_, h_b.data(), cap_slots_, st);
+            h2d(d_send_, h_e.data(), cap_slots_, st);
+            HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * cap_slots_, st));
+            HIP_TRY(hipMemsetAsync(d_tot_ + cap_slots_, 0, sizeof(uint64_t), st));
+            cnt = dalloc<uint32_t>(tmp, n + 1);
+            moff = dalloc<uint64_t>(tmp, n + 1);
+            HIP_TRY(hipMemsetAsync(cnt + n, 0, sizeof(uint32_t), st));
+            if (F_->max_ne > 65535)
+                throw Error{HGX_ERR_UNSUPPORTED, "states above 65535 edges need the large-state join"};
+            MatchArgs ma{};
+            ma.P = F_->v; ma.runs = d_runs_; ma.active = d_active_; ma.cnt = cnt; ma.moff = moff;
+            ma.max_ne = std::max(1u, F_->max_ne);
+            ma.max_nextv = std::max(1u, F_->max_nextv);
+            ma.lds_per_wave = (match_lds_per_wave(ma.max_ne, ma.max_nextv, F_->v.A) + 15u) & ~15u;
+            ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
+            ma.budget = d_budget_;
+            ma.run_acc = (unsigned long long*)d_run_acc_;
+            const uint32_t wpb_m = waves_per_block(ma.lds_per_wave, ctx_->max_lds);
+            const uint64_t match_bytes = F_->n_edges * (4ull * F_->v.A + 1) + (uint64_t)n * 24;
+            timed(ctx_, st, "k_match_count", match_bytes + n * 4ull, n,
+                  [&] { return launch_match(ma, false, ctx_->num_cu * 8, wpb_m, st); });
+            scan_u32_to_u64(tmp, cnt, moff, n, st);
+            HIP_TRY(launch_run_totals(moff, d_sbeg_, d_send_, cap_slots_, d_tot_, st));
+            std::vector<uint64_t> htot(cap_slots_ + 1);
+            d2h(htot.data(), d_tot_, cap_slots_ + 1, st);
+            HIP_TRY(hipStreamSynchronize(st));
+            parent_edges = htot[cap_slots_];
+            bool masked = false;
+            for (uint32_t i = 0; i < nF; ++i) {
+                RunRec& r = *F_runs_[i];
+                if (!h_active[r.slot]) continue;
+                const uint64_t t = htot[r.slot];
+                if (r.h.cap && r.states + t > r.h.cap) {
+                    r.truncated = true; r.done = true; h_active[r.slot] = 0; masked = true;
+                } else if (t == 0) {
+                    r.done = true; h_active[r.slot] = 0;
+                } else {
+                    tot[i] = t;
+                    for (auto& R : r.h.rules) K = std::max<uint32_t>(K, R.n_lhs);
+                }
+            }
+            if (masked) {
+                h2d(d_active_, h_active.data(), cap_slots_, st);
+                HIP_TRY(launch_mask_counts(F_->v.run, d_active_, n, cnt, st));
+                scan_u32_to_u64(tmp, cnt, moff, n, st);
+            }
+            for (uint32_t i = 0; i < nF; ++i) M += tot[i];
+            if (M > 0xFFFFFFF0ull) throw Error{HGX_ERR_UNSUPPORTED, "more than 2^32 events in one generation"};
+            if (M) {
+                // uncapped maps grow ahead of the inserts
+                for (uint32_t i = 0; i < nF; ++i) {
+                    if (!tot[i]) continue;
+                    RunRec& r = *F_runs_[i];
+                    const uint64_t need = 2 * (r.states + tot[i]);
+                    if (need <= r.map_cap) continue;
+                    uint64_t c = r.map_cap;
+                    while (c < need) c <<= 1;
+                    if (c > 0x80000000ull) throw Error{HGX_ERR_UNSUPPORTED, "state map above 2^31 slots"};
+                    uint64_t* keys = (uint64_t*)slab_.alloc(8 * c);
+                    uint32_t* vals = (uint32_t*)slab_.alloc(4 * c);
+                    HIP_TRY(hipMemsetAsync(keys, 0, 8 * c, st));
+                    HIP_TRY(launch_rehash(r.map_keys, r.map_vals, r.map_cap, keys, vals, (uint32_t)(c - 1), st));
+                    slab_.free(r.map_keys);
+                    slab_.free(r.map_vals);
+                    r.map_keys = keys; r.map_vals = vals; r.map_cap = (uint32_t)c;
+                    DRun& d = h_runs_[r.slot];
+                    d.map_keys = keys; d.map_vals = vals; d.map_mask = (uint32_t)(c - 1);
+                }
+                mv.n = (uint32_t)M; mv.K = K;
+                mv.parent = dalloc<uint32_t>(tmp, M);
+                mv.rule = dalloc<uint32_t>(tmp, M);
+                mv.q = dalloc<uint16_t>(tmp, M * K);
+                mv.cne = dalloc<uint32_t>(tmp, M + 1);
+                HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
+                ma.M = mv;
+                ma.parent_edges = nullptr;
+                timed(ctx_, st, "k_match_emit", match_bytes + M * (12ull + 2ull * K), n,
+                      [&] { return launch_match(ma, true, ctx_->num_cu * 8, wpb_m, st); });
+            }
+        }
+    }
+
+    // ---- 2. admissions: queued requests join the generation built now ----
+    std::vector<std::shared_ptr<RunRec>> admitted;
+    const uint64_t soft = (uint64_t)(pool_->budget() * soft_budget_frac);
+    while (!pending_.empty()) {
+        auto r = pending_.front();
+        if (!live_.empty() && live_footprint_ + r->footprint > soft) break;
+        pending_.pop_front();
+        admit(r);
+        admitted.push_back(r);
+    }
+    uint32_t NR = 0;
+    uint64_t NER = 0;
+    for (auto& r : admitted) { NR += (uint32_t)r->h.roots.size(); NER += r->h.root_edges; }
+
+    // ---- 3. the child generation: children of the frontier, then the new roots ----
+    std::vector<std::shared_ptr<RunRec>> next_runs;
+    std::vector<uint32_t> next_b, next_e;
+    std::vector<std::shared_ptr<RunRec>> done_now;
+    std::shared_ptr<DevGen> C;
+    // host staging for the roots; must outlive the stream synchronisation below
+    std::vector<uint32_t> r_run, r_ne, r_nextv, r_sid, r_ev;
+    std::vector<uint64_t> r_eoff;
+    std::vector<uint8_t> r_depth, r_ar;
+    const uint32_t nC = (uint32_t)M + NR;
+    if (nC) {
+        C = std::make_shared<DevGen>(pool_);
+        uint32_t A = 1, lstride = 1, max_ne = 0, max_nextv = 0;
+        bool keep = false;
+        for (uint32_t i = 0; i < nF; ++i) {
+            if (!tot[i]) continue;
+            const RunRec& r = *F_runs_[i];
+            A = std::max(A, r.h.max_arity);
+            lstride = std::max(lstride, r.h.steps);
+            keep |= (r.h.flags & HGX_REC_STATES) != 0;
+        }
+        if (M) {
+            uint32_t gain = 0, nw = 0;
+            for (uint32_t i = 0; i < nF; ++i)
+                if (tot[i]) { gain = std::max(gain, F_runs_[i]->h.max_gain); nw = std::max(nw, F_runs_[i]->h.max_new); }
+            max_ne = F_->max_ne + gain;
+            max_nextv = F_->max_nextv + nw;
+        }
+        for (auto& r : admitted) {
+            A = std::max(A, r->h.max_arity);
+            lstride = std::max(lstride, r->h.steps);
+            keep |= (r->h.flags & HGX_REC_STATES) != 0;
+            for (size_t k = 0; k < r->h.roots.size(); ++k) {
+                max_ne = std::max<uint32_t>(max_ne, (uint32_t)r->h.roots[k].size());
+                max_nextv = std::max(max_nextv, r->h.root_nv[k]);
+            }
+        }
+        C->keep_edges = keep;
+        C->max_ne = max_ne;
+        C->max_nextv = max_nextv;
+        GenView& v = C->v;
+        v.n = nC; v.A = A; v.lstride = lstride;
+        v.run = dalloc<uint32_t>(C->meta, nC);
+        v.eoff = dalloc<uint64_t>(C->meta, nC + 1);
+        v.ne = dalloc<uint32_t>(C->meta, nC);
+        v.nextv = dalloc<uint32_t>(C->meta, nC);
+        v.sid = dalloc<uint32_t>(C->meta, nC);
+        v.hash = dalloc<uint64_t>(C->meta, nC);
+        v.lanc = dalloc<uint64_t>(C->meta, (size_t)nC * lstride);
+        v.depth = dalloc<uint8_t>(C->meta, nC);
+        if (M) {
+            C->ev.in_state = dalloc<uint32_t>(C->meta, M);
+            C->ev.rule = dalloc<uint32_t>(C->meta, M);
+            C->ev.prod = dalloc<uint64_t>(C->meta, M);
+            C->ev.tr = dalloc<uint64_t>(C->meta, M);
+            scan_u32_to_u64(tmp, mv.cne, v.eoff, (uint32_t)M, st);
+        }
+        uint64_t EC = 0;
+        if (M) {
+            d2h(&EC, v.eoff + M, 1, st);
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+        C->n_edges = EC + NER;
+        v.ev = dalloc<uint32_t>(C->edges, (EC + NER) * A);
+        v.ar = dalloc<uint8_t>(C->edges, EC + NER);
+        v.ps = dalloc<uint8_t>(C->edges, EC + NER);
+
+        // per-run child ranges and event numbering
+        uint32_t o = 0;
+        std::vector<uint32_t> cb(nF, 0), ce(nF, 0);
+        for (uint32_t i = 0; i < nF; ++i) {
+            if (!tot[i]) continue;
+            RunRec& r = *F_runs_[i];
+            cb[i] = o; o += (uint32_t)tot[i]; ce[i] = o;
+            h_runs_[r.slot].ev_base = (uint32_t)r.events;
+            h_runs_[r.slot].child_begin = cb[i];
+        }
+        h2d(d_runs_, h_runs_.data(), cap_slots_, st);
+
+        if (M) {
+            ExpandArgs ea{};
+            ea.P = F_->v; ea.C = v; ea.M = mv; ea.E = C->ev; ea.runs = d_runs_; ea.ctr = d_ctr_;
+            ea.max_child_ne = std::max(1u, max_ne);
+            ea.max_nv = std::max(1u, std::min(max_nextv, max_ne * A));
+            ea.max_nextv = std::max(1u, max_nextv);
+            ea.lds_per_wave = expand_lds_per_wave(ea.max_child_ne, ea.max_nv, ea.max_nextv, A);
+            const uint32_t wpb_e = place_workspace(tmp, ea.lds_per_wave, (uint32_t)M, ctx_->max_lds,
+                                                   ctx_->num_cu * 8, &ea.scratch);
+            const uint64_t bytes = M * (8ull + 2ull * K) + parent_edges * (4ull * F_->v.A + 2) +
+                                   EC * (4ull * A + 2) + M * (28ull + 8ull * lstride) + M * 24ull + M * 12ull;
+            timed(ctx_, st, "k_expand", bytes, M, [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, st); });
+        }
+
+        // new roots: uploaded, hashed, inserted into their maps
+        if (NR) {
+            r_run.resize(NR); r_ne.resize(NR); r_nextv.resize(NR); r_sid.resize(NR);
+            r_eoff.resize(NR + 1); r_depth.assign(NR, 0); r_ar.resize(NER); r_ev.assign(NER * A, 0);
+            uint32_t s = 0;
+            uint64_t e = 0;
+            for (auto& r : admitted) {
+                for (size_t k = 0; k < r->h.roots.size(); ++k, ++s) {
+                    r_run[s] = r->slot;
+                    r_ne[s] = (uint32_t)r->h.roots[k].size();
+                    r_nextv[s] = r->h.root_nv[k];
+                    r_sid[s] = (uint32_t)k;
+                    r_eoff[s] = EC + e;
+                    for (auto& ed : r->h.roots[k]) {
+                        r_ar[e] = (uint8_t)ed.size();
+                        for (size_t x = 0; x < ed.size(); ++x) r_ev[e * A + x] = ed[x];
+                        ++e;
+                    }
+                }
+            }
+            r_eoff[NR] = EC + e;
+            h2d(v.run + M, r_run.data(), NR, st);
+            h2d(v.ne + M, r_ne.data(), NR, st);
+            h2d(v.nextv + M, r_nextv.data(), NR, st);
+            h2d(v.sid + M, r_sid.data(), NR, st);
+            h2d(v.depth + M, r_depth.data(), NR, st);
+            h2d(v.eoff + M, r_eoff.data(), NR + 1, st);
+            h2d(v.ar + EC, r_ar.data(), NER, st);
+            h2d(v.ev + EC * A, r_ev.data(), NER * A, st);
+            if (NER) HIP_TRY(hipMemsetAsync(v.ps + EC, 0, NER, st));
+            uint32_t rmax_ne = 1, rmax_nv = 1;
+            for (auto& r : admitted)
+                for (size_t k = 0; k < r->h.roots.size(); ++k) {
+                    rmax_ne = std::max<uint32_t>(rmax_ne, (uint32_t)r->h.roots[k].size());
+                    rmax_nv = std::max(rmax_nv, r->h.root_nv[k]);
+                }
+            WLStatesArgs w{};
+            w.n = NR; w.A = A; w.state_off = v.eoff + M; w.ar = v.ar; w.ev = v.ev; w.nv = v.nextv + M;
+            w.out = v.hash + M; w.run_of = v.run + M; w.sid_of = v.sid + M; w.runs = d_runs_; w.ctr = d_ctr_;
+            w.max_ne = rmax_ne; w.max_nv = rmax_nv;
+            w.lds_per_wave = wl_states_lds_per_wave(w.max_ne, w.max_nv, A);
+            const uint32_t wpb = place_workspace(tmp, w.lds_per_wave, NR, ctx_->max_lds, ctx_->num_cu * 8,
+                                                 &w.scratch);
+            timed(ctx_, st, "k_wl_states", NER * (4ull * A + 1) + NR * 36ull, NR,
+                  [&] { return launch_wl_states(w, ctx_->num_cu * 8, wpb, st); });
+        }
+
+        // branchial relation of this generation's events
+        std::vector<uint64_t> h_poff;
+        if (M) {
+            bool any_b = false, all_b = true, any_rel = false;
+            std::vector<uint8_t> bflag(cap_slots_, 0);
+            for (uint32_t i = 0; i < nF; ++i) {
+                if (!tot[i]) continue;
+                const RunRec& r = *F_runs_[i];
+                const bool b = (r.h.flags & HGX_REC_BRANCHIAL) != 0;
+                bflag[r.slot] = b ? 1 : 0;
+                any_b |= b; all_b &= b;
+                any_rel |= b && (r.h.flags & HGX_REC_RELATIONS);
+            }
+            if (any_b) {
+                uint32_t* bcnt = cnt;
+                if (!all_b) {
+                    bcnt = dalloc<uint32_t>(tmp, n + 1);
+                    HIP_TRY(hipMemcpyAsync(bcnt, cnt, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
+                    uint8_t* d_bf = dalloc<uint8_t>(tmp, cap_slots_);
+                    h2d(d_bf, bflag.data(), cap_slots_, st);
+                    HIP_TRY(launch_mask_counts(F_->v.run, d_bf, n, bcnt, st));
+                    HIP_TRY(hipStreamSynchronize(st));
+                }
+                BranchialArgs ba{};
+                ba.P = F_->v; ba.M = mv; ba.runs = d_runs_; ba.cnt = bcnt; ba.moff = moff; ba.ctr = d_ctr_;
+                ba.emit = 0; ba.lds_per_wave = branchial_lds_per_wave();
+                uint32_t* pcnt = nullptr;
+                if (any_rel) {
+                    pcnt = dalloc<uint32_t>(tmp, n + 1);
+                    HIP_TRY(hipMemsetAsync(pcnt, 0, sizeof(uint32_t) * (n + 1), st));
+                }
+                ba.pcnt = pcnt;
+                timed(ctx_, st, "k_branchial", M * (6ull + 2ull * K) + n * 12ull, n,
+                      [&] { return launch_branchial(ba, ctx_->num_cu * 8, 2, st); });
+                if (any_rel) {
+                    uint64_t* poff = dalloc<uint64_t>(tmp, n + 1);
+                    scan_u32_to_u64(tmp, pcnt, poff, n, st);
+                    h_poff.resize(n + 1);
+                    d2h(h_poff.data(), poff, n + 1, st);
+                    HIP_TRY(hipStreamSynchronize(st));
+                    const uint64_t NB = h_poff[n];
+                    if (NB) {
+                        C->br_pairs = dalloc<uint32_t>(C->meta, 2 * NB);
+                        ba.emit = 1; ba.poff = poff; ba.pairs = C->br_pairs;
+                        HIP_TRY(launch_branchial(ba, ctx_->num_cu * 8, 2, st));
+                    }
+                }
+            }
+        }
+
+        // bookkeeping
+        for (uint32_t i = 0; i < nF; ++i) {
+            if (!tot[i]) continue;
+            auto& r = F_runs_[i];
+            Segment sg{C, cb[i], ce[i], 0, 0};
+            if (!h_poff.empty()) { sg.br_b = h_poff[F_b_[i]]; sg.br_e = h_poff[F_e_[i]]; }
+            r->segs.push_back(sg);
+            r->states += tot[i];
+            r->events += tot[i];
+            r->depth += 1;
+            r->steps_done = r->depth;
+            if (r->depth >= r->h.steps) r->done = true;
+            next_runs.push_back(r);
+            next_b.push_back(cb[i]);
+            next_e.push_back(ce[i]);
+        }
+        uint32_t s = (uint32_t)M;
+        for (auto& r : admitted) {
+            const uint32_t nr = (uint32_t)r->h.roots.size();
+            r->segs.push_back(Segment{C, s, s + nr, 0, 0});
+            r->states = nr;
+            if (r->h.steps == 0) r->done = true;
+            next_runs.push_back(r);
+            next_b.push_back(s);
+            next_e.push_back(s + nr);
+            s += nr;
+        }
+    }
+
+    // ---- 4. finished runs, slot recycling, the new frontier ----
+    for (auto& r : F_runs_)
+        if (r->done && live_.count(r->id)) done_now.push_back(r);
+    for (auto& r : next_runs)
+        if (r->done && live_.count(r->id) &&
+            std::find(done_now.begin(), done_now.end(), r) == done_now.end())
+            done_now.push_back(r);
+    std::vector<uint32_t> releasable;
+    releasable.swap(release_next_);
+    finalize(done_now);                        // synchronises the stream
+    if (done_now.empty()) HIP_TRY(hipStreamSynchronize(st));
+    prof_resolve(ctx_);
+    for (uint32_t s : releasable) free_slots_.push_back(s);
+    if (F_ && !F_->keep_edges) F_->edges.release();
+    F_ = C;
+    F_runs_.swap(next_runs);
+    F_b_.swap(next_b);
+    F_e_.swap(next_e);
+    // a frontier holding only finished runs is dead
+    bool live_in_F = false;
+    for (auto& r : F_runs_) live_in_F |= !r->done;
+    if (!live_in_F) {
+        if (F_ && !F_->keep_edges) F_->edges.release();
+        F_.reset();
+        F_runs_.clear(); F_b_.clear(); F_e_.clear();
+        for (uint32_t s2 : release_next_) free_slots_.push_back(s2);
+        release_next_.clear();
+    }
+    return live_or_queued();
+}
+
+// ------------------------------------------------------------------------------------------
+// result readback
+// ------------------------------------------------------------------------------------------
+
+struct RunTables {
+    std::vector<uint64_t> hash;
+    std::vector<uint32_t> step, creator;
+    std::vector<uint32_t> ev_in, ev_rule, ev_step;
+    std::vector<uint64_t> ev_prod, ev_tr;
+};
+
+static RunTables run_tables(const RunRec& R) {
+    RunTables T;
+    const uint64_t ns = R.states;
+    const uint32_t nroots = (uint32_t)R.h.roots.size();
+    const uint64_t ne = ns >= nroots ? ns - nroots : 0;
+    T.hash.resize(ns); T.step.resize(ns); T.creator.assign(ns, 0xFFFFFFFFu);
+    T.ev_in.resize(ne); T.ev_rule.resize(ne); T.ev_step.resize(ne); T.ev_prod.resize(ne); T.ev_tr.resize(ne);
+    for (size_t k = 0; k < R.segs.size(); ++k) {
+        const Segment& sg = R.segs[k];
+        const uint32_t m = sg.e - sg.b;
+        if (!m) continue;
+        std::vector<uint64_t> h(m);
+        std::vector<uint32_t> sid(m);
+        std::vector<uint8_t> dep(m);
+        d2h(h.data(), sg.g->v.hash + sg.b, m, R.st);
+        d2h(sid.data(), sg.g->v.sid + sg.b, m, R.st);
+        d2h(dep.data(), sg.g->v.depth + sg.b, m, R.st);
+        std::vector<uint32_t> in, rule;
+        std::vector<uint64_t> prod, tr;
+        const bool events = k > 0;                  // the first segment holds the roots
+        if (events) {
+            in.resize(m); rule.resize(m); prod.resize(m); tr.resize(m);
+            d2h(in.data(), sg.g->ev.in_state + sg.b, m, R.st);
+            d2h(rule.data(), sg.g->ev.rule + sg.b, m, R.st);
+            d2h(prod.data(), sg.g->ev.prod + sg.b, m, R.st);
+            d2h(tr.data(), sg.g->ev.tr + sg.b, m, R.st);
+        }
+        HIP_TRY(hipStreamSynchronize(R.st));
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t s = sid[i];
+            if (s >= ns) throw Error{HGX_ERR_DEVICE, "state id out of range in readback"};
+            T.hash[s] = h[i];
+            T.step[s] = dep[i];
+            if (events) {
+                const uint32_t e = s - nroots;
+                T.creator[s] = e;
+                T.ev_in[e] = in[i]; T.ev_rule[e] = rule[i]; T.ev_step[e] = dep[i];
+                T.ev_prod[e] = prod[i]; T.ev_tr[e] = tr[i];
+            }
+        }
+    }
+    return T;
+}
+
+}  // namespace hgx
+
+using namespace hgx;
+
+static hgx_result* make_result(std::shared_ptr<RunRec> r) {
+    auto* out = new hgx_result();
+    out->run = r;
+    out->device_ns = (uint64_t)(r->device_ms * 1e6);
+    out->latency_ns = (uint64_t)std::chrono::duration<double, std::nano>(r->t_done - r->t_submit).count();
+    return out;
+}
+
+extern "C" int hgx_evolve_batch(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_result** results) {
+    if (!ctx || (!runs && n) || (!results && n)) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        std::vector<RunHost> hs;
+        hs.reserve(n);
+        for (uint32_t i = 0; i < n; ++i) hs.push_back(parse_run(&runs[i]));
+        Engine E(ctx);
+        E.soft_budget_frac = 0.8;
+        E.reserve_slots(n + 1);
+        for (uint32_t i = 0; i < n; ++i) E.submit(std::move(hs[i]), i + 1);
+        while (E.live_or_queued()) E.step();
+        for (uint32_t i = 0; i < n; ++i) results[i] = make_result(E.take(i + 1));
+        return HGX_OK;
+    });
+}
+
+extern "C" int hgx_request_submit(hgx_ctx* ctx, const hgx_run_desc* run, uint64_t* request_id) {
+    if (!ctx || !run || !request_id) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        RunHost h = parse_run(run);
+        if (!ctx->serve) { HIP_TRY(hipSetDevice(ctx->device)); ctx->serve = std::make_unique<Engine>(ctx); }
+        const uint64_t id = ctx->next_request_id++;
+        ctx->serve->submit(std::move(h), id);
+        *request_id = id;
+        return HGX_OK;
+    });
+}
+
+extern "C" int hgx_serve_step(hgx_ctx* ctx, uint32_t* live) {
+    if (!ctx) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        uint32_t l = 0;
+        if (ctx->serve) { HIP_TRY(hipSetDevice(ctx->device)); l = ctx->serve->step(); }
+        if (live) *live = l;
+        return HGX_OK;
+    });
+}
+
+extern "C" int hgx_request_poll(hgx_ctx* ctx, uint64_t id, hgx_result** out) {
+    if (!ctx || !out) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    *out = nullptr;
+    if (!ctx->serve || !ctx->serve->known(id)) { ctx->err = "unknown request id"; return HGX_ERR_INVALID; }
+    auto r = ctx->serve->take(id);
+    if (!r) return HGX_ERR_NOT_READY;
+    *out = make_result(r);
+    return HGX_OK;
+}
+
+extern "C" int hgx_request_wait(hgx_ctx* ctx, uint64_t id, hgx_result** out) {
+    if (!ctx || !out) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        *out = nullptr;
+        if (!ctx->serve || !ctx->serve->known(id)) { ctx->err = "unknown request id"; return HGX_ERR_INVALID; }
+        HIP_TRY(hipSetDevice(ctx->device));
+        while (true) {
+            auto r = ctx->serve->take(id);
+            if (r) { *out = make_result(r); return HGX_OK; }
+            ctx->serve->step();
+        }
+    });
+}
+
+extern "C" int hgx_serve_drain(hgx_ctx* ctx, uint64_t* ids, hgx_result** results, uint32_t cap, uint32_t* n) {
+    if (!ctx || !n || (cap && (!ids || !results))) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    uint32_t k = 0;
+    if (ctx->serve) {
+        auto& fin = ctx->serve->finished;
+        for (auto it = fin.begin(); it != fin.end() && k < cap;) {
+            ids[k] = it->first;
+            results[k] = make_result(it->second);
+            ++k;
+            it = fin.erase(it);
+        }
+    }
+    *n = k;
+    return HGX_OK;
+}
+
+extern "C" uint64_t hgx_request_footprint(const hgx_run_desc* run) {
+    try {
+        return footprint_bytes(parse_run(run));
+    } catch (...) {
+        return 0;
+    }
+}
+
+extern "C" int hgx_result_counts(const hgx_result* r, hgx_counts* out) {
+    if (!r || !out) return HGX_ERR_INVALID;
+    *out = r->run->counts;
+    return HGX_OK;
+}
+
+extern "C" int hgx_result_counts_many(const hgx_result* const* rs, uint32_t n, hgx_counts* out) {
+    if ((!rs || !out) && n) return HGX_ERR_INVALID;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!rs[i]) return HGX_ERR_INVALID;
+        out[i] = rs[i]->run->counts;
+    }
+    return HGX_OK;
+}
+
+extern "C" int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint64_t* latency_ns) {
+    if (!r) return HGX_ERR_INVALID;
+    if (device_ns) *device_ns = r->device_ns;
+    if (latency_ns) *latency_ns = r->latency_ns;
+    return HGX_OK;
+}
+
+extern "C" void hgx_result_release(hgx_result* r) { delete r; }
+
+extern "C" int hgx_result_states(const hgx_result* r, uint64_t* hash, uint32_t* step, uint32_t* creator,
+                                 uint64_t cap) {
+    if (!r || cap < r->run->counts.raw_states) return HGX_ERR_INVALID;
+    try {
+        RunTables T = run_tables(*r->run);
+        for (uint64_t s = 0; s < r->run->states; ++s) {
+            if (hash) hash[s] = T.hash[s];
+            if (step) step[s] = T.step[s];
+            if (creator) creator[s] = T.creator[s];
+        }
+        return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_result_events(const hgx_result* r, hgx_event* out, uint64_t cap) {
+    if (!r || (!out && r->run->events) || cap < r->run->events) return HGX_ERR_INVALID;
+    try {
+        RunTables T = run_tables(*r->run);
+        const uint32_t nroots = (uint32_t)r->run->h.roots.size();
+        for (uint64_t e = 0; e < r->run->events; ++e)
+            out[e] = hgx_event{T.ev_in[e], (uint32_t)(e + nroots), T.ev_rule[e], T.ev_step[e]};
+        return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_result_causal(const hgx_result* r, hgx_pair* pairs, uint8_t* in_tr, uint64_t cap) {
+    if (!r) return HGX_ERR_INVALID;
+    const RunRec& R = *r->run;
+    if (!(R.h.flags & HGX_REC_CAUSAL) || cap < R.counts.causal_pairs) return HGX_ERR_INVALID;
+    try {
+        RunTables T = run_tables(R);
+        uint64_t o = 0;
+        for (uint64_t c = 0; c < R.events; ++c) {
+            for (uint64_t m = T.ev_prod[c]; m; m &= m - 1) {
+                const uint32_t s = (uint32_t)__builtin_ctzll(m);
+                uint32_t x = T.ev_in[c];                 // walk up the lineage to generation s
+                while (T.step[x] > s) x = T.ev_in[T.creator[x]];
+                if (o >= cap) return HGX_ERR_INVALID;
+                if (pairs) pairs[o] = hgx_pair{T.creator[x], (uint32_t)c};
+                if (in_tr) in_tr[o] = (uint8_t)((T.ev_tr[c] >> s) & 1u);
+                ++o;
+            }
+        }
+        return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64_t cap) {
+    if (!r) return HGX_ERR_INVALID;
+    const RunRec& R = *r->run;
+    if (!(R.h.flags & HGX_REC_BRANCHIAL) || !(R.h.flags & HGX_REC_RELATIONS)) return HGX_ERR_INVALID;
+    if (cap < R.counts.branchial_pairs) return HGX_ERR_INVALID;
+    try {
+        uint64_t o = 0;
+        for (const Segment& sg : R.segs) {
+            if (sg.br_e <= sg.br_b || !sg.g->br_pairs) continue;
+            std::vector<uint32_t> tmp(2 * (sg.br_e - sg.br_b));
+            d2h(tmp.data(), sg.g->br_pairs + 2 * sg.br_b, tmp.size(), R.st);
+            HIP_TRY(hipStreamSynchronize(R.st));
+            for (uint64_t i = 0; i < sg.br_e - sg.br_b; ++i) {
+                if (o >= cap) return HGX_ERR_INVALID;
+                pairs[o++] = hgx_pair{tmp[2 * i], tmp[2 * i + 1]};
+            }
+        }
+        return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_result_state_edges(const hgx_result* r, uint64_t* state_offsets, uint32_t* edge_offsets,
+                                      uint32_t* vertices, uint64_t* n_edges_total, uint64_t* n_vertices_total) {
+    if (!r) return HGX_ERR_INVALID;
+    const RunRec& R = *r->run;
+    if (!(R.h.flags & HGX_REC_STATES)) return HGX_ERR_INVALID;
+    try {
+        uint64_t ne_tot = 0, nv_tot = 0, s_idx = 0;
+        for (const Segment& sg : R.segs) {
+            const uint32_t m = sg.e - sg.b;
+            if (!m) continue;
+            const uint32_t A = sg.g->v.A;
+            std::vector<uint64_t> eo(m + 1);
+            d2h(eo.data(), sg.g->v.eoff + sg.b, m + 1, R.st);
+            HIP_TRY(hipStreamSynchronize(R.st));
+            const uint64_t E0 = eo[0], E1 = eo[m];
+            std::vector<uint8_t> ar(E1 - E0);
+            std::vector<uint32_t> ev((E1 - E0) * A);
+            d2h(ar.data(), sg.g->v.ar + E0, ar.size(), R.st);
+            d2h(ev.data(), sg.g->v.ev + E0 * A, ev.size(), R.st);
+            HIP_TRY(hipStreamSynchronize(R.st));
+            for (uint32_t i = 0; i < m; ++i) {
+                if (state_offsets) state_offsets[s_idx] = ne_tot;
+                ++s_idx;
+                for (uint64_t e = eo[i]; e < eo[i + 1]; ++e) {
+                    const uint32_t a = ar[e - E0];
+                    if (edge_offsets) edge_offsets[ne_tot] = (uint32_t)nv_tot;
+                    if (vertices) for (uint32_t k = 0; k < a; ++k) vertices[nv_tot + k] = ev[(e - E0) * A + k];
+                    nv_tot += a;
+                    ++ne_tot;
+                }
+            }
+        }
+        if (state_offsets) state_offsets[s_idx] = ne_tot;
+        if (edge_offsets) edge_offsets[ne_tot] = (uint32_t)nv_tot;
+        if (n_edges_total) *n_edges_total = ne_tot;
+        if (n_vertices_total) *n_vertices_total = nv_tot;
+        return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_match_count(hgx_ctx* ctx, const uint8_t*, uint32_t, uint32_t, uint32_t, const uint32_t*,
+                               const uint32_t*, uint32_t, uint64_t*, uint32_t*, uint64_t, float*) {
+    if (ctx) ctx->err = "large-state join not built in this library";
+    return HGX_ERR_UNSUPPORTED;
+}

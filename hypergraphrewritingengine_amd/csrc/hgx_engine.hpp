@@ -1,0 +1,121 @@
+// hgx_engine.hpp -- the generation engine behind both the batch and the served path.
+//
+// The engine holds a FRONTIER: one device generation of states drawn from every admitted
+// request, each at its own depth. step() advances all of them by one generation -- join,
+// cap decision, rewrite + WL + dedup + causal, branchial -- and admits queued requests by
+// appending their roots to the generation it builds, so a request that arrives while others
+// run joins the very next launch. A request that finishes leaves its counters and a view of
+// its generations behind; its slot, rules and state map are recycled.
+#pragma once
+
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "hgx_runtime.hpp"
+
+namespace hgx {
+
+// One generation on the device. `meta` holds per-state and per-event arrays (kept while any
+// result may read them); `edges` holds the edge pool (released once the next generation has
+// been built from it, unless a request asked to keep its states' edges).
+struct DevGen {
+    std::shared_ptr<ChunkPool> pool;
+    Arena meta;
+    Arena edges;
+    GenView v{};
+    EventView ev{};
+    uint64_t n_edges = 0;
+    uint32_t max_ne = 0, max_nextv = 0;
+    uint32_t* br_pairs = nullptr;    // branchial pairs of the events that created these states
+    bool keep_edges = false;
+    explicit DevGen(std::shared_ptr<ChunkPool> p)
+        : pool(p), meta(p.get(), (size_t)1 << 20), edges(p.get(), (size_t)4 << 20) {}
+};
+
+struct Segment {
+    std::shared_ptr<DevGen> g;       // generation holding this run's states
+    uint32_t b, e;                   // their index range in it
+    uint64_t br_b, br_e;             // this run's branchial pairs in g->br_pairs
+};
+
+using Clock = std::chrono::steady_clock;
+
+struct RunRec {
+    uint64_t id = 0;
+    RunHost h;
+    uint32_t slot = 0xFFFFFFFFu;
+    DRule* d_rules = nullptr;
+    uint64_t* map_keys = nullptr;
+    uint32_t* map_vals = nullptr;
+    uint32_t map_cap = 0;
+    uint64_t states = 0, events = 0;
+    uint32_t depth = 0, steps_done = 0;
+    bool truncated = false, done = false, admitted = false;
+    uint64_t footprint = 0;
+    std::vector<Segment> segs;       // every state of the run, in state-id order
+    hgx_counts counts{};
+    hipStream_t st = nullptr;        // the context's stream (contexts outlive results)
+    Clock::time_point t_submit, t_admit, t_done;
+    double device_ms = 0;
+};
+
+class Engine {
+public:
+    explicit Engine(hgx_ctx* ctx);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    uint64_t submit(RunHost&& h, uint64_t id);
+    void reserve_slots(uint32_t n) { grow_slots(n); }
+    // One generation of every admitted request, plus admissions. Returns live + queued runs.
+    uint32_t step();
+    uint32_t live_or_queued() const { return (uint32_t)(live_.size() + pending_.size()); }
+    // A finished run (removed from the finished set), or null.
+    std::shared_ptr<RunRec> take(uint64_t id);
+    bool known(uint64_t id) const;
+    std::map<uint64_t, std::shared_ptr<RunRec>> finished;
+    double soft_budget_frac = 0.6;
+
+private:
+    hgx_ctx* ctx_;
+    hipStream_t st_;
+    std::shared_ptr<ChunkPool> pool_;
+    SlabAlloc slab_;
+    std::unique_ptr<Arena> slot_arena_;
+    uint32_t cap_slots_ = 0;
+    DRun* d_runs_ = nullptr;
+    uint64_t* d_ctr_ = nullptr;
+    uint8_t* d_active_ = nullptr;
+    uint32_t* d_budget_ = nullptr;
+    uint64_t* d_run_acc_ = nullptr;
+    uint32_t* d_sbeg_ = nullptr;
+    uint32_t* d_send_ = nullptr;
+    uint64_t* d_tot_ = nullptr;               // [cap_slots + 1]
+    std::vector<DRun> h_runs_;
+    std::vector<uint32_t> free_slots_;
+    std::vector<uint32_t> release_next_;      // slots whose states are still in the frontier
+    std::map<uint64_t, std::shared_ptr<RunRec>> live_;
+    std::deque<std::shared_ptr<RunRec>> pending_;
+    std::shared_ptr<DevGen> F_;               // current frontier
+    std::vector<std::shared_ptr<RunRec>> F_runs_;   // runs with states in F, in F order
+    std::vector<uint32_t> F_b_, F_e_;         // their ranges, parallel to F_runs_
+    uint64_t live_footprint_ = 0;
+
+    void grow_slots(uint32_t need);
+    uint32_t take_slot();
+    void admit(std::shared_ptr<RunRec>& r);
+    void finalize(std::vector<std::shared_ptr<RunRec>>& runs);
+};
+
+void engine_destroy(std::unique_ptr<Engine>& e);
+
+}  // namespace hgx
+
+struct hgx_result {
+    std::shared_ptr<hgx::RunRec> run;
+    uint64_t device_ns = 0, latency_ns = 0;
+};

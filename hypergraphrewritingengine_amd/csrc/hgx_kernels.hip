@@ -45,6 +45,7 @@ struct JoinLane {
     uint32_t matched[16];   // edge index bound at each depth
     uint32_t smask[17];     // bound-variable mask on entry to each depth
     uint32_t cidx[16];      // next candidate at each depth
+    uint32_t chi[16];       // end of the candidate range at each depth
     uint32_t bind[32];
 };
 
@@ -59,10 +60,24 @@ __device__ inline bool bind_edge(const uint32_t* ev, uint32_t ea, const uint8_t*
     return true;
 }
 
-// Depth-first completion of one seed; returns the number of matches. With out != nullptr
-// every match is written at out_base + k (k-th match of this seed).
+// Candidates of depth d: the incidence list of the pivot vertex (a vertex the pattern edge
+// shares with the bound prefix; the list holds every edge containing it, ascending), or every
+// edge when the pattern edge shares nothing. Either way the candidates are tried in ascending
+// edge order, so the matches come out in the same order as an all-edge scan.
+__device__ inline void cand_range(const DRule& R, uint32_t d, const JoinLane& L, const uint32_t* iend,
+                                  uint32_t ne, uint32_t& lo, uint32_t& hi) {
+    const uint32_t pv = R.piv[d];
+    if (pv == 0xFFu) { lo = 0; hi = ne; return; }
+    const uint32_t x = L.bind[R.lv[R.order[d]][pv]];
+    lo = x ? iend[x - 1] : 0u;
+    hi = iend[x];
+}
+
+// Depth-first completion of one seed; returns the number of matches. With q_out != nullptr
+// every match is written at out_base + k (the k-th match of this seed).
 __device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const uint8_t* sa,
-                                     uint32_t ne, uint32_t A, uint32_t seed, JoinLane& L,
+                                     uint32_t ne, uint32_t A, const uint32_t* iend, const uint16_t* ie,
+                                     uint32_t seed, JoinLane& L,
                                      uint16_t* q_out, uint32_t K, uint32_t out_base,
                                      uint32_t* par_out, uint32_t* rule_out, uint32_t* cne_out,
                                      uint32_t parent, uint32_t rule_id, uint32_t child_ne) {
@@ -80,13 +95,15 @@ __device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const u
         return 1;
     }
     L.smask[1] = mask;
-    L.cidx[1] = 0;
+    cand_range(R, 1, L, iend, ne, L.cidx[1], L.chi[1]);
     uint32_t d = 1;
     while (d >= 1) {
         const uint32_t p = R.order[d];
+        const bool indexed = R.piv[d] != 0xFFu;
         bool advanced = false;
-        while (L.cidx[d] < ne) {
-            const uint32_t e = L.cidx[d]++;
+        while (L.cidx[d] < L.chi[d]) {
+            const uint32_t j = L.cidx[d]++;
+            const uint32_t e = indexed ? (uint32_t)ie[j] : j;
             bool taken = false;
             for (uint32_t t = 0; t < d; ++t) taken |= (L.matched[t] == e);
             if (taken) continue;
@@ -103,7 +120,7 @@ __device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const u
                 continue;
             }
             L.smask[d + 1] = m;
-            L.cidx[d + 1] = 0;
+            cand_range(R, d + 1, L, iend, ne, L.cidx[d + 1], L.chi[d + 1]);
             ++d;
             advanced = true;
             break;
@@ -113,31 +130,74 @@ __device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const u
     return found;
 }
 
+uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A) {
+    auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
+    return al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
+           al(max_ne * A * 2) + 64;
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    uint8_t* my = lds + (size_t)w * a.lds_per_wave;
+    uint8_t* p = lds + (size_t)w * a.lds_per_wave;
     const uint32_t A = a.P.A;
-    uint32_t* sv = (uint32_t*)my;                              // [ne * A] vertices
-    uint8_t* sa = my + (size_t)a.max_ne * A * 4;               // [ne] arity
-    uint16_t* seeds = (uint16_t*)(sa + ((a.max_ne + 7u) & ~7u));   // [ne]
+    uint32_t* sv = (uint32_t*)carve(p, a.max_ne * A * 4);      // vertices
+    uint8_t* sa = carve(p, a.max_ne);                           // arity
+    uint16_t* seeds = (uint16_t*)carve(p, a.max_ne * 2);
+    uint32_t* iend = (uint32_t*)carve(p, (a.max_nextv + 1) * 4); // incidence list ends per vertex
+    uint16_t* ie = (uint16_t*)carve(p, a.max_ne * A * 2);       // incidence lists
     JoinLane L;
 
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
     for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
         const uint32_t run = a.P.run[s];
         if (!a.active[run]) { if (!EMIT && l == 0) a.cnt[s] = 0; continue; }
+        const uint64_t budget = EMIT ? 0 : (uint64_t)a.budget[run];
+        if (!EMIT && a.run_acc[run] > budget) { if (l == 0) a.cnt[s] = 0; continue; }
         const uint32_t ne = a.P.ne[s];
         const uint64_t eo = a.P.eoff[s];
+        const uint32_t nvid = a.P.nextv[s];                     // labels are < nvid
         for (uint32_t i = l; i < ne * A; i += WAVE) sv[i] = a.P.ev[eo * A + i];
         for (uint32_t i = l; i < ne; i += WAVE) sa[i] = a.P.ar[eo + i];
+        for (uint32_t v = l; v <= nvid; v += WAVE) iend[v] = 0;
         wave_sync();
+        // Incidence index (vertex -> ascending edges, one entry per (vertex, edge)): counting
+        // sort with LDS atomics, then each list put in edge order.
+        for (uint32_t e = l; e < ne; e += WAVE)
+            for (uint32_t k = 0; k < sa[e]; ++k) {
+                const uint32_t x = sv[e * A + k];
+                bool dup = false;
+                for (uint32_t k2 = 0; k2 < k; ++k2) dup |= (sv[e * A + k2] == x);
+                if (!dup) atomicAdd(&iend[x], 1u);
+            }
+        wave_sync();
+        wave_excl_scan_lds(iend, nvid + 1);                     // iend[x] = start of x's list
+        for (uint32_t e = l; e < ne; e += WAVE)
+            for (uint32_t k = 0; k < sa[e]; ++k) {
+                const uint32_t x = sv[e * A + k];
+                bool dup = false;
+                for (uint32_t k2 = 0; k2 < k; ++k2) dup |= (sv[e * A + k2] == x);
+                if (!dup) ie[atomicAdd(&iend[x], 1u)] = (uint16_t)e;   // -> iend[x] = end
+            }
+        wave_sync();
+        for (uint32_t x = l; x < nvid; x += WAVE) {
+            const uint32_t b = x ? iend[x - 1] : 0u, n = iend[x] - b;
+            for (uint32_t i = 1; i < n; ++i) {
+                const uint16_t key = ie[b + i];
+                uint32_t j = i;
+                while (j > 0 && ie[b + j - 1] > key) { ie[b + j] = ie[b + j - 1]; --j; }
+                ie[b + j] = key;
+            }
+        }
+        wave_sync();
+
         const DRun& RN = a.runs[run];
         uint32_t total = 0;
+        bool stop = false;
         uint32_t out = EMIT ? (uint32_t)a.moff[s] : 0u;
-        for (uint32_t r = RN.rule_begin; r < RN.rule_begin + RN.n_rules; ++r) {
-            const DRule& R = a.rules[r];
+        for (uint32_t r = 0; r < RN.n_rules && !stop; ++r) {
+            const DRule& R = RN.rules[r];
             const uint32_t p0 = R.order[0];
             const uint32_t child_ne = ne - R.n_lhs + R.n_rhs;
             // SCAN: compact the seeds of position order[0] in edge order.
@@ -159,16 +219,24 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                 const uint32_t i = base + l;
                 uint32_t c = 0;
                 if (i < nseed)
-                    c = join_seed(R, sv, sa, ne, A, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
+                    c = join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
                                   nullptr, 0, 0, 0);
                 if (EMIT) {
                     const uint32_t inc = wave_incl_scan_u32(c);
                     if (i < nseed && c)
-                        join_seed(R, sv, sa, ne, A, seeds[i], L, a.M.q, a.M.K, out + inc - c,
+                        join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, a.M.q, a.M.K, out + inc - c,
                                   a.M.parent, a.M.rule, a.M.cne, s, r, child_ne);
                     out += __shfl(inc, WAVE - 1, WAVE);
                 } else {
-                    total += wave_sum_u32(c);
+                    // The run's shared tally: once it passes what the run may still take, the
+                    // generation is over the cap whatever the other states hold, so stop. Every
+                    // state's count equals what it added, so the totals stay over the cap.
+                    const uint32_t round = wave_sum_u32(c);
+                    total += round;
+                    unsigned long long before = 0;
+                    if (l == 0 && round) before = atomicAdd(&a.run_acc[run], (unsigned long long)round);
+                    before = __shfl(before, 0, WAVE);
+                    if (before + round > budget) { stop = true; break; }
                 }
             }
             wave_sync();
@@ -204,8 +272,10 @@ __device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
 __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    uint8_t* my = lds + (size_t)w * a.lds_per_wave;
-    const uint32_t A = a.P.A;
+    const uint32_t wpb_ = blockDim.x / WAVE;
+    uint8_t* my = a.scratch ? a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave
+                            : lds + (size_t)w * a.lds_per_wave;
+    const uint32_t PA = a.P.A, A = a.C.A;                  // parent / child vertex strides
     const WLSizes z = WLSizes::make(a.max_child_ne, a.max_nv, A);
     WLSpace s;
     uint8_t* p = my;
@@ -232,16 +302,17 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
     for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t par = a.M.parent[c];
         const uint32_t rid = a.M.rule[c];
-        const DRule& R = a.rules[rid];
         const uint32_t run = a.P.run[par];
         if (run != acc_run) { flush(); acc_run = run; }
         DRun* RN = &a.runs[run];
+        const DRule& R = RN->rules[rid];
         const uint32_t nl = R.n_lhs, nr = R.n_rhs;
         const uint32_t pne = a.P.ne[par];
         const uint64_t peo = a.P.eoff[par];
         const uint64_t ceo = a.C.eoff[c];
         const uint32_t cne = pne - nl + nr;
-        const uint32_t step = a.C.step;                      // generation of the child
+        const uint32_t pdep = a.P.depth[par];
+        const uint32_t step = pdep + 1;                       // generation of the child
         const uint16_t* q = a.M.q + (size_t)c * a.M.K;
 
         // Parent edges minus the consumed ones, order kept (SparseBitset::derive).
@@ -255,7 +326,7 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
                 const uint32_t pos = kept + lanes_below(bal);
                 const uint8_t ar = a.P.ar[peo + i];
                 for (uint32_t k = 0; k < A; ++k) {
-                    const uint32_t v = a.P.ev[(peo + i) * A + k];
+                    const uint32_t v = k < PA ? a.P.ev[(peo + i) * PA + k] : 0u;
                     a.C.ev[(ceo + pos) * A + k] = v;
                     gv[pos * A + k] = v;
                 }
@@ -277,7 +348,7 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
                     const uint32_t var = R.rv[l][k];
                     if (R.lhs_mask & (1u << var)) {
                         const uint32_t se = q[R.var_src_edge[var]];
-                        v = a.P.ev[(peo + se) * A + R.var_src_pos[var]];
+                        v = a.P.ev[(peo + se) * PA + R.var_src_pos[var]];
                     } else {
                         v = fresh_base + __popc(R.new_mask & ((1u << var) - 1u));
                     }
@@ -315,8 +386,9 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
             a.C.nextv[c] = nextv;
             a.C.sid[c] = child_sid;
             a.C.hash[c] = h;
+            a.C.depth[c] = (uint8_t)step;
             a.E.in_state[c] = a.P.sid[par];
-            a.E.rule[c] = rid - RN->rule_begin;
+            a.E.rule[c] = rid;
             acc[CTR_EVENTS] += 1;
             if (map_insert(RN, h, child_sid)) { acc[CTR_CANON] += 1; acc[CTR_FINGERPRINT] += mix64(h); }
             // Causal: one triple per consumed edge with a producer; the producers are events of
@@ -330,7 +402,7 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
                 psq[t] = a.P.ps[peo + q[t]];
                 if (psq[t]) { ++triples; prod |= 1ull << psq[t]; }
             }
-            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.step;   // steps 1..P.step
+            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.lstride;   // generations 1..pdep
             uint64_t tr = prod;
             if (RN->flags & 0x1u) {
                 for (uint64_t m = prod; m; m &= m - 1) {
@@ -363,8 +435,8 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         // Lineage table of the child: the parent's, then this event's ancestor mask.
         anc_c = __shfl(anc_c, 0, WAVE);
         {
-            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.step;
-            uint64_t* clan = a.C.lanc + (size_t)c * step;
+            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.lstride;
+            uint64_t* clan = a.C.lanc + (size_t)c * a.C.lstride;
             for (uint32_t t = l; t < step; t += WAVE) clan[t] = (t + 1 == step) ? anc_c : plan[t];
         }
         wave_sync();
@@ -374,14 +446,35 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
 
 // ------------------------------------------------------------------------------------------
 // Branchial pairs: two events of the SAME input state whose consumed edge sets intersect,
-// one pair each (causal_graph.cpp:324-351). One wave per parent; consumed sets as 64-bit
-// position masks when the parent has at most 64 edges, position lists otherwise.
+// one pair each (causal_graph.cpp:324-351). One wave per parent.
+//
+// Parents of at most 64 edges: the parent's matches are taken in chunks of BR_CHUNK; for each
+// chunk an LDS bitset per edge position marks the chunk's matches that consume it. Match i
+// then meets every later match j of the chunk through the OR of the bitsets of its own edges:
+// one popcount per 64 candidates instead of one test per pair. Larger parents compare
+// consumed-position lists pairwise.
 // ------------------------------------------------------------------------------------------
+
+constexpr uint32_t BR_WORDS = 16;                  // 64-bit words per edge bitset
+constexpr uint32_t BR_CHUNK = BR_WORDS * 64;        // matches per chunk
+constexpr uint32_t BR_MASKS = 1024;                 // consumed masks cached in LDS
+
+uint32_t branchial_lds_per_wave() { return 64 * BR_WORDS * 8 + BR_MASKS * 8 + 64; }
+
+__device__ inline uint64_t consumed_mask(const BranchialArgs& a, const DRun& RN, uint32_t m) {
+    const uint32_t nl = RN.rules[a.M.rule[m]].n_lhs;
+    const uint16_t* q = a.M.q + (size_t)m * a.M.K;
+    uint64_t msk = 0;
+    for (uint32_t t = 0; t < nl; ++t) msk |= 1ull << q[t];
+    return msk;
+}
 
 __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    uint64_t* masks = (uint64_t*)(lds + (size_t)w * a.lds_per_wave);
+    uint8_t* my = lds + (size_t)w * a.lds_per_wave;
+    uint64_t* bits = (uint64_t*)my;                      // [64][BR_WORDS]
+    uint64_t* mcache = bits + 64 * BR_WORDS;             // [BR_MASKS]
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
     for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
         const uint32_t cnt = a.cnt[s];
@@ -389,52 +482,101 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
         const uint32_t run = a.P.run[s];
         const uint32_t m0 = (uint32_t)a.moff[s];
         const uint32_t ne = a.P.ne[s];
-        const bool small = ne <= 64 && cnt <= a.mask_cap;
-        if (small) {
-            for (uint32_t i = l; i < cnt; i += WAVE) {
-                const uint32_t nl = a.rules[a.M.rule[m0 + i]].n_lhs;
-                uint64_t m = 0;
-                for (uint32_t t = 0; t < nl; ++t) m |= 1ull << a.M.q[(size_t)(m0 + i) * a.M.K + t];
-                masks[i] = m;
-            }
-            wave_sync();
-        }
-        auto overlap = [&](uint32_t i, uint32_t j) -> bool {
-            if (small) return (masks[i] & masks[j]) != 0;
-            const uint16_t* qi = a.M.q + (size_t)(m0 + i) * a.M.K;
-            const uint16_t* qj = a.M.q + (size_t)(m0 + j) * a.M.K;
-            const uint32_t ni = a.rules[a.M.rule[m0 + i]].n_lhs, nj = a.rules[a.M.rule[m0 + j]].n_lhs;
-            for (uint32_t x = 0; x < ni; ++x)
-                for (uint32_t y = 0; y < nj; ++y)
-                    if (qi[x] == qj[y]) return true;
-            return false;
-        };
         const DRun& RN = a.runs[run];
-        if (a.emit == 0) {
-            uint32_t c = 0;
-            for (uint32_t i = l; i < cnt; i += WAVE)
-                for (uint32_t j = i + 1; j < cnt; ++j) c += overlap(i, j) ? 1u : 0u;
-            const uint32_t tot = wave_sum_u32(c);
-            if (l == 0) {
-                if (tot) atomicAdd((unsigned long long*)&a.ctr[(size_t)run * CTR_N + CTR_BRANCHIAL],
-                                   (unsigned long long)tot);
-                if (a.pcnt) a.pcnt[s] = tot;
+        const uint32_t ev0 = RN.ev_base + (m0 - RN.child_begin);
+        uint64_t out = a.emit ? a.poff[s] : 0;
+        uint32_t total = 0;
+        if (ne <= 64) {
+            const uint32_t ncache = cnt < BR_MASKS ? cnt : BR_MASKS;
+            for (uint32_t i = l; i < ncache; i += WAVE) mcache[i] = consumed_mask(a, RN, m0 + i);
+            wave_sync();
+            auto mask_of = [&](uint32_t i) -> uint64_t {
+                return i < BR_MASKS ? mcache[i] : consumed_mask(a, RN, m0 + i);
+            };
+            for (uint32_t cb = 0; cb < cnt; cb += BR_CHUNK) {
+                const uint32_t ce = min(cnt, cb + BR_CHUNK);
+                for (uint32_t x = l; x < 64 * BR_WORDS; x += WAVE) bits[x] = 0;
+                wave_sync();
+                for (uint32_t j = cb + l; j < ce; j += WAVE) {
+                    const uint32_t r = j - cb;
+                    for (uint64_t mk = mask_of(j); mk; mk &= mk - 1)
+                        atomicOr((unsigned long long*)&bits[__builtin_ctzll(mk) * BR_WORDS + (r >> 6)],
+                                 1ull << (r & 63));
+                }
+                wave_sync();
+                const uint32_t words = (ce - cb + 63) >> 6;
+                for (uint32_t base = 0; base < ce; base += WAVE) {
+                    const uint32_t i = base + l;
+                    uint32_t c = 0;
+                    uint64_t mi = 0;
+                    if (i < ce) {
+                        mi = mask_of(i);
+                        for (uint32_t wd = 0; wd < words; ++wd) {
+                            const uint32_t jb = cb + wd * 64;
+                            if (jb + 63 <= i) continue;
+                            uint64_t word = 0;
+                            for (uint64_t mk = mi; mk; mk &= mk - 1) word |= bits[__builtin_ctzll(mk) * BR_WORDS + wd];
+                            if (jb <= i) word &= ~((2ull << (i - jb)) - 1ull);
+                            c += __popcll(word);
+                        }
+                    }
+                    if (a.emit) {
+                        const uint32_t inc = wave_incl_scan_u32(c);
+                        uint64_t o = out + inc - c;
+                        if (c) {
+                            for (uint32_t wd = 0; wd < words; ++wd) {
+                                const uint32_t jb = cb + wd * 64;
+                                if (jb + 63 <= i) continue;
+                                uint64_t word = 0;
+                                for (uint64_t mk = mi; mk; mk &= mk - 1) word |= bits[__builtin_ctzll(mk) * BR_WORDS + wd];
+                                if (jb <= i) word &= ~((2ull << (i - jb)) - 1ull);
+                                for (; word; word &= word - 1) {
+                                    a.pairs[2 * o] = ev0 + i;
+                                    a.pairs[2 * o + 1] = ev0 + jb + __builtin_ctzll(word);
+                                    ++o;
+                                }
+                            }
+                        }
+                        out += __shfl(inc, WAVE - 1, WAVE);
+                    } else {
+                        total += c;
+                    }
+                }
+                wave_sync();
             }
         } else {
-            // Materialise: rows in lane order, each row's pairs in column order.
-            uint64_t out = a.poff[s];
-            const uint32_t ev0 = RN.ev_base + (m0 - RN.child_begin);
+            auto overlap = [&](uint32_t i, uint32_t j) -> bool {
+                const uint16_t* qi = a.M.q + (size_t)(m0 + i) * a.M.K;
+                const uint16_t* qj = a.M.q + (size_t)(m0 + j) * a.M.K;
+                const uint32_t ni = RN.rules[a.M.rule[m0 + i]].n_lhs, nj = RN.rules[a.M.rule[m0 + j]].n_lhs;
+                for (uint32_t x = 0; x < ni; ++x)
+                    for (uint32_t y = 0; y < nj; ++y)
+                        if (qi[x] == qj[y]) return true;
+                return false;
+            };
             for (uint32_t base = 0; base < cnt; base += WAVE) {
                 const uint32_t i = base + l;
                 uint32_t c = 0;
                 if (i < cnt)
                     for (uint32_t j = i + 1; j < cnt; ++j) c += overlap(i, j) ? 1u : 0u;
-                const uint32_t inc = wave_incl_scan_u32(c);
-                uint64_t o = out + inc - c;
-                if (i < cnt && c)
-                    for (uint32_t j = i + 1; j < cnt; ++j)
-                        if (overlap(i, j)) { a.pairs[2 * o] = ev0 + i; a.pairs[2 * o + 1] = ev0 + j; ++o; }
-                out += __shfl(inc, WAVE - 1, WAVE);
+                if (a.emit) {
+                    const uint32_t inc = wave_incl_scan_u32(c);
+                    uint64_t o = out + inc - c;
+                    if (c)
+                        for (uint32_t j = i + 1; j < cnt; ++j)
+                            if (overlap(i, j)) { a.pairs[2 * o] = ev0 + i; a.pairs[2 * o + 1] = ev0 + j; ++o; }
+                    out += __shfl(inc, WAVE - 1, WAVE);
+                } else {
+                    total += c;
+                }
+            }
+        }
+        if (!a.emit) {
+            const uint32_t tot = wave_sum_u32(total);
+            if (l == 0) {
+                if (tot) atomicAdd((unsigned long long*)&a.ctr[(size_t)run * CTR_N + CTR_BRANCHIAL],
+                                   (unsigned long long)tot);
+                if (a.pcnt) a.pcnt[s] = tot;
             }
         }
         wave_sync();
@@ -449,7 +591,9 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
 __global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    uint8_t* p = lds + (size_t)w * a.lds_per_wave;
+    const uint32_t wpb_ = blockDim.x / WAVE;
+    uint8_t* p = a.scratch ? a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave
+                           : lds + (size_t)w * a.lds_per_wave;
     const uint32_t A = a.A;
     const WLSizes z = WLSizes::make(a.max_ne, a.max_nv, A);
     WLSpace s;
@@ -514,6 +658,9 @@ __global__ void k_rehash(const uint64_t* old_keys, const uint32_t* old_vals, uin
 // launchers
 // ------------------------------------------------------------------------------------------
 
+static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap);
+uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap) { return grid_for(items, wpb, grid_cap); }
+
 static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
     uint32_t g = (items + per_block - 1) / per_block;
     if (g > cap) g = cap;
@@ -536,7 +683,7 @@ uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t ma
 
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
     uint32_t grid = grid_for(a.M.n, wpb, grid_cap);
-    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
     hipLaunchKernelGGL(k_expand, dim3(grid), dim3(64 * wpb), lds, st, a);
     return hipGetLastError();
 }
@@ -555,7 +702,7 @@ uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A) {
 
 hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
     const uint32_t grid = grid_for(a.n, wpb, grid_cap);
-    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
     hipLaunchKernelGGL(k_wl_states, dim3(grid), dim3(64 * wpb), lds, st, a);
     return hipGetLastError();
 }

@@ -10,7 +10,6 @@ namespace hgx {
 
 struct MatchArgs {
     GenView P;
-    const DRule* rules;
     const DRun* runs;
     const uint8_t* active;     // [runs] does this run advance this generation
     uint32_t* cnt;             // count pass: [P.n] matches per state
@@ -19,6 +18,9 @@ struct MatchArgs {
     uint32_t max_ne;           // largest state in P (LDS staging)
     uint32_t lds_per_wave;
     unsigned long long* parent_edges;   // count pass: += edges of each state with a match
+    const uint32_t* budget;    // count pass: [runs] matches a run may still take
+    unsigned long long* run_acc;   // count pass: [runs] running tally (zeroed per generation)
+    uint32_t max_nextv;        // bound on vertex labels in P (incidence index)
 };
 
 struct ExpandArgs {
@@ -26,19 +28,19 @@ struct ExpandArgs {
     GenView C;                 // children (eoff filled; the rest written here)
     MatchView M;
     EventView E;
-    const DRule* rules;
     DRun* runs;
     uint64_t* ctr;             // [runs * CTR_N]
     uint32_t max_child_ne;
     uint32_t max_nv;           // bound on distinct vertices of a child
     uint32_t max_nextv;        // bound on vertex labels of a child
     uint32_t lds_per_wave;
+    uint8_t* scratch;          // when set: per-wave workspace in global memory instead of LDS
+                               //   (states too large for LDS), lds_per_wave bytes per wave
 };
 
 struct BranchialArgs {
     GenView P;
     MatchView M;
-    const DRule* rules;
     const DRun* runs;
     const uint32_t* cnt;       // [P.n] matches per parent
     const uint64_t* moff;      // [P.n + 1]
@@ -47,8 +49,7 @@ struct BranchialArgs {
     uint32_t* pcnt;            // [P.n] pairs per parent (may be null in count mode)
     const uint64_t* poff;      // [P.n + 1] exclusive offsets (emit)
     uint32_t* pairs;           // [2 * total] event ids within the run
-    uint32_t mask_cap;         // masks that fit the per-wave LDS
-    uint32_t lds_per_wave;
+    uint32_t lds_per_wave;     // branchial_lds_per_wave()
 };
 
 struct WLStatesArgs {
@@ -65,11 +66,16 @@ struct WLStatesArgs {
     uint64_t* ctr;
     uint32_t max_ne, max_nv;
     uint32_t lds_per_wave;
+    uint8_t* scratch;          // as ExpandArgs::scratch
 };
 
+uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A);
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A);
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+// grid used by the launchers for `items` work items (for sizing global scratch)
+uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap);
+uint32_t branchial_lds_per_wave();
 hipError_t launch_branchial(const BranchialArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
 uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A);
 hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);

@@ -1,23 +1,40 @@
-// hgx_runtime.hpp -- host-side device-memory management shared by the batch and served paths.
+// hgx_runtime.hpp -- host-side pieces shared by the runtime files: errors, device memory
+// (chunk pool, bump arenas, slab allocator), request parsing, scan helper, the context.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
-struct hgx_ctx;
+#include "../../include/hgx.h"
+#include "hgx_device.hpp"
 
 namespace hgx {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            throw ::hgx::Error{HGX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
 
 struct Chunk {
     void* p;
     size_t size;
 };
 
-// Large device chunks, allocated on first use and then recycled: the steady state of a serving
-// loop allocates nothing from the HIP runtime. Bounded by the context's HBM budget.
+// Device memory chunks, allocated on first use and then recycled (best fit), bounded by the
+// context's HBM budget. A serving loop in steady state allocates nothing from the runtime.
 class ChunkPool {
 public:
     ChunkPool(uint64_t budget, size_t chunk_bytes) : budget_(budget), chunk_bytes_(chunk_bytes) {}
@@ -26,19 +43,20 @@ public:
     void put(const Chunk& c);
     uint64_t budget() const { return budget_; }
     uint64_t allocated() const { return total_; }
+    uint64_t in_use() const { return in_use_; }
 
 private:
     std::mutex mu_;
     std::vector<Chunk> free_;
     uint64_t budget_;
     size_t chunk_bytes_;
-    uint64_t total_ = 0;
+    uint64_t total_ = 0, in_use_ = 0;
 };
 
 // Bump allocation over pool chunks; everything returns to the pool at once.
 class Arena {
 public:
-    explicit Arena(ChunkPool* pool) : pool_(pool) {}
+    explicit Arena(ChunkPool* pool, size_t min_chunk = 0) : pool_(pool), min_chunk_(min_chunk) {}
     ~Arena();
     Arena(const Arena&) = delete;
     Arena& operator=(const Arena&) = delete;
@@ -48,12 +66,120 @@ public:
 
 private:
     ChunkPool* pool_;
+    size_t min_chunk_;
     std::vector<Chunk> chunks_;
     size_t used_ = 0;
     uint64_t bytes_ = 0;
 };
 
-struct ServeState;
-void serve_destroy(hgx_ctx* ctx);
+// Power-of-two size classes carved from 64 MiB pool chunks: per-request rules and state maps,
+// which come and go in any order on the served path.
+class SlabAlloc {
+public:
+    explicit SlabAlloc(ChunkPool* pool) : pool_(pool) {}
+    ~SlabAlloc();
+    void* alloc(size_t bytes);
+    void free(void* p);
+
+private:
+    ChunkPool* pool_;
+    std::vector<Chunk> slabs_;
+    std::map<uint32_t, std::vector<void*>> free_;     // class -> free blocks
+    std::map<void*, uint32_t> live_;                  // block -> class
+    std::map<void*, Chunk> big_;                      // blocks above the largest class
+    size_t slab_used_ = 0;
+};
+
+template <class T>
+T* dalloc(Arena& a, size_t n) { return (T*)a.alloc(sizeof(T) * (n ? n : 1)); }
+
+template <class T>
+void h2d(T* dst, const T* src, size_t n, hipStream_t st) {
+    if (n) HIP_TRY(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
+}
+
+template <class T>
+void d2h(T* dst, const T* src, size_t n, hipStream_t st) {
+    if (n) HIP_TRY(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+}
+
+// Exclusive prefix sum of n uint32 values (in[n] must read 0) into n+1 uint64 offsets.
+void scan_u32_to_u64(Arena& ar, const uint32_t* in, uint64_t* out, uint32_t n, hipStream_t st);
+
+// One parsed request.
+struct RunHost {
+    std::vector<DRule> rules;
+    std::vector<std::vector<std::vector<uint32_t>>> roots;   // renumbered per root
+    std::vector<uint32_t> root_nv;
+    uint32_t steps = 0, cap = 0, flags = 0;
+    uint32_t max_arity = 1, max_lhs = 1, max_gain = 0, max_new = 0;
+    uint64_t root_edges = 0;
+};
+RunHost parse_run(const hgx_run_desc* d);
+uint64_t footprint_bytes(const RunHost& h);
+
+uint32_t waves_per_block(uint32_t lds_per_wave, uint32_t max_lds);
+// Workspace placement for a wave-per-state kernel: LDS when it fits, else a global-memory slice
+// per wave from `ar` (4 waves per block). Returns waves per block; *scratch set in the second case.
+uint32_t place_workspace(Arena& ar, uint32_t per_wave, uint32_t items, uint32_t max_lds, uint32_t grid_cap,
+                         uint8_t** scratch);
+
+struct ProfPending {
+    std::string name;
+    hipEvent_t a, b;
+    uint64_t bytes, items;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<ProfPending> pend;
+    std::map<std::string, hgx_kernel_stat> acc;
+};
+
+class Engine;
+
+}  // namespace hgx
+
+struct hgx_ctx {
+    hgx::Profiler prof;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::shared_ptr<hgx::ChunkPool> pool;
+    std::string err;
+    std::string info;
+    uint32_t num_cu = 256;
+    uint32_t max_lds = 65536;
+    std::recursive_mutex mu;
+    std::unique_ptr<hgx::Engine> serve;          // the served path's engine
+    uint64_t next_request_id = 1;
+};
+
+namespace hgx {
+
+// Launch with optional event timing; `bytes` are the launch's algorithmic HBM bytes.
+template <class F>
+void timed(hgx_ctx* ctx, hipStream_t st, const char* name, uint64_t bytes, uint64_t items, F&& launch) {
+    if (!ctx->prof.on) { HIP_TRY(launch()); return; }
+    ProfPending p{name, nullptr, nullptr, bytes, items};
+    HIP_TRY(hipEventCreate(&p.a));
+    HIP_TRY(hipEventCreate(&p.b));
+    HIP_TRY(hipEventRecord(p.a, st));
+    HIP_TRY(launch());
+    HIP_TRY(hipEventRecord(p.b, st));
+    ctx->prof.pend.push_back(p);
+}
+void prof_resolve(hgx_ctx* ctx);
+
+template <class F>
+int guarded(hgx_ctx* ctx, F&& f) {
+    try {
+        return f();
+    } catch (const Error& e) {
+        if (ctx) ctx->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return HGX_ERR_DEVICE;
+    }
+}
 
 }  // namespace hgx

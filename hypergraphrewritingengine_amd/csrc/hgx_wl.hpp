@@ -47,9 +47,10 @@ struct WLSizes {
         s.OCC = s.NE * A;
         s.NV = nv ? nv : 1;
         if (s.NV > s.OCC) s.NV = s.OCC;
-        s.NN = s.OCC * (A > 1 ? A : 2);
+        // per-vertex segments: sum over vertices of max(nn, min(deg, cap)) <= OCC * max(A-1, 1)
+        s.NN = s.OCC * (A > 2 ? A - 1 : 1);
         uint32_t ht = 64;
-        while (ht < 2 * s.NV) ht <<= 1;
+        while (ht < s.NV + s.NV / 4 + 1) ht <<= 1;     // load factor <= 0.8
         s.HT = ht;
         return s;
     }

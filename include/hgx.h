@@ -168,12 +168,17 @@ int hgx_serve_step(hgx_ctx* ctx, uint32_t* live);
 /* HGX_OK and *out set when done; HGX_ERR_NOT_READY while running. */
 int hgx_request_poll(hgx_ctx* ctx, uint64_t request_id, hgx_result** out);
 int hgx_request_wait(hgx_ctx* ctx, uint64_t request_id, hgx_result** out);
+/* Results of requests that finished and were not yet taken, up to cap of them; each is
+ * removed from the finished set (ids[i] names results[i]). */
+int hgx_serve_drain(hgx_ctx* ctx, uint64_t* ids, hgx_result** results, uint32_t cap, uint32_t* n);
 /* Footprint estimate (bytes of HBM) the admission control charges for a request. */
 uint64_t hgx_request_footprint(const hgx_run_desc* run);
 
 /* ---- results ------------------------------------------------------------------------- */
 
 int hgx_result_counts(const hgx_result* r, hgx_counts* out);
+/* The same for n results at once (one call for a whole batch). */
+int hgx_result_counts_many(const hgx_result* const* rs, uint32_t n, hgx_counts* out);
 /* Per raw state: WL hash, step, and the event that created it (0xFFFFFFFF for roots).
  * Any pointer may be NULL. cap is the capacity in elements; returns HGX_ERR_INVALID if
  * cap < raw_states. */

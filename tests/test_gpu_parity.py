@@ -151,3 +151,30 @@ def test_edge_cases(ctx):
     assert res.counts["events"] == 0
     with pytest.raises(hgx.HGXError):
         ctx.evolve(Request([r], [[1, 2]], 99))
+
+
+def test_served_results_equal_alone_and_in_the_mix(ctx):
+    """Continuous batching: requests arriving while others are on the device are admitted into
+    the next generation; every result equals the same request evolved alone (and the oracle)."""
+    reqs = [W.served_run(i, cap=1024, flags=REL | TR) for i in range(24)]
+    reqs[3] = W.config1(steps=4, flags=REL | TR)
+    ids = {}
+    got = {}
+    # staggered arrivals: a few requests per generation step
+    pending = list(range(len(reqs)))
+    while pending or ids:
+        for _ in range(3):
+            if pending:
+                i = pending.pop(0)
+                ids[ctx.submit(reqs[i])] = i
+        ctx.serve_step()
+        for rid, res in ctx.drain().items():
+            got[ids.pop(rid)] = res
+    assert sorted(got) == list(range(len(reqs)))
+    for i, req in enumerate(reqs):
+        assert_exact(got[i], po.evolve(req))
+        alone = ctx.evolve(req)
+        assert alone.counts == got[i].counts
+    # wait() drives the loop by itself
+    rid = ctx.submit(reqs[5])
+    assert ctx.wait(rid).counts == got[5].counts
