@@ -163,6 +163,17 @@ class Result:
             pass
 
 
+class PreparedBatch:
+    """hgx_run_desc records of a list of requests, with the buffers they point into."""
+
+    def __init__(self, requests):
+        self._keep: list = []
+        self.n = len(requests)
+        self.descs = (abi.RunDesc * max(self.n, 1))()
+        for i, r in enumerate(requests):
+            self.descs[i] = abi.make_run_desc(r, self._keep)
+
+
 class Context:
     """An hgx_ctx on one device: the HBM budget, the stream, the served-request queue."""
 
@@ -187,13 +198,14 @@ class Context:
         self._lib.hgx_device_info(self._h, buf, 512)
         return buf.value.decode()
 
+    def prepare(self, requests) -> "PreparedBatch":
+        """Encode requests into hgx_run_desc records once, for repeated evolve_batch calls."""
+        return PreparedBatch(requests)
+
     def evolve_batch(self, requests):
-        """Evolve requests together on the device; returns one Result per request."""
-        keep: list = []
-        n = len(requests)
-        descs = (abi.RunDesc * max(n, 1))()
-        for i, r in enumerate(requests):
-            descs[i] = abi.make_run_desc(r, keep)
+        """Evolve requests (a list or a PreparedBatch) together on the device; one Result each."""
+        pb = requests if isinstance(requests, PreparedBatch) else PreparedBatch(requests)
+        n, descs = pb.n, pb.descs
         outs = (C.c_void_p * max(n, 1))()
         rc = self._lib.hgx_evolve_batch(self._h, descs, n, outs)
         if rc != abi.HGX_OK:
@@ -228,6 +240,27 @@ class Context:
         if rc != abi.HGX_OK:
             raise self._err(rc)
         return out
+
+    def match(self, rules, rule_index: int, edges, want_edges: bool = False, cap: int = 0):
+        """All matches of one rule in one (large) state on the device: count, and up to `cap`
+        matches as edge indices in pattern order. Returns (count, matches or None, device ms)."""
+        prog = np.frombuffer(encode_rules(rules), np.uint8).copy()
+        offs, verts = encode_state(edges)
+        if verts.size == 0:
+            verts = np.zeros(1, np.uint32)
+        k = len(rules[rule_index].lhs)
+        out = np.zeros(max(cap, 1) * k, np.uint32) if want_edges else None
+        cnt = C.c_uint64()
+        ms = C.c_float()
+        rc = self._lib.hgx_match_count(self._h, prog.ctypes.data, len(prog), len(rules), rule_index,
+                                       offs.ctypes.data, verts.ctypes.data, len(edges), C.byref(cnt),
+                                       out.ctypes.data if want_edges else None,
+                                       cap if want_edges else 0, C.byref(ms))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        n = cnt.value
+        mat = out[: min(n, cap) * k].reshape(-1, k) if want_edges else None
+        return n, mat, ms.value
 
     # ---- served path: continuous batching ----
 

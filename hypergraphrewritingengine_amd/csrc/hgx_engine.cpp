@@ -822,8 +822,3 @@ extern "C" int hgx_result_state_edges(const hgx_result* r, uint64_t* state_offse
     }
 }
 
-extern "C" int hgx_match_count(hgx_ctx* ctx, const uint8_t*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                               const uint32_t*, uint32_t, uint64_t*, uint32_t*, uint64_t, float*) {
-    if (ctx) ctx->err = "large-state join not built in this library";
-    return HGX_ERR_UNSUPPORTED;
-}

@@ -33,7 +33,8 @@ struct WLSpace {
     uint32_t* comp;      // [NV]
     uint64_t* cur;       // [NV]
     uint64_t* nxt;       // [NV]
-    uint64_t* nbr;       // [NN] per-vertex neighbour segments
+    uint64_t* nbr;       // [NN] per-vertex neighbour segments (values being sorted)
+    uint32_t* nb;        // [NN] per-vertex neighbour lists: (local id << 4) | position
     uint64_t* hset;      // [HT]
     uint32_t ht_mask;
 };
@@ -58,7 +59,7 @@ struct WLSizes {
     __host__ __device__ uint32_t bytes(uint32_t A) const {
         auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
         return al(NE) + al(NE * A * 2) + al((NV + 1) * 4) + al(OCC * 4) + al((NV + 1) * 4) +
-               al(NV * 4) + 8 * (NV + NV + NN + HT);
+               al(NV * 4) + al(NN * 4) + 8 * (NV + NV + NN + HT);
     }
 };
 
@@ -80,6 +81,7 @@ __device__ inline void wl_bind(WLSpace& s, uint8_t*& p, const WLSizes& z, uint32
     s.nxt = (uint64_t*)carve(p, z.NV * 8);
     s.nbr = (uint64_t*)carve(p, z.NN * 8);
     s.hset = (uint64_t*)carve(p, z.HT * 8);
+    s.nb = (uint32_t*)carve(p, z.NN * 4);
     s.ht_mask = z.HT - 1;
 }
 
@@ -97,6 +99,22 @@ __device__ inline uint32_t wave_excl_scan_lds(uint32_t* a, uint32_t n) {
     wave_sync();
     return carry;
 }
+
+// Batcher's odd-even merge network for 8 keys (19 comparators), in registers. Unused slots
+// hold ~0 and sort last, so the first n entries are the n keys in ascending order -- the order
+// isort_u64 (core.hpp:54-61) produces.
+#define HGX_CE(a, b) { const uint64_t lo_ = x[a] < x[b] ? x[a] : x[b]; \
+                       const uint64_t hi_ = x[a] < x[b] ? x[b] : x[a]; x[a] = lo_; x[b] = hi_; }
+__device__ inline void sort8(uint64_t (&x)[8]) {
+    HGX_CE(0, 1) HGX_CE(2, 3) HGX_CE(4, 5) HGX_CE(6, 7)
+    HGX_CE(0, 2) HGX_CE(1, 3) HGX_CE(4, 6) HGX_CE(5, 7)
+    HGX_CE(1, 2) HGX_CE(5, 6) HGX_CE(0, 4) HGX_CE(3, 7)
+    HGX_CE(1, 5) HGX_CE(2, 6)
+    HGX_CE(1, 4) HGX_CE(3, 6)
+    HGX_CE(2, 4) HGX_CE(3, 5)
+    HGX_CE(3, 4)
+}
+#undef HGX_CE
 
 __device__ inline void isort_lds(uint64_t* a, uint32_t n) {   // core.hpp:54-61
     for (uint32_t i = 1; i < n; ++i) {
@@ -180,43 +198,78 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
     wave_sync();
     wave_excl_scan_lds(s.cursor, nV + 1);   // cursor[v] = segment offset
 
-    // ---- initial colours (wl_core.hpp:93-107) ----
+    // ---- initial colours (wl_core.hpp:93-107), and each vertex's neighbour list, fixed for
+    //      every refinement round: (neighbour, position) for each occurrence and other position
+    //      (nbr_cap never truncates it: nbr_cap is the largest such count) ----
     for (uint32_t v = l; v < nV; v += WAVE) {
         const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
-        uint64_t* seg = s.nbr + s.cursor[v];
+        const uint32_t so = s.cursor[v];
+        uint64_t* seg = s.nbr + so;
         uint64_t h = fnv(FNV_OFFSET, deg);
-        uint32_t cnt = 0;
-        for (uint32_t j = 0; j < deg && cnt < nbr_cap; ++j) {
-            const uint32_t ep = s.occ_ep[b + j];
-            seg[cnt++] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+        const uint32_t cnt = deg < nbr_cap ? deg : nbr_cap;
+        if (cnt <= 8) {
+            uint64_t x[8];
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) {
+                uint64_t key = ~0ull;
+                if (i < cnt) {
+                    const uint32_t ep = s.occ_ep[b + i];
+                    key = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+                }
+                x[i] = key;
+            }
+            sort8(x);
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i)
+                if (i < cnt) { h = fnv(h, x[i] >> 8); h = fnv(h, x[i] & 0xFFu); }
+        } else {
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t ep = s.occ_ep[b + j];
+                seg[j] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+            }
+            isort_lds(seg, cnt);
+            for (uint32_t i = 0; i < cnt; ++i) { h = fnv(h, seg[i] >> 8); h = fnv(h, seg[i] & 0xFFu); }
         }
-        isort_lds(seg, cnt);
-        for (uint32_t i = 0; i < cnt; ++i) { h = fnv(h, seg[i] >> 8); h = fnv(h, seg[i] & 0xFFu); }
         s.cur[v] = h;
+        uint32_t nn = 0;
+        for (uint32_t j = 0; j < deg; ++j) {
+            const uint32_t ep = s.occ_ep[b + j];
+            const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
+            const uint32_t a = s.e_ar[e];
+            for (uint32_t k = 0; k < a; ++k)
+                if (k != mypos) s.nb[so + nn++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
+        }
+        s.comp[v] = nn;                       // comp is free until the component phase
     }
     wave_sync();
     uint64_t* cur = s.cur;
     uint64_t* nxt = s.nxt;
     uint32_t prev_d = wl_ndistinct(s, cur);
 
-    // ---- refinement (wl_core.hpp:119-141) ----
+    // ---- refinement (wl_core.hpp:119-141): sorted multiset of fnv(colour, position) over the
+    //      neighbour list, folded in order -- in registers for up to 8 neighbours ----
     for (uint32_t it = 0; it < WL_MAX_ITERS; ++it) {
         for (uint32_t v = l; v < nV; v += WAVE) {
-            const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
-            uint64_t* seg = s.nbr + s.cursor[v];
-            uint32_t nn = 0;
-            for (uint32_t j = 0; j < deg; ++j) {
-                const uint32_t ep = s.occ_ep[b + j];
-                const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
-                const uint32_t a = s.e_ar[e];
-                for (uint32_t k = 0; k < a; ++k) {
-                    if (k == mypos) continue;
-                    if (nn < nbr_cap) seg[nn++] = fnv(cur[s.e_lv[e * A + k]], k);
-                }
-            }
-            isort_lds(seg, nn);
+            const uint32_t so = s.cursor[v], nn = s.comp[v];
             uint64_t h = cur[v];
-            for (uint32_t i = 0; i < nn; ++i) h = fnv(h, seg[i]);
+            if (nn <= 8) {
+                uint64_t x[8];
+#pragma unroll
+                for (uint32_t i = 0; i < 8; ++i) {
+                    uint64_t val = ~0ull;
+                    if (i < nn) { const uint32_t t = s.nb[so + i]; val = fnv(cur[t >> 4], t & 15u); }
+                    x[i] = val;
+                }
+                sort8(x);
+#pragma unroll
+                for (uint32_t i = 0; i < 8; ++i)
+                    if (i < nn) h = fnv(h, x[i]);
+            } else {
+                uint64_t* seg = s.nbr + so;
+                for (uint32_t i = 0; i < nn; ++i) { const uint32_t t = s.nb[so + i]; seg[i] = fnv(cur[t >> 4], t & 15u); }
+                isort_lds(seg, nn);
+                for (uint32_t i = 0; i < nn; ++i) h = fnv(h, seg[i]);
+            }
             nxt[v] = h;
         }
         wave_sync();
