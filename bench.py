@@ -123,8 +123,8 @@ def main():
     canon = sum(c["canonical_states"] for c in counts)
     raw = sum(c["raw_states"] for c in counts)
     value = canon / (ms_per_step / 1000.0)
-    print(f"[bench] batched: {ms_per_step:.2f} ms/step, {canon} canonical / {raw} raw states",
-          file=sys.stderr, flush=True)
+    print(f"[bench] batched: {ms_per_step:.2f} ms/step, {canon} canonical / {raw} raw states; "
+          f"last pass {ctx.last_timing}", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: one more pass with every launch event-timed
     ctx.profile(True)

@@ -204,10 +204,14 @@ class Context:
 
     def evolve_batch(self, requests):
         """Evolve requests (a list or a PreparedBatch) together on the device; one Result each."""
+        import time as _t
+        t0 = _t.perf_counter()
         pb = requests if isinstance(requests, PreparedBatch) else PreparedBatch(requests)
         n, descs = pb.n, pb.descs
         outs = (C.c_void_p * max(n, 1))()
+        t1 = _t.perf_counter()
         rc = self._lib.hgx_evolve_batch(self._h, descs, n, outs)
+        t2 = _t.perf_counter()
         if rc != abi.HGX_OK:
             raise self._err(rc)
         cnts = (abi.Counts * max(n, 1))()
@@ -217,8 +221,12 @@ class Context:
         arr = np.frombuffer(cnts, dtype=np.dtype([(k, np.uint64 if t is C.c_uint64 else np.uint32)
                                                    for k, t in abi.Counts._fields_]), count=n)
         names = arr.dtype.names
-        return [Result(self._lib, C.c_void_p(outs[i]), self,
-                       dict(zip(names, map(int, arr[i])))) for i in range(n)]
+        cols = {k: arr[k].tolist() for k in names}
+        out = [Result(self._lib, C.c_void_p(outs[i]), self, {k: cols[k][i] for k in names})
+               for i in range(n)]
+        self.last_timing = {"prepare_s": t1 - t0, "library_s": t2 - t1,
+                            "results_s": _t.perf_counter() - t2}
+        return out
 
     def evolve(self, request):
         return self.evolve_batch([request])[0]

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 
@@ -10,6 +12,16 @@
 #include "hgx_kernels.hpp"
 
 namespace hgx {
+
+// HGX_TRACE=1 prints a host-side phase breakdown of every generation step to stderr (the role of
+// the reference's HG_GPU_DBG_TIME, gpu/src/evolve.cu:358).
+static bool trace_on() {
+    static const bool on = [] { const char* e = std::getenv("HGX_TRACE"); return e && *e && *e != '0'; }();
+    return on;
+}
+static double ms_since(Clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
 
 Engine::Engine(hgx_ctx* ctx) : ctx_(ctx), st_(ctx->stream), pool_(ctx->pool), slab_(ctx->pool.get()) {
     grow_slots(256);
@@ -81,18 +93,18 @@ std::shared_ptr<RunRec> Engine::take(uint64_t id) {
     return r;
 }
 
+// Host side of an admission; the device side (rules copied, map and counters zeroed) is one
+// k_admit launch for all of a step's admissions.
 void Engine::admit(std::shared_ptr<RunRec>& r) {
     const uint32_t slot = take_slot();
     r->slot = slot;
     r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->h.rules.size());
-    h2d(r->d_rules, r->h.rules.data(), r->h.rules.size(), st_);
     uint64_t want = r->h.cap ? (uint64_t)r->h.cap + r->h.roots.size() : std::max<uint64_t>(512, r->h.roots.size());
     uint64_t mc = 64;
     while (mc < 2 * want) mc <<= 1;
     r->map_cap = (uint32_t)mc;
     r->map_keys = (uint64_t*)slab_.alloc(8 * mc);
     r->map_vals = (uint32_t*)slab_.alloc(4 * mc);
-    HIP_TRY(hipMemsetAsync(r->map_keys, 0, 8 * mc, st_));
     DRun& d = h_runs_[slot];
     std::memset(&d, 0, sizeof(d));
     d.rules = r->d_rules;
@@ -102,7 +114,6 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     d.map_keys = r->map_keys;
     d.map_vals = r->map_vals;
     d.map_mask = (uint32_t)(mc - 1);
-    HIP_TRY(hipMemsetAsync(d_ctr_ + (size_t)slot * CTR_N, 0, sizeof(uint64_t) * CTR_N, st_));
     r->admitted = true;
     r->t_admit = Clock::now();
     live_[r->id] = r;
@@ -146,6 +157,8 @@ void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
 }
 
 uint32_t Engine::step() {
+    const auto t_step = Clock::now();
+    double t_count = 0, t_emit = 0, t_child = 0;
     hipStream_t st = st_;
     Arena tmp(pool_.get(), (size_t)4 << 20);
     const uint32_t n = F_ ? F_->v.n : 0;
@@ -200,6 +213,7 @@ uint32_t Engine::step() {
             std::vector<uint64_t> htot(cap_slots_ + 1);
             d2h(htot.data(), d_tot_, cap_slots_ + 1, st);
             HIP_TRY(hipStreamSynchronize(st));
+            t_count = ms_since(t_step);
             parent_edges = htot[cap_slots_];
             bool masked = false;
             for (uint32_t i = 0; i < nF; ++i) {
@@ -256,6 +270,7 @@ uint32_t Engine::step() {
         }
     }
 
+    t_emit = ms_since(t_step);
     // ---- 2. admissions: queued requests join the generation built now ----
     std::vector<std::shared_ptr<RunRec>> admitted;
     const uint64_t soft = (uint64_t)(pool_->budget() * soft_budget_frac);
@@ -269,6 +284,26 @@ uint32_t Engine::step() {
     uint32_t NR = 0;
     uint64_t NER = 0;
     for (auto& r : admitted) { NR += (uint32_t)r->h.roots.size(); NER += r->h.root_edges; }
+    std::vector<DRule> a_rules;                 // staging; outlives the synchronisation below
+    std::vector<AdmitItem> a_items;
+    if (!admitted.empty()) {
+        for (auto& r : admitted) {
+            AdmitItem it{};
+            it.rule_src = (uint32_t)a_rules.size();
+            it.n_rules = (uint32_t)r->h.rules.size();
+            it.rule_dst = r->d_rules;
+            it.map_keys = r->map_keys;
+            it.map_cap = r->map_cap;
+            it.slot = r->slot;
+            a_items.push_back(it);
+            a_rules.insert(a_rules.end(), r->h.rules.begin(), r->h.rules.end());
+        }
+        DRule* d_stage = dalloc<DRule>(tmp, a_rules.size());
+        AdmitItem* d_items = dalloc<AdmitItem>(tmp, a_items.size());
+        h2d(d_stage, a_rules.data(), a_rules.size(), st);
+        h2d(d_items, a_items.data(), a_items.size(), st);
+        HIP_TRY(launch_admit(d_items, (uint32_t)a_items.size(), d_stage, d_ctr_, st));
+    }
 
     // ---- 3. the child generation: children of the frontier, then the new roots ----
     std::vector<std::shared_ptr<RunRec>> next_runs;
@@ -489,6 +524,7 @@ uint32_t Engine::step() {
         }
     }
 
+    t_child = ms_since(t_step);
     // ---- 4. finished runs, slot recycling, the new frontier ----
     for (auto& r : F_runs_)
         if (r->done && live_.count(r->id)) done_now.push_back(r);
@@ -517,6 +553,11 @@ uint32_t Engine::step() {
         for (uint32_t s2 : release_next_) free_slots_.push_back(s2);
         release_next_.clear();
     }
+    if (trace_on())
+        std::fprintf(stderr, "[hgx] step: frontier %u states, %llu children, %u admitted, %zu finished | "
+                             "count+sync %.3f ms, +emit %.3f, +child gen %.3f, total %.3f ms\n",
+                     n, (unsigned long long)M, (unsigned)admitted.size(), done_now.size(), t_count, t_emit,
+                     t_child, ms_since(t_step));
     return live_or_queued();
 }
 
@@ -592,15 +633,18 @@ extern "C" int hgx_evolve_batch(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&]() -> int {
         HIP_TRY(hipSetDevice(ctx->device));
+        const auto t0 = Clock::now();
         std::vector<RunHost> hs;
         hs.reserve(n);
         for (uint32_t i = 0; i < n; ++i) hs.push_back(parse_run(&runs[i]));
+        if (trace_on()) std::fprintf(stderr, "[hgx] batch: parsed %u requests in %.3f ms\n", n, ms_since(t0));
         Engine E(ctx);
         E.soft_budget_frac = 0.8;
         E.reserve_slots(n + 1);
         for (uint32_t i = 0; i < n; ++i) E.submit(std::move(hs[i]), i + 1);
         while (E.live_or_queued()) E.step();
         for (uint32_t i = 0; i < n; ++i) results[i] = make_result(E.take(i + 1));
+        if (trace_on()) std::fprintf(stderr, "[hgx] batch: total %.3f ms\n", ms_since(t0));
         return HGX_OK;
     });
 }

@@ -626,6 +626,23 @@ __global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
     }
 }
 
+// Admissions of one step: one block per request.
+__global__ void k_admit(const AdmitItem* items, const DRule* staging, uint64_t* ctr) {
+    const AdmitItem it = items[blockIdx.x];
+    const uint32_t* src = (const uint32_t*)(staging + it.rule_src);
+    uint32_t* dst = (uint32_t*)it.rule_dst;
+    const uint32_t words = it.n_rules * (uint32_t)(sizeof(DRule) / 4);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    for (uint64_t i = threadIdx.x; i < it.map_cap; i += blockDim.x) it.map_keys[i] = 0;
+    if (threadIdx.x < CTR_N) ctr[(size_t)it.slot * CTR_N + threadIdx.x] = 0;
+}
+
+hipError_t launch_admit(const AdmitItem* items, uint32_t n, const DRule* staging, uint64_t* ctr, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_admit, dim3(n), dim3(256), 0, st, items, staging, ctr);
+    return hipGetLastError();
+}
+
 // Per-run match totals of a generation: runs' states are contiguous ranges.
 __global__ void k_run_totals(const uint64_t* moff, const uint32_t* run_sbeg, const uint32_t* run_send,
                              uint32_t nruns, uint64_t* tot) {

@@ -70,6 +70,18 @@ struct WLStatesArgs {
 };
 
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A);
+// One admitted request: its rules (copied from a packed staging array), its state map (zeroed)
+// and its counter row (zeroed) -- every admission of a step in one launch.
+struct AdmitItem {
+    uint32_t rule_src;         // first rule in the staging array
+    uint32_t n_rules;
+    DRule* rule_dst;
+    uint64_t* map_keys;
+    uint64_t map_cap;
+    uint32_t slot, pad;
+};
+hipError_t launch_admit(const AdmitItem* items, uint32_t n, const DRule* staging, uint64_t* ctr, hipStream_t st);
+
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A);
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
