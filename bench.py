@@ -84,6 +84,98 @@ def cpu_baseline(reqs, gpu_counts, budget_s, threads):
             "raw_states_per_s": raw / secs if secs else None, "count_mismatches": mismatch}
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, as recorded from rocprofv3 FETCH_SIZE
+    and WRITE_SIZE passes (scaled by a device copy of known size) in the committed profile
+    summaries: lines `traffic_per_launch <kernel> <bytes>`."""
+    import glob
+    import re
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc*.md"))):
+        for line in open(f):
+            m = re.match(r"^traffic_per_launch\s+(\S+)\s+(\d+)", line.strip().strip("`"))
+            if m and m.group(1) == kernel:
+                return {"bytes": int(m.group(2)), "source": os.path.relpath(f, ROOT)}
+    return None
+
+
+def poisson_arrivals(rate, seconds, seed):
+    """Arrival times (s) of a Poisson process of `rate` req/s over `seconds`."""
+    import random
+    rnd = random.Random(seed)
+    t, out = 0.0, []
+    while True:
+        t += rnd.expovariate(rate)
+        if t >= seconds:
+            return out
+        out.append(t)
+
+
+def serve_window(ctx, rate, seconds, first, seed):
+    """Serving under Poisson arrivals: requests W.served_run(first + k) arrive at the k-th
+    arrival time, join the next generation launch (continuous batching) and are drained as
+    they finish. Latency = completion - arrival, on the host clock."""
+    from hypergraphrewritingengine_amd import workloads as W
+    arr = poisson_arrivals(rate, seconds, seed)
+    reqs = [W.served_run(first + k) for k in range(len(arr))]
+    n = len(reqs)
+    idx_of, lat, done_at = {}, [0.0] * n, [0.0] * n
+    canon = raw = 0
+    i = done = 0
+    steps = 0
+    t0 = time.perf_counter()
+    while done < n:
+        now = time.perf_counter() - t0
+        while i < n and arr[i] <= now:
+            idx_of[ctx.submit(reqs[i])] = i
+            i += 1
+        if i == done:                       # idle: wait for the next arrival
+            time.sleep(max(0.0, min(arr[i] - now, 0.001)))
+            continue
+        ctx.serve_step()
+        steps += 1
+        fin = ctx.drain()
+        now = time.perf_counter() - t0
+        for rid, res in fin.items():
+            k = idx_of.pop(rid)
+            c = res.counts
+            canon += c["canonical_states"]
+            raw += c["raw_states"]
+            lat[k] = now - arr[k]
+            done_at[k] = now
+            res.close()
+            done += 1
+    wall = max(done_at) if n else 0.0
+    return {"rate_rps": rate, "window_s": seconds, "requests": n, "launch_steps": steps,
+            "canonical_states_per_s": round(canon / wall, 1) if wall else None,
+            "raw_states_per_s": round(raw / wall, 1) if wall else None,
+            "p50_ms": round(pct(lat, 50) * 1e3, 3) if n else None,
+            "p99_ms": round(pct(lat, 99) * 1e3, 3) if n else None,
+            "wall_s": round(wall, 3)}, reqs, arr
+
+
+def plain_queue(ctx, reqs, arr, budget_s):
+    """The same arrivals served one request at a time (each request evolved alone, in arrival
+    order, the next starting when the previous finishes): a single-server queue whose service
+    times are measured live. Stops after budget_s of wall time; when it stops early the
+    latencies cover the requests served so far (a lower bound on the full window's queueing)."""
+    t0 = time.perf_counter()
+    lat, canon = [], 0
+    for r, a in zip(reqs, arr):
+        now = time.perf_counter() - t0
+        if now > budget_s:
+            break
+        if now < a:
+            time.sleep(a - now)
+        out = ctx.evolve(r)
+        lat.append(time.perf_counter() - t0 - a)
+        canon += out.counts["canonical_states"]
+        out.close()
+    wall = time.perf_counter() - t0
+    return {"canonical_states_per_s": round(canon / wall, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+            "p99_ms": round(pct(lat, 99) * 1e3, 3), "wall_s": round(wall, 3),
+            "served": len(lat), "of": len(reqs)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +186,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-rates", default="1000,5000",
+                    help="Poisson arrival rates (req/s) of the serving windows; '' to skip")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-plain-seconds", type=float, default=8.0)
+    ap.add_argument("--no-serve-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (a known-size kernel for scaling "
+                         "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the batched path shards whole "
@@ -106,6 +206,16 @@ def main():
 
     ctx = hgx.Context(0)
     print(f"[bench] {ctx.info}", file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {n} bytes device-to-device", file=sys.stderr, flush=True)
+        del src, dst
     reqs = [W.batched_run(i) for i in range(args.runs)]
     batch = ctx.prepare(reqs)               # host-side request records, built once
 
@@ -141,6 +251,11 @@ def main():
                                 "alg_GBps": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 2)}
                             for k, v in stats.items()}}
 
+    tr = pmc_traffic(dom_name)
+    if tr:
+        roofline["traffic"] = tr["bytes"]
+        roofline["traffic_source"] = tr["source"]
+
     # plain path: the same requests one at a time, serially
     plain = None
     if not args.no_plain:
@@ -156,6 +271,19 @@ def main():
         plain = {"canonical_states_per_s": pc / tot, "sample_runs": len(sample),
                  "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3)}
         print(f"[bench] plain: {plain}", file=sys.stderr, flush=True)
+
+    # serving: continuous batching under Poisson arrivals, against the plain one-at-a-time queue
+    serving = []
+    rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
+    for k, rate in enumerate(rates):
+        first = 1_000_000 * (k + 1)
+        sv, sreqs, sarr = serve_window(ctx, rate, args.serve_seconds, first, seed=k + 1)
+        if not args.no_serve_plain:
+            sv["plain"] = plain_queue(ctx, sreqs, sarr, args.serve_plain_seconds)
+            sv["speedup_vs_plain"] = (round(sv["canonical_states_per_s"] / sv["plain"]["canonical_states_per_s"], 2)
+                                      if sv["canonical_states_per_s"] else None)
+        serving.append(sv)
+        print(f"[bench] serve: {sv}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -184,6 +312,7 @@ def main():
                        "plain_p50": plain["p50_ms"] if plain else None,
                        "plain_p99": plain["p99_ms"] if plain else None},
         "plain_gpu": plain,
+        "serving": serving,
         "speedup_vs_plain": round(value / plain["canonical_states_per_s"], 2) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -269,12 +269,16 @@ __device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
     }
 }
 
+// GWS selects the workspace at compile time (global scratch for states too large for LDS), so
+// the LDS variant addresses its workspace with LDS instructions rather than flat ones.
+template <bool GWS>
 __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
     const uint32_t wpb_ = blockDim.x / WAVE;
-    uint8_t* my = a.scratch ? a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave
-                            : lds + (size_t)w * a.lds_per_wave;
+    uint8_t* my;
+    if constexpr (GWS) my = a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave;
+    else my = lds + (size_t)w * a.lds_per_wave;
     const uint32_t PA = a.P.A, A = a.C.A;                  // parent / child vertex strides
     const WLSizes z = WLSizes::make(a.max_child_ne, a.max_nv, A);
     WLSpace s;
@@ -588,12 +592,14 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
 // each into its run's state map (roots).
 // ------------------------------------------------------------------------------------------
 
+template <bool GWS>
 __global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
     const uint32_t wpb_ = blockDim.x / WAVE;
-    uint8_t* p = a.scratch ? a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave
-                           : lds + (size_t)w * a.lds_per_wave;
+    uint8_t* p;
+    if constexpr (GWS) p = a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave;
+    else p = lds + (size_t)w * a.lds_per_wave;
     const uint32_t A = a.A;
     const WLSizes z = WLSizes::make(a.max_ne, a.max_nv, A);
     WLSpace s;
@@ -701,7 +707,8 @@ uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t ma
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
     uint32_t grid = grid_for(a.M.n, wpb, grid_cap);
     const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(64 * wpb), lds, st, a);
+    if (a.scratch) hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
+    else hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(64 * wpb), lds, st, a);
     return hipGetLastError();
 }
 
@@ -720,7 +727,8 @@ uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A) {
 hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
     const uint32_t grid = grid_for(a.n, wpb, grid_cap);
     const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
-    hipLaunchKernelGGL(k_wl_states, dim3(grid), dim3(64 * wpb), lds, st, a);
+    if (a.scratch) hipLaunchKernelGGL(k_wl_states<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
+    else hipLaunchKernelGGL(k_wl_states<false>, dim3(grid), dim3(64 * wpb), lds, st, a);
     return hipGetLastError();
 }
 
