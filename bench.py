@@ -98,6 +98,93 @@ def pmc_traffic(kernel):
     return None
 
 
+def copy_calibration(n):
+    """One device-to-device copy of n bytes (the runtime's copy kernel reads n and writes n
+    bytes): the known-size launch that scales FETCH_SIZE / WRITE_SIZE counter readings."""
+    import ctypes as C
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    ok = (hip.hipMemset(a, 1, C.c_size_t(n)) == 0 and hip.hipDeviceSynchronize() == 0 and
+          hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0 and hip.hipDeviceSynchronize() == 0)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if not ok:
+        raise RuntimeError("calibration copy failed")
+    print(f"[bench] copied {n} bytes device-to-device", file=sys.stderr, flush=True)
+
+
+def dominant(stats):
+    name, st = max(stats.items(), key=lambda kv: kv[1]["total_ms"])
+    gbs = st["alg_bytes"] / (st["total_ms"] * 1e-3) / 1e9
+    return {"kernel": name, "achieved_GBps": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 5),
+            "avg_launch_ms": round(st["total_ms"] / st["launches"], 4),
+            "alg_bytes_per_launch": st["alg_bytes"] // st["launches"]}
+
+
+def config2_line(ctx, steps, cpu_steps, threads, no_cpu):
+    """One large evolution: rule {{x,y},{x,z}} -> {{x,z},{x,w},{y,w},{z,w}} from {{1,1},{1,1}},
+    full multiway with causal and branchial relations."""
+    from hypergraphrewritingengine_amd import workloads as W
+    ctx.evolve(W.config2(steps=min(steps, 4))).close()
+    req = W.config2(steps=steps)
+    t = time.perf_counter()
+    r = ctx.evolve(req)
+    wall = time.perf_counter() - t
+    c = r.counts
+    r.close()
+    ctx.profile(True)
+    ctx.evolve(req).close()
+    stats = ctx.profile_read()
+    ctx.profile(False)
+    out = {"steps": steps, "wall_ms": round(wall * 1e3, 3), "raw_states": c["raw_states"],
+           "canonical_states": c["canonical_states"], "branchial_pairs": c["branchial_pairs"],
+           "causal_pairs": c["causal_pairs"],
+           "raw_states_per_s": round(c["raw_states"] / wall, 1),
+           "canonical_states_per_s": round(c["canonical_states"] / wall, 1),
+           "roofline": dominant(stats)}
+    if not no_cpu:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            small = W.config2(steps=cpu_steps)
+            rc, _ = po.ref_evolve(small, threads=threads, reps=1)
+            g = ctx.evolve(small)
+            gc = g.counts
+            g.close()
+            out["cpu_reference"] = {"steps": cpu_steps, "threads": threads, "s": round(rc["median_s"], 4),
+                                    "raw_states_per_s": round(rc["raw_states"] / rc["median_s"], 1),
+                                    "counts_match": all(gc[k] == rc[k] for k in
+                                                        ("raw_states", "canonical_states", "causal_pairs",
+                                                         "branchial_pairs"))}
+    return out
+
+
+def config3_line(ctx, name, rule, edges, no_cpu):
+    """All matches of one rule in one large random state (device CSR index + seeded join)."""
+    ctx.match([rule], 0, edges)
+    t = time.perf_counter()
+    n, _, dev_ms = ctx.match([rule], 0, edges)
+    wall = time.perf_counter() - t
+    ctx.profile(True)
+    ctx.match([rule], 0, edges)
+    stats = ctx.profile_read()
+    ctx.profile(False)
+    out = {"edges": len(edges), "matches": n, "device_ms": round(dev_ms, 4),
+           "wall_ms": round(wall * 1e3, 3), "matches_per_s": round(n / (dev_ms * 1e-3), 1),
+           "kernels": {k: {"ms": round(v["total_ms"], 4),
+                           "alg_GBps": round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 2)}
+                       for k, v in stats.items()},
+           "roofline": dominant(stats)}
+    if not no_cpu:
+        from oracle import pyoracle as po
+        t = time.perf_counter()
+        cn = po.match([rule], 0, edges)
+        out["cpu_port"] = {"s": round(time.perf_counter() - t, 4), "threads": 1, "count_match": cn == n}
+    return out
+
+
 def poisson_arrivals(rate, seconds, seed):
     """Arrival times (s) of a Poisson process of `rate` req/s over `seconds`."""
     import random
@@ -191,6 +278,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-plain-seconds", type=float, default=8.0)
     ap.add_argument("--no-serve-plain", action="store_true")
+    ap.add_argument("--no-others", action="store_true", help="skip the config 2 / 3 lines")
+    ap.add_argument("--config2-steps", type=int, default=6)
+    ap.add_argument("--forward", type=int, default=-1, choices=(-1, 0, 1),
+                    help="incremental match forwarding: 1 on, 0 off, -1 automatic")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a known-size kernel for scaling "
                          "hardware byte counters)")
@@ -205,17 +296,10 @@ def main():
     from hypergraphrewritingengine_amd import workloads as W
 
     ctx = hgx.Context(0)
+    ctx.set_forwarding(args.forward)
     print(f"[bench] {ctx.info}", file=sys.stderr, flush=True)
     if args.copy_calibration:
-        import torch
-        n = args.copy_calibration << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"[bench] copied {n} bytes device-to-device", file=sys.stderr, flush=True)
-        del src, dst
+        copy_calibration(args.copy_calibration << 20)
     reqs = [W.batched_run(i) for i in range(args.runs)]
     batch = ctx.prepare(reqs)               # host-side request records, built once
 
@@ -285,6 +369,17 @@ def main():
         serving.append(sv)
         print(f"[bench] serve: {sv}", file=sys.stderr, flush=True)
 
+    # the other measured workloads: one large evolution, and the large-state join
+    others = {}
+    if not args.no_others:
+        threads = min(os.cpu_count() or 1, 16)
+        others["config2"] = config2_line(ctx, args.config2_steps, 4, threads, args.no_cpu)
+        print(f"[bench] config2: {others['config2']}", file=sys.stderr, flush=True)
+        for name, mk in (("config3a", W.config3a), ("config3b", W.config3b)):
+            rule, edges = mk()
+            others[name] = config3_line(ctx, name, rule, edges, args.no_cpu)
+            print(f"[bench] {name}: {others[name]}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -306,13 +401,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched config: 4096 seeded multiway runs, 2-3 edge binary LHS, "
                                "3-64 edge connected inits, 3-6 steps, 4096-state cap per run",
-                   "runs": args.runs, "parallelism": "single GPU, one batch per launch sequence"},
+                   "runs": args.runs, "parallelism": "single GPU, one batch per launch sequence",
+                   "forwarding": {-1: "automatic", 0: "off", 1: "on"}[args.forward]},
         "raw_states_per_s": round(raw / (ms_per_step / 1000.0), 1),
         "latency_ms": {"batched_p50": round(ms_per_step, 3), "batched_p99": round(ms_per_step, 3),
                        "plain_p50": plain["p50_ms"] if plain else None,
                        "plain_p99": plain["p99_ms"] if plain else None},
         "plain_gpu": plain,
         "serving": serving,
+        "other_workloads": others,
         "speedup_vs_plain": round(value / plain["canonical_states_per_s"], 2) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -30,7 +30,11 @@ EXPORTED_SYMBOLS = [
     "hgx_result_events", "hgx_result_state_edges", "hgx_result_causal", "hgx_result_branchial",
     "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
+    "hgx_ctx_set_option",
 ]
+
+OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
+OPTION_DELTA_CAP = 2   # HGX_OPTION_DELTA_CAP
 
 
 class HGXError(RuntimeError):
@@ -83,6 +87,7 @@ def load_library():
     L.hgx_result_counts_many.argtypes = [vp, C.c_uint32, C.POINTER(abi.Counts)]
     L.hgx_serve_drain.argtypes = [vp, C.POINTER(C.c_uint64), vp, C.c_uint32, C.POINTER(C.c_uint32)]
     L.hgx_profile_enable.argtypes = [vp, C.c_int]
+    L.hgx_ctx_set_option.argtypes = [vp, C.c_uint32, C.c_int64]
     L.hgx_profile_read.argtypes = [vp, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)]
     _LIB = L
     return L
@@ -197,6 +202,20 @@ class Context:
         buf = C.create_string_buffer(512)
         self._lib.hgx_device_info(self._h, buf, 512)
         return buf.value.decode()
+
+    def set_forwarding(self, mode: int):
+        """Incremental match forwarding: 1 on, 0 off (full join for every state), -1 automatic.
+        Results never depend on it."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_FORWARD, int(mode))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def set_delta_cap(self, n: int):
+        """Delta matches one forwarded state may stage on chip (1..4096); beyond it the state
+        takes the full join. Results never depend on it."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_DELTA_CAP, int(n))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
 
     def prepare(self, requests) -> "PreparedBatch":
         """Encode requests into hgx_run_desc records once, for repeated evolve_batch calls."""

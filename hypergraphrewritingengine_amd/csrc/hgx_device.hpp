@@ -46,6 +46,10 @@ struct DRule {
     uint8_t var_src_pos[32];
     uint8_t piv[16];          // per join depth: a position sharing a bound variable, or 0xFF
     uint32_t lhs_mask, rhs_mask, new_mask, pad1;
+    // Anchored join orders for delta joins: aord[p] starts at LHS position p and adds the
+    // remaining positions greedily by shared variables; apiv[p] are its pivots.
+    uint8_t aord[16][16];
+    uint8_t apiv[16][16];
 };
 
 // Per request (run) state the kernels read and update.

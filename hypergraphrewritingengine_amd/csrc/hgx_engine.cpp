@@ -161,6 +161,7 @@ uint32_t Engine::step() {
     double t_count = 0, t_emit = 0, t_child = 0;
     hipStream_t st = st_;
     Arena tmp(pool_.get(), (size_t)4 << 20);
+    auto keep = std::make_unique<Arena>(pool_.get(), (size_t)4 << 20);   // this step's match list
     const uint32_t n = F_ ? F_->v.n : 0;
     const uint32_t nF = (uint32_t)F_runs_.size();
     std::vector<uint64_t> tot(nF, 0);
@@ -175,6 +176,8 @@ uint32_t Engine::step() {
     if (n) {
         std::vector<uint32_t> h_budget(cap_slots_, 0), h_b(cap_slots_, 0), h_e(cap_slots_, 0);
         bool any = false;
+        uint32_t KL = 1;
+        bool small = true;
         for (uint32_t i = 0; i < nF; ++i) {
             RunRec& r = *F_runs_[i];
             h_b[r.slot] = F_b_[i];
@@ -182,6 +185,10 @@ uint32_t Engine::step() {
             if (r.done || r.depth >= r.h.steps) { r.done = true; continue; }
             h_active[r.slot] = 1;
             h_budget[r.slot] = !r.h.cap ? 0xFFFFFFFFu : (r.states >= r.h.cap ? 0u : (uint32_t)(r.h.cap - r.states));
+            for (auto& R : r.h.rules) {
+                KL = std::max<uint32_t>(KL, R.n_lhs);
+                small &= R.n_lhs <= 4 && (R.lhs_mask >> 16) == 0;
+            }
             any = true;
         }
         if (any) {
@@ -192,7 +199,7 @@ uint32_t Engine::step() {
             HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * cap_slots_, st));
             HIP_TRY(hipMemsetAsync(d_tot_ + cap_slots_, 0, sizeof(uint64_t), st));
             cnt = dalloc<uint32_t>(tmp, n + 1);
-            moff = dalloc<uint64_t>(tmp, n + 1);
+            moff = dalloc<uint64_t>(*keep, n + 1);
             HIP_TRY(hipMemsetAsync(cnt + n, 0, sizeof(uint32_t), st));
             if (F_->max_ne > 65535)
                 throw Error{HGX_ERR_UNSUPPORTED, "states above 65535 edges need the large-state join"};
@@ -200,7 +207,16 @@ uint32_t Engine::step() {
             ma.P = F_->v; ma.runs = d_runs_; ma.active = d_active_; ma.cnt = cnt; ma.moff = moff;
             ma.max_ne = std::max(1u, F_->max_ne);
             ma.max_nextv = std::max(1u, F_->max_nextv);
-            ma.lds_per_wave = (match_lds_per_wave(ma.max_ne, ma.max_nextv, F_->v.A) + 15u) & ~15u;
+            // forwarding: on when asked, or (automatic) for every state the previous
+            // generation made
+            const bool fwd = prev_.n > 0 && prev_.n <= n && ctx_->forward != 0;
+            ma.fwd_n = fwd ? prev_.n : 0;
+            ma.pm_parent = prev_.parent; ma.pm_rule = prev_.rule; ma.pm_q = prev_.q; ma.pm_K = prev_.K;
+            ma.pm_moff = prev_.moff;
+            ma.KL = KL;
+            ma.small = small ? 1u : 0u;
+            ma.dmax = fwd ? std::min(ctx_->delta_cap, 16384u / (2u * KL + 8u)) : 0u;   // <= 16 KiB LDS
+            ma.lds_per_wave = (match_lds_per_wave(ma.max_ne, ma.max_nextv, F_->v.A, ma.KL, ma.dmax) + 15u) & ~15u;
             ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
             ma.budget = d_budget_;
             ma.run_acc = (unsigned long long*)d_run_acc_;
@@ -257,9 +273,9 @@ uint32_t Engine::step() {
                     d.map_keys = keys; d.map_vals = vals; d.map_mask = (uint32_t)(c - 1);
                 }
                 mv.n = (uint32_t)M; mv.K = K;
-                mv.parent = dalloc<uint32_t>(tmp, M);
-                mv.rule = dalloc<uint32_t>(tmp, M);
-                mv.q = dalloc<uint16_t>(tmp, M * K);
+                mv.parent = dalloc<uint32_t>(*keep, M);
+                mv.rule = dalloc<uint32_t>(*keep, M);
+                mv.q = dalloc<uint16_t>(*keep, M * K);
                 mv.cne = dalloc<uint32_t>(tmp, M + 1);
                 HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
                 ma.M = mv;
@@ -540,6 +556,13 @@ uint32_t Engine::step() {
     for (uint32_t s : releasable) free_slots_.push_back(s);
     if (F_ && !F_->keep_edges) F_->edges.release();
     F_ = C;
+    if (M) {
+        prev_.ar = std::move(keep);
+        prev_.n = (uint32_t)M; prev_.K = K;
+        prev_.parent = mv.parent; prev_.rule = mv.rule; prev_.q = mv.q; prev_.moff = moff;
+    } else {
+        prev_ = PrevMatches{};
+    }
     F_runs_.swap(next_runs);
     F_b_.swap(next_b);
     F_e_.swap(next_e);
@@ -549,6 +572,7 @@ uint32_t Engine::step() {
     if (!live_in_F) {
         if (F_ && !F_->keep_edges) F_->edges.release();
         F_.reset();
+        prev_ = PrevMatches{};
         F_runs_.clear(); F_b_.clear(); F_e_.clear();
         for (uint32_t s2 : release_next_) free_slots_.push_back(s2);
         release_next_.clear();

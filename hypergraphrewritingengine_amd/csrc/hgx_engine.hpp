@@ -104,6 +104,16 @@ private:
     std::vector<std::shared_ptr<RunRec>> F_runs_;   // runs with states in F, in F order
     std::vector<uint32_t> F_b_, F_e_;         // their ranges, parallel to F_runs_
     uint64_t live_footprint_ = 0;
+    // The previous generation's match list, kept one generation longer: state s < prev_.n of
+    // the frontier was made by match s, and its parent's matches seed its own (forwarding).
+    struct PrevMatches {
+        std::unique_ptr<Arena> ar;
+        uint32_t n = 0, K = 1;
+        uint32_t* parent = nullptr;
+        uint32_t* rule = nullptr;
+        uint16_t* q = nullptr;
+        uint64_t* moff = nullptr;
+    } prev_;
 
     void grow_slots(uint32_t need);
     uint32_t take_slot();

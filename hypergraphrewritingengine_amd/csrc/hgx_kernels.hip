@@ -41,15 +41,21 @@ __device__ inline uint32_t xcd_block() {
 //           enumerates, so event numbering is identical across engines and runs.
 // ------------------------------------------------------------------------------------------
 
-struct JoinLane {
-    uint32_t matched[16];   // edge index bound at each depth
-    uint32_t smask[17];     // bound-variable mask on entry to each depth
-    uint32_t cidx[16];      // next candidate at each depth
-    uint32_t chi[16];       // end of the candidate range at each depth
-    uint32_t bind[32];
+// Per-lane depth-first state. KM bounds the LHS size and VM the pattern variable ids; the small
+// instance (LHS <= 4 edges, variables < 16) is sized so the compiler keeps it in registers
+// instead of scratch memory.
+template <int KM, int VM>
+struct JoinLaneT {
+    uint32_t matched[KM];   // edge index bound at each depth
+    uint32_t smask[KM + 1]; // bound-variable mask on entry to each depth
+    uint32_t cidx[KM];      // next candidate at each depth
+    uint32_t chi[KM];       // end of the candidate range at each depth
+    uint32_t bind[VM];
 };
+using JoinLaneBig = JoinLaneT<16, 32>;
+using JoinLaneSmall = JoinLaneT<4, 16>;
 
-__device__ inline bool bind_edge(const uint32_t* ev, uint32_t ea, const uint8_t* pv, uint32_t pa,
+__device__ __forceinline__ bool bind_edge(const uint32_t* ev, uint32_t ea, const uint8_t* pv, uint32_t pa,
                                  uint32_t* bind, uint32_t& mask) {   // match_core.hpp:32-49
     if (ea != pa) return false;
     for (uint32_t i = 0; i < ea; ++i) {
@@ -64,7 +70,8 @@ __device__ inline bool bind_edge(const uint32_t* ev, uint32_t ea, const uint8_t*
 // shares with the bound prefix; the list holds every edge containing it, ascending), or every
 // edge when the pattern edge shares nothing. Either way the candidates are tried in ascending
 // edge order, so the matches come out in the same order as an all-edge scan.
-__device__ inline void cand_range(const DRule& R, uint32_t d, const JoinLane& L, const uint32_t* iend,
+template <class JoinLane>
+__device__ __forceinline__ void cand_range(const DRule& R, uint32_t d, const JoinLane& L, const uint32_t* iend,
                                   uint32_t ne, uint32_t& lo, uint32_t& hi) {
     const uint32_t pv = R.piv[d];
     if (pv == 0xFFu) { lo = 0; hi = ne; return; }
@@ -75,7 +82,8 @@ __device__ inline void cand_range(const DRule& R, uint32_t d, const JoinLane& L,
 
 // Depth-first completion of one seed; returns the number of matches. With q_out != nullptr
 // every match is written at out_base + k (the k-th match of this seed).
-__device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const uint8_t* sa,
+template <class JoinLane>
+__device__ __forceinline__ uint32_t join_seed(const DRule& R, const uint32_t* sv, const uint8_t* sa,
                                      uint32_t ne, uint32_t A, const uint32_t* iend, const uint16_t* ie,
                                      uint32_t seed, JoinLane& L,
                                      uint16_t* q_out, uint32_t K, uint32_t out_base,
@@ -130,13 +138,95 @@ __device__ inline uint32_t join_seed(const DRule& R, const uint32_t* sv, const u
     return found;
 }
 
-uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A) {
-    auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
-    return al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
-           al(max_ne * A * 2) + 64;
+// Order of two matches of one state, as a full join emits them: rule, then the edge
+// positions in the rule's join order (q arrays are in LHS order).
+__device__ __forceinline__ bool match_less(const DRun& RN, uint32_t ra, const uint16_t* qa, uint32_t rb,
+                                  const uint16_t* qb) {
+    if (ra != rb) return ra < rb;
+    const DRule& R = RN.rules[ra];
+    for (uint32_t t = 0; t < R.n_lhs; ++t) {
+        const uint32_t o = R.order[t];
+        if (qa[o] != qb[o]) return qa[o] < qb[o];
+    }
+    return false;
 }
 
-template <bool EMIT>
+// Delta matches (find_delta_matches, hypergraph/include/hypergraph/pattern_matcher.hpp:493-537):
+// pattern edge p bound to child edge `anchor` (a produced edge), the rest completed depth-first
+// in the anchored order aord[p]. LHS positions below p take kept edges only (positions below
+// `kept`), so a match that uses produced edges is found exactly once -- from its lowest LHS
+// position holding a produced edge. With d_out the k-th match is written at d_out + k * stride
+// as (rule, edge positions in LHS order).
+template <class JoinLane>
+__device__ __forceinline__ uint32_t join_anchor(const DRule& R, uint32_t p, const uint32_t* sv, const uint8_t* sa,
+                                       uint32_t ne, uint32_t A, const uint32_t* iend, const uint16_t* ie,
+                                       uint32_t kept, uint32_t anchor, JoinLane& L, uint16_t* d_out,
+                                       uint32_t stride, uint32_t rule_id) {
+    const uint32_t n = R.n_lhs;
+    const uint8_t* ord = R.aord[p];
+    const uint8_t* piv = R.apiv[p];
+    uint32_t mask = 0;
+    if (!bind_edge(sv + anchor * A, sa[anchor], R.lv[p], R.la[p], L.bind, mask)) return 0;
+    L.matched[0] = anchor;
+    auto range = [&](uint32_t d, uint32_t& lo, uint32_t& hi) {
+        const uint32_t pv = piv[d];
+        if (pv == 0xFFu) { lo = 0; hi = ne; return; }
+        const uint32_t x = L.bind[R.lv[ord[d]][pv]];
+        lo = x ? iend[x - 1] : 0u;
+        hi = iend[x];
+    };
+    uint32_t found = 0;
+    if (n == 1) {
+        if (d_out) { d_out[0] = (uint16_t)rule_id; d_out[1 + p] = (uint16_t)anchor; }
+        return 1;
+    }
+    L.smask[1] = mask;
+    range(1, L.cidx[1], L.chi[1]);
+    uint32_t d = 1;
+    while (d >= 1) {
+        const uint32_t pe = ord[d];
+        const bool indexed = piv[d] != 0xFFu;
+        const uint32_t lim = pe < p ? kept : ne;      // candidates ascend: stop at the limit
+        bool advanced = false;
+        while (L.cidx[d] < L.chi[d]) {
+            const uint32_t j = L.cidx[d]++;
+            const uint32_t e = indexed ? (uint32_t)ie[j] : j;
+            if (e >= lim) { L.cidx[d] = L.chi[d]; break; }
+            bool taken = false;
+            for (uint32_t t = 0; t < d; ++t) taken |= (L.matched[t] == e);
+            if (taken) continue;
+            uint32_t m = L.smask[d];
+            if (!bind_edge(sv + e * A, sa[e], R.lv[pe], R.la[pe], L.bind, m)) continue;
+            L.matched[d] = e;
+            if (d + 1 == n) {
+                if (d_out) {
+                    uint16_t* o = d_out + (size_t)found * stride;
+                    o[0] = (uint16_t)rule_id;
+                    for (uint32_t t = 0; t < n; ++t) o[1 + ord[t]] = (uint16_t)L.matched[t];
+                }
+                ++found;
+                continue;
+            }
+            L.smask[d + 1] = m;
+            range(d + 1, L.cidx[d + 1], L.chi[d + 1]);
+            ++d;
+            advanced = true;
+            break;
+        }
+        if (!advanced) --d;
+    }
+    return found;
+}
+
+uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax) {
+    auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
+    uint32_t b = al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
+                 al(max_ne * A * 2) + 64;
+    if (dmax) b += al(dmax * (KL + 1) * 2) + al(dmax * 2) + al((dmax + 1) * 4);
+    return b;
+}
+
+template <bool EMIT, class JoinLane>
 __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
@@ -147,6 +237,10 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     uint16_t* seeds = (uint16_t*)carve(p, a.max_ne * 2);
     uint32_t* iend = (uint32_t*)carve(p, (a.max_nextv + 1) * 4); // incidence list ends per vertex
     uint16_t* ie = (uint16_t*)carve(p, a.max_ne * A * 2);       // incidence lists
+    const uint32_t dstride = a.KL + 1;                          // delta match: rule + positions
+    uint16_t* dlist = (uint16_t*)carve(p, a.dmax * dstride * 2);   // empty when dmax = 0
+    uint16_t* dperm = (uint16_t*)carve(p, a.dmax * 2);
+    uint32_t* dhist = (uint32_t*)carve(p, a.dmax ? (a.dmax + 1) * 4 : 0);
     JoinLane L;
 
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
@@ -196,7 +290,124 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
         uint32_t total = 0;
         bool stop = false;
         uint32_t out = EMIT ? (uint32_t)a.moff[s] : 0u;
-        for (uint32_t r = 0; r < RN.n_rules && !stop; ++r) {
+        bool full = true;
+        if (s < a.fwd_n) {
+            // ---- incremental forwarding (parallel_evolution.cpp:591-756): the parent's
+            //      matches that avoid the consumed edges, plus delta joins at the produced
+            //      edges, merged into the full join's order ----
+            const DRule& CR = RN.rules[a.pm_rule[s]];
+            const uint32_t nc = CR.n_lhs, nprod = CR.n_rhs;
+            const uint32_t kept = ne - nprod;                   // kept edges come first
+            const uint16_t* cq = a.pm_q + (size_t)s * a.pm_K;  // consumed, parent positions
+            const uint32_t par = a.pm_parent[s];
+            const uint64_t lb = a.pm_moff[par], le = a.pm_moff[par + 1];
+            uint32_t ncombo = 0;                                // (rule, anchor, produced edge)
+            for (uint32_t r = 0; r < RN.n_rules; ++r) ncombo += RN.rules[r].n_lhs * nprod;
+            auto combo = [&](uint32_t i, uint32_t& r, uint32_t& pp, uint32_t& j) {
+                r = 0;
+                while (i >= RN.rules[r].n_lhs * nprod) { i -= RN.rules[r].n_lhs * nprod; ++r; }
+                pp = i / nprod;
+                j = i % nprod;
+            };
+            uint32_t nd = 0;
+            for (uint32_t base = 0; base < ncombo; base += WAVE) {
+                const uint32_t i = base + l;
+                uint32_t c = 0;
+                if (i < ncombo) {
+                    uint32_t r, pp, j;
+                    combo(i, r, pp, j);
+                    c = join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                }
+                nd += wave_sum_u32(c);
+            }
+            if (!EMIT || (a.dmax && nd <= a.dmax)) {
+                full = false;
+                if (EMIT) {
+                    uint32_t o = 0;
+                    for (uint32_t base = 0; base < ncombo; base += WAVE) {
+                        const uint32_t i = base + l;
+                        uint32_t c = 0, r = 0, pp = 0, j = 0;
+                        if (i < ncombo) {
+                            combo(i, r, pp, j);
+                            c = join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                        }
+                        const uint32_t inc = wave_incl_scan_u32(c);
+                        if (c)
+                            join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L,
+                                        dlist + (size_t)(o + inc - c) * dstride, dstride, r);
+                        o += __shfl(inc, WAVE - 1, WAVE);
+                    }
+                    wave_sync();
+                    for (uint32_t i = l; i < nd; i += WAVE) {        // rank = place in order
+                        const uint16_t* di = dlist + (size_t)i * dstride;
+                        uint32_t rank = 0;
+                        for (uint32_t j = 0; j < nd; ++j) {
+                            const uint16_t* dj = dlist + (size_t)j * dstride;
+                            rank += match_less(RN, dj[0], dj + 1, di[0], di + 1) ? 1u : 0u;
+                        }
+                        dperm[rank] = (uint16_t)i;
+                    }
+                    for (uint32_t i = l; i <= nd; i += WAVE) dhist[i] = 0;
+                    wave_sync();
+                }
+                uint32_t nf = 0;
+                for (uint64_t base = lb; base < le; base += WAVE) {
+                    const uint64_t m = base + l;
+                    bool fw = false;
+                    uint32_t rm = 0, nm = 0;
+                    uint16_t qm[16];
+                    if (m < le) {
+                        rm = a.pm_rule[m];
+                        nm = RN.rules[rm].n_lhs;
+                        fw = true;
+                        for (uint32_t t = 0; t < nm; ++t) {
+                            const uint32_t x = a.pm_q[m * a.pm_K + t];
+                            uint32_t below = 0;
+                            for (uint32_t u = 0; u < nc; ++u) {
+                                const uint32_t y = cq[u];
+                                fw &= (x != y);
+                                below += (y < x) ? 1u : 0u;
+                            }
+                            qm[t] = (uint16_t)(x - below);   // position among the kept edges
+                        }
+                    }
+                    const uint64_t bal = __ballot(fw);
+                    if (EMIT && fw) {
+                        uint32_t lo = 0, hi = nd;                   // delta matches before it
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            const uint16_t* dm = dlist + (size_t)dperm[mid] * dstride;
+                            if (match_less(RN, dm[0], dm + 1, rm, qm)) lo = mid + 1; else hi = mid;
+                        }
+                        const uint32_t o2 = out + nf + lanes_below(bal) + lo;
+                        for (uint32_t t = 0; t < nm; ++t) a.M.q[(size_t)o2 * a.M.K + t] = qm[t];
+                        a.M.parent[o2] = s;
+                        a.M.rule[o2] = rm;
+                        a.M.cne[o2] = ne - nm + RN.rules[rm].n_rhs;
+                        atomicAdd(&dhist[lo], 1u);
+                    }
+                    nf += __popcll(bal);
+                }
+                if (EMIT) {
+                    wave_sync();
+                    wave_excl_scan_lds(dhist, nd + 1);        // dhist[k + 1] = forwarded before delta k
+                    for (uint32_t k = l; k < nd; k += WAVE) {
+                        const uint16_t* dm = dlist + (size_t)dperm[k] * dstride;
+                        const uint32_t rr = dm[0], nm = RN.rules[rr].n_lhs;
+                        const uint32_t o2 = out + dhist[k + 1] + k;
+                        for (uint32_t t = 0; t < nm; ++t) a.M.q[(size_t)o2 * a.M.K + t] = dm[1 + t];
+                        a.M.parent[o2] = s;
+                        a.M.rule[o2] = rr;
+                        a.M.cne[o2] = ne - nm + RN.rules[rr].n_rhs;
+                    }
+                    wave_sync();
+                } else {
+                    total = nf + nd;
+                    if (l == 0 && total) atomicAdd(&a.run_acc[run], (unsigned long long)total);
+                }
+            }
+        }
+        for (uint32_t r = 0; full && r < RN.n_rules && !stop; ++r) {
             const DRule& R = RN.rules[r];
             const uint32_t p0 = R.order[0];
             const uint32_t child_ne = ne - R.n_lhs + R.n_rhs;
@@ -248,8 +459,10 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     }
 }
 
-template __global__ void k_match<false>(MatchArgs);
-template __global__ void k_match<true>(MatchArgs);
+template __global__ void k_match<false, JoinLaneBig>(MatchArgs);
+template __global__ void k_match<true, JoinLaneBig>(MatchArgs);
+template __global__ void k_match<false, JoinLaneSmall>(MatchArgs);
+template __global__ void k_match<true, JoinLaneSmall>(MatchArgs);
 
 // ------------------------------------------------------------------------------------------
 // Rewrite + hash + dedup + event + causal, one wave per child (Rewriter::apply,
@@ -465,7 +678,7 @@ constexpr uint32_t BR_MASKS = 1024;                 // consumed masks cached in 
 
 uint32_t branchial_lds_per_wave() { return 64 * BR_WORDS * 8 + BR_MASKS * 8 + 64; }
 
-__device__ inline uint64_t consumed_mask(const BranchialArgs& a, const DRun& RN, uint32_t m) {
+__device__ __forceinline__ uint64_t consumed_mask(const BranchialArgs& a, const DRun& RN, uint32_t m) {
     const uint32_t nl = RN.rules[a.M.rule[m]].n_lhs;
     const uint16_t* q = a.M.q + (size_t)m * a.M.K;
     uint64_t msk = 0;
@@ -694,8 +907,13 @@ static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
     const uint32_t grid = grid_for(a.P.n, wpb, grid_cap);
     const size_t lds = (size_t)a.lds_per_wave * wpb;
-    if (emit) hipLaunchKernelGGL(k_match<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
-    else hipLaunchKernelGGL(k_match<false>, dim3(grid), dim3(64 * wpb), lds, st, a);
+    if (a.small) {
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneSmall>), dim3(grid), dim3(64 * wpb), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneSmall>), dim3(grid), dim3(64 * wpb), lds, st, a);
+    } else {
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneBig>), dim3(grid), dim3(64 * wpb), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneBig>), dim3(grid), dim3(64 * wpb), lds, st, a);
+    }
     return hipGetLastError();
 }
 

@@ -21,6 +21,19 @@ struct MatchArgs {
     const uint32_t* budget;    // count pass: [runs] matches a run may still take
     unsigned long long* run_acc;   // count pass: [runs] running tally (zeroed per generation)
     uint32_t max_nextv;        // bound on vertex labels in P (incidence index)
+    // Incremental forwarding. States [0, fwd_n) of P are the children of the previous
+    // generation's matches (state s was made by match s there); their matches are the
+    // parent's matches that avoid the consumed edges, renumbered, plus delta joins anchored at
+    // the produced edges -- the same list, in the same order, as a full join.
+    uint32_t fwd_n;
+    const uint32_t* pm_parent; // [fwd_n] previous-frontier index of each state's parent
+    const uint32_t* pm_rule;   // [fwd_n] rule of each previous match
+    const uint16_t* pm_q;      // [fwd_n * pm_K] edge positions of each previous match
+    uint32_t pm_K;
+    const uint64_t* pm_moff;   // [previous frontier + 1] each parent's match range
+    uint32_t KL;               // largest LHS of the frontier's active runs
+    uint32_t dmax;             // delta matches a state may stage (else it takes the full join)
+    uint32_t small;            // every active rule has <= 4 LHS edges and variables < 16
 };
 
 struct ExpandArgs {
@@ -69,7 +82,7 @@ struct WLStatesArgs {
     uint8_t* scratch;          // as ExpandArgs::scratch
 };
 
-uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A);
+uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax);
 // One admitted request: its rules (copied from a packed staging array), its state map (zeroed)
 // and its counter row (zeroed) -- every admission of a step in one launch.
 struct AdmitItem {

@@ -213,6 +213,27 @@ static void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<
                     if (bound & (1u << R.lv[pe][i])) { R.piv[d] = (uint8_t)i; break; }
             bound |= var_mask(R.lv[pe], R.la[pe]);
         }
+        // anchored orders: position p first, then most shared bound variables (ties: lower
+        // position); the choice only affects join cost, never the set of matches
+        std::memset(R.aord, 0, sizeof(R.aord));
+        std::memset(R.apiv, 0xFF, sizeof(R.apiv));
+        for (int p0 = 0; p0 < R.n_lhs; ++p0) {
+            bool used[16] = {};
+            R.aord[p0][0] = (uint8_t)p0; used[p0] = true;
+            uint32_t bnd = var_mask(R.lv[p0], R.la[p0]);
+            for (int d = 1; d < R.n_lhs; ++d) {
+                int pick = -1, ps = -1;
+                for (int e = 0; e < R.n_lhs; ++e) {
+                    if (used[e]) continue;
+                    const int sh = __builtin_popcount(var_mask(R.lv[e], R.la[e]) & bnd);
+                    if (sh > ps) { pick = e; ps = sh; }
+                }
+                R.aord[p0][d] = (uint8_t)pick; used[pick] = true;
+                for (int i = 0; i < R.la[pick]; ++i)
+                    if (bnd & (1u << R.lv[pick][i])) { R.apiv[p0][d] = (uint8_t)i; break; }
+                bnd |= var_mask(R.lv[pick], R.la[pick]);
+            }
+        }
         out.push_back(R);
     }
     if (k != len) throw Error{HGX_ERR_INVALID, "rule program has trailing bytes"};
@@ -391,6 +412,21 @@ extern "C" size_t hgx_last_error(const hgx_ctx* ctx, char* buf, size_t len) {
 
 extern "C" size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len) {
     return copy_str(ctx ? ctx->info : std::string(), buf, len);
+}
+
+extern "C" int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value) {
+    if (!ctx) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (option == HGX_OPTION_FORWARD && value >= -1 && value <= 1) {
+        ctx->forward = (int)value;
+        return HGX_OK;
+    }
+    if (option == HGX_OPTION_DELTA_CAP && value >= 1 && value <= 4096) {
+        ctx->delta_cap = (uint32_t)value;
+        return HGX_OK;
+    }
+    ctx->err = "unknown option or value";
+    return HGX_ERR_INVALID;
 }
 
 

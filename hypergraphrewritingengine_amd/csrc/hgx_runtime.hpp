@@ -151,6 +151,8 @@ struct hgx_ctx {
     std::recursive_mutex mu;
     std::unique_ptr<hgx::Engine> serve;          // the served path's engine
     uint64_t next_request_id = 1;
+    int forward = -1;                            // HGX_OPTION_FORWARD
+    uint32_t delta_cap = 256;                    // HGX_OPTION_DELTA_CAP
 };
 
 namespace hgx {

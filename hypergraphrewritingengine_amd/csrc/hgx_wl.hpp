@@ -63,13 +63,13 @@ struct WLSizes {
     }
 };
 
-__device__ inline uint8_t* carve(uint8_t*& p, uint32_t bytes) {
+__device__ __forceinline__ uint8_t* carve(uint8_t*& p, uint32_t bytes) {
     uint8_t* r = p;
     p += (bytes + 7u) & ~7u;
     return r;
 }
 
-__device__ inline void wl_bind(WLSpace& s, uint8_t*& p, const WLSizes& z, uint32_t A) {
+__device__ __forceinline__ void wl_bind(WLSpace& s, uint8_t*& p, const WLSizes& z, uint32_t A) {
     s.A = A;
     s.e_ar = carve(p, z.NE);
     s.e_lv = (uint16_t*)carve(p, z.NE * A * 2);
@@ -86,7 +86,7 @@ __device__ inline void wl_bind(WLSpace& s, uint8_t*& p, const WLSizes& z, uint32
 }
 
 // Exclusive prefix sum of a[0..n) in LDS, in place; returns the total.
-__device__ inline uint32_t wave_excl_scan_lds(uint32_t* a, uint32_t n) {
+__device__ __forceinline__ uint32_t wave_excl_scan_lds(uint32_t* a, uint32_t n) {
     const uint32_t l = lane_id();
     uint32_t carry = 0;
     for (uint32_t base = 0; base < n; base += WAVE) {
@@ -116,7 +116,7 @@ __device__ inline void sort8(uint64_t (&x)[8]) {
 }
 #undef HGX_CE
 
-__device__ inline void isort_lds(uint64_t* a, uint32_t n) {   // core.hpp:54-61
+__device__ __forceinline__ void isort_lds(uint64_t* a, uint32_t n) {   // core.hpp:54-61
     for (uint32_t i = 1; i < n; ++i) {
         const uint64_t key = a[i];
         uint32_t j = i;
@@ -126,7 +126,7 @@ __device__ inline void isort_lds(uint64_t* a, uint32_t n) {   // core.hpp:54-61
 }
 
 // Exact number of distinct values in cur[0..nV).
-__device__ inline uint32_t wl_ndistinct(WLSpace& s, const uint64_t* cur) {
+__device__ __forceinline__ uint32_t wl_ndistinct(WLSpace& s, const uint64_t* cur) {
     const uint32_t l = lane_id();
     for (uint32_t i = l; i <= s.ht_mask; i += WAVE) s.hset[i] = 0;
     wave_sync();

@@ -149,6 +149,18 @@ size_t hgx_last_error(const hgx_ctx* ctx, char* buf, size_t len);
 /* Kernel/device description string (arch, CU count, HBM), for reports. */
 size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
 
+/* Context options (none changes any result, only how it is computed):
+ *   HGX_OPTION_FORWARD  incremental match forwarding for states made by the previous
+ *                       generation (ParallelEvolutionEngine's forwarding, enabled there by
+ *                       rule_analysis forwarding_pays, hypergraph/src/rule_analysis.cpp:134):
+ *                       1 on, 0 off (full join for every state), -1 automatic (default).
+ *   HGX_OPTION_DELTA_CAP delta matches one forwarded state may stage on chip, 1..4096
+ *                       (default 256); a state with more takes the full join.
+ * Returns HGX_ERR_INVALID for an unknown option or value. */
+#define HGX_OPTION_FORWARD 1
+#define HGX_OPTION_DELTA_CAP 2
+int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value);
+
 /* ---- synchronous batch: the plain and the batched path -------------------------------- */
 
 /* Evolve n independent requests together on the device, each with its own rules, roots,

@@ -106,6 +106,8 @@ public:
         ctx_.reset(c);
     }
     hgx_ctx* get() const { return ctx_.get(); }
+    // Incremental match forwarding: 1 on, 0 off, -1 automatic (results never depend on it).
+    void set_forwarding(int mode) { detail::check(hgx_ctx_set_option(ctx_.get(), HGX_OPTION_FORWARD, mode), ctx_.get()); }
     std::string info() const {
         char buf[512] = {0};
         hgx_device_info(ctx_.get(), buf, sizeof(buf));
