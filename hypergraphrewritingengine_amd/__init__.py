@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = [
     "hgx_result_events", "hgx_result_state_edges", "hgx_result_causal", "hgx_result_branchial",
     "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
-    "hgx_ctx_set_option",
+    "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -88,6 +88,10 @@ def load_library():
     L.hgx_serve_drain.argtypes = [vp, C.POINTER(C.c_uint64), vp, C.c_uint32, C.POINTER(C.c_uint32)]
     L.hgx_profile_enable.argtypes = [vp, C.c_int]
     L.hgx_ctx_set_option.argtypes = [vp, C.c_uint32, C.c_int64]
+    L.hgx_batch_prepare.argtypes = [vp, C.POINTER(abi.RunDesc), C.c_uint32, C.POINTER(vp)]
+    L.hgx_batch_evolve.argtypes = [vp, vp, C.POINTER(vp)]
+    L.hgx_batch_release.argtypes = [vp]
+    L.hgx_batch_release.restype = None
     L.hgx_profile_read.argtypes = [vp, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)]
     _LIB = L
     return L
@@ -169,14 +173,34 @@ class Result:
 
 
 class PreparedBatch:
-    """hgx_run_desc records of a list of requests, with the buffers they point into."""
+    """hgx_run_desc records of a list of requests, with the buffers they point into, and (when
+    made by Context.prepare) the requests parsed and staged in device memory (hgx_batch)."""
 
-    def __init__(self, requests):
+    def __init__(self, requests, ctx=None):
         self._keep: list = []
         self.n = len(requests)
         self.descs = (abi.RunDesc * max(self.n, 1))()
         for i, r in enumerate(requests):
             self.descs[i] = abi.make_run_desc(r, self._keep)
+        self._ctx = ctx
+        self._h = None
+        if ctx is not None:
+            h = C.c_void_p()
+            rc = ctx._lib.hgx_batch_prepare(ctx._h, self.descs, self.n, C.byref(h))
+            if rc != abi.HGX_OK:
+                raise ctx._err(rc)
+            self._h = h
+
+    def close(self):
+        if self._h:
+            self._ctx._lib.hgx_batch_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -218,8 +242,9 @@ class Context:
             raise self._err(rc)
 
     def prepare(self, requests) -> "PreparedBatch":
-        """Encode requests into hgx_run_desc records once, for repeated evolve_batch calls."""
-        return PreparedBatch(requests)
+        """Parse requests and stage their rules and initial states in device memory once
+        (hgx_batch_prepare), for repeated evolve_batch calls from device-resident inputs."""
+        return PreparedBatch(requests, self)
 
     def evolve_batch(self, requests):
         """Evolve requests (a list or a PreparedBatch) together on the device; one Result each."""
@@ -229,7 +254,10 @@ class Context:
         n, descs = pb.n, pb.descs
         outs = (C.c_void_p * max(n, 1))()
         t1 = _t.perf_counter()
-        rc = self._lib.hgx_evolve_batch(self._h, descs, n, outs)
+        if pb._h is not None:
+            rc = self._lib.hgx_batch_evolve(self._h, pb._h, outs)
+        else:
+            rc = self._lib.hgx_evolve_batch(self._h, descs, n, outs)
         t2 = _t.perf_counter()
         if rc != abi.HGX_OK:
             raise self._err(rc)

@@ -68,11 +68,11 @@ uint32_t Engine::take_slot() {
     return s;
 }
 
-uint64_t Engine::submit(RunHost&& h, uint64_t id) {
+uint64_t Engine::submit(std::shared_ptr<const RunHost> h, uint64_t id) {
     auto r = std::make_shared<RunRec>();
     r->id = id;
-    r->h = std::move(h);
-    r->footprint = footprint_bytes(r->h);
+    r->hp = std::move(h);
+    r->footprint = footprint_bytes(*r->hp);
     r->st = st_;
     r->t_submit = Clock::now();
     pending_.push_back(r);
@@ -98,8 +98,8 @@ std::shared_ptr<RunRec> Engine::take(uint64_t id) {
 void Engine::admit(std::shared_ptr<RunRec>& r) {
     const uint32_t slot = take_slot();
     r->slot = slot;
-    r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->h.rules.size());
-    uint64_t want = r->h.cap ? (uint64_t)r->h.cap + r->h.roots.size() : std::max<uint64_t>(512, r->h.roots.size());
+    r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->hp->rules.size());
+    uint64_t want = r->hp->cap ? (uint64_t)r->hp->cap + r->hp->roots.size() : std::max<uint64_t>(512, r->hp->roots.size());
     uint64_t mc = 64;
     while (mc < 2 * want) mc <<= 1;
     r->map_cap = (uint32_t)mc;
@@ -108,9 +108,9 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     DRun& d = h_runs_[slot];
     std::memset(&d, 0, sizeof(d));
     d.rules = r->d_rules;
-    d.n_rules = (uint32_t)r->h.rules.size();
-    d.n_roots = (uint32_t)r->h.roots.size();
-    d.flags = r->h.flags;
+    d.n_rules = (uint32_t)r->hp->rules.size();
+    d.n_roots = (uint32_t)r->hp->roots.size();
+    d.flags = r->hp->flags;
     d.map_keys = r->map_keys;
     d.map_vals = r->map_vals;
     d.map_mask = (uint32_t)(mc - 1);
@@ -134,13 +134,13 @@ void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
         c.events = r->events;
         c.canonical_states = x[CTR_CANON];
         c.fingerprint = x[CTR_FINGERPRINT];
-        const bool rec_causal = r->h.flags & HGX_REC_CAUSAL;
+        const bool rec_causal = r->hp->flags & HGX_REC_CAUSAL;
         c.causal_triples = rec_causal ? x[CTR_TRIPLES] : 0;
         c.causal_pairs = rec_causal ? x[CTR_PAIRS] : 0;
-        const bool tr = r->h.flags & HGX_OPT_TR;
+        const bool tr = r->hp->flags & HGX_OPT_TR;
         c.causal_pairs_tr = tr ? x[CTR_PAIRS_TR] : c.causal_pairs;
         c.causal_triples_tr = tr ? x[CTR_TRIPLES_TR] : c.causal_triples;
-        c.branchial_pairs = (r->h.flags & HGX_REC_BRANCHIAL) ? x[CTR_BRANCHIAL] : 0;
+        c.branchial_pairs = (r->hp->flags & HGX_REC_BRANCHIAL) ? x[CTR_BRANCHIAL] : 0;
         c.steps_done = r->steps_done;
         c.truncated = r->truncated ? 1u : 0u;
         r->t_done = now;
@@ -182,10 +182,10 @@ uint32_t Engine::step() {
             RunRec& r = *F_runs_[i];
             h_b[r.slot] = F_b_[i];
             h_e[r.slot] = F_e_[i];
-            if (r.done || r.depth >= r.h.steps) { r.done = true; continue; }
+            if (r.done || r.depth >= r.hp->steps) { r.done = true; continue; }
             h_active[r.slot] = 1;
-            h_budget[r.slot] = !r.h.cap ? 0xFFFFFFFFu : (r.states >= r.h.cap ? 0u : (uint32_t)(r.h.cap - r.states));
-            for (auto& R : r.h.rules) {
+            h_budget[r.slot] = !r.hp->cap ? 0xFFFFFFFFu : (r.states >= r.hp->cap ? 0u : (uint32_t)(r.hp->cap - r.states));
+            for (auto& R : r.hp->rules) {
                 KL = std::max<uint32_t>(KL, R.n_lhs);
                 small &= R.n_lhs <= 4 && (R.lhs_mask >> 16) == 0;
             }
@@ -236,13 +236,13 @@ uint32_t Engine::step() {
                 RunRec& r = *F_runs_[i];
                 if (!h_active[r.slot]) continue;
                 const uint64_t t = htot[r.slot];
-                if (r.h.cap && r.states + t > r.h.cap) {
+                if (r.hp->cap && r.states + t > r.hp->cap) {
                     r.truncated = true; r.done = true; h_active[r.slot] = 0; masked = true;
                 } else if (t == 0) {
                     r.done = true; h_active[r.slot] = 0;
                 } else {
                     tot[i] = t;
-                    for (auto& R : r.h.rules) K = std::max<uint32_t>(K, R.n_lhs);
+                    for (auto& R : r.hp->rules) K = std::max<uint32_t>(K, R.n_lhs);
                 }
             }
             if (masked) {
@@ -297,28 +297,40 @@ uint32_t Engine::step() {
         admit(r);
         admitted.push_back(r);
     }
-    uint32_t NR = 0;
-    uint64_t NER = 0;
-    for (auto& r : admitted) { NR += (uint32_t)r->h.roots.size(); NER += r->h.root_edges; }
+    // requests with host inputs first, then prepared ones (device-resident inputs)
+    std::stable_partition(admitted.begin(), admitted.end(),
+                          [](const std::shared_ptr<RunRec>& r) { return !r->hp->staged.rules; });
+    uint32_t NR = 0, NRh = 0;
+    uint64_t NER = 0, NERh = 0;
+    for (auto& r : admitted) {
+        NR += (uint32_t)r->hp->roots.size(); NER += r->hp->root_edges;
+        if (!r->hp->staged.rules) { NRh += (uint32_t)r->hp->roots.size(); NERh += r->hp->root_edges; }
+    }
     std::vector<DRule> a_rules;                 // staging; outlives the synchronisation below
     std::vector<AdmitItem> a_items;
     if (!admitted.empty()) {
+        size_t nh = 0;
+        for (auto& r : admitted) if (!r->hp->staged.rules) nh += r->hp->rules.size();
+        DRule* d_stage = nh ? dalloc<DRule>(tmp, nh) : nullptr;
         for (auto& r : admitted) {
             AdmitItem it{};
-            it.rule_src = (uint32_t)a_rules.size();
-            it.n_rules = (uint32_t)r->h.rules.size();
+            if (r->hp->staged.rules) {
+                it.rule_src = r->hp->staged.rules;
+            } else {
+                it.rule_src = d_stage + a_rules.size();
+                a_rules.insert(a_rules.end(), r->hp->rules.begin(), r->hp->rules.end());
+            }
+            it.n_rules = (uint32_t)r->hp->rules.size();
             it.rule_dst = r->d_rules;
             it.map_keys = r->map_keys;
             it.map_cap = r->map_cap;
             it.slot = r->slot;
             a_items.push_back(it);
-            a_rules.insert(a_rules.end(), r->h.rules.begin(), r->h.rules.end());
         }
-        DRule* d_stage = dalloc<DRule>(tmp, a_rules.size());
         AdmitItem* d_items = dalloc<AdmitItem>(tmp, a_items.size());
-        h2d(d_stage, a_rules.data(), a_rules.size(), st);
+        if (nh) h2d(d_stage, a_rules.data(), a_rules.size(), st);
         h2d(d_items, a_items.data(), a_items.size(), st);
-        HIP_TRY(launch_admit(d_items, (uint32_t)a_items.size(), d_stage, d_ctr_, st));
+        HIP_TRY(launch_admit(d_items, (uint32_t)a_items.size(), d_ctr_, st));
     }
 
     // ---- 3. the child generation: children of the frontier, then the new roots ----
@@ -330,6 +342,7 @@ uint32_t Engine::step() {
     std::vector<uint32_t> r_run, r_ne, r_nextv, r_sid, r_ev;
     std::vector<uint64_t> r_eoff;
     std::vector<uint8_t> r_depth, r_ar;
+    std::vector<RootCopy> r_copy;
     const uint32_t nC = (uint32_t)M + NR;
     if (nC) {
         C = std::make_shared<DevGen>(pool_);
@@ -338,24 +351,24 @@ uint32_t Engine::step() {
         for (uint32_t i = 0; i < nF; ++i) {
             if (!tot[i]) continue;
             const RunRec& r = *F_runs_[i];
-            A = std::max(A, r.h.max_arity);
-            lstride = std::max(lstride, r.h.steps);
-            keep |= (r.h.flags & HGX_REC_STATES) != 0;
+            A = std::max(A, r.hp->max_arity);
+            lstride = std::max(lstride, r.hp->steps);
+            keep |= (r.hp->flags & HGX_REC_STATES) != 0;
         }
         if (M) {
             uint32_t gain = 0, nw = 0;
             for (uint32_t i = 0; i < nF; ++i)
-                if (tot[i]) { gain = std::max(gain, F_runs_[i]->h.max_gain); nw = std::max(nw, F_runs_[i]->h.max_new); }
+                if (tot[i]) { gain = std::max(gain, F_runs_[i]->hp->max_gain); nw = std::max(nw, F_runs_[i]->hp->max_new); }
             max_ne = F_->max_ne + gain;
             max_nextv = F_->max_nextv + nw;
         }
         for (auto& r : admitted) {
-            A = std::max(A, r->h.max_arity);
-            lstride = std::max(lstride, r->h.steps);
-            keep |= (r->h.flags & HGX_REC_STATES) != 0;
-            for (size_t k = 0; k < r->h.roots.size(); ++k) {
-                max_ne = std::max<uint32_t>(max_ne, (uint32_t)r->h.roots[k].size());
-                max_nextv = std::max(max_nextv, r->h.root_nv[k]);
+            A = std::max(A, r->hp->max_arity);
+            lstride = std::max(lstride, r->hp->steps);
+            keep |= (r->hp->flags & HGX_REC_STATES) != 0;
+            for (size_t k = 0; k < r->hp->roots.size(); ++k) {
+                max_ne = std::max<uint32_t>(max_ne, (uint32_t)r->hp->roots[k].size());
+                max_nextv = std::max(max_nextv, r->hp->root_nv[k]);
             }
         }
         C->keep_edges = keep;
@@ -410,45 +423,69 @@ uint32_t Engine::step() {
             const uint32_t wpb_e = place_workspace(tmp, ea.lds_per_wave, (uint32_t)M, ctx_->max_lds,
                                                    ctx_->num_cu * 8, &ea.scratch);
             const uint64_t bytes = M * (8ull + 2ull * K) + parent_edges * (4ull * F_->v.A + 2) +
-                                   EC * (4ull * A + 2) + M * (28ull + 8ull * lstride) + M * 24ull + M * 12ull;
+                                   EC * (4ull * A + 2) + M * (28ull + 8ull * lstride) + M * 24ull;
             timed(ctx_, st, "k_expand", bytes, M, [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, st); });
+            timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
+                  [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });
         }
 
-        // new roots: uploaded, hashed, inserted into their maps
+        // new roots: uploaded (host inputs) or copied on the device (prepared inputs), then
+        // hashed and inserted into their maps
         if (NR) {
-            r_run.resize(NR); r_ne.resize(NR); r_nextv.resize(NR); r_sid.resize(NR);
-            r_eoff.resize(NR + 1); r_depth.assign(NR, 0); r_ar.resize(NER); r_ev.assign(NER * A, 0);
-            uint32_t s = 0;
-            uint64_t e = 0;
-            for (auto& r : admitted) {
-                for (size_t k = 0; k < r->h.roots.size(); ++k, ++s) {
-                    r_run[s] = r->slot;
-                    r_ne[s] = (uint32_t)r->h.roots[k].size();
-                    r_nextv[s] = r->h.root_nv[k];
-                    r_sid[s] = (uint32_t)k;
-                    r_eoff[s] = EC + e;
-                    for (auto& ed : r->h.roots[k]) {
-                        r_ar[e] = (uint8_t)ed.size();
-                        for (size_t x = 0; x < ed.size(); ++x) r_ev[e * A + x] = ed[x];
-                        ++e;
+            if (NRh) {
+                r_run.resize(NRh); r_ne.resize(NRh); r_nextv.resize(NRh); r_sid.resize(NRh);
+                r_eoff.resize(NRh + 1); r_depth.assign(NRh, 0); r_ar.resize(NERh); r_ev.assign(NERh * A, 0);
+                uint32_t s = 0;
+                uint64_t e = 0;
+                for (auto& r : admitted) {
+                    if (r->hp->staged.rules) continue;
+                    for (size_t k = 0; k < r->hp->roots.size(); ++k, ++s) {
+                        r_run[s] = r->slot;
+                        r_ne[s] = (uint32_t)r->hp->roots[k].size();
+                        r_nextv[s] = r->hp->root_nv[k];
+                        r_sid[s] = (uint32_t)k;
+                        r_eoff[s] = EC + e;
+                        for (auto& ed : r->hp->roots[k]) {
+                            r_ar[e] = (uint8_t)ed.size();
+                            for (size_t x = 0; x < ed.size(); ++x) r_ev[e * A + x] = ed[x];
+                            ++e;
+                        }
                     }
                 }
+                r_eoff[NRh] = EC + e;
+                h2d(v.run + M, r_run.data(), NRh, st);
+                h2d(v.ne + M, r_ne.data(), NRh, st);
+                h2d(v.nextv + M, r_nextv.data(), NRh, st);
+                h2d(v.sid + M, r_sid.data(), NRh, st);
+                h2d(v.depth + M, r_depth.data(), NRh, st);
+                h2d(v.eoff + M, r_eoff.data(), NRh + 1, st);
+                if (NERh) {
+                    h2d(v.ar + EC, r_ar.data(), NERh, st);
+                    h2d(v.ev + EC * A, r_ev.data(), NERh * A, st);
+                    HIP_TRY(hipMemsetAsync(v.ps + EC, 0, NERh, st));
+                }
             }
-            r_eoff[NR] = EC + e;
-            h2d(v.run + M, r_run.data(), NR, st);
-            h2d(v.ne + M, r_ne.data(), NR, st);
-            h2d(v.nextv + M, r_nextv.data(), NR, st);
-            h2d(v.sid + M, r_sid.data(), NR, st);
-            h2d(v.depth + M, r_depth.data(), NR, st);
-            h2d(v.eoff + M, r_eoff.data(), NR + 1, st);
-            h2d(v.ar + EC, r_ar.data(), NER, st);
-            h2d(v.ev + EC * A, r_ev.data(), NER * A, st);
-            if (NER) HIP_TRY(hipMemsetAsync(v.ps + EC, 0, NER, st));
+            if (NR > NRh) {
+                uint32_t s = (uint32_t)M + NRh;
+                uint64_t e = EC + NERh;
+                for (auto& r : admitted) {
+                    const StagedSrc& g = r->hp->staged;
+                    if (!g.rules) continue;
+                    RootCopy c{g.ne, g.nextv, g.eoff, g.ar, g.ev, g.A, (uint32_t)r->hp->roots.size(),
+                               r->slot, s, e};
+                    r_copy.push_back(c);
+                    s += (uint32_t)r->hp->roots.size();
+                    e += r->hp->root_edges;
+                }
+                RootCopy* d_copy = dalloc<RootCopy>(tmp, r_copy.size());
+                h2d(d_copy, r_copy.data(), r_copy.size(), st);
+                HIP_TRY(launch_copy_roots(d_copy, (uint32_t)r_copy.size(), v, st));
+            }
             uint32_t rmax_ne = 1, rmax_nv = 1;
             for (auto& r : admitted)
-                for (size_t k = 0; k < r->h.roots.size(); ++k) {
-                    rmax_ne = std::max<uint32_t>(rmax_ne, (uint32_t)r->h.roots[k].size());
-                    rmax_nv = std::max(rmax_nv, r->h.root_nv[k]);
+                for (size_t k = 0; k < r->hp->roots.size(); ++k) {
+                    rmax_ne = std::max<uint32_t>(rmax_ne, (uint32_t)r->hp->roots[k].size());
+                    rmax_nv = std::max(rmax_nv, r->hp->root_nv[k]);
                 }
             WLStatesArgs w{};
             w.n = NR; w.A = A; w.state_off = v.eoff + M; w.ar = v.ar; w.ev = v.ev; w.nv = v.nextv + M;
@@ -469,10 +506,10 @@ uint32_t Engine::step() {
             for (uint32_t i = 0; i < nF; ++i) {
                 if (!tot[i]) continue;
                 const RunRec& r = *F_runs_[i];
-                const bool b = (r.h.flags & HGX_REC_BRANCHIAL) != 0;
+                const bool b = (r.hp->flags & HGX_REC_BRANCHIAL) != 0;
                 bflag[r.slot] = b ? 1 : 0;
                 any_b |= b; all_b &= b;
-                any_rel |= b && (r.h.flags & HGX_REC_RELATIONS);
+                any_rel |= b && (r.hp->flags & HGX_REC_RELATIONS);
             }
             if (any_b) {
                 uint32_t* bcnt = cnt;
@@ -522,17 +559,17 @@ uint32_t Engine::step() {
             r->events += tot[i];
             r->depth += 1;
             r->steps_done = r->depth;
-            if (r->depth >= r->h.steps) r->done = true;
+            if (r->depth >= r->hp->steps) r->done = true;
             next_runs.push_back(r);
             next_b.push_back(cb[i]);
             next_e.push_back(ce[i]);
         }
         uint32_t s = (uint32_t)M;
         for (auto& r : admitted) {
-            const uint32_t nr = (uint32_t)r->h.roots.size();
+            const uint32_t nr = (uint32_t)r->hp->roots.size();
             r->segs.push_back(Segment{C, s, s + nr, 0, 0});
             r->states = nr;
-            if (r->h.steps == 0) r->done = true;
+            if (r->hp->steps == 0) r->done = true;
             next_runs.push_back(r);
             next_b.push_back(s);
             next_e.push_back(s + nr);
@@ -599,7 +636,7 @@ struct RunTables {
 static RunTables run_tables(const RunRec& R) {
     RunTables T;
     const uint64_t ns = R.states;
-    const uint32_t nroots = (uint32_t)R.h.roots.size();
+    const uint32_t nroots = (uint32_t)R.hp->roots.size();
     const uint64_t ne = ns >= nroots ? ns - nroots : 0;
     T.hash.resize(ns); T.step.resize(ns); T.creator.assign(ns, 0xFFFFFFFFu);
     T.ev_in.resize(ne); T.ev_rule.resize(ne); T.ev_step.resize(ne); T.ev_prod.resize(ne); T.ev_tr.resize(ne);
@@ -665,12 +702,112 @@ extern "C" int hgx_evolve_batch(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t
         Engine E(ctx);
         E.soft_budget_frac = 0.8;
         E.reserve_slots(n + 1);
-        for (uint32_t i = 0; i < n; ++i) E.submit(std::move(hs[i]), i + 1);
+        for (uint32_t i = 0; i < n; ++i) E.submit(std::make_shared<const RunHost>(std::move(hs[i])), i + 1);
         while (E.live_or_queued()) E.step();
         for (uint32_t i = 0; i < n; ++i) results[i] = make_result(E.take(i + 1));
         if (trace_on()) std::fprintf(stderr, "[hgx] batch: total %.3f ms\n", ms_since(t0));
         return HGX_OK;
     });
+}
+
+// ---- prepared batches: inputs parsed and staged in device memory once --------------------
+
+struct hgx_batch {
+    hgx_ctx* ctx = nullptr;
+    std::vector<std::shared_ptr<const RunHost>> runs;
+    std::unique_ptr<Arena> mem;                // device copies of rules and initial states
+};
+
+extern "C" int hgx_batch_prepare(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_batch** out) {
+    if (!ctx || (!runs && n) || !out) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        std::vector<RunHost> hs;
+        hs.reserve(n);
+        for (uint32_t i = 0; i < n; ++i) hs.push_back(parse_run(&runs[i]));
+        uint64_t nrules = 0, nroots = 0, nedges = 0;
+        uint32_t A = 1;
+        for (auto& h : hs) {
+            nrules += h.rules.size(); nroots += h.roots.size(); nedges += h.root_edges;
+            for (auto& st : h.roots) for (auto& e : st) A = std::max<uint32_t>(A, (uint32_t)e.size());
+        }
+        std::vector<DRule> rules;
+        rules.reserve(nrules);
+        std::vector<uint32_t> ne, nv, ev(std::max<uint64_t>(1, nedges * A), 0);
+        std::vector<uint64_t> eoff;
+        std::vector<uint8_t> ar(std::max<uint64_t>(1, nedges), 0);
+        std::vector<uint64_t> rule_b, root_b;
+        uint64_t e = 0;
+        for (auto& h : hs) {
+            rule_b.push_back(rules.size());
+            root_b.push_back(ne.size());
+            rules.insert(rules.end(), h.rules.begin(), h.rules.end());
+            for (size_t k = 0; k < h.roots.size(); ++k) {
+                ne.push_back((uint32_t)h.roots[k].size());
+                nv.push_back(h.root_nv[k]);
+                eoff.push_back(e);
+                for (auto& ed : h.roots[k]) {
+                    ar[e] = (uint8_t)ed.size();
+                    for (size_t x = 0; x < ed.size(); ++x) ev[e * A + x] = ed[x];
+                    ++e;
+                }
+            }
+        }
+        eoff.push_back(e);
+        auto b = std::make_unique<hgx_batch>();
+        b->ctx = ctx;
+        b->mem = std::make_unique<Arena>(ctx->pool.get(), (size_t)1 << 20);
+        const hipStream_t st = ctx->stream;
+        DRule* d_rules = dalloc<DRule>(*b->mem, std::max<uint64_t>(1, nrules));
+        uint32_t* d_ne = dalloc<uint32_t>(*b->mem, std::max<uint64_t>(1, nroots));
+        uint32_t* d_nv = dalloc<uint32_t>(*b->mem, std::max<uint64_t>(1, nroots));
+        uint64_t* d_eoff = dalloc<uint64_t>(*b->mem, nroots + 1);
+        uint8_t* d_ar = dalloc<uint8_t>(*b->mem, ar.size());
+        uint32_t* d_ev = dalloc<uint32_t>(*b->mem, ev.size());
+        if (nrules) h2d(d_rules, rules.data(), nrules, st);
+        if (nroots) { h2d(d_ne, ne.data(), nroots, st); h2d(d_nv, nv.data(), nroots, st); }
+        h2d(d_eoff, eoff.data(), nroots + 1, st);
+        h2d(d_ar, ar.data(), ar.size(), st);
+        h2d(d_ev, ev.data(), ev.size(), st);
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t i = 0; i < n; ++i) {
+            StagedSrc& sg = hs[i].staged;
+            sg.rules = d_rules + rule_b[i];
+            sg.ne = d_ne + root_b[i];
+            sg.nextv = d_nv + root_b[i];
+            sg.eoff = d_eoff + root_b[i];
+            sg.ar = d_ar;
+            sg.ev = d_ev;
+            sg.A = A;
+            b->runs.push_back(std::make_shared<const RunHost>(std::move(hs[i])));
+        }
+        *out = b.release();
+        return HGX_OK;
+    });
+}
+
+extern "C" int hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* b, hgx_result** results) {
+    if (!ctx || !b || b->ctx != ctx || (!results && !b->runs.empty())) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const uint32_t n = (uint32_t)b->runs.size();
+        Engine E(ctx);
+        E.soft_budget_frac = 0.8;
+        E.reserve_slots(n + 1);
+        for (uint32_t i = 0; i < n; ++i) E.submit(b->runs[i], i + 1);
+        while (E.live_or_queued()) E.step();
+        for (uint32_t i = 0; i < n; ++i) results[i] = make_result(E.take(i + 1));
+        return HGX_OK;
+    });
+}
+
+extern "C" void hgx_batch_release(hgx_batch* b) {
+    if (!b) return;
+    hgx_ctx* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    delete b;
 }
 
 extern "C" int hgx_request_submit(hgx_ctx* ctx, const hgx_run_desc* run, uint64_t* request_id) {
@@ -680,7 +817,7 @@ extern "C" int hgx_request_submit(hgx_ctx* ctx, const hgx_run_desc* run, uint64_
         RunHost h = parse_run(run);
         if (!ctx->serve) { HIP_TRY(hipSetDevice(ctx->device)); ctx->serve = std::make_unique<Engine>(ctx); }
         const uint64_t id = ctx->next_request_id++;
-        ctx->serve->submit(std::move(h), id);
+        ctx->serve->submit(std::make_shared<const RunHost>(std::move(h)), id);
         *request_id = id;
         return HGX_OK;
     });
@@ -792,7 +929,7 @@ extern "C" int hgx_result_events(const hgx_result* r, hgx_event* out, uint64_t c
     if (!r || (!out && r->run->events) || cap < r->run->events) return HGX_ERR_INVALID;
     try {
         RunTables T = run_tables(*r->run);
-        const uint32_t nroots = (uint32_t)r->run->h.roots.size();
+        const uint32_t nroots = (uint32_t)r->run->hp->roots.size();
         for (uint64_t e = 0; e < r->run->events; ++e)
             out[e] = hgx_event{T.ev_in[e], (uint32_t)(e + nroots), T.ev_rule[e], T.ev_step[e]};
         return HGX_OK;
@@ -804,7 +941,7 @@ extern "C" int hgx_result_events(const hgx_result* r, hgx_event* out, uint64_t c
 extern "C" int hgx_result_causal(const hgx_result* r, hgx_pair* pairs, uint8_t* in_tr, uint64_t cap) {
     if (!r) return HGX_ERR_INVALID;
     const RunRec& R = *r->run;
-    if (!(R.h.flags & HGX_REC_CAUSAL) || cap < R.counts.causal_pairs) return HGX_ERR_INVALID;
+    if (!(R.hp->flags & HGX_REC_CAUSAL) || cap < R.counts.causal_pairs) return HGX_ERR_INVALID;
     try {
         RunTables T = run_tables(R);
         uint64_t o = 0;
@@ -828,7 +965,7 @@ extern "C" int hgx_result_causal(const hgx_result* r, hgx_pair* pairs, uint8_t* 
 extern "C" int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64_t cap) {
     if (!r) return HGX_ERR_INVALID;
     const RunRec& R = *r->run;
-    if (!(R.h.flags & HGX_REC_BRANCHIAL) || !(R.h.flags & HGX_REC_RELATIONS)) return HGX_ERR_INVALID;
+    if (!(R.hp->flags & HGX_REC_BRANCHIAL) || !(R.hp->flags & HGX_REC_RELATIONS)) return HGX_ERR_INVALID;
     if (cap < R.counts.branchial_pairs) return HGX_ERR_INVALID;
     try {
         uint64_t o = 0;
@@ -852,7 +989,7 @@ extern "C" int hgx_result_state_edges(const hgx_result* r, uint64_t* state_offse
                                       uint32_t* vertices, uint64_t* n_edges_total, uint64_t* n_vertices_total) {
     if (!r) return HGX_ERR_INVALID;
     const RunRec& R = *r->run;
-    if (!(R.h.flags & HGX_REC_STATES)) return HGX_ERR_INVALID;
+    if (!(R.hp->flags & HGX_REC_STATES)) return HGX_ERR_INVALID;
     try {
         uint64_t ne_tot = 0, nv_tot = 0, s_idx = 0;
         for (const Segment& sg : R.segs) {

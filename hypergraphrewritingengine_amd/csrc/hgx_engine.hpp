@@ -45,7 +45,7 @@ using Clock = std::chrono::steady_clock;
 
 struct RunRec {
     uint64_t id = 0;
-    RunHost h;
+    std::shared_ptr<const RunHost> hp;   // parsed request (shared with a prepared batch)
     uint32_t slot = 0xFFFFFFFFu;
     DRule* d_rules = nullptr;
     uint64_t* map_keys = nullptr;
@@ -69,7 +69,7 @@ public:
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
-    uint64_t submit(RunHost&& h, uint64_t id);
+    uint64_t submit(std::shared_ptr<const RunHost> h, uint64_t id);
     void reserve_slots(uint32_t n) { grow_slots(n); }
     // One generation of every admitted request, plus admissions. Returns live + queued runs.
     uint32_t step();

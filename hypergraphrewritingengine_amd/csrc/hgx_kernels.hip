@@ -607,7 +607,6 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
             a.E.in_state[c] = a.P.sid[par];
             a.E.rule[c] = rid;
             acc[CTR_EVENTS] += 1;
-            if (map_insert(RN, h, child_sid)) { acc[CTR_CANON] += 1; acc[CTR_FINGERPRINT] += mix64(h); }
             // Causal: one triple per consumed edge with a producer; the producers are events of
             // this lineage, named by their generation (rewriter.cpp:121-191,
             // causal_graph.cpp:187-234). Transitive reduction: (P, C) is redundant exactly when
@@ -659,6 +658,27 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         wave_sync();
     }
     flush();
+}
+
+// State identity (Hypergraph::create_or_get_canonical_state, hypergraph/src/hypergraph.cpp:154-254):
+// every child's hash into its request's state map, one lane per child, after k_expand -- so no
+// wave of k_expand waits on a global compare-and-swap chain. A key inserted first counts a
+// canonical state and adds to the fingerprint.
+__global__ void k_dedup(GenView C, uint32_t n, DRun* runs, uint64_t* ctr) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t run = C.run[c];
+    const uint64_t h = C.hash[c];
+    if (map_insert(&runs[run], h, C.sid[c])) {
+        atomicAdd((unsigned long long*)&ctr[(size_t)run * CTR_N + CTR_CANON], 1ull);
+        atomicAdd((unsigned long long*)&ctr[(size_t)run * CTR_N + CTR_FINGERPRINT], (unsigned long long)mix64(h));
+    }
+}
+
+hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_dedup, dim3((n + 255) / 256), dim3(256), 0, st, C, n, runs, ctr);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -846,9 +866,9 @@ __global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
 }
 
 // Admissions of one step: one block per request.
-__global__ void k_admit(const AdmitItem* items, const DRule* staging, uint64_t* ctr) {
+__global__ void k_admit(const AdmitItem* items, uint64_t* ctr) {
     const AdmitItem it = items[blockIdx.x];
-    const uint32_t* src = (const uint32_t*)(staging + it.rule_src);
+    const uint32_t* src = (const uint32_t*)it.rule_src;
     uint32_t* dst = (uint32_t*)it.rule_dst;
     const uint32_t words = it.n_rules * (uint32_t)(sizeof(DRule) / 4);
     for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
@@ -856,9 +876,37 @@ __global__ void k_admit(const AdmitItem* items, const DRule* staging, uint64_t* 
     if (threadIdx.x < CTR_N) ctr[(size_t)it.slot * CTR_N + threadIdx.x] = 0;
 }
 
-hipError_t launch_admit(const AdmitItem* items, uint32_t n, const DRule* staging, uint64_t* ctr, hipStream_t st) {
+hipError_t launch_admit(const AdmitItem* items, uint32_t n, uint64_t* ctr, hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_admit, dim3(n), dim3(256), 0, st, items, staging, ctr);
+    hipLaunchKernelGGL(k_admit, dim3(n), dim3(256), 0, st, items, ctr);
+    return hipGetLastError();
+}
+
+// One block per prepared request: its roots' records and edges into the generation.
+__global__ void k_copy_roots(const RootCopy* items, GenView v) {
+    const RootCopy it = items[blockIdx.x];
+    const uint64_t e0 = it.eoff[0], e1 = it.eoff[it.n_roots];
+    for (uint32_t k = threadIdx.x; k <= it.n_roots; k += blockDim.x) {
+        v.eoff[it.dst_state + k] = it.dst_edge + (it.eoff[k] - e0);
+        if (k < it.n_roots) {
+            v.run[it.dst_state + k] = it.slot;
+            v.ne[it.dst_state + k] = it.ne[k];
+            v.nextv[it.dst_state + k] = it.nextv[k];
+            v.sid[it.dst_state + k] = k;
+            v.depth[it.dst_state + k] = 0;
+        }
+    }
+    for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const uint64_t d = it.dst_edge + (e - e0);
+        v.ar[d] = it.ar[e];
+        v.ps[d] = 0;
+        for (uint32_t x = 0; x < v.A; ++x) v.ev[d * v.A + x] = x < it.src_A ? it.ev[e * it.src_A + x] : 0u;
+    }
+}
+
+hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_roots, dim3(n), dim3(256), 0, st, items, v);
     return hipGetLastError();
 }
 

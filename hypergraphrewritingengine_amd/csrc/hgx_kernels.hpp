@@ -86,18 +86,30 @@ uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uin
 // One admitted request: its rules (copied from a packed staging array), its state map (zeroed)
 // and its counter row (zeroed) -- every admission of a step in one launch.
 struct AdmitItem {
-    uint32_t rule_src;         // first rule in the staging array
-    uint32_t n_rules;
+    const DRule* rule_src;     // its rules in device memory (staging array or prepared batch)
+    uint32_t n_rules, pad0;
     DRule* rule_dst;
     uint64_t* map_keys;
     uint64_t map_cap;
     uint32_t slot, pad;
 };
-hipError_t launch_admit(const AdmitItem* items, uint32_t n, const DRule* staging, uint64_t* ctr, hipStream_t st);
+hipError_t launch_admit(const AdmitItem* items, uint32_t n, uint64_t* ctr, hipStream_t st);
+// Initial states of prepared requests, copied from the batch's device arrays into a generation.
+struct RootCopy {
+    const uint32_t* ne;
+    const uint32_t* nextv;
+    const uint64_t* eoff;
+    const uint8_t* ar;
+    const uint32_t* ev;
+    uint32_t src_A, n_roots, slot, dst_state;
+    uint64_t dst_edge;
+};
+hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v, hipStream_t st);
 
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A);
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr, hipStream_t st);
 // grid used by the launchers for `items` work items (for sizing global scratch)
 uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap);
 uint32_t branchial_lds_per_wave();

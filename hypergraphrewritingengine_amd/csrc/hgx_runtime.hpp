@@ -107,7 +107,19 @@ void d2h(T* dst, const T* src, size_t n, hipStream_t st) {
 void scan_u32_to_u64(Arena& ar, const uint32_t* in, uint64_t* out, uint32_t n, hipStream_t st);
 
 // One parsed request.
+// Device copies of a prepared request's rules and initial states (hgx_batch_prepare).
+struct StagedSrc {
+    const DRule* rules = nullptr;      // null: the request's inputs are on the host only
+    const uint32_t* ne = nullptr;      // [roots] from this request's first root
+    const uint32_t* nextv = nullptr;
+    const uint64_t* eoff = nullptr;    // [roots + 1] offsets into the batch edge arrays
+    const uint8_t* ar = nullptr;       // the batch edge arrays (arity, vertices at stride A)
+    const uint32_t* ev = nullptr;
+    uint32_t A = 0;
+};
+
 struct RunHost {
+    StagedSrc staged;
     std::vector<DRule> rules;
     std::vector<std::vector<std::vector<uint32_t>>> roots;   // renumbered per root
     std::vector<uint32_t> root_nv;

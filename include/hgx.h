@@ -168,6 +168,16 @@ int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value);
  * depend on which other requests share the batch. */
 int hgx_evolve_batch(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_result** results);
 
+/* Prepared batches: the requests are parsed and their rules and initial states staged in
+ * device memory once (hgx_batch_prepare); hgx_batch_evolve then runs them from device-resident
+ * inputs, any number of times, with results identical to hgx_evolve_batch on the same
+ * requests. No counterpart in the reference: it is the batched path of the service with its
+ * inputs already on the device. A batch must be released before its context is destroyed. */
+typedef struct hgx_batch hgx_batch;
+int  hgx_batch_prepare(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_batch** out);
+int  hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* batch, hgx_result** results);
+void hgx_batch_release(hgx_batch* batch);
+
 /* ---- served path: continuous batching ------------------------------------------------- */
 
 /* Queue a request. It is admitted when its HBM footprint estimate fits what running

@@ -178,3 +178,26 @@ def test_served_results_equal_alone_and_in_the_mix(ctx):
     # wait() drives the loop by itself
     rid = ctx.submit(reqs[5])
     assert ctx.wait(rid).counts == got[5].counts
+
+
+def test_prepared_batch_equals_host_batch(ctx):
+    """hgx_batch_prepare stages rules and initial states in device memory once; evolving the
+    prepared batch (twice) gives what hgx_evolve_batch gives on the same requests."""
+    reqs = [W.batched_run(i, cap=1024, flags=REL | TR) for i in range(40)]
+    reqs += [W.config1(steps=4, flags=REL),
+             Request([rule_from_lists([[0, 1], [1, 2]], [[0, 2], [2, 3], [3, 1]])],
+                     [[[0, 1], [1, 2]], [[5, 5], [5, 6]], []], 3, flags=REL | TR, multi_root=True),
+             Request([rule_from_lists([[0, 1, 2]], [[0, 1], [1, 2, 3]])], [[1, 2, 3], [3, 4, 5]], 3,
+                     flags=REL)]
+    host = ctx.evolve_batch(reqs)
+    prep = ctx.prepare(reqs)
+    for _ in range(2):
+        dev = ctx.evolve_batch(prep)
+        for a, b in zip(dev, host):
+            assert a.counts == b.counts
+            for x, y in zip(a.states(), b.states()):
+                assert np.array_equal(x, y)
+            assert np.array_equal(a.events(), b.events())
+    for i in (0, 40, 41, 42):
+        assert_exact(dev[i], po.evolve(reqs[i]))
+    prep.close()
