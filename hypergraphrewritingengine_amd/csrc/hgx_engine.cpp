@@ -114,6 +114,12 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     d.map_keys = r->map_keys;
     d.map_vals = r->map_vals;
     d.map_mask = (uint32_t)(mc - 1);
+    r->ne_bound = 0;
+    r->nv_bound = 0;
+    for (size_t k = 0; k < r->hp->roots.size(); ++k) {
+        r->ne_bound = std::max<uint32_t>(r->ne_bound, (uint32_t)r->hp->roots[k].size());
+        r->nv_bound = std::max(r->nv_bound, r->hp->root_nv[k]);
+    }
     r->admitted = true;
     r->t_admit = Clock::now();
     live_[r->id] = r;
@@ -156,12 +162,23 @@ void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
     }
 }
 
+namespace {
+// Children of the requests whose child-edge bound falls in one size class.
+struct ExpandClass {
+    uint32_t ne = 0, nextv = 0;
+    uint64_t n = 0;
+    std::vector<uint32_t> rb, rpre{0};
+};
+constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
+}  // namespace
+
 uint32_t Engine::step() {
     const auto t_step = Clock::now();
     double t_count = 0, t_emit = 0, t_child = 0;
     hipStream_t st = st_;
     Arena tmp(pool_.get(), (size_t)4 << 20);
     auto keep = std::make_unique<Arena>(pool_.get(), (size_t)4 << 20);   // this step's match list
+    std::vector<ExpandClass> mcls_count, mcls_emit;   // host range lists of the join launches
     const uint32_t n = F_ ? F_->v.n : 0;
     const uint32_t nF = (uint32_t)F_runs_.size();
     std::vector<uint64_t> tot(nF, 0);
@@ -207,8 +224,9 @@ uint32_t Engine::step() {
             ma.P = F_->v; ma.runs = d_runs_; ma.active = d_active_; ma.cnt = cnt; ma.moff = moff;
             ma.max_ne = std::max(1u, F_->max_ne);
             ma.max_nextv = std::max(1u, F_->max_nextv);
-            // forwarding: on when asked, or (automatic) for every state the previous
-            // generation made
+            // forwarding: on when asked; automatic forwards the count pass (cheaper than a
+            // full join there) and emits by full join (the merge costs more than it saves on
+            // small states)
             const bool fwd = prev_.n > 0 && prev_.n <= n && ctx_->forward != 0;
             ma.fwd_n = fwd ? prev_.n : 0;
             ma.pm_parent = prev_.parent; ma.pm_rule = prev_.rule; ma.pm_q = prev_.q; ma.pm_K = prev_.K;
@@ -216,14 +234,49 @@ uint32_t Engine::step() {
             ma.KL = KL;
             ma.small = small ? 1u : 0u;
             ma.dmax = fwd ? std::min(ctx_->delta_cap, 16384u / (2u * KL + 8u)) : 0u;   // <= 16 KiB LDS
-            ma.lds_per_wave = (match_lds_per_wave(ma.max_ne, ma.max_nextv, F_->v.A, ma.KL, ma.dmax) + 15u) & ~15u;
             ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
             ma.budget = d_budget_;
             ma.run_acc = (unsigned long long*)d_run_acc_;
-            const uint32_t wpb_m = waves_per_block(ma.lds_per_wave, ctx_->max_lds);
             const uint64_t match_bytes = F_->n_edges * (4ull * F_->v.A + 1) + (uint64_t)n * 24;
-            timed(ctx_, st, "k_match_count", match_bytes + n * 4ull, n,
-                  [&] { return launch_match(ma, false, ctx_->num_cu * 8, wpb_m, st); });
+            // One launch per size class of the requests taking part (by their bound on state
+            // edges), each with LDS staging sized for its class; states outside the launches
+            // (requests not advancing) keep the zero count set here.
+            auto launch_classes = [&](bool emit, std::vector<ExpandClass>& cls, const char* name,
+                                      uint64_t bytes) {
+                cls.assign(4, ExpandClass{});
+                for (uint32_t i = 0; i < nF; ++i) {
+                    const RunRec& r = *F_runs_[i];
+                    if (emit ? !tot[i] : !h_active[r.slot]) continue;
+                    if (F_e_[i] == F_b_[i]) continue;
+                    int k = 0;
+                    while (r.ne_bound > kClassEdges[k]) ++k;
+                    ExpandClass& x = cls[k];
+                    x.ne = std::max(x.ne, r.ne_bound);
+                    x.nextv = std::max(x.nextv, r.nv_bound);
+                    x.rb.push_back(F_b_[i]);
+                    x.n += F_e_[i] - F_b_[i];
+                    x.rpre.push_back((uint32_t)x.n);
+                }
+                for (auto& x : cls) {
+                    if (!x.n) continue;
+                    MatchArgs mc = ma;
+                    mc.max_ne = std::max(1u, std::min(x.ne, F_->max_ne));
+                    mc.max_nextv = std::max(1u, std::min(x.nextv, F_->max_nextv));
+                    mc.lds_per_wave = (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax) + 15u) & ~15u;
+                    if (x.n != n) {
+                        uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
+                        h2d(d_r, x.rb.data(), x.rb.size(), st);
+                        h2d(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
+                        mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
+                        mc.n_items = (uint32_t)x.n;
+                    }
+                    const uint32_t wpb_m = waves_per_block(mc.lds_per_wave, ctx_->max_lds);
+                    timed(ctx_, st, name, (uint64_t)((double)bytes * x.n / n), x.n,
+                          [&] { return launch_match(mc, emit, ctx_->num_cu * 8, wpb_m, st); });
+                }
+            };
+            HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
+            launch_classes(false, mcls_count, "k_match_count", match_bytes + n * 4ull);
             scan_u32_to_u64(tmp, cnt, moff, n, st);
             HIP_TRY(launch_run_totals(moff, d_sbeg_, d_send_, cap_slots_, d_tot_, st));
             std::vector<uint64_t> htot(cap_slots_ + 1);
@@ -280,8 +333,11 @@ uint32_t Engine::step() {
                 HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
                 ma.M = mv;
                 ma.parent_edges = nullptr;
-                timed(ctx_, st, "k_match_emit", match_bytes + M * (12ull + 2ull * K), n,
-                      [&] { return launch_match(ma, true, ctx_->num_cu * 8, wpb_m, st); });
+                if (ma.fwd_n && ctx_->forward != 1) {
+                    ma.fwd_n = 0;
+                    ma.dmax = 0;
+                }
+                launch_classes(true, mcls_emit, "k_match_emit", match_bytes + M * (12ull + 2ull * K));
             }
         }
     }
@@ -343,6 +399,7 @@ uint32_t Engine::step() {
     std::vector<uint64_t> r_eoff;
     std::vector<uint8_t> r_depth, r_ar;
     std::vector<RootCopy> r_copy;
+    ExpandClass cls[4];                         // host range lists of the expand launches
     const uint32_t nC = (uint32_t)M + NR;
     if (nC) {
         C = std::make_shared<DevGen>(pool_);
@@ -414,17 +471,44 @@ uint32_t Engine::step() {
         h2d(d_runs_, h_runs_.data(), cap_slots_, st);
 
         if (M) {
-            ExpandArgs ea{};
-            ea.P = F_->v; ea.C = v; ea.M = mv; ea.E = C->ev; ea.runs = d_runs_; ea.ctr = d_ctr_;
-            ea.max_child_ne = std::max(1u, max_ne);
-            ea.max_nv = std::max(1u, std::min(max_nextv, max_ne * A));
-            ea.max_nextv = std::max(1u, max_nextv);
-            ea.lds_per_wave = expand_lds_per_wave(ea.max_child_ne, ea.max_nv, ea.max_nextv, A);
-            const uint32_t wpb_e = place_workspace(tmp, ea.lds_per_wave, (uint32_t)M, ctx_->max_lds,
-                                                   ctx_->num_cu * 8, &ea.scratch);
+            // Size classes: children grouped by their request's bound on child edges, each class
+            // launched with a workspace sized for it, so one large request in the mix does not
+            // push every child onto the large-state workspace.
+            for (uint32_t i = 0; i < nF; ++i) {
+                if (!tot[i]) continue;
+                const RunRec& r = *F_runs_[i];
+                const uint32_t cne = r.ne_bound + r.hp->max_gain, cnv = r.nv_bound + r.hp->max_new;
+                int k = 0;
+                while (cne > kClassEdges[k]) ++k;
+                ExpandClass& x = cls[k];
+                x.ne = std::max(x.ne, cne);
+                x.nextv = std::max(x.nextv, cnv);
+                x.rb.push_back(cb[i]);
+                x.n += ce[i] - cb[i];
+                x.rpre.push_back((uint32_t)x.n);
+            }
             const uint64_t bytes = M * (8ull + 2ull * K) + parent_edges * (4ull * F_->v.A + 2) +
                                    EC * (4ull * A + 2) + M * (28ull + 8ull * lstride) + M * 24ull;
-            timed(ctx_, st, "k_expand", bytes, M, [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, st); });
+            for (auto& x : cls) {
+                if (!x.n) continue;
+                ExpandArgs ea{};
+                ea.P = F_->v; ea.C = v; ea.M = mv; ea.E = C->ev; ea.runs = d_runs_; ea.ctr = d_ctr_;
+                ea.max_child_ne = std::max(1u, std::min(x.ne, max_ne));
+                ea.max_nextv = std::max(1u, std::min(x.nextv, max_nextv));
+                ea.max_nv = std::max(1u, std::min(ea.max_nextv, ea.max_child_ne * A));
+                ea.lds_per_wave = expand_lds_per_wave(ea.max_child_ne, ea.max_nv, ea.max_nextv, A);
+                ea.n_items = (uint32_t)x.n;
+                if (x.n != M) {
+                    uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
+                    h2d(d_r, x.rb.data(), x.rb.size(), st);
+                    h2d(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
+                    ea.rb = d_r; ea.rpre = d_r + x.rb.size(); ea.nrange = (uint32_t)x.rb.size();
+                }
+                const uint32_t wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds,
+                                                       ctx_->num_cu * 8, &ea.scratch);
+                timed(ctx_, st, "k_expand", (uint64_t)((double)bytes * x.n / M), x.n,
+                      [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, st); });
+            }
             timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });
         }
@@ -532,6 +616,34 @@ uint32_t Engine::step() {
                 ba.pcnt = pcnt;
                 timed(ctx_, st, "k_branchial", M * (6ull + 2ull * K) + n * 12ull, n,
                       [&] { return launch_branchial(ba, ctx_->num_cu * 8, 2, st); });
+                // parents above 64 edges: per-edge lists of consuming matches
+                BrListArgs bl{};
+                const bool large = F_->max_ne > 64;
+                if (large) {
+                    const uint64_t E = F_->n_edges;
+                    if (E >= 0xFFFFFFFFull || M * K >= 0xFFFFFFFFull)
+                        throw Error{HGX_ERR_UNSUPPORTED, "branchial lists above 2^32 entries"};
+                    bl.P = F_->v; bl.M = mv; bl.runs = d_runs_; bl.cnt = bcnt; bl.moff = moff;
+                    bl.ctr = d_ctr_; bl.pcnt = pcnt;
+                    bl.ecnt = dalloc<uint32_t>(tmp, E + 1);
+                    bl.ecur = dalloc<uint32_t>(tmp, E + 1);
+                    uint64_t* eoff = dalloc<uint64_t>(tmp, E + 1);
+                    bl.eoff = eoff;
+                    bl.elist = dalloc<uint32_t>(tmp, std::max<uint64_t>(1, M * K));
+                    bl.bc = dalloc<uint32_t>(tmp, M + 1);
+                    HIP_TRY(hipMemsetAsync(bl.ecnt, 0, sizeof(uint32_t) * (E + 1), st));
+                    HIP_TRY(hipMemsetAsync(bl.ecur, 0, sizeof(uint32_t) * (E + 1), st));
+                    HIP_TRY(hipMemsetAsync(bl.bc, 0, sizeof(uint32_t) * (M + 1), st));
+                    const uint64_t lbytes = M * (12ull + 2ull * K) * 4 + E * 20ull;
+                    timed(ctx_, st, "k_branchial_lists", lbytes, M, [&] {
+                        hipError_t e = launch_brl(bl, 0, E, st);
+                        if (e == hipSuccess) { scan_u32_to_u64(tmp, bl.ecnt, eoff, (uint32_t)E, st); e = hipGetLastError(); }
+                        if (e == hipSuccess) e = launch_brl(bl, 1, E, st);
+                        if (e == hipSuccess) e = launch_brl(bl, 2, E, st);
+                        if (e == hipSuccess) e = launch_brl(bl, 3, E, st);
+                        return e;
+                    });
+                }
                 if (any_rel) {
                     uint64_t* poff = dalloc<uint64_t>(tmp, n + 1);
                     scan_u32_to_u64(tmp, pcnt, poff, n, st);
@@ -543,6 +655,12 @@ uint32_t Engine::step() {
                         C->br_pairs = dalloc<uint32_t>(C->meta, 2 * NB);
                         ba.emit = 1; ba.poff = poff; ba.pairs = C->br_pairs;
                         HIP_TRY(launch_branchial(ba, ctx_->num_cu * 8, 2, st));
+                        if (large) {
+                            uint64_t* bo = dalloc<uint64_t>(tmp, M + 1);
+                            scan_u32_to_u64(tmp, bl.bc, bo, (uint32_t)M, st);
+                            bl.bo = bo; bl.poff = poff; bl.pairs = C->br_pairs;
+                            HIP_TRY(launch_brl(bl, 4, F_->n_edges, st));
+                        }
                     }
                 }
             }
@@ -559,6 +677,8 @@ uint32_t Engine::step() {
             r->events += tot[i];
             r->depth += 1;
             r->steps_done = r->depth;
+            r->ne_bound += r->hp->max_gain;
+            r->nv_bound += r->hp->max_new;
             if (r->depth >= r->hp->steps) r->done = true;
             next_runs.push_back(r);
             next_b.push_back(cb[i]);

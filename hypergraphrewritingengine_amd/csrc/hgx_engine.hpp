@@ -53,6 +53,7 @@ struct RunRec {
     uint32_t map_cap = 0;
     uint64_t states = 0, events = 0;
     uint32_t depth = 0, steps_done = 0;
+    uint32_t ne_bound = 0, nv_bound = 0;   // bounds on edges / vertex labels of its frontier states
     bool truncated = false, done = false, admitted = false;
     uint64_t footprint = 0;
     std::vector<Segment> segs;       // every state of the run, in state-id order

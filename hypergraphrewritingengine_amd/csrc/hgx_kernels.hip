@@ -244,7 +244,14 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     JoinLane L;
 
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
-    for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
+    const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
+    for (uint32_t ls = blockIdx.x * (blockDim.x / WAVE) + w; ls < n_items; ls += waves) {
+        uint32_t s = ls;
+        if (a.nrange) {                                         // subset: logical -> state
+            uint32_t lo = 0, hi = a.nrange;
+            while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= ls) lo = mid; else hi = mid; }
+            s = a.rb[lo] + (ls - a.rpre[lo]);
+        }
         const uint32_t run = a.P.run[s];
         if (!a.active[run]) { if (!EMIT && l == 0) a.cnt[s] = 0; continue; }
         const uint64_t budget = EMIT ? 0 : (uint64_t)a.budget[run];
@@ -503,8 +510,14 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
     // Contiguous chunk of children per wave; consecutive chunks on one XCD.
     const uint32_t wpb = blockDim.x / WAVE;
     const uint32_t gw = xcd_block() * wpb + w, nw = gridDim.x * wpb;
-    const uint32_t per = (a.M.n + nw - 1) / nw;
-    const uint32_t c0 = gw * per, c1 = min(a.M.n, c0 + per);
+    const uint32_t per = (a.n_items + nw - 1) / nw;
+    const uint32_t c0 = gw * per, c1 = min(a.n_items, c0 + per);
+    uint32_t rr = 0;                                       // range of logical index c (subsets)
+    if (a.nrange) {
+        uint32_t lo = 0, hi = a.nrange;                    // last range starting at or before c0
+        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= c0) lo = mid; else hi = mid; }
+        rr = lo;
+    }
 
     uint32_t acc_run = INV;
     uint64_t acc[CTR_N] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -516,7 +529,12 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         for (int k = 0; k < CTR_N; ++k) acc[k] = 0;
     };
 
-    for (uint32_t c = c0; c < c1; ++c) {
+    for (uint32_t lc = c0; lc < c1; ++lc) {
+        uint32_t c = lc;
+        if (a.nrange) {
+            while (lc >= a.rpre[rr + 1]) ++rr;
+            c = a.rb[rr] + (lc - a.rpre[rr]);
+        }
         const uint32_t par = a.M.parent[c];
         const uint32_t rid = a.M.rule[c];
         const uint32_t run = a.P.run[par];
@@ -531,6 +549,15 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         const uint32_t pdep = a.P.depth[par];
         const uint32_t step = pdep + 1;                       // generation of the child
         const uint16_t* q = a.M.q + (size_t)c * a.M.K;
+        // Causal inputs, fetched now so their latency hides under the rewrite and the hash:
+        // lane t < nl holds the producer generation of consumed edge t (0: none) and that
+        // producer's causal-ancestor mask from the parent's lineage table.
+        uint32_t ps_t = 0;
+        uint64_t anc_t = 0;
+        if (l < nl) {
+            ps_t = a.P.ps[peo + q[l]];
+            if (ps_t) anc_t = a.P.lanc[(size_t)par * a.P.lstride + ps_t - 1];
+        }
 
         // Parent edges minus the consumed ones, order kept (SparseBitset::derive).
         uint32_t kept = 0;
@@ -593,10 +620,37 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         wave_sync();
         const uint64_t h = wl_hash_wave(s);
 
-        // Identity and bookkeeping (lane 0): event id, map insert, causal relation.
+        // Causal: one triple per consumed edge with a producer; the producers are events of
+        // this lineage, named by their generation (rewriter.cpp:121-191,
+        // causal_graph.cpp:187-234). Transitive reduction: (P, C) is redundant exactly when P
+        // is a causal ancestor of another producer of C. Every lane computes the same masks
+        // from the nl lanes holding the consumed edges' producers.
+        uint64_t prod = 0, anc_c = 0, others = 0;
+        for (uint32_t u = 0; u < nl; ++u) {
+            const uint32_t pu = __shfl(ps_t, u, WAVE);
+            const uint64_t au = ((uint64_t)(uint32_t)__shfl((uint32_t)(anc_t >> 32), u, WAVE) << 32) |
+                                (uint32_t)__shfl((uint32_t)anc_t, u, WAVE);
+            if (pu) {
+                prod |= 1ull << pu;
+                anc_c |= (1ull << pu) | au;
+                if (pu != ps_t) others |= au;
+            }
+        }
+        const bool rec_causal = (RN->flags & 0x1u) != 0;
+        const bool drop = (RN->flags & 0x100u) && l < nl && ps_t && ((others >> ps_t) & 1ull);
+        const uint64_t dropped = __ballot(drop);
+        uint64_t tr = prod;
+        for (uint32_t u = 0; u < nl; ++u) {
+            const uint32_t pu = __shfl(ps_t, u, WAVE);
+            if ((dropped >> u) & 1ull) tr &= ~(1ull << pu);
+        }
+        const uint32_t triples = (uint32_t)__popcll(__ballot(l < nl && ps_t != 0));
+        const uint32_t triples_tr = (uint32_t)__popcll(__ballot(l < nl && ps_t != 0 && ((tr >> ps_t) & 1ull)));
+        if (!rec_causal) { prod = 0; tr = 0; anc_c = 0; }
+
+        // Identity and bookkeeping (lane 0): the child's record and its event.
         const uint32_t ev_id = RN->ev_base + (c - RN->child_begin);
         const uint32_t child_sid = RN->n_roots + ev_id;
-        uint64_t anc_c = 0;
         if (l == 0) {
             a.C.run[c] = run;
             a.C.ne[c] = cne;
@@ -607,49 +661,16 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
             a.E.in_state[c] = a.P.sid[par];
             a.E.rule[c] = rid;
             acc[CTR_EVENTS] += 1;
-            // Causal: one triple per consumed edge with a producer; the producers are events of
-            // this lineage, named by their generation (rewriter.cpp:121-191,
-            // causal_graph.cpp:187-234). Transitive reduction: (P, C) is redundant exactly when
-            // P is a causal ancestor of another producer of C.
-            uint64_t prod = 0;
-            uint32_t triples = 0;
-            uint8_t psq[16];
-            for (uint32_t t = 0; t < nl; ++t) {
-                psq[t] = a.P.ps[peo + q[t]];
-                if (psq[t]) { ++triples; prod |= 1ull << psq[t]; }
-            }
-            const uint64_t* plan = a.P.lanc + (size_t)par * a.P.lstride;   // generations 1..pdep
-            uint64_t tr = prod;
-            if (RN->flags & 0x1u) {
-                for (uint64_t m = prod; m; m &= m - 1) {
-                    const uint32_t sp = __builtin_ctzll(m);
-                    anc_c |= (1ull << sp) | plan[sp - 1];
-                }
-                if (RN->flags & 0x100u) {
-                    for (uint64_t m = prod; m; m &= m - 1) {
-                        const uint32_t sp = __builtin_ctzll(m);
-                        uint64_t others = 0;
-                        for (uint64_t m2 = prod; m2; m2 &= m2 - 1) {
-                            const uint32_t sy = __builtin_ctzll(m2);
-                            if (sy != sp) others |= plan[sy - 1];
-                        }
-                        if (others & (1ull << sp)) tr &= ~(1ull << sp);
-                    }
-                }
-                uint32_t triples_tr = 0;
-                for (uint32_t t = 0; t < nl; ++t) triples_tr += (psq[t] && ((tr >> psq[t]) & 1ull)) ? 1u : 0u;
+            if (rec_causal) {
                 acc[CTR_TRIPLES] += triples;
                 acc[CTR_PAIRS] += __popcll(prod);
                 acc[CTR_PAIRS_TR] += __popcll(tr);
                 acc[CTR_TRIPLES_TR] += triples_tr;
-            } else {
-                prod = 0; tr = 0;
             }
             a.E.prod[c] = prod;
             a.E.tr[c] = tr;
         }
         // Lineage table of the child: the parent's, then this event's ancestor mask.
-        anc_c = __shfl(anc_c, 0, WAVE);
         {
             const uint64_t* plan = a.P.lanc + (size_t)par * a.P.lstride;
             uint64_t* clan = a.C.lanc + (size_t)c * a.C.lstride;
@@ -658,6 +679,112 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         wave_sync();
     }
     flush();
+}
+
+// ------------------------------------------------------------------------------------------
+// Branchial pairs of parents above 64 edges. Every consumed edge slot (the parent's edge in
+// the generation's edge pool) gets the ascending list of the matches consuming it; match i
+// meets each partner j > i through the list of the first of its own edges that j also
+// consumes. The work is the sum over edges of (list length)^2, not (matches per parent)^2.
+// One thread per match throughout.
+// ------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ bool brl_large(const BrListArgs& a, uint32_t m, uint32_t& par) {
+    par = a.M.parent[m];
+    return a.P.ne[par] > 64 && a.cnt[par] >= 2;
+}
+
+__global__ void k_brl_count(BrListArgs a) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t par;
+    if (m >= a.M.n || !brl_large(a, m, par)) return;
+    const uint32_t nl = a.runs[a.P.run[par]].rules[a.M.rule[m]].n_lhs;
+    const uint16_t* q = a.M.q + (size_t)m * a.M.K;
+    for (uint32_t k = 0; k < nl; ++k) atomicAdd(&a.ecnt[a.P.eoff[par] + q[k]], 1u);
+}
+
+__global__ void k_brl_scatter(BrListArgs a) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t par;
+    if (m >= a.M.n || !brl_large(a, m, par)) return;
+    const uint32_t nl = a.runs[a.P.run[par]].rules[a.M.rule[m]].n_lhs;
+    const uint16_t* q = a.M.q + (size_t)m * a.M.K;
+    for (uint32_t k = 0; k < nl; ++k) {
+        const uint64_t e = a.P.eoff[par] + q[k];
+        a.elist[a.eoff[e] + atomicAdd(&a.ecur[e], 1u)] = m;
+    }
+}
+
+__global__ void k_brl_sort(BrListArgs a, uint64_t nslots) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nslots) return;
+    const uint64_t b = a.eoff[e], n = a.eoff[e + 1] - b;
+    uint32_t* L = a.elist + b;
+    for (uint64_t i = 1; i < n; ++i) {
+        const uint32_t key = L[i];
+        uint64_t j = i;
+        while (j > 0 && L[j - 1] > key) { L[j] = L[j - 1]; --j; }
+        L[j] = key;
+    }
+}
+
+template <bool EMIT>
+__global__ void k_brl_pairs(BrListArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t par;
+    if (i >= a.M.n || !brl_large(a, i, par)) return;
+    const uint32_t run = a.P.run[par];
+    const DRun& RN = a.runs[run];
+    const uint32_t nl = RN.rules[a.M.rule[i]].n_lhs;
+    const uint16_t* qi = a.M.q + (size_t)i * a.M.K;
+    const uint64_t e0 = a.P.eoff[par];
+    uint32_t c = 0;
+    uint64_t o = 0;
+    if (EMIT) {
+        const uint32_t mb = (uint32_t)a.moff[par];
+        o = a.poff[par] + (a.bo[i] - a.bo[mb]);
+    }
+    for (uint32_t k = 0; k < nl; ++k) {
+        const uint64_t e = e0 + qi[k];
+        for (uint64_t x = a.eoff[e]; x < a.eoff[e + 1]; ++x) {
+            const uint32_t j = a.elist[x];
+            if (j <= i) continue;
+            // count j here only if this is the first of i's edges that j consumes
+            const uint16_t* qj = a.M.q + (size_t)j * a.M.K;
+            const uint32_t nj = RN.rules[a.M.rule[j]].n_lhs;
+            bool earlier = false;
+            for (uint32_t k2 = 0; k2 < k && !earlier; ++k2)
+                for (uint32_t y = 0; y < nj; ++y) earlier |= (qj[y] == qi[k2]);
+            if (earlier) continue;
+            if (EMIT) {
+                a.pairs[2 * o] = RN.ev_base + (i - RN.child_begin);
+                a.pairs[2 * o + 1] = RN.ev_base + (j - RN.child_begin);
+                ++o;
+            }
+            ++c;
+        }
+    }
+    if (!EMIT) {
+        a.bc[i] = c;
+        if (c) {
+            atomicAdd((unsigned long long*)&a.ctr[(size_t)run * CTR_N + CTR_BRANCHIAL], (unsigned long long)c);
+            if (a.pcnt) atomicAdd(&a.pcnt[par], c);
+        }
+    }
+}
+
+hipError_t launch_brl(const BrListArgs& a, int phase, uint64_t nslots, hipStream_t st) {
+    const uint32_t n = a.M.n;
+    if (!n) return hipSuccess;
+    const dim3 g((n + 255) / 256), b(256);
+    switch (phase) {
+        case 0: hipLaunchKernelGGL(k_brl_count, g, b, 0, st, a); break;
+        case 1: hipLaunchKernelGGL(k_brl_scatter, g, b, 0, st, a); break;
+        case 2: hipLaunchKernelGGL(k_brl_sort, dim3((uint32_t)((nslots + 255) / 256)), b, 0, st, a, nslots); break;
+        case 3: hipLaunchKernelGGL(k_brl_pairs<false>, g, b, 0, st, a); break;
+        default: hipLaunchKernelGGL(k_brl_pairs<true>, g, b, 0, st, a); break;
+    }
+    return hipGetLastError();
 }
 
 // State identity (Hypergraph::create_or_get_canonical_state, hypergraph/src/hypergraph.cpp:154-254):
@@ -688,8 +815,8 @@ hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr,
 // Parents of at most 64 edges: the parent's matches are taken in chunks of BR_CHUNK; for each
 // chunk an LDS bitset per edge position marks the chunk's matches that consume it. Match i
 // then meets every later match j of the chunk through the OR of the bitsets of its own edges:
-// one popcount per 64 candidates instead of one test per pair. Larger parents compare
-// consumed-position lists pairwise.
+// one popcount per 64 candidates instead of one test per pair. Larger parents take the
+// per-edge match lists above (k_brl_*).
 // ------------------------------------------------------------------------------------------
 
 constexpr uint32_t BR_WORDS = 16;                  // 64-bit words per edge bitset
@@ -715,7 +842,7 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
     for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
         const uint32_t cnt = a.cnt[s];
-        if (cnt < 2) { if (a.emit == 0 && a.pcnt && l == 0) a.pcnt[s] = 0; continue; }
+        if (cnt < 2 || a.P.ne[s] > 64) { if (a.emit == 0 && a.pcnt && l == 0) a.pcnt[s] = 0; continue; }
         const uint32_t run = a.P.run[s];
         const uint32_t m0 = (uint32_t)a.moff[s];
         const uint32_t ne = a.P.ne[s];
@@ -723,7 +850,7 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
         const uint32_t ev0 = RN.ev_base + (m0 - RN.child_begin);
         uint64_t out = a.emit ? a.poff[s] : 0;
         uint32_t total = 0;
-        if (ne <= 64) {
+        {
             const uint32_t ncache = cnt < BR_MASKS ? cnt : BR_MASKS;
             for (uint32_t i = l; i < ncache; i += WAVE) mcache[i] = consumed_mask(a, RN, m0 + i);
             wave_sync();
@@ -780,32 +907,6 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
                     }
                 }
                 wave_sync();
-            }
-        } else {
-            auto overlap = [&](uint32_t i, uint32_t j) -> bool {
-                const uint16_t* qi = a.M.q + (size_t)(m0 + i) * a.M.K;
-                const uint16_t* qj = a.M.q + (size_t)(m0 + j) * a.M.K;
-                const uint32_t ni = RN.rules[a.M.rule[m0 + i]].n_lhs, nj = RN.rules[a.M.rule[m0 + j]].n_lhs;
-                for (uint32_t x = 0; x < ni; ++x)
-                    for (uint32_t y = 0; y < nj; ++y)
-                        if (qi[x] == qj[y]) return true;
-                return false;
-            };
-            for (uint32_t base = 0; base < cnt; base += WAVE) {
-                const uint32_t i = base + l;
-                uint32_t c = 0;
-                if (i < cnt)
-                    for (uint32_t j = i + 1; j < cnt; ++j) c += overlap(i, j) ? 1u : 0u;
-                if (a.emit) {
-                    const uint32_t inc = wave_incl_scan_u32(c);
-                    uint64_t o = out + inc - c;
-                    if (c)
-                        for (uint32_t j = i + 1; j < cnt; ++j)
-                            if (overlap(i, j)) { a.pairs[2 * o] = ev0 + i; a.pairs[2 * o + 1] = ev0 + j; ++o; }
-                    out += __shfl(inc, WAVE - 1, WAVE);
-                } else {
-                    total += c;
-                }
             }
         }
         if (!a.emit) {
@@ -953,7 +1054,9 @@ static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
 }
 
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
-    const uint32_t grid = grid_for(a.P.n, wpb, grid_cap);
+    const uint32_t items = a.nrange ? a.n_items : a.P.n;
+    if (!items) return hipSuccess;
+    const uint32_t grid = grid_for(items, wpb, grid_cap);
     const size_t lds = (size_t)a.lds_per_wave * wpb;
     if (a.small) {
         if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneSmall>), dim3(grid), dim3(64 * wpb), lds, st, a);
@@ -971,7 +1074,8 @@ uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t ma
 }
 
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
-    uint32_t grid = grid_for(a.M.n, wpb, grid_cap);
+    if (!a.n_items) return hipSuccess;
+    uint32_t grid = grid_for(a.n_items, wpb, grid_cap);
     const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
     if (a.scratch) hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
     else hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(64 * wpb), lds, st, a);

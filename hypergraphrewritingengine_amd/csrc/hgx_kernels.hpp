@@ -34,6 +34,11 @@ struct MatchArgs {
     uint32_t KL;               // largest LHS of the frontier's active runs
     uint32_t dmax;             // delta matches a state may stage (else it takes the full join)
     uint32_t small;            // every active rule has <= 4 LHS edges and variables < 16
+    // Optional subset of P's states, as ExpandArgs' ranges (nrange = 0: all of P).
+    const uint32_t* rb;
+    const uint32_t* rpre;
+    uint32_t nrange;
+    uint32_t n_items;
 };
 
 struct ExpandArgs {
@@ -49,6 +54,12 @@ struct ExpandArgs {
     uint32_t lds_per_wave;
     uint8_t* scratch;          // when set: per-wave workspace in global memory instead of LDS
                                //   (states too large for LDS), lds_per_wave bytes per wave
+    // Optional subset of the children: ranges [rb[r], rb[r] + rpre[r+1] - rpre[r]) for
+    // r < nrange (rpre = exclusive prefix of the lengths). nrange = 0: all of M.
+    const uint32_t* rb;
+    const uint32_t* rpre;
+    uint32_t nrange;
+    uint32_t n_items;          // children this launch covers
 };
 
 struct BranchialArgs {
@@ -64,6 +75,27 @@ struct BranchialArgs {
     uint32_t* pairs;           // [2 * total] event ids within the run
     uint32_t lds_per_wave;     // branchial_lds_per_wave()
 };
+
+// Branchial pairs of parents above 64 edges (per-edge lists of consuming matches).
+struct BrListArgs {
+    GenView P;
+    MatchView M;
+    const DRun* runs;
+    const uint32_t* cnt;       // [P.n] matches per parent (zero for runs without branchial)
+    const uint64_t* moff;      // [P.n + 1]
+    uint32_t* ecnt;            // [edge slots] list lengths
+    const uint64_t* eoff;      // [edge slots + 1] list offsets (scan of ecnt)
+    uint32_t* ecur;            // [edge slots] scatter cursors (zeroed)
+    uint32_t* elist;           // [entries] consuming matches, ascending per slot after the sort
+    uint32_t* bc;              // [M.n] pairs found per match (count phase)
+    const uint64_t* bo;        // [M.n + 1] exclusive scan of bc (emit phase)
+    uint64_t* ctr;
+    uint32_t* pcnt;            // [P.n] pairs per parent, added to (may be null)
+    const uint64_t* poff;      // [P.n + 1] (emit)
+    uint32_t* pairs;           // (emit)
+};
+// phase 0 count, 1 scatter, 2 sort lists, 3 pairs (count), 4 pairs (emit)
+hipError_t launch_brl(const BrListArgs& a, int phase, uint64_t nslots, hipStream_t st);
 
 struct WLStatesArgs {
     uint32_t n;

@@ -36,6 +36,10 @@ struct WLSpace {
     uint64_t* nbr;       // [NN] per-vertex neighbour segments (values being sorted)
     uint32_t* nb;        // [NN] per-vertex neighbour lists: (local id << 4) | position
     uint64_t* hset;      // [HT]
+    uint32_t* nnv;       // [NV] neighbour-list length per vertex: sum of (arity - 1) over occurrences
+    uint32_t* oown;      // [OCC] vertex owning each occurrence slot
+    uint32_t* own;       // [NN] vertex owning each neighbour-segment slot
+    uint64_t* srt;       // [NN] segments in sorted order
     uint32_t ht_mask;
 };
 
@@ -59,7 +63,8 @@ struct WLSizes {
     __host__ __device__ uint32_t bytes(uint32_t A) const {
         auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
         return al(NE) + al(NE * A * 2) + al((NV + 1) * 4) + al(OCC * 4) + al((NV + 1) * 4) +
-               al(NV * 4) + al(NN * 4) + 8 * (NV + NV + NN + HT);
+               al(NV * 4) + al(NN * 4) + 8 * (NV + NV + NN + HT) +
+               al(NV * 4) + al(OCC * 4) + al(NN * 4) + 8 * NN;
     }
 };
 
@@ -82,6 +87,10 @@ __device__ __forceinline__ void wl_bind(WLSpace& s, uint8_t*& p, const WLSizes& 
     s.nbr = (uint64_t*)carve(p, z.NN * 8);
     s.hset = (uint64_t*)carve(p, z.HT * 8);
     s.nb = (uint32_t*)carve(p, z.NN * 4);
+    s.nnv = (uint32_t*)carve(p, z.NV * 4);
+    s.oown = (uint32_t*)carve(p, z.OCC * 4);
+    s.own = (uint32_t*)carve(p, z.NN * 4);
+    s.srt = (uint64_t*)carve(p, z.NN * 8);
     s.ht_mask = z.HT - 1;
 }
 
@@ -116,18 +125,44 @@ __device__ inline void sort8(uint64_t (&x)[8]) {
 }
 #undef HGX_CE
 
-__device__ __forceinline__ void isort_lds(uint64_t* a, uint32_t n) {   // core.hpp:54-61
-    for (uint32_t i = 1; i < n; ++i) {
-        const uint64_t key = a[i];
-        uint32_t j = i;
-        while (j > 0 && a[j - 1] > key) { a[j] = a[j - 1]; --j; }
-        a[j] = key;
+// Every segment [so[v], so[v] + cnt[v]) of vals, sorted ascending into the same places of out:
+// an entry's place is the number of smaller entries of its segment (equal ones by position), so
+// the lanes share a long segment instead of one lane sorting it (the order isort_u64,
+// core.hpp:54-61, produces). own[j] names the segment slot j belongs to; total = slots.
+template <class T>
+__device__ __forceinline__ void seg_rank_sort(const T* vals, T* out, const uint32_t* own, const uint32_t* so,
+                                              const uint32_t* cnt, uint32_t total) {
+    for (uint32_t j = lane_id(); j < total; j += WAVE) {
+        const uint32_t v = own[j];
+        const uint32_t b = so[v], n = cnt ? cnt[v] : so[v + 1] - b;     // null cnt: so is a CSR
+        if (j - b >= n) continue;
+        const T x = vals[j];
+        uint32_t r = 0;
+#pragma unroll 4
+        for (uint32_t i = 0; i < n; ++i) {
+            const T y = vals[b + i];
+            r += (y < x || (y == x && b + i < j)) ? 1u : 0u;
+        }
+        out[b + r] = x;
     }
+    wave_sync();
 }
 
 // Exact number of distinct values in cur[0..nV).
 __device__ __forceinline__ uint32_t wl_ndistinct(WLSpace& s, const uint64_t* cur) {
     const uint32_t l = lane_id();
+    if (s.nV <= WAVE) {
+        // one colour per lane: a lane counts when no lower lane holds its colour
+        const uint64_t x = l < s.nV ? cur[l] : 0ull;
+        const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)(x >> 32);
+        bool first = l < s.nV;
+        for (uint32_t u = 0; u < s.nV; ++u) {
+            const uint32_t ulo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, (int)u);
+            const uint32_t uhi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, (int)u);
+            first &= !(u < l && ulo == xlo && uhi == xhi);
+        }
+        return (uint32_t)__popcll(__ballot(first));
+    }
     for (uint32_t i = l; i <= s.ht_mask; i += WAVE) s.hset[i] = 0;
     wave_sync();
     uint32_t cnt = 0, zero = 0;
@@ -155,13 +190,27 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
     if (nE == 0) return EMPTY_STATE_HASH;     // hypergraph.cpp:743-745
     if (nV == 0) return 0;                    // wl_core.hpp:85
 
-    // ---- occurrence CSR (wl_core.hpp:87-97) ----
-    for (uint32_t v = l; v <= nV; v += WAVE) s.occ_off[v] = 0;
+    // ---- occurrence CSR (wl_core.hpp:87-97), with each vertex's neighbour-list length ----
+    for (uint32_t v = l; v <= nV; v += WAVE) {
+        s.occ_off[v] = 0;
+        if (v < nV) s.nnv[v] = 0;
+    }
     wave_sync();
-    for (uint32_t e = l; e < nE; e += WAVE)
-        for (uint32_t p = 0; p < s.e_ar[e]; ++p) atomicAdd(&s.occ_off[s.e_lv[e * A + p]], 1u);
+    uint32_t amin = 0xFFu, amax = 0;
+    for (uint32_t e = l; e < nE; e += WAVE) {
+        const uint32_t a = s.e_ar[e];
+        amin = a < amin ? a : amin;
+        amax = a > amax ? a : amax;
+        for (uint32_t p = 0; p < a; ++p) {
+            const uint32_t x = s.e_lv[e * A + p];
+            atomicAdd(&s.occ_off[x], 1u);
+            if (a > 1) atomicAdd(&s.nnv[x], a - 1);   // wl_hash.hpp:201-210
+        }
+    }
+    const bool uniform = wave_max_u32(amax) == (0xFFu - wave_max_u32(0xFFu - amin));
+    const uint32_t ua = wave_max_u32(amax);
     wave_sync();
-    wave_excl_scan_lds(s.occ_off, nV + 1);
+    const uint32_t nocc = wave_excl_scan_lds(s.occ_off, nV + 1);
     for (uint32_t v = l; v <= nV; v += WAVE) s.cursor[v] = s.occ_off[v];
     wave_sync();
     for (uint32_t e = l; e < nE; e += WAVE)
@@ -169,45 +218,47 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
             const uint32_t w = atomicAdd(&s.cursor[s.e_lv[e * A + p]], 1u);
             s.occ_ep[w] = (e << 8) | p;
         }
+    for (uint32_t v = l; v < nV; v += WAVE)
+        for (uint32_t i = s.occ_off[v]; i < s.occ_off[v + 1]; ++i) s.oown[i] = v;
     wave_sync();
     // Each bucket into (edge, pos) order -- the order the reference's counting sort leaves.
-    uint32_t nn_max = 0;
-    for (uint32_t v = l; v < nV; v += WAVE) {
-        const uint32_t b = s.occ_off[v], n = s.occ_off[v + 1] - b;
-        for (uint32_t i = 1; i < n; ++i) {
-            const uint32_t key = s.occ_ep[b + i];
-            uint32_t j = i;
-            while (j > 0 && s.occ_ep[b + j - 1] > key) { s.occ_ep[b + j] = s.occ_ep[b + j - 1]; --j; }
-            s.occ_ep[b + j] = key;
-        }
-        uint32_t nn = 0;   // wl_hash.hpp:201-210: sum of (arity - 1) over the occurrences
-        for (uint32_t i = 0; i < n; ++i) nn += s.e_ar[s.occ_ep[b + i] >> 8] - 1u;
-        s.cursor[v] = nn;
-        nn_max = nn > nn_max ? nn : nn_max;
+    // The order only matters where the initial colour truncates a vertex's occurrences
+    // (deg > nbr_cap), which needs unary edges: with every arity >= 2, nn >= deg for each
+    // vertex, and every later use of a bucket sorts what it reads.
+    const bool unary = (0xFFu - wave_max_u32(0xFFu - amin)) <= 1;
+    if (unary) {
+        seg_rank_sort(s.occ_ep, s.nb, s.oown, s.occ_off, (const uint32_t*)nullptr, nocc);
+        for (uint32_t i = l; i < nocc; i += WAVE) s.occ_ep[i] = s.nb[i];
     }
+    uint32_t nn_max = 0;
+    for (uint32_t v = l; v < nV; v += WAVE) nn_max = s.nnv[v] > nn_max ? s.nnv[v] : nn_max;
     uint32_t nbr_cap = wave_max_u32(nn_max);
     if (nbr_cap < 1) nbr_cap = 1;
     wave_sync();
     // Segment sizes: room for the refinement list (nn) and the initial list (min(deg, cap)).
+    uint32_t cnt_max = 0;
     for (uint32_t v = l; v < nV; v += WAVE) {
         const uint32_t deg = s.occ_off[v + 1] - s.occ_off[v];
         const uint32_t init = deg < nbr_cap ? deg : nbr_cap;
-        const uint32_t nn = s.cursor[v];
+        const uint32_t nn = s.nnv[v];
         s.cursor[v] = nn > init ? nn : init;
+        cnt_max = init > cnt_max ? init : cnt_max;
     }
+    cnt_max = wave_max_u32(cnt_max);
     wave_sync();
-    wave_excl_scan_lds(s.cursor, nV + 1);   // cursor[v] = segment offset
+    wave_excl_scan_lds(s.cursor, nV + 1);                       // cursor[v] = segment offset
+    const uint32_t nslots = s.cursor[nV];                       // (cursor[nV] held no size)
+    for (uint32_t v = l; v < nV; v += WAVE)
+        for (uint32_t k = s.cursor[v]; k < s.cursor[v + 1]; ++k) s.own[k] = v;
+    wave_sync();
 
-    // ---- initial colours (wl_core.hpp:93-107), and each vertex's neighbour list, fixed for
-    //      every refinement round: (neighbour, position) for each occurrence and other position
-    //      (nbr_cap never truncates it: nbr_cap is the largest such count) ----
-    for (uint32_t v = l; v < nV; v += WAVE) {
-        const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
-        const uint32_t so = s.cursor[v];
-        uint64_t* seg = s.nbr + so;
-        uint64_t h = fnv(FNV_OFFSET, deg);
-        const uint32_t cnt = deg < nbr_cap ? deg : nbr_cap;
-        if (cnt <= 8) {
+    // ---- initial colours (wl_core.hpp:93-107): fnv(deg), then the first min(deg, cap)
+    //      occurrences' (arity, position) keys in ascending order ----
+    if (cnt_max <= 8) {
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
+            const uint32_t cnt = deg < nbr_cap ? deg : nbr_cap;
+            uint64_t h = fnv(FNV_OFFSET, deg);
             uint64_t x[8];
 #pragma unroll
             for (uint32_t i = 0; i < 8; ++i) {
@@ -222,24 +273,55 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
 #pragma unroll
             for (uint32_t i = 0; i < 8; ++i)
                 if (i < cnt) { h = fnv(h, x[i] >> 8); h = fnv(h, x[i] & 0xFFu); }
-        } else {
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t ep = s.occ_ep[b + j];
-                seg[j] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+            s.cur[v] = h;
+        }
+    } else {
+        for (uint32_t v = l; v < nV; v += WAVE) {       // init counts, kept in comp until the components
+            const uint32_t deg = s.occ_off[v + 1] - s.occ_off[v];
+            s.comp[v] = deg < nbr_cap ? deg : nbr_cap;
+        }
+        wave_sync();
+        for (uint32_t j = l; j < nslots; j += WAVE) {
+            const uint32_t v = s.own[j], i = j - s.cursor[v];
+            if (i < s.comp[v]) {
+                const uint32_t ep = s.occ_ep[s.occ_off[v] + i];
+                s.nbr[j] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
             }
-            isort_lds(seg, cnt);
-            for (uint32_t i = 0; i < cnt; ++i) { h = fnv(h, seg[i] >> 8); h = fnv(h, seg[i] & 0xFFu); }
         }
-        s.cur[v] = h;
-        uint32_t nn = 0;
-        for (uint32_t j = 0; j < deg; ++j) {
-            const uint32_t ep = s.occ_ep[b + j];
-            const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
-            const uint32_t a = s.e_ar[e];
-            for (uint32_t k = 0; k < a; ++k)
-                if (k != mypos) s.nb[so + nn++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
+        wave_sync();
+        seg_rank_sort(s.nbr, s.srt, s.own, s.cursor, s.comp, nslots);
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t so = s.cursor[v], cnt = s.comp[v];
+            uint64_t h = fnv(FNV_OFFSET, s.occ_off[v + 1] - s.occ_off[v]);
+            for (uint32_t i = 0; i < cnt; ++i) { const uint64_t x = s.srt[so + i]; h = fnv(h, x >> 8); h = fnv(h, x & 0xFFu); }
+            s.cur[v] = h;
         }
-        s.comp[v] = nn;                       // comp is free until the component phase
+    }
+
+    // ---- each vertex's neighbour list, fixed for every refinement round: (neighbour, position)
+    //      for each occurrence and other position (nbr_cap never truncates it: nbr_cap is the
+    //      largest such count) ----
+    if (uniform && ua > 1) {
+        // one lane per occurrence: the i-th occurrence of v fills slots i*(a-1) .. of v's segment
+        for (uint32_t j = l; j < nocc; j += WAVE) {
+            const uint32_t v = s.oown[j], i = j - s.occ_off[v];
+            const uint32_t ep = s.occ_ep[j], e = ep >> 8, mypos = ep & 0xFFu;
+            uint32_t o = s.cursor[v] + i * (ua - 1);
+            for (uint32_t k = 0; k < ua; ++k)
+                if (k != mypos) s.nb[o++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
+        }
+    } else {
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
+            uint32_t o = s.cursor[v];
+            for (uint32_t j = 0; j < deg; ++j) {
+                const uint32_t ep = s.occ_ep[b + j];
+                const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
+                const uint32_t a = s.e_ar[e];
+                for (uint32_t k = 0; k < a; ++k)
+                    if (k != mypos) s.nb[o++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
+            }
+        }
     }
     wave_sync();
     uint64_t* cur = s.cur;
@@ -247,12 +329,13 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
     uint32_t prev_d = wl_ndistinct(s, cur);
 
     // ---- refinement (wl_core.hpp:119-141): sorted multiset of fnv(colour, position) over the
-    //      neighbour list, folded in order -- in registers for up to 8 neighbours ----
+    //      neighbour list, folded in order -- in registers when every list has at most 8
+    //      entries, else sorted segment by segment across the wave ----
     for (uint32_t it = 0; it < WL_MAX_ITERS; ++it) {
-        for (uint32_t v = l; v < nV; v += WAVE) {
-            const uint32_t so = s.cursor[v], nn = s.comp[v];
-            uint64_t h = cur[v];
-            if (nn <= 8) {
+        if (nbr_cap <= 8) {
+            for (uint32_t v = l; v < nV; v += WAVE) {
+                const uint32_t so = s.cursor[v], nn = s.nnv[v];
+                uint64_t h = cur[v];
                 uint64_t x[8];
 #pragma unroll
                 for (uint32_t i = 0; i < 8; ++i) {
@@ -264,13 +347,22 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
 #pragma unroll
                 for (uint32_t i = 0; i < 8; ++i)
                     if (i < nn) h = fnv(h, x[i]);
-            } else {
-                uint64_t* seg = s.nbr + so;
-                for (uint32_t i = 0; i < nn; ++i) { const uint32_t t = s.nb[so + i]; seg[i] = fnv(cur[t >> 4], t & 15u); }
-                isort_lds(seg, nn);
-                for (uint32_t i = 0; i < nn; ++i) h = fnv(h, seg[i]);
+                nxt[v] = h;
             }
-            nxt[v] = h;
+        } else {
+            for (uint32_t j = l; j < nslots; j += WAVE) {
+                const uint32_t v = s.own[j];
+                if (j - s.cursor[v] < s.nnv[v]) { const uint32_t t = s.nb[j]; s.nbr[j] = fnv(cur[t >> 4], t & 15u); }
+            }
+            wave_sync();
+            seg_rank_sort(s.nbr, s.srt, s.own, s.cursor, s.nnv, nslots);
+            for (uint32_t v = l; v < nV; v += WAVE) {
+                const uint32_t so = s.cursor[v], nn = s.nnv[v];
+                uint64_t h = cur[v];
+#pragma unroll 4
+                for (uint32_t i = 0; i < nn; ++i) h = fnv(h, s.srt[so + i]);
+                nxt[v] = h;
+            }
         }
         wave_sync();
         uint64_t* t = cur; cur = nxt; nxt = t;

@@ -201,3 +201,20 @@ def test_prepared_batch_equals_host_batch(ctx):
     for i in (0, 40, 41, 42):
         assert_exact(dev[i], po.evolve(reqs[i]))
     prep.close()
+
+
+@pytest.mark.parametrize("seed,steps,cap", [(0, 2, 0), (1, 2, 4000), (2, 1, 0)])
+def test_branchial_large_parents_exact(ctx, seed, steps, cap):
+    """Parents above 64 edges take the per-edge match-list path for branchial pairs: exact
+    pairs against the oracle, with hyperedges, repeated edges and a two-edge rule whose
+    matches share both edges in reverse order."""
+    rnd = random.Random(seed)
+    n = 70 + 40 * seed
+    init = [[rnd.randrange(40), rnd.randrange(40)] for _ in range(n)]
+    init += [[3, 4, 5], [5, 4, 3], [7, 7]]
+    rules = [rule_from_lists([[0, 1], [0, 2]], [[0, 2], [0, 3], [1, 3], [2, 3]], 0),
+             rule_from_lists([[0, 1, 2]], [[0, 1], [1, 2]], 1)]
+    req = Request(rules, init, steps, state_cap=cap, flags=REL | TR)
+    res = ctx.evolve(req)
+    assert res.counts["branchial_pairs"] > 0
+    assert_exact(res, po.evolve(req))
