@@ -43,6 +43,7 @@ void Engine::grow_slots(uint32_t need) {
     uint8_t* active = dalloc<uint8_t>(*na, cap);
     uint32_t* budget = dalloc<uint32_t>(*na, cap);
     uint64_t* acc = dalloc<uint64_t>(*na, cap);
+    uint64_t* redges = dalloc<uint64_t>(*na, cap);
     uint32_t* sb = dalloc<uint32_t>(*na, cap);
     uint32_t* se = dalloc<uint32_t>(*na, cap);
     uint64_t* tot = dalloc<uint64_t>(*na, cap + 1);
@@ -57,6 +58,7 @@ void Engine::grow_slots(uint32_t need) {
     }
     slot_arena_ = std::move(na);
     d_runs_ = runs; d_ctr_ = ctr; d_active_ = active; d_budget_ = budget; d_run_acc_ = acc;
+    d_run_edges_ = redges;
     d_sbeg_ = sb; d_send_ = se; d_tot_ = tot;
     cap_slots_ = cap;
 }
@@ -179,6 +181,7 @@ uint32_t Engine::step() {
     Arena tmp(pool_.get(), (size_t)4 << 20);
     auto keep = std::make_unique<Arena>(pool_.get(), (size_t)4 << 20);   // this step's match list
     std::vector<ExpandClass> mcls_count, mcls_emit;   // host range lists of the join launches
+    std::vector<uint64_t> h_edges;                      // per-slot child edges of this generation
     const uint32_t n = F_ ? F_->v.n : 0;
     const uint32_t nF = (uint32_t)F_runs_.size();
     std::vector<uint64_t> tot(nF, 0);
@@ -214,6 +217,7 @@ uint32_t Engine::step() {
             h2d(d_sbeg_, h_b.data(), cap_slots_, st);
             h2d(d_send_, h_e.data(), cap_slots_, st);
             HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * cap_slots_, st));
+            HIP_TRY(hipMemsetAsync(d_run_edges_, 0, sizeof(uint64_t) * cap_slots_, st));
             HIP_TRY(hipMemsetAsync(d_tot_ + cap_slots_, 0, sizeof(uint64_t), st));
             cnt = dalloc<uint32_t>(tmp, n + 1);
             moff = dalloc<uint64_t>(*keep, n + 1);
@@ -237,6 +241,7 @@ uint32_t Engine::step() {
             ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
             ma.budget = d_budget_;
             ma.run_acc = (unsigned long long*)d_run_acc_;
+            ma.run_edges = (unsigned long long*)d_run_edges_;
             const uint64_t match_bytes = F_->n_edges * (4ull * F_->v.A + 1) + (uint64_t)n * 24;
             // One launch per size class of the requests taking part (by their bound on state
             // edges), each with LDS staging sized for its class; states outside the launches
@@ -280,7 +285,9 @@ uint32_t Engine::step() {
             scan_u32_to_u64(tmp, cnt, moff, n, st);
             HIP_TRY(launch_run_totals(moff, d_sbeg_, d_send_, cap_slots_, d_tot_, st));
             std::vector<uint64_t> htot(cap_slots_ + 1);
+            h_edges.assign(cap_slots_, 0);
             d2h(htot.data(), d_tot_, cap_slots_ + 1, st);
+            d2h(h_edges.data(), d_run_edges_, cap_slots_, st);
             HIP_TRY(hipStreamSynchronize(st));
             t_count = ms_since(t_step);
             parent_edges = htot[cap_slots_];
@@ -448,11 +455,9 @@ uint32_t Engine::step() {
             C->ev.tr = dalloc<uint64_t>(C->meta, M);
             scan_u32_to_u64(tmp, mv.cne, v.eoff, (uint32_t)M, st);
         }
-        uint64_t EC = 0;
-        if (M) {
-            d2h(&EC, v.eoff + M, 1, st);
-            HIP_TRY(hipStreamSynchronize(st));
-        }
+        uint64_t EC = 0;                         // child edges, from the count pass's per-run sums
+        for (uint32_t i = 0; i < nF; ++i)
+            if (tot[i]) EC += h_edges[F_runs_[i]->slot];
         C->n_edges = EC + NER;
         v.ev = dalloc<uint32_t>(C->edges, (EC + NER) * A);
         v.ar = dalloc<uint8_t>(C->edges, EC + NER);

@@ -93,6 +93,7 @@ private:
     uint8_t* d_active_ = nullptr;
     uint32_t* d_budget_ = nullptr;
     uint64_t* d_run_acc_ = nullptr;
+    uint64_t* d_run_edges_ = nullptr;
     uint32_t* d_sbeg_ = nullptr;
     uint32_t* d_send_ = nullptr;
     uint64_t* d_tot_ = nullptr;               // [cap_slots + 1]

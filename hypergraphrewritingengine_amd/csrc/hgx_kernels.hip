@@ -221,7 +221,7 @@ __device__ __forceinline__ uint32_t join_anchor(const DRule& R, uint32_t p, cons
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax) {
     auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
     uint32_t b = al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
-                 al(max_ne * A * 2) + 64;
+                 al(max_ne * A * 2) * 2 + al(max_ne * A * 4) + 64;
     if (dmax) b += al(dmax * (KL + 1) * 2) + al(dmax * 2) + al((dmax + 1) * 4);
     return b;
 }
@@ -237,6 +237,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     uint16_t* seeds = (uint16_t*)carve(p, a.max_ne * 2);
     uint32_t* iend = (uint32_t*)carve(p, (a.max_nextv + 1) * 4); // incidence list ends per vertex
     uint16_t* ie = (uint16_t*)carve(p, a.max_ne * A * 2);       // incidence lists
+    uint16_t* ie2 = (uint16_t*)carve(p, a.max_ne * A * 2);      // the same, before ordering
+    uint32_t* iown = (uint32_t*)carve(p, a.max_ne * A * 4);     // vertex owning each list slot
     const uint32_t dstride = a.KL + 1;                          // delta match: rule + positions
     uint16_t* dlist = (uint16_t*)carve(p, a.dmax * dstride * 2);   // empty when dmax = 0
     uint16_t* dperm = (uint16_t*)carve(p, a.dmax * 2);
@@ -273,28 +275,34 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                 if (!dup) atomicAdd(&iend[x], 1u);
             }
         wave_sync();
-        wave_excl_scan_lds(iend, nvid + 1);                     // iend[x] = start of x's list
+        const uint32_t nslot = wave_excl_scan_lds(iend, nvid + 1);   // iend[x] = start of x's list
+        for (uint32_t x = l; x < nvid; x += WAVE)
+            for (uint32_t i = iend[x]; i < iend[x + 1]; ++i) iown[i] = x;
+        wave_sync();
         for (uint32_t e = l; e < ne; e += WAVE)
             for (uint32_t k = 0; k < sa[e]; ++k) {
                 const uint32_t x = sv[e * A + k];
                 bool dup = false;
                 for (uint32_t k2 = 0; k2 < k; ++k2) dup |= (sv[e * A + k2] == x);
-                if (!dup) ie[atomicAdd(&iend[x], 1u)] = (uint16_t)e;   // -> iend[x] = end
+                if (!dup) ie2[atomicAdd(&iend[x], 1u)] = (uint16_t)e;   // -> iend[x] = end
             }
         wave_sync();
-        for (uint32_t x = l; x < nvid; x += WAVE) {
+        // each list into ascending edge order: an entry's place is the number of smaller entries
+        // of its list, so a hub vertex's list is placed by as many lanes as it has entries
+        for (uint32_t j = l; j < nslot; j += WAVE) {
+            const uint32_t x = iown[j];
             const uint32_t b = x ? iend[x - 1] : 0u, n = iend[x] - b;
-            for (uint32_t i = 1; i < n; ++i) {
-                const uint16_t key = ie[b + i];
-                uint32_t j = i;
-                while (j > 0 && ie[b + j - 1] > key) { ie[b + j] = ie[b + j - 1]; --j; }
-                ie[b + j] = key;
-            }
+            const uint16_t key = ie2[j];
+            uint32_t r = 0;
+#pragma unroll 4
+            for (uint32_t i = 0; i < n; ++i) r += ie2[b + i] < key ? 1u : 0u;
+            ie[b + r] = key;
         }
         wave_sync();
 
         const DRun& RN = a.runs[run];
         uint32_t total = 0;
+        uint64_t edges_sum = 0;                                // count pass: edges of the children
         bool stop = false;
         uint32_t out = EMIT ? (uint32_t)a.moff[s] : 0u;
         bool full = true;
@@ -317,6 +325,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                 j = i % nprod;
             };
             uint32_t nd = 0;
+            uint64_t de = 0;                                    // child edges of the delta matches
             for (uint32_t base = 0; base < ncombo; base += WAVE) {
                 const uint32_t i = base + l;
                 uint32_t c = 0;
@@ -324,6 +333,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                     uint32_t r, pp, j;
                     combo(i, r, pp, j);
                     c = join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                    de += (uint64_t)c * (ne - RN.rules[r].n_lhs + RN.rules[r].n_rhs);
                 }
                 nd += wave_sum_u32(c);
             }
@@ -379,6 +389,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                         }
                     }
                     const uint64_t bal = __ballot(fw);
+                    if (!EMIT && fw) de += ne - nm + RN.rules[rm].n_rhs;
                     if (EMIT && fw) {
                         uint32_t lo = 0, hi = nd;                   // delta matches before it
                         while (lo < hi) {
@@ -410,6 +421,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                     wave_sync();
                 } else {
                     total = nf + nd;
+                    edges_sum = wave_sum_u64(de);
                     if (l == 0 && total) atomicAdd(&a.run_acc[run], (unsigned long long)total);
                 }
             }
@@ -451,6 +463,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                     // state's count equals what it added, so the totals stay over the cap.
                     const uint32_t round = wave_sum_u32(c);
                     total += round;
+                    edges_sum += (uint64_t)round * child_ne;
                     unsigned long long before = 0;
                     if (l == 0 && round) before = atomicAdd(&a.run_acc[run], (unsigned long long)round);
                     before = __shfl(before, 0, WAVE);
@@ -462,6 +475,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
         if (!EMIT && l == 0) {
             a.cnt[s] = total;
             if (total && a.parent_edges) atomicAdd(a.parent_edges, (unsigned long long)ne);
+            if (edges_sum) atomicAdd(&a.run_edges[run], (unsigned long long)edges_sum);
         }
     }
 }

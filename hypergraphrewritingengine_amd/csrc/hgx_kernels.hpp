@@ -20,6 +20,7 @@ struct MatchArgs {
     unsigned long long* parent_edges;   // count pass: += edges of each state with a match
     const uint32_t* budget;    // count pass: [runs] matches a run may still take
     unsigned long long* run_acc;   // count pass: [runs] running tally (zeroed per generation)
+    unsigned long long* run_edges; // count pass: [runs] edges of all children (zeroed per generation)
     uint32_t max_nextv;        // bound on vertex labels in P (incidence index)
     // Incremental forwarding. States [0, fwd_n) of P are the children of the previous
     // generation's matches (state s was made by match s there); their matches are the
