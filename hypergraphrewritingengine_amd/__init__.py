@@ -100,15 +100,23 @@ def load_library():
 class Result:
     """One request's result (hgx_result*). Arrays are read back lazily."""
 
-    def __init__(self, lib, handle, ctx, counts=None):
+    def __init__(self, lib, handle, ctx, counts=None, row=None):
         self._lib = lib
         self._h = handle
         self._ctx = ctx           # keep the context alive while results exist
-        if counts is None:
+        self._row = row           # (structured array, index) of a batch's counts, read lazily
+        self._counts = counts
+        if counts is None and row is None:
             c = abi.Counts()
             self._check(lib.hgx_result_counts(handle, C.byref(c)))
-            counts = c.as_dict()
-        self.counts = counts
+            self._counts = c.as_dict()
+
+    @property
+    def counts(self) -> dict:
+        if self._counts is None:
+            arr, i = self._row
+            self._counts = {k: int(arr[k][i]) for k in arr.dtype.names}
+        return self._counts
 
     def _check(self, rc):
         if rc != abi.HGX_OK:
@@ -267,10 +275,7 @@ class Context:
             raise self._err(rc)
         arr = np.frombuffer(cnts, dtype=np.dtype([(k, np.uint64 if t is C.c_uint64 else np.uint32)
                                                    for k, t in abi.Counts._fields_]), count=n)
-        names = arr.dtype.names
-        cols = {k: arr[k].tolist() for k in names}
-        out = [Result(self._lib, C.c_void_p(outs[i]), self, {k: cols[k][i] for k in names})
-               for i in range(n)]
+        out = [Result(self._lib, C.c_void_p(outs[i]), self, row=(arr, i)) for i in range(n)]
         self.last_timing = {"prepare_s": t1 - t0, "library_s": t2 - t1,
                             "results_s": _t.perf_counter() - t2}
         return out

@@ -205,6 +205,7 @@ uint32_t Engine::step() {
             if (r.done || r.depth >= r.hp->steps) { r.done = true; continue; }
             h_active[r.slot] = 1;
             h_budget[r.slot] = !r.hp->cap ? 0xFFFFFFFFu : (r.states >= r.hp->cap ? 0u : (uint32_t)(r.hp->cap - r.states));
+            small &= r.hp->rules.size() <= 2;
             for (auto& R : r.hp->rules) {
                 KL = std::max<uint32_t>(KL, R.n_lhs);
                 small &= R.n_lhs <= 4 && (R.lhs_mask >> 16) == 0;
@@ -267,7 +268,8 @@ uint32_t Engine::step() {
                     MatchArgs mc = ma;
                     mc.max_ne = std::max(1u, std::min(x.ne, F_->max_ne));
                     mc.max_nextv = std::max(1u, std::min(x.nextv, F_->max_nextv));
-                    mc.lds_per_wave = (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax) + 15u) & ~15u;
+                    mc.lds_per_wave = (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax,
+                                                          mc.small != 0) + 15u) & ~15u;
                     if (x.n != n) {
                         uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
                         h2d(d_r, x.rb.data(), x.rb.size(), st);

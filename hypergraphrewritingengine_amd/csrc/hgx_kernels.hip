@@ -13,6 +13,8 @@
 // MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hgx_device.hpp"
 #include "hgx_kernels.hpp"
 #include "hgx_wl.hpp"
@@ -41,22 +43,41 @@ __device__ inline uint32_t xcd_block() {
 //           enumerates, so event numbering is identical across engines and runs.
 // ------------------------------------------------------------------------------------------
 
-// Per-lane depth-first state. KM bounds the LHS size and VM the pattern variable ids; the small
-// instance (LHS <= 4 edges, variables < 16) is sized so the compiler keeps it in registers
-// instead of scratch memory.
+// Per-lane depth-first state: the edge bound at each depth, the bound-variable mask on entry to
+// each depth, the next candidate and the end of the candidate range at each depth, and the
+// variable bindings. Rules up to 16 LHS edges and variable ids up to 31 keep it per lane in
+// private memory (JoinLaneBig); the common rules -- at most 4 LHS edges, variables below 16 --
+// keep it in LDS, interleaved across the wave (element i of a lane at word i * 64 + lane), so
+// every step of the depth-first search is an LDS access rather than a private-memory one.
 template <int KM, int VM>
 struct JoinLaneT {
-    uint32_t matched[KM];   // edge index bound at each depth
-    uint32_t smask[KM + 1]; // bound-variable mask on entry to each depth
-    uint32_t cidx[KM];      // next candidate at each depth
-    uint32_t chi[KM];       // end of the candidate range at each depth
+    uint32_t matched[KM];
+    uint32_t smask[KM + 1];
+    uint32_t cidx[KM];
+    uint32_t chi[KM];
     uint32_t bind[VM];
 };
 using JoinLaneBig = JoinLaneT<16, 32>;
-using JoinLaneSmall = JoinLaneT<4, 16>;
 
+struct LaneArr {
+    uint32_t* p;            // this lane's element 0
+    __device__ __forceinline__ uint32_t& operator[](uint32_t i) const { return p[i * WAVE]; }
+};
+struct JoinLaneLds {
+    static constexpr uint32_t WORDS = 4 + 5 + 4 + 4 + 16;
+    LaneArr matched, smask, cidx, chi, bind;
+    __device__ __forceinline__ void at(uint32_t* block, uint32_t lane) {
+        matched.p = block + lane;
+        smask.p = matched.p + 4 * WAVE;
+        cidx.p = smask.p + 5 * WAVE;
+        chi.p = cidx.p + 4 * WAVE;
+        bind.p = chi.p + 4 * WAVE;
+    }
+};
+
+template <class Bind>
 __device__ __forceinline__ bool bind_edge(const uint32_t* ev, uint32_t ea, const uint8_t* pv, uint32_t pa,
-                                 uint32_t* bind, uint32_t& mask) {   // match_core.hpp:32-49
+                                 Bind bind, uint32_t& mask) {   // match_core.hpp:32-49
     if (ea != pa) return false;
     for (uint32_t i = 0; i < ea; ++i) {
         const uint32_t var = pv[i], bit = 1u << var;
@@ -140,10 +161,10 @@ __device__ __forceinline__ uint32_t join_seed(const DRule& R, const uint32_t* sv
 
 // Order of two matches of one state, as a full join emits them: rule, then the edge
 // positions in the rule's join order (q arrays are in LHS order).
-__device__ __forceinline__ bool match_less(const DRun& RN, uint32_t ra, const uint16_t* qa, uint32_t rb,
+__device__ __forceinline__ bool match_less(const DRule* rules, uint32_t ra, const uint16_t* qa, uint32_t rb,
                                   const uint16_t* qb) {
     if (ra != rb) return ra < rb;
-    const DRule& R = RN.rules[ra];
+    const DRule& R = rules[ra];
     for (uint32_t t = 0; t < R.n_lhs; ++t) {
         const uint32_t o = R.order[t];
         if (qa[o] != qb[o]) return qa[o] < qb[o];
@@ -218,10 +239,12 @@ __device__ __forceinline__ uint32_t join_anchor(const DRule& R, uint32_t p, cons
     return found;
 }
 
-uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax) {
+uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
+                            bool small) {
     auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
     uint32_t b = al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
                  al(max_ne * A * 2) * 2 + al(max_ne * A * 4) + 64;
+    if (small) b += JoinLaneLds::WORDS * WAVE * 4 + 2 * (uint32_t)sizeof(DRule);
     if (dmax) b += al(dmax * (KL + 1) * 2) + al(dmax * 2) + al((dmax + 1) * 4);
     return b;
 }
@@ -244,6 +267,14 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     uint16_t* dperm = (uint16_t*)carve(p, a.dmax * 2);
     uint32_t* dhist = (uint32_t*)carve(p, a.dmax ? (a.dmax + 1) * 4 : 0);
     JoinLane L;
+    // Small rules (<= 4 LHS edges, <= 2 rules per request): the request's rules staged in LDS per
+    // state, so every pattern read in the depth-first search is an LDS read.
+    constexpr bool SMALL = std::is_same<JoinLane, JoinLaneLds>::value;
+    DRule* lr = nullptr;
+    if constexpr (SMALL) {
+        L.at((uint32_t*)carve(p, JoinLaneLds::WORDS * WAVE * 4), l);
+        lr = (DRule*)carve(p, 2 * sizeof(DRule));
+    }
 
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
@@ -301,6 +332,17 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
         wave_sync();
 
         const DRun& RN = a.runs[run];
+        const DRule* rules;
+        if constexpr (SMALL) {
+            const uint32_t* src = (const uint32_t*)RN.rules;
+            uint32_t* dst = (uint32_t*)lr;
+            const uint32_t words = RN.n_rules * (uint32_t)(sizeof(DRule) / 4);
+            for (uint32_t i = l; i < words; i += WAVE) dst[i] = src[i];
+            wave_sync();
+            rules = lr;
+        } else {
+            rules = RN.rules;
+        }
         uint32_t total = 0;
         uint64_t edges_sum = 0;                                // count pass: edges of the children
         bool stop = false;
@@ -310,17 +352,17 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             // ---- incremental forwarding (parallel_evolution.cpp:591-756): the parent's
             //      matches that avoid the consumed edges, plus delta joins at the produced
             //      edges, merged into the full join's order ----
-            const DRule& CR = RN.rules[a.pm_rule[s]];
+            const DRule& CR = rules[a.pm_rule[s]];
             const uint32_t nc = CR.n_lhs, nprod = CR.n_rhs;
             const uint32_t kept = ne - nprod;                   // kept edges come first
             const uint16_t* cq = a.pm_q + (size_t)s * a.pm_K;  // consumed, parent positions
             const uint32_t par = a.pm_parent[s];
             const uint64_t lb = a.pm_moff[par], le = a.pm_moff[par + 1];
             uint32_t ncombo = 0;                                // (rule, anchor, produced edge)
-            for (uint32_t r = 0; r < RN.n_rules; ++r) ncombo += RN.rules[r].n_lhs * nprod;
+            for (uint32_t r = 0; r < RN.n_rules; ++r) ncombo += rules[r].n_lhs * nprod;
             auto combo = [&](uint32_t i, uint32_t& r, uint32_t& pp, uint32_t& j) {
                 r = 0;
-                while (i >= RN.rules[r].n_lhs * nprod) { i -= RN.rules[r].n_lhs * nprod; ++r; }
+                while (i >= rules[r].n_lhs * nprod) { i -= rules[r].n_lhs * nprod; ++r; }
                 pp = i / nprod;
                 j = i % nprod;
             };
@@ -332,8 +374,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                 if (i < ncombo) {
                     uint32_t r, pp, j;
                     combo(i, r, pp, j);
-                    c = join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
-                    de += (uint64_t)c * (ne - RN.rules[r].n_lhs + RN.rules[r].n_rhs);
+                    c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                    de += (uint64_t)c * (ne - rules[r].n_lhs + rules[r].n_rhs);
                 }
                 nd += wave_sum_u32(c);
             }
@@ -346,11 +388,11 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                         uint32_t c = 0, r = 0, pp = 0, j = 0;
                         if (i < ncombo) {
                             combo(i, r, pp, j);
-                            c = join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                            c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
                         }
                         const uint32_t inc = wave_incl_scan_u32(c);
                         if (c)
-                            join_anchor(RN.rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L,
+                            join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L,
                                         dlist + (size_t)(o + inc - c) * dstride, dstride, r);
                         o += __shfl(inc, WAVE - 1, WAVE);
                     }
@@ -360,7 +402,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                         uint32_t rank = 0;
                         for (uint32_t j = 0; j < nd; ++j) {
                             const uint16_t* dj = dlist + (size_t)j * dstride;
-                            rank += match_less(RN, dj[0], dj + 1, di[0], di + 1) ? 1u : 0u;
+                            rank += match_less(rules, dj[0], dj + 1, di[0], di + 1) ? 1u : 0u;
                         }
                         dperm[rank] = (uint16_t)i;
                     }
@@ -375,7 +417,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                     uint16_t qm[16];
                     if (m < le) {
                         rm = a.pm_rule[m];
-                        nm = RN.rules[rm].n_lhs;
+                        nm = rules[rm].n_lhs;
                         fw = true;
                         for (uint32_t t = 0; t < nm; ++t) {
                             const uint32_t x = a.pm_q[m * a.pm_K + t];
@@ -389,19 +431,19 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                         }
                     }
                     const uint64_t bal = __ballot(fw);
-                    if (!EMIT && fw) de += ne - nm + RN.rules[rm].n_rhs;
+                    if (!EMIT && fw) de += ne - nm + rules[rm].n_rhs;
                     if (EMIT && fw) {
                         uint32_t lo = 0, hi = nd;                   // delta matches before it
                         while (lo < hi) {
                             const uint32_t mid = (lo + hi) >> 1;
                             const uint16_t* dm = dlist + (size_t)dperm[mid] * dstride;
-                            if (match_less(RN, dm[0], dm + 1, rm, qm)) lo = mid + 1; else hi = mid;
+                            if (match_less(rules, dm[0], dm + 1, rm, qm)) lo = mid + 1; else hi = mid;
                         }
                         const uint32_t o2 = out + nf + lanes_below(bal) + lo;
                         for (uint32_t t = 0; t < nm; ++t) a.M.q[(size_t)o2 * a.M.K + t] = qm[t];
                         a.M.parent[o2] = s;
                         a.M.rule[o2] = rm;
-                        a.M.cne[o2] = ne - nm + RN.rules[rm].n_rhs;
+                        a.M.cne[o2] = ne - nm + rules[rm].n_rhs;
                         atomicAdd(&dhist[lo], 1u);
                     }
                     nf += __popcll(bal);
@@ -411,12 +453,12 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                     wave_excl_scan_lds(dhist, nd + 1);        // dhist[k + 1] = forwarded before delta k
                     for (uint32_t k = l; k < nd; k += WAVE) {
                         const uint16_t* dm = dlist + (size_t)dperm[k] * dstride;
-                        const uint32_t rr = dm[0], nm = RN.rules[rr].n_lhs;
+                        const uint32_t rr = dm[0], nm = rules[rr].n_lhs;
                         const uint32_t o2 = out + dhist[k + 1] + k;
                         for (uint32_t t = 0; t < nm; ++t) a.M.q[(size_t)o2 * a.M.K + t] = dm[1 + t];
                         a.M.parent[o2] = s;
                         a.M.rule[o2] = rr;
-                        a.M.cne[o2] = ne - nm + RN.rules[rr].n_rhs;
+                        a.M.cne[o2] = ne - nm + rules[rr].n_rhs;
                     }
                     wave_sync();
                 } else {
@@ -427,7 +469,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             }
         }
         for (uint32_t r = 0; full && r < RN.n_rules && !stop; ++r) {
-            const DRule& R = RN.rules[r];
+            const DRule& R = rules[r];
             const uint32_t p0 = R.order[0];
             const uint32_t child_ne = ne - R.n_lhs + R.n_rhs;
             // SCAN: compact the seeds of position order[0] in edge order.
@@ -482,8 +524,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
 
 template __global__ void k_match<false, JoinLaneBig>(MatchArgs);
 template __global__ void k_match<true, JoinLaneBig>(MatchArgs);
-template __global__ void k_match<false, JoinLaneSmall>(MatchArgs);
-template __global__ void k_match<true, JoinLaneSmall>(MatchArgs);
+template __global__ void k_match<false, JoinLaneLds>(MatchArgs);
+template __global__ void k_match<true, JoinLaneLds>(MatchArgs);
 
 // ------------------------------------------------------------------------------------------
 // Rewrite + hash + dedup + event + causal, one wave per child (Rewriter::apply,
@@ -1073,8 +1115,8 @@ hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32
     const uint32_t grid = grid_for(items, wpb, grid_cap);
     const size_t lds = (size_t)a.lds_per_wave * wpb;
     if (a.small) {
-        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneSmall>), dim3(grid), dim3(64 * wpb), lds, st, a);
-        else hipLaunchKernelGGL((k_match<false, JoinLaneSmall>), dim3(grid), dim3(64 * wpb), lds, st, a);
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneLds>), dim3(grid), dim3(64 * wpb), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneLds>), dim3(grid), dim3(64 * wpb), lds, st, a);
     } else {
         if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneBig>), dim3(grid), dim3(64 * wpb), lds, st, a);
         else hipLaunchKernelGGL((k_match<false, JoinLaneBig>), dim3(grid), dim3(64 * wpb), lds, st, a);

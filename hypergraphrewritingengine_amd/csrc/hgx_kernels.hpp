@@ -34,7 +34,8 @@ struct MatchArgs {
     const uint64_t* pm_moff;   // [previous frontier + 1] each parent's match range
     uint32_t KL;               // largest LHS of the frontier's active runs
     uint32_t dmax;             // delta matches a state may stage (else it takes the full join)
-    uint32_t small;            // every active rule has <= 4 LHS edges and variables < 16
+    uint32_t small;            // every active request has <= 2 rules, each with <= 4 LHS edges
+                               //   and variables < 16
     // Optional subset of P's states, as ExpandArgs' ranges (nrange = 0: all of P).
     const uint32_t* rb;
     const uint32_t* rpre;
@@ -115,7 +116,8 @@ struct WLStatesArgs {
     uint8_t* scratch;          // as ExpandArgs::scratch
 };
 
-uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax);
+uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
+                            bool small);
 // One admitted request: its rules (copied from a packed staging array), its state map (zeroed)
 // and its counter row (zeroed) -- every admission of a step in one launch.
 struct AdmitItem {
