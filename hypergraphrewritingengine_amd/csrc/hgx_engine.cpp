@@ -229,9 +229,9 @@ uint32_t Engine::step() {
             ma.P = F_->v; ma.runs = d_runs_; ma.active = d_active_; ma.cnt = cnt; ma.moff = moff;
             ma.max_ne = std::max(1u, F_->max_ne);
             ma.max_nextv = std::max(1u, F_->max_nextv);
-            // forwarding: on when asked; automatic forwards the count pass (cheaper than a
-            // full join there) and emits by full join (the merge costs more than it saves on
-            // small states)
+            // forwarding: on when asked; automatic forwards the count pass of states above 128
+            // edges (measured cheaper than their full join) and joins everything else in full
+            // (cheaper for small states, and for the emit pass, where the merge costs more)
             const bool fwd = prev_.n > 0 && prev_.n <= n && ctx_->forward != 0;
             ma.fwd_n = fwd ? prev_.n : 0;
             ma.pm_parent = prev_.parent; ma.pm_rule = prev_.rule; ma.pm_q = prev_.q; ma.pm_K = prev_.K;
@@ -266,6 +266,7 @@ uint32_t Engine::step() {
                 for (auto& x : cls) {
                     if (!x.n) continue;
                     MatchArgs mc = ma;
+                    if (ctx_->forward == -1 && x.ne <= kClassEdges[0]) { mc.fwd_n = 0; mc.dmax = 0; }
                     mc.max_ne = std::max(1u, std::min(x.ne, F_->max_ne));
                     mc.max_nextv = std::max(1u, std::min(x.nextv, F_->max_nextv));
                     mc.lds_per_wave = (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax,
