@@ -50,6 +50,10 @@ struct DRule {
     // remaining positions greedily by shared variables; apiv[p] are its pivots.
     uint8_t aord[16][16];
     uint8_t apiv[16][16];
+    // Binding sources in the join order: jsrc[d][i] = (d' << 4) | i' when the variable at slot i
+    // of pattern edge order[d] first appeared at slot i' of depth d' (an earlier depth, or an
+    // earlier slot of the same edge), 0xFF where it appears first.
+    uint8_t jsrc[16][16];
 };
 
 // Per request (run) state the kernels read and update.

@@ -239,6 +239,93 @@ __device__ __forceinline__ uint32_t join_anchor(const DRule& R, uint32_t p, cons
     return found;
 }
 
+// Depth-first completion of one seed with the search state in registers, for rules of at most
+// N = 4 LHS edges: every depth is its own template level, so the edges bound so far are a
+// register array indexed by constants, and a variable's value is read from the vertex slot
+// where it was first bound (DRule::jsrc) instead of from a binding table. Same candidates in
+// the same order as join_seed, so the same matches in the same order.
+template <int N>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&m)[N], uint32_t d) {
+    uint32_t r = m[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) r = (d == (uint32_t)k) ? m[k] : r;
+    return r;
+}
+
+struct RegJoinOut {
+    uint16_t* q;
+    uint32_t K, base, *parent, *rule, *cne;
+    uint32_t state, rule_id, child_ne;
+};
+
+template <int N, int D, bool OUT>
+__device__ __forceinline__ uint32_t reg_join(const DRule& R, const uint32_t* sv, const uint8_t* sa, uint32_t ne,
+                                             uint32_t A, const uint32_t* iend, const uint16_t* ie,
+                                             uint32_t (&m)[N], const RegJoinOut& o, uint32_t found) {
+    if constexpr (D == N) {
+        if constexpr (OUT) {
+            const size_t at = (size_t)(o.base + found);
+#pragma unroll
+            for (int t = 0; t < N; ++t) o.q[at * o.K + R.order[t]] = (uint16_t)m[t];
+            o.parent[at] = o.state; o.rule[at] = o.rule_id; o.cne[at] = o.child_ne;
+        }
+        return found + 1;
+    } else {
+        const uint32_t pe = R.order[D], pa = R.la[pe];
+        uint32_t lo = 0, hi = ne;
+        const uint32_t pv = R.piv[D];
+        const bool indexed = pv != 0xFFu;
+        if (indexed) {
+            const uint32_t src = R.jsrc[D][pv];
+            const uint32_t x = sv[pick<N>(m, src >> 4) * A + (src & 15u)];
+            lo = x ? iend[x - 1] : 0u;
+            hi = iend[x];
+        }
+        for (uint32_t j = lo; j < hi; ++j) {
+            const uint32_t e = indexed ? (uint32_t)ie[j] : j;
+            bool ok = sa[e] == pa;
+#pragma unroll
+            for (int t = 0; t < D; ++t) ok &= (m[t] != e);
+            if (!ok) continue;
+            for (uint32_t i = 0; i < pa && ok; ++i) {
+                const uint32_t src = R.jsrc[D][i];
+                if (src == 0xFFu) continue;
+                const uint32_t sd = src >> 4, si = src & 15u;
+                const uint32_t want = sd == (uint32_t)D ? sv[e * A + si] : sv[pick<N>(m, sd) * A + si];
+                ok = sv[e * A + i] == want;
+            }
+            if (!ok) continue;
+            m[D] = e;
+            found = reg_join<N, D + 1, OUT>(R, sv, sa, ne, A, iend, ie, m, o, found);
+        }
+        return found;
+    }
+}
+
+template <int N, bool OUT>
+__device__ __forceinline__ uint32_t reg_join_seed(const DRule& R, const uint32_t* sv, const uint8_t* sa,
+                                                  uint32_t ne, uint32_t A, const uint32_t* iend,
+                                                  const uint16_t* ie, uint32_t seed, const RegJoinOut& o) {
+    uint32_t m[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) m[t] = 0xFFFFFFFFu;
+    m[0] = seed;
+    return reg_join<N, 1, OUT>(R, sv, sa, ne, A, iend, ie, m, o, 0u);
+}
+
+// seeds already satisfy position order[0]'s pattern (the SCAN checks it)
+template <bool OUT>
+__device__ __forceinline__ uint32_t reg_join_any(const DRule& R, const uint32_t* sv, const uint8_t* sa,
+                                                 uint32_t ne, uint32_t A, const uint32_t* iend,
+                                                 const uint16_t* ie, uint32_t seed, const RegJoinOut& o) {
+    switch (R.n_lhs) {
+        case 1: return reg_join_seed<1, OUT>(R, sv, sa, ne, A, iend, ie, seed, o);
+        case 2: return reg_join_seed<2, OUT>(R, sv, sa, ne, A, iend, ie, seed, o);
+        case 3: return reg_join_seed<3, OUT>(R, sv, sa, ne, A, iend, ie, seed, o);
+        default: return reg_join_seed<4, OUT>(R, sv, sa, ne, A, iend, ie, seed, o);
+    }
+}
+
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
                             bool small) {
     auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
@@ -490,14 +577,23 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             for (uint32_t base = 0; base < nseed; base += WAVE) {
                 const uint32_t i = base + l;
                 uint32_t c = 0;
-                if (i < nseed)
-                    c = join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
-                                  nullptr, 0, 0, 0);
+                if (i < nseed) {
+                    if constexpr (SMALL) c = reg_join_any<false>(R, sv, sa, ne, A, iend, ie, seeds[i], RegJoinOut{});
+                    else c = join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
+                                       nullptr, 0, 0, 0);
+                }
                 if (EMIT) {
                     const uint32_t inc = wave_incl_scan_u32(c);
-                    if (i < nseed && c)
-                        join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, a.M.q, a.M.K, out + inc - c,
-                                  a.M.parent, a.M.rule, a.M.cne, s, r, child_ne);
+                    if (i < nseed && c) {
+                        if constexpr (SMALL) {
+                            const RegJoinOut ro{a.M.q, a.M.K, out + inc - c, a.M.parent, a.M.rule, a.M.cne,
+                                                s, r, child_ne};
+                            reg_join_any<true>(R, sv, sa, ne, A, iend, ie, seeds[i], ro);
+                        } else {
+                            join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, a.M.q, a.M.K, out + inc - c,
+                                      a.M.parent, a.M.rule, a.M.cne, s, r, child_ne);
+                        }
+                    }
                     out += __shfl(inc, WAVE - 1, WAVE);
                 } else {
                     // The run's shared tally: once it passes what the run may still take, the

@@ -213,6 +213,19 @@ static void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<
                     if (bound & (1u << R.lv[pe][i])) { R.piv[d] = (uint8_t)i; break; }
             bound |= var_mask(R.lv[pe], R.la[pe]);
         }
+        std::memset(R.jsrc, 0xFF, sizeof(R.jsrc));
+        {
+            uint8_t first[32];
+            std::memset(first, 0xFF, sizeof(first));
+            for (int d = 0; d < R.n_lhs; ++d) {
+                const int pe = R.order[d];
+                for (int i = 0; i < R.la[pe]; ++i) {
+                    const uint8_t v = R.lv[pe][i];
+                    if (first[v] != 0xFF) R.jsrc[d][i] = first[v];
+                    else first[v] = (uint8_t)((d << 4) | i);
+                }
+            }
+        }
         // anchored orders: position p first, then most shared bound variables (ties: lower
         // position); the choice only affects join cost, never the set of matches
         std::memset(R.aord, 0, sizeof(R.aord));

@@ -190,140 +190,203 @@ __device__ inline uint64_t wl_hash_wave(WLSpace& s) {
     if (nE == 0) return EMPTY_STATE_HASH;     // hypergraph.cpp:743-745
     if (nV == 0) return 0;                    // wl_core.hpp:85
 
-    // ---- occurrence CSR (wl_core.hpp:87-97), with each vertex's neighbour-list length ----
-    for (uint32_t v = l; v <= nV; v += WAVE) {
-        s.occ_off[v] = 0;
-        if (v < nV) s.nnv[v] = 0;
-    }
-    wave_sync();
-    uint32_t amin = 0xFFu, amax = 0;
+    // Edges of one arity 2..4 (the common case) and fewer than 256 of them: a vertex's sorted
+    // initial keys (arity, position) are its occurrence counts per position in position order,
+    // and nothing truncates (nbr_cap = max (arity-1)*deg >= deg), so the occurrence CSR is not
+    // needed: count per position, size each neighbour segment (arity-1)*deg, and let every
+    // occurrence write its neighbours straight into its vertex's segment (the segment is sorted
+    // before it is folded, so the order of writes does not matter).
+    uint32_t amn = 0xFFu, amx = 0;
     for (uint32_t e = l; e < nE; e += WAVE) {
         const uint32_t a = s.e_ar[e];
-        amin = a < amin ? a : amin;
-        amax = a > amax ? a : amax;
-        for (uint32_t p = 0; p < a; ++p) {
-            const uint32_t x = s.e_lv[e * A + p];
-            atomicAdd(&s.occ_off[x], 1u);
-            if (a > 1) atomicAdd(&s.nnv[x], a - 1);   // wl_hash.hpp:201-210
-        }
+        amn = a < amn ? a : amn;
+        amx = a > amx ? a : amx;
     }
-    const bool uniform = wave_max_u32(amax) == (0xFFu - wave_max_u32(0xFFu - amin));
-    const uint32_t ua = wave_max_u32(amax);
-    wave_sync();
-    const uint32_t nocc = wave_excl_scan_lds(s.occ_off, nV + 1);
-    for (uint32_t v = l; v <= nV; v += WAVE) s.cursor[v] = s.occ_off[v];
-    wave_sync();
-    for (uint32_t e = l; e < nE; e += WAVE)
-        for (uint32_t p = 0; p < s.e_ar[e]; ++p) {
-            const uint32_t w = atomicAdd(&s.cursor[s.e_lv[e * A + p]], 1u);
-            s.occ_ep[w] = (e << 8) | p;
-        }
-    for (uint32_t v = l; v < nV; v += WAVE)
-        for (uint32_t i = s.occ_off[v]; i < s.occ_off[v + 1]; ++i) s.oown[i] = v;
-    wave_sync();
-    // Each bucket into (edge, pos) order -- the order the reference's counting sort leaves.
-    // The order only matters where the initial colour truncates a vertex's occurrences
-    // (deg > nbr_cap), which needs unary edges: with every arity >= 2, nn >= deg for each
-    // vertex, and every later use of a bucket sorts what it reads.
-    const bool unary = (0xFFu - wave_max_u32(0xFFu - amin)) <= 1;
-    if (unary) {
-        seg_rank_sort(s.occ_ep, s.nb, s.oown, s.occ_off, (const uint32_t*)nullptr, nocc);
-        for (uint32_t i = l; i < nocc; i += WAVE) s.occ_ep[i] = s.nb[i];
-    }
-    uint32_t nn_max = 0;
-    for (uint32_t v = l; v < nV; v += WAVE) nn_max = s.nnv[v] > nn_max ? s.nnv[v] : nn_max;
-    uint32_t nbr_cap = wave_max_u32(nn_max);
-    if (nbr_cap < 1) nbr_cap = 1;
-    wave_sync();
-    // Segment sizes: room for the refinement list (nn) and the initial list (min(deg, cap)).
-    uint32_t cnt_max = 0;
-    for (uint32_t v = l; v < nV; v += WAVE) {
-        const uint32_t deg = s.occ_off[v + 1] - s.occ_off[v];
-        const uint32_t init = deg < nbr_cap ? deg : nbr_cap;
-        const uint32_t nn = s.nnv[v];
-        s.cursor[v] = nn > init ? nn : init;
-        cnt_max = init > cnt_max ? init : cnt_max;
-    }
-    cnt_max = wave_max_u32(cnt_max);
-    wave_sync();
-    wave_excl_scan_lds(s.cursor, nV + 1);                       // cursor[v] = segment offset
-    const uint32_t nslots = s.cursor[nV];                       // (cursor[nV] held no size)
-    for (uint32_t v = l; v < nV; v += WAVE)
-        for (uint32_t k = s.cursor[v]; k < s.cursor[v + 1]; ++k) s.own[k] = v;
-    wave_sync();
-
-    // ---- initial colours (wl_core.hpp:93-107): fnv(deg), then the first min(deg, cap)
-    //      occurrences' (arity, position) keys in ascending order ----
-    if (cnt_max <= 8) {
-        for (uint32_t v = l; v < nV; v += WAVE) {
-            const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
-            const uint32_t cnt = deg < nbr_cap ? deg : nbr_cap;
-            uint64_t h = fnv(FNV_OFFSET, deg);
-            uint64_t x[8];
-#pragma unroll
-            for (uint32_t i = 0; i < 8; ++i) {
-                uint64_t key = ~0ull;
-                if (i < cnt) {
-                    const uint32_t ep = s.occ_ep[b + i];
-                    key = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
-                }
-                x[i] = key;
+    amx = wave_max_u32(amx);
+    amn = 0xFFu - wave_max_u32(0xFFu - amn);
+    const bool fast = amn == amx && amx >= 2 && amx <= 4 && nE < 256;
+    uint32_t nbr_cap = 1, nslots = 0;
+    if (fast) {
+        const uint32_t ua = amx;
+        uint32_t* deg = s.occ_off;                     // [NV]: occurrences, then write cursors
+        uint32_t* pcnt = s.comp;                       // [NV]: occurrences per position, 8 bits each
+        for (uint32_t v = l; v < nV; v += WAVE) { deg[v] = 0; pcnt[v] = 0; }
+        wave_sync();
+        for (uint32_t e = l; e < nE; e += WAVE)
+            for (uint32_t p = 0; p < ua; ++p) {
+                const uint32_t x = s.e_lv[e * A + p];
+                atomicAdd(&deg[x], 1u);
+                atomicAdd(&pcnt[x], 1u << (8 * p));
             }
-            sort8(x);
-#pragma unroll
-            for (uint32_t i = 0; i < 8; ++i)
-                if (i < cnt) { h = fnv(h, x[i] >> 8); h = fnv(h, x[i] & 0xFFu); }
-            s.cur[v] = h;
+        wave_sync();
+        uint32_t nn_max = 0;
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t nn = deg[v] * (ua - 1);
+            s.nnv[v] = nn;
+            s.cursor[v] = nn;
+            nn_max = nn > nn_max ? nn : nn_max;
         }
+        nbr_cap = wave_max_u32(nn_max);
+        if (l == 0) s.cursor[nV] = 0;
+        wave_sync();
+        wave_excl_scan_lds(s.cursor, nV + 1);          // cursor[v] = segment offset
+        nslots = s.cursor[nV];
+        for (uint32_t v = l; v < nV; v += WAVE) {
+            const uint32_t b = s.cursor[v], d = deg[v];
+            for (uint32_t k = b; k < b + s.nnv[v]; ++k) s.own[k] = v;
+            // initial colour (wl_core.hpp:93-107): fnv(deg), then (arity, position) per occurrence
+            // in ascending key order -- position order, each position as often as it occurs
+            uint64_t h = fnv(FNV_OFFSET, d);
+            const uint32_t pc = pcnt[v];
+            for (uint32_t p = 0; p < ua; ++p)
+                for (uint32_t i = 0; i < ((pc >> (8 * p)) & 0xFFu); ++i) { h = fnv(h, ua); h = fnv(h, p); }
+            s.cur[v] = h;
+            deg[v] = b;                                // write cursor
+        }
+        wave_sync();
+        for (uint32_t e = l; e < nE; e += WAVE)
+            for (uint32_t p = 0; p < ua; ++p) {
+                uint32_t o = atomicAdd(&deg[s.e_lv[e * A + p]], ua - 1);
+                for (uint32_t k = 0; k < ua; ++k)
+                    if (k != p) s.nb[o++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
+            }
+        wave_sync();
     } else {
-        for (uint32_t v = l; v < nV; v += WAVE) {       // init counts, kept in comp until the components
+        // ---- occurrence CSR (wl_core.hpp:87-97), with each vertex's neighbour-list length ----
+        for (uint32_t v = l; v <= nV; v += WAVE) {
+            s.occ_off[v] = 0;
+            if (v < nV) s.nnv[v] = 0;
+        }
+        wave_sync();
+        uint32_t amin = 0xFFu, amax = 0;
+        for (uint32_t e = l; e < nE; e += WAVE) {
+            const uint32_t a = s.e_ar[e];
+            amin = a < amin ? a : amin;
+            amax = a > amax ? a : amax;
+            for (uint32_t p = 0; p < a; ++p) {
+                const uint32_t x = s.e_lv[e * A + p];
+                atomicAdd(&s.occ_off[x], 1u);
+                if (a > 1) atomicAdd(&s.nnv[x], a - 1);   // wl_hash.hpp:201-210
+            }
+        }
+        const bool uniform = wave_max_u32(amax) == (0xFFu - wave_max_u32(0xFFu - amin));
+        const uint32_t ua = wave_max_u32(amax);
+        wave_sync();
+        const uint32_t nocc = wave_excl_scan_lds(s.occ_off, nV + 1);
+        for (uint32_t v = l; v <= nV; v += WAVE) s.cursor[v] = s.occ_off[v];
+        wave_sync();
+        for (uint32_t e = l; e < nE; e += WAVE)
+            for (uint32_t p = 0; p < s.e_ar[e]; ++p) {
+                const uint32_t w = atomicAdd(&s.cursor[s.e_lv[e * A + p]], 1u);
+                s.occ_ep[w] = (e << 8) | p;
+            }
+        for (uint32_t v = l; v < nV; v += WAVE)
+            for (uint32_t i = s.occ_off[v]; i < s.occ_off[v + 1]; ++i) s.oown[i] = v;
+        wave_sync();
+        // Each bucket into (edge, pos) order -- the order the reference's counting sort leaves.
+        // The order only matters where the initial colour truncates a vertex's occurrences
+        // (deg > nbr_cap), which needs unary edges: with every arity >= 2, nn >= deg for each
+        // vertex, and every later use of a bucket sorts what it reads.
+        const bool unary = (0xFFu - wave_max_u32(0xFFu - amin)) <= 1;
+        if (unary) {
+            seg_rank_sort(s.occ_ep, s.nb, s.oown, s.occ_off, (const uint32_t*)nullptr, nocc);
+            for (uint32_t i = l; i < nocc; i += WAVE) s.occ_ep[i] = s.nb[i];
+        }
+        uint32_t nn_max = 0;
+        for (uint32_t v = l; v < nV; v += WAVE) nn_max = s.nnv[v] > nn_max ? s.nnv[v] : nn_max;
+        nbr_cap = wave_max_u32(nn_max);
+        if (nbr_cap < 1) nbr_cap = 1;
+        wave_sync();
+        // Segment sizes: room for the refinement list (nn) and the initial list (min(deg, cap)).
+        uint32_t cnt_max = 0;
+        for (uint32_t v = l; v < nV; v += WAVE) {
             const uint32_t deg = s.occ_off[v + 1] - s.occ_off[v];
-            s.comp[v] = deg < nbr_cap ? deg : nbr_cap;
+            const uint32_t init = deg < nbr_cap ? deg : nbr_cap;
+            const uint32_t nn = s.nnv[v];
+            s.cursor[v] = nn > init ? nn : init;
+            cnt_max = init > cnt_max ? init : cnt_max;
         }
+        cnt_max = wave_max_u32(cnt_max);
         wave_sync();
-        for (uint32_t j = l; j < nslots; j += WAVE) {
-            const uint32_t v = s.own[j], i = j - s.cursor[v];
-            if (i < s.comp[v]) {
-                const uint32_t ep = s.occ_ep[s.occ_off[v] + i];
-                s.nbr[j] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+        wave_excl_scan_lds(s.cursor, nV + 1);                       // cursor[v] = segment offset
+        nslots = s.cursor[nV];                                      // (cursor[nV] held no size)
+        for (uint32_t v = l; v < nV; v += WAVE)
+            for (uint32_t k = s.cursor[v]; k < s.cursor[v + 1]; ++k) s.own[k] = v;
+        wave_sync();
+
+        // ---- initial colours (wl_core.hpp:93-107): fnv(deg), then the first min(deg, cap)
+        //      occurrences' (arity, position) keys in ascending order ----
+        if (cnt_max <= 8) {
+            for (uint32_t v = l; v < nV; v += WAVE) {
+                const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
+                const uint32_t cnt = deg < nbr_cap ? deg : nbr_cap;
+                uint64_t h = fnv(FNV_OFFSET, deg);
+                uint64_t x[8];
+    #pragma unroll
+                for (uint32_t i = 0; i < 8; ++i) {
+                    uint64_t key = ~0ull;
+                    if (i < cnt) {
+                        const uint32_t ep = s.occ_ep[b + i];
+                        key = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+                    }
+                    x[i] = key;
+                }
+                sort8(x);
+    #pragma unroll
+                for (uint32_t i = 0; i < 8; ++i)
+                    if (i < cnt) { h = fnv(h, x[i] >> 8); h = fnv(h, x[i] & 0xFFu); }
+                s.cur[v] = h;
+            }
+        } else {
+            for (uint32_t v = l; v < nV; v += WAVE) {       // init counts, kept in comp until the components
+                const uint32_t deg = s.occ_off[v + 1] - s.occ_off[v];
+                s.comp[v] = deg < nbr_cap ? deg : nbr_cap;
+            }
+            wave_sync();
+            for (uint32_t j = l; j < nslots; j += WAVE) {
+                const uint32_t v = s.own[j], i = j - s.cursor[v];
+                if (i < s.comp[v]) {
+                    const uint32_t ep = s.occ_ep[s.occ_off[v] + i];
+                    s.nbr[j] = ((uint64_t)s.e_ar[ep >> 8] << 8) | (uint64_t)(ep & 0xFFu);
+                }
+            }
+            wave_sync();
+            seg_rank_sort(s.nbr, s.srt, s.own, s.cursor, s.comp, nslots);
+            for (uint32_t v = l; v < nV; v += WAVE) {
+                const uint32_t so = s.cursor[v], cnt = s.comp[v];
+                uint64_t h = fnv(FNV_OFFSET, s.occ_off[v + 1] - s.occ_off[v]);
+                for (uint32_t i = 0; i < cnt; ++i) { const uint64_t x = s.srt[so + i]; h = fnv(h, x >> 8); h = fnv(h, x & 0xFFu); }
+                s.cur[v] = h;
             }
         }
-        wave_sync();
-        seg_rank_sort(s.nbr, s.srt, s.own, s.cursor, s.comp, nslots);
-        for (uint32_t v = l; v < nV; v += WAVE) {
-            const uint32_t so = s.cursor[v], cnt = s.comp[v];
-            uint64_t h = fnv(FNV_OFFSET, s.occ_off[v + 1] - s.occ_off[v]);
-            for (uint32_t i = 0; i < cnt; ++i) { const uint64_t x = s.srt[so + i]; h = fnv(h, x >> 8); h = fnv(h, x & 0xFFu); }
-            s.cur[v] = h;
-        }
-    }
 
-    // ---- each vertex's neighbour list, fixed for every refinement round: (neighbour, position)
-    //      for each occurrence and other position (nbr_cap never truncates it: nbr_cap is the
-    //      largest such count) ----
-    if (uniform && ua > 1) {
-        // one lane per occurrence: the i-th occurrence of v fills slots i*(a-1) .. of v's segment
-        for (uint32_t j = l; j < nocc; j += WAVE) {
-            const uint32_t v = s.oown[j], i = j - s.occ_off[v];
-            const uint32_t ep = s.occ_ep[j], e = ep >> 8, mypos = ep & 0xFFu;
-            uint32_t o = s.cursor[v] + i * (ua - 1);
-            for (uint32_t k = 0; k < ua; ++k)
-                if (k != mypos) s.nb[o++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
-        }
-    } else {
-        for (uint32_t v = l; v < nV; v += WAVE) {
-            const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
-            uint32_t o = s.cursor[v];
-            for (uint32_t j = 0; j < deg; ++j) {
-                const uint32_t ep = s.occ_ep[b + j];
-                const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
-                const uint32_t a = s.e_ar[e];
-                for (uint32_t k = 0; k < a; ++k)
+        // ---- each vertex's neighbour list, fixed for every refinement round: (neighbour, position)
+        //      for each occurrence and other position (nbr_cap never truncates it: nbr_cap is the
+        //      largest such count) ----
+        if (uniform && ua > 1) {
+            // one lane per occurrence: the i-th occurrence of v fills slots i*(a-1) .. of v's segment
+            for (uint32_t j = l; j < nocc; j += WAVE) {
+                const uint32_t v = s.oown[j], i = j - s.occ_off[v];
+                const uint32_t ep = s.occ_ep[j], e = ep >> 8, mypos = ep & 0xFFu;
+                uint32_t o = s.cursor[v] + i * (ua - 1);
+                for (uint32_t k = 0; k < ua; ++k)
                     if (k != mypos) s.nb[o++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
             }
+        } else {
+            for (uint32_t v = l; v < nV; v += WAVE) {
+                const uint32_t b = s.occ_off[v], deg = s.occ_off[v + 1] - b;
+                uint32_t o = s.cursor[v];
+                for (uint32_t j = 0; j < deg; ++j) {
+                    const uint32_t ep = s.occ_ep[b + j];
+                    const uint32_t e = ep >> 8, mypos = ep & 0xFFu;
+                    const uint32_t a = s.e_ar[e];
+                    for (uint32_t k = 0; k < a; ++k)
+                        if (k != mypos) s.nb[o++] = ((uint32_t)s.e_lv[e * A + k] << 4) | k;
+                }
+            }
         }
+        wave_sync();
     }
-    wave_sync();
     uint64_t* cur = s.cur;
     uint64_t* nxt = s.nxt;
     uint32_t prev_d = wl_ndistinct(s, cur);
