@@ -17,7 +17,7 @@ from .rules import (Request, Rule, RuleBuilder, encode_roots, encode_rules, enco
 
 __all__ = ["Context", "Result", "Request", "Rule", "RuleBuilder", "make_rule", "rule_from_lists",
            "encode_rules", "encode_state", "encode_roots", "library_path", "load_library",
-           "HGXError", "EXPORTED_SYMBOLS"]
+           "HGXError", "EXPORTED_SYMBOLS", "wxf_validate"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = [
     "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
+    "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -92,6 +93,11 @@ def load_library():
     L.hgx_batch_evolve.argtypes = [vp, vp, C.POINTER(vp)]
     L.hgx_batch_release.argtypes = [vp]
     L.hgx_batch_release.restype = None
+    L.hgx_result_consumed.argtypes = [vp, vp, vp, C.c_uint64]
+    L.hgx_run_wxf.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]
+    L.hgx_buffer_free.argtypes = [C.POINTER(C.c_uint8)]
+    L.hgx_buffer_free.restype = None
+    L.hgx_wxf_validate.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
     L.hgx_profile_read.argtypes = [vp, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)]
     _LIB = L
     return L
@@ -178,6 +184,14 @@ class Result:
             self.close()
         except Exception:
             pass
+
+
+def wxf_validate(job: bytes):
+    """Parse and check a WXF job without a device: (True, "") or (False, reason)."""
+    L = load_library()
+    buf = C.create_string_buffer(1024)
+    rc = L.hgx_wxf_validate(job, len(job), buf, 1024)
+    return rc == abi.HGX_OK, buf.value.decode()
 
 
 class PreparedBatch:
@@ -321,6 +335,18 @@ class Context:
         n = cnt.value
         mat = out[: min(n, cap) * k].reshape(-1, k) if want_edges else None
         return n, mat, ms.value
+
+    def run_wxf(self, job: bytes) -> bytes:
+        """The paclet's job envelope: a WXF job in, the WXF result out (hgx_run_wxf)."""
+        out = C.POINTER(C.c_uint8)()
+        n = C.c_size_t()
+        rc = self._lib.hgx_run_wxf(self._h, job, len(job), C.byref(out), C.byref(n))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        try:
+            return C.string_at(out, n.value)
+        finally:
+            self._lib.hgx_buffer_free(out)
 
     # ---- served path: continuous batching ----
 

@@ -457,6 +457,11 @@ uint32_t Engine::step() {
             C->ev.prod = dalloc<uint64_t>(C->meta, M);
             C->ev.tr = dalloc<uint64_t>(C->meta, M);
             scan_u32_to_u64(tmp, mv.cne, v.eoff, (uint32_t)M, st);
+            if (keep) {                          // consumed positions, for event edge readback
+                C->cq = dalloc<uint16_t>(C->meta, M * K);
+                C->cK = K;
+                HIP_TRY(hipMemcpyAsync(C->cq, mv.q, sizeof(uint16_t) * M * K, hipMemcpyDeviceToDevice, st));
+            }
         }
         uint64_t EC = 0;                         // child edges, from the count pass's per-run sums
         for (uint32_t i = 0; i < nF; ++i)
@@ -1107,6 +1112,42 @@ extern "C" int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64
                 pairs[o++] = hgx_pair{tmp[2 * i], tmp[2 * i + 1]};
             }
         }
+        return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_result_consumed(const hgx_result* r, uint32_t* offsets, uint16_t* positions, uint64_t cap) {
+    if (!r) return HGX_ERR_INVALID;
+    const RunRec& R = *r->run;
+    if (!(R.hp->flags & HGX_REC_STATES)) return HGX_ERR_INVALID;
+    try {
+        uint64_t o = 0, ev = 0;
+        for (size_t k = 1; k < R.segs.size(); ++k) {            // segment 0 holds the roots
+            const Segment& sg = R.segs[k];
+            const uint32_t m = sg.e - sg.b;
+            if (!m) continue;
+            const uint32_t K = sg.g->cK;
+            if (!sg.g->cq || !K) throw Error{HGX_ERR_INVALID, "consumed positions were not kept"};
+            std::vector<uint16_t> q((size_t)m * K);
+            std::vector<uint32_t> rule(m);
+            d2h(q.data(), sg.g->cq + (size_t)sg.b * K, q.size(), R.st);
+            d2h(rule.data(), sg.g->ev.rule + sg.b, m, R.st);
+            HIP_TRY(hipStreamSynchronize(R.st));
+            for (uint32_t i = 0; i < m; ++i, ++ev) {
+                const uint32_t nl = R.hp->rules.at(rule[i]).n_lhs;
+                if (offsets) offsets[ev] = (uint32_t)o;
+                for (uint32_t t = 0; t < nl; ++t) {
+                    if (positions) {
+                        if (o >= cap) return HGX_ERR_INVALID;
+                        positions[o] = q[(size_t)i * K + t];
+                    }
+                    ++o;
+                }
+            }
+        }
+        if (offsets) offsets[ev] = (uint32_t)o;
         return HGX_OK;
     } catch (const Error& e) {
         return e.code;

@@ -30,6 +30,8 @@ struct DevGen {
     uint64_t n_edges = 0;
     uint32_t max_ne = 0, max_nextv = 0;
     uint32_t* br_pairs = nullptr;    // branchial pairs of the events that created these states
+    uint16_t* cq = nullptr;          // with kept edges: consumed edge positions of each event
+    uint32_t cK = 0;                 //   (stride)
     bool keep_edges = false;
     explicit DevGen(std::shared_ptr<ChunkPool> p)
         : pool(p), meta(p.get(), (size_t)1 << 20), edges(p.get(), (size_t)4 << 20) {}

@@ -1,0 +1,459 @@
+// hgx_job.cpp -- the paclet's job envelope: a WXF job in, a WXF result out, evolved on the device.
+//
+// The counterpart of run_rewriting_core (paclet_source/hg_core.hpp:21-22) on the multiway path
+// this library serves: the job keys and option semantics of hypergraph_ffi.cpp:136-376 (parse)
+// and the result layout of hg_gpu_backend.cpp:362-640 (States, Events, CausalEdges,
+// BranchialEdges, Num*). It is written over the library's own C ABI (hgx_evolve_batch and the
+// hgx_result_* readbacks).
+//
+// Served: full multiway evolution (CanonicalizeStates -> None, CanonicalizeEvents -> None), any
+// rule set, one or more initial states, causal edges with or without transitive reduction,
+// branchial edges, RequestedData selection. Options that change what is explored (sampling,
+// caps, canonical exploration, sessions, genesis events, graph properties) are refused with an
+// error rather than ignored, so a caller never receives a different evolution than it asked
+// for; options with no effect on the result (ShowProgress, RandomSeed, ...) are accepted and
+// reported under "Warnings" as skipped, with the reference's Kind/Count/Context schema.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "hgx_runtime.hpp"
+#include "hgx_wxf.hpp"
+
+namespace hgx {
+namespace job {
+
+using wxf::Value;
+
+struct Job {
+    std::vector<std::vector<std::vector<int64_t>>> states;          // initial states as given
+    std::vector<std::pair<std::vector<std::vector<int64_t>>, std::vector<std::vector<int64_t>>>> rules;
+    int64_t steps = 1;
+    bool tr = true;                          // CausalTransitiveReduction (FFI default True)
+    bool inc_states = true, inc_events = true, inc_events_min = false, inc_causal = true,
+         inc_branchial = true, inc_nstates = true, inc_nevents = true, inc_ncausal = true,
+         inc_nbranchial = true, inc_hashes = false;
+    std::vector<std::pair<std::string, std::string>> warnings;   // (kind, context)
+};
+
+struct JobError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+static int64_t as_int(const Value& v, const char* what) {
+    if (v.kind != Value::Int) throw JobError(std::string(what) + " must be an integer");
+    return v.i;
+}
+
+static bool as_bool(const Value& v, const char* what) {
+    if (v.is_symbol("True")) return true;
+    if (v.is_symbol("False")) return false;
+    throw JobError(std::string(what) + " must be True or False");
+}
+
+static std::string as_name(const Value& v) {
+    if (v.kind == Value::String || v.kind == Value::Symbol) return v.s;
+    throw JobError("expected a string or symbol");
+}
+
+static const Value& need_list(const Value& v, const char* what) {
+    if (!v.is_list()) throw JobError(std::string(what) + " must be a list");
+    return v;
+}
+
+// {{v...}, ...}
+static std::vector<std::vector<int64_t>> edges_of(const Value& v, const char* what) {
+    need_list(v, what);
+    std::vector<std::vector<int64_t>> out;
+    for (size_t k = 0; k < v.nargs(); ++k) {
+        const Value& e = need_list(v.arg(k), what);
+        std::vector<int64_t> ed;
+        for (size_t j = 0; j < e.nargs(); ++j) ed.push_back(as_int(e.arg(j), what));
+        out.push_back(std::move(ed));
+    }
+    return out;
+}
+
+static Job parse(const Value& root) {
+    if (root.kind != Value::Assoc) throw JobError("the job must be an Association");
+    Job j;
+    bool have_rules = false;
+    for (const auto& kv : root.rules) {
+        const std::string key = as_name(kv.first);
+        const Value& val = kv.second;
+        if (key == "InitialStates") {
+            need_list(val, "InitialStates");
+            for (size_t k = 0; k < val.nargs(); ++k) j.states.push_back(edges_of(val.arg(k), "InitialStates"));
+        } else if (key == "InitialEdges") {
+            j.states.push_back(edges_of(val, "InitialEdges"));
+        } else if (key == "Rules") {
+            if (val.kind != Value::Assoc) throw JobError("Rules must be an Association of name -> Rule[lhs, rhs]");
+            for (const auto& r : val.rules) {
+                const Value& rv = r.second;
+                if (!rv.is_head("Rule") || rv.nargs() != 2) throw JobError("Expected Rule[lhs, rhs]");
+                j.rules.emplace_back(edges_of(rv.arg(0), "rule LHS"), edges_of(rv.arg(1), "rule RHS"));
+            }
+            have_rules = true;
+        } else if (key == "Steps") {
+            j.steps = as_int(val, "Steps");
+            if (j.steps < 0) throw JobError("Steps must be non-negative, got " + std::to_string(j.steps));
+        } else if (key == "Options") {
+            if (val.kind != Value::Assoc) throw JobError("Options must be an Association");
+            for (const auto& o : val.rules) {
+                const std::string ok = as_name(o.first);
+                const Value& ov = o.second;
+                auto refuse = [&](const std::string& why) {
+                    throw JobError("option " + ok + ": " + why + " (this backend evolves the full multiway system)");
+                };
+                if (ok == "CanonicalizeStates") {
+                    const std::string m = as_name(ov);
+                    if (m != "None" && m != "False") refuse("only None is served");
+                } else if (ok == "CanonicalizeEvents") {
+                    if (!(ov.kind == Value::Symbol && ov.s == "None")) refuse("only None is served");
+                } else if (ok == "CausalTransitiveReduction") {
+                    j.tr = as_bool(ov, ok.c_str());
+                } else if (ok == "IncludeCanonicalHashes") {
+                    j.inc_hashes = as_bool(ov, ok.c_str());
+                } else if (ok == "ExploreFromCanonicalStatesOnly" || ok == "ShowGenesisEvents" ||
+                           ok == "QuotientInitialStates") {
+                    if (as_bool(ov, ok.c_str())) refuse("True is not served");
+                } else if (ok == "MaxSuccessorStatesPerParent" || ok == "MaxStatesPerStep" ||
+                           ok == "MatchesPerStep" || ok == "MatchesPerStateRule") {
+                    if (as_int(ov, ok.c_str()) != 0) refuse("caps are not served");
+                } else if (ok == "ExplorationProbability" || ok == "TransitionRate") {
+                    const double x = ov.kind == Value::Real ? ov.r : (double)as_int(ov, ok.c_str());
+                    if (x != 1.0) refuse("sampling is not served");
+                } else if (ok == "RuleWeights") {
+                    if (!ov.is_list() || ov.nargs() != 0) refuse("sampling is not served");
+                } else if (ok == "GraphProperties") {
+                    if (!ov.is_list() || ov.nargs() != 0) refuse("graph properties are not served");
+                } else if (ok == "BranchialStep") {
+                    if (as_int(ov, ok.c_str()) != 0) refuse("per-step branchial selection is not served");
+                } else if (ok == "RequestedData") {
+                    need_list(ov, "RequestedData");
+                    j.inc_states = j.inc_events = j.inc_events_min = j.inc_causal = j.inc_branchial = false;
+                    j.inc_nstates = j.inc_nevents = j.inc_ncausal = j.inc_nbranchial = false;
+                    for (size_t k = 0; k < ov.nargs(); ++k) {
+                        const std::string c = as_name(ov.arg(k));
+                        if (c == "States") j.inc_states = true;
+                        else if (c == "Events") j.inc_events = true;
+                        else if (c == "EventsMinimal") j.inc_events_min = true;
+                        else if (c == "CausalEdges") j.inc_causal = true;
+                        else if (c == "BranchialEdges") j.inc_branchial = true;
+                        else if (c == "NumStates") j.inc_nstates = true;
+                        else if (c == "NumEvents") j.inc_nevents = true;
+                        else if (c == "NumCausalEdges") j.inc_ncausal = true;
+                        else if (c == "NumBranchialEdges") j.inc_nbranchial = true;
+                        else refuse("component " + c + " is not served");
+                    }
+                } else {
+                    // ShowProgress, RandomSeed, UniformRandom, EdgeDeduplication and unknown keys:
+                    // nothing here depends on them
+                    j.warnings.emplace_back("OptionSkipped", ok);
+                }
+            }
+        } else if (key == "Op") {
+            const std::string op = as_name(val);
+            if (!op.empty() && op != "Evolve") throw JobError("Op " + op + ": sessions are not served");
+        } else if (key == "Session" || key == "From" || key == "Delivery") {
+            throw JobError(key + ": sessions are not served");
+        } else {
+            j.warnings.emplace_back("KeySkipped", key);
+        }
+    }
+    if (!have_rules || j.rules.empty()) throw JobError("the job has no Rules");
+    return j;
+}
+
+// Rule program bytes (per rule: n_lhs, n_rhs, then (arity, vars...) per LHS then RHS edge), with
+// the FFI's checks: variables 0..31, arity <= 16, at most 16 edges per side, empty edges dropped,
+// a non-empty LHS (hypergraph_ffi.cpp:520-590).
+static std::vector<uint8_t> program(const Job& j) {
+    std::vector<uint8_t> out;
+    for (size_t r = 0; r < j.rules.size(); ++r) {
+        auto side = [&](const std::vector<std::vector<int64_t>>& edges, const char* name,
+                        std::vector<std::vector<uint8_t>>& dst) {
+            for (size_t e = 0; e < edges.size(); ++e) {
+                if (edges[e].empty()) continue;
+                if (dst.size() >= HGX_MAX_PATTERN_EDGES)
+                    throw JobError("rule " + std::to_string(r) + " " + name + " has more than 16 edges");
+                if (edges[e].size() > HGX_MAX_ARITY)
+                    throw JobError("rule " + std::to_string(r) + " " + name + " edge " + std::to_string(e) +
+                                   " has arity above 16");
+                std::vector<uint8_t> ed;
+                for (int64_t v : edges[e]) {
+                    if (v < 0) throw JobError("rule " + std::to_string(r) + " " + name + " edge " +
+                                              std::to_string(e) + " has a negative pattern variable");
+                    if (v >= HGX_MAX_VARS) throw JobError("rule " + std::to_string(r) + " " + name +
+                                                          " uses pattern variable " + std::to_string(v) +
+                                                          ", but the maximum is 31");
+                    ed.push_back((uint8_t)v);
+                }
+                dst.push_back(std::move(ed));
+            }
+        };
+        std::vector<std::vector<uint8_t>> lhs, rhs;
+        side(j.rules[r].first, "LHS", lhs);
+        side(j.rules[r].second, "RHS", rhs);
+        if (lhs.empty()) throw JobError("rule " + std::to_string(r) + " has an empty left-hand side");
+        out.push_back((uint8_t)lhs.size());
+        out.push_back((uint8_t)rhs.size());
+        for (auto* sd : {&lhs, &rhs})
+            for (auto& ed : *sd) {
+                out.push_back((uint8_t)ed.size());
+                out.insert(out.end(), ed.begin(), ed.end());
+            }
+    }
+    return out;
+}
+
+// Initial states as the FFI takes them: negative vertices and empty edges dropped, empty states
+// dropped (hypergraph_ffi.cpp:598-636); vertex renumbering happens in the library.
+static void roots(const Job& j, std::vector<uint32_t>& eoff, std::vector<uint32_t>& verts,
+                  std::vector<uint32_t>& roff) {
+    eoff.assign(1, 0);
+    roff.assign(1, 0);
+    for (const auto& st : j.states) {
+        uint32_t kept = 0;
+        for (const auto& e : st) {
+            uint32_t n = 0;
+            for (int64_t v : e) {
+                if (v < 0) continue;
+                if (v > 0xFFFFFFFFll) throw JobError("vertex id above 2^32-1");
+                verts.push_back((uint32_t)v);
+                ++n;
+            }
+            if (!n) continue;
+            if (n > HGX_MAX_ARITY) throw JobError("initial edge with arity above 16");
+            eoff.push_back((uint32_t)verts.size());
+            ++kept;
+        }
+        if (kept) roff.push_back((uint32_t)(eoff.size() - 1));
+    }
+}
+
+static Value ilist(const std::vector<int64_t>& xs) {
+    Value v = Value::list();
+    v.args.reserve(xs.size() + 1);
+    for (int64_t x : xs) v.push(Value::integer(x));
+    return v;
+}
+
+struct Run {
+    hgx_counts c{};
+    std::vector<uint64_t> hash;
+    std::vector<uint32_t> step, creator;
+    std::vector<hgx_event> events;
+    std::vector<hgx_pair> causal, branchial;
+    std::vector<uint8_t> causal_tr;
+    std::vector<uint64_t> soff;
+    std::vector<uint32_t> eoff, verts, coff;
+    std::vector<uint16_t> cpos;
+};
+
+static void check(int rc, hgx_ctx* ctx, const char* what) {
+    if (rc == HGX_OK) return;
+    char buf[512] = {0};
+    hgx_last_error(ctx, buf, sizeof(buf));
+    throw JobError(std::string(what) + " failed (status " + std::to_string(rc) + "): " + buf);
+}
+
+static Value evolve(hgx_ctx* ctx, const Job& j) {
+    const std::vector<uint8_t> prog = program(j);
+    std::vector<uint32_t> eoff, verts, roff;
+    roots(j, eoff, verts, roff);
+    const uint32_t n_roots = (uint32_t)roff.size() - 1;
+    if (n_roots == 0) throw JobError("the job has no non-empty initial state");
+    if (j.steps > HGX_MAX_STEPS) throw JobError("Steps above " + std::to_string(HGX_MAX_STEPS));
+    uint32_t flags = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL | HGX_REC_RELATIONS;
+    if (j.inc_states || j.inc_events) flags |= HGX_REC_STATES;
+    if (j.tr) flags |= HGX_OPT_TR;
+    hgx_run_desc d{};
+    d.rules = prog.data(); d.rules_len = (uint32_t)prog.size(); d.n_rules = (uint32_t)j.rules.size();
+    d.edge_offsets = eoff.data(); d.vertices = verts.empty() ? nullptr : verts.data();
+    d.n_edges = (uint32_t)(eoff.size() - 1);
+    d.root_offsets = roff.data(); d.n_roots = n_roots;
+    d.steps = (uint32_t)j.steps; d.state_cap = 0; d.flags = flags;
+    hgx_result* res = nullptr;
+    check(hgx_evolve_batch(ctx, &d, 1, &res), ctx, "evolution");
+    Run R;
+    struct Release { hgx_result* r; ~Release() { hgx_result_release(r); } } rel{res};
+    check(hgx_result_counts(res, &R.c), ctx, "counts");
+    const uint64_t ns = R.c.raw_states, ne = R.c.events;
+    R.hash.resize(ns); R.step.resize(ns); R.creator.resize(ns);
+    check(hgx_result_states(res, R.hash.data(), R.step.data(), R.creator.data(), ns), ctx, "states");
+    R.events.resize(ne);
+    if (ne) check(hgx_result_events(res, R.events.data(), ne), ctx, "events");
+    R.causal.resize(R.c.causal_pairs); R.causal_tr.resize(R.c.causal_pairs);
+    if (R.c.causal_pairs) check(hgx_result_causal(res, R.causal.data(), R.causal_tr.data(), R.c.causal_pairs), ctx, "causal");
+    R.branchial.resize(R.c.branchial_pairs);
+    if (R.c.branchial_pairs) check(hgx_result_branchial(res, R.branchial.data(), R.c.branchial_pairs), ctx, "branchial");
+    if (flags & HGX_REC_STATES) {
+        uint64_t te = 0, tv = 0;
+        check(hgx_result_state_edges(res, nullptr, nullptr, nullptr, &te, &tv), ctx, "state edges");
+        R.soff.resize(ns + 1); R.eoff.resize(te + 1); R.verts.resize(tv);
+        check(hgx_result_state_edges(res, R.soff.data(), R.eoff.data(), R.verts.data(), nullptr, nullptr), ctx,
+              "state edges");
+        R.coff.resize(ne + 1);
+        check(hgx_result_consumed(res, R.coff.data(), nullptr, 0), ctx, "consumed edges");
+        R.cpos.resize(R.coff[ne]);
+        check(hgx_result_consumed(res, R.coff.data(), R.cpos.data(), R.cpos.size()), ctx, "consumed edges");
+    }
+
+    // The representative of each WL class: its lowest state id (the first writer in id order)
+    std::unordered_map<uint64_t, uint32_t> rep_of_hash;
+    std::vector<int64_t> rep(ns);
+    for (uint64_t s = 0; s < ns; ++s) {
+        auto it = rep_of_hash.emplace(R.hash[s], (uint32_t)s).first;
+        rep[s] = it->second;
+    }
+
+    Value out = Value::assoc();
+    if (j.inc_states) {
+        Value sts = Value::assoc();
+        sts.rules.reserve(ns);
+        for (uint64_t s = 0; s < ns; ++s) {
+            Value sa = Value::assoc();
+            sa.put("Id", Value::integer((int64_t)s));
+            sa.put("CanonicalId", Value::integer(rep[s]));
+            sa.put("ContentStateId", Value::integer(rep[s]));
+            sa.put("Step", Value::integer(R.step[s]));
+            Value edges = Value::list();
+            for (uint64_t e = R.soff[s], k = 0; e < R.soff[s + 1]; ++e, ++k) {
+                Value ed = Value::list();
+                ed.push(Value::integer((int64_t)k));
+                for (uint32_t x = R.eoff[e]; x < R.eoff[e + 1]; ++x) ed.push(Value::integer(R.verts[x]));
+                edges.push(std::move(ed));
+            }
+            sa.put("Edges", std::move(edges));
+            sa.put("IsInitial", Value::boolean(s < n_roots));
+            if (j.inc_hashes) sa.put("CanonicalHash", Value::integer((int64_t)R.hash[s]));
+            sts.put((int64_t)s, std::move(sa));
+        }
+        out.put("States", std::move(sts));
+    }
+    if (j.inc_events || j.inc_events_min) {
+        Value evs = Value::assoc();
+        evs.rules.reserve(ne);
+        for (uint64_t e = 0; e < ne; ++e) {
+            const hgx_event& x = R.events[e];
+            Value ea = Value::assoc();
+            ea.put("Id", Value::integer((int64_t)e));
+            if (j.inc_events) {
+                ea.put("CanonicalId", Value::integer((int64_t)e));
+                ea.put("RuleIndex", Value::integer(x.rule));
+            }
+            ea.put("InputState", Value::integer(x.input_state));
+            ea.put("OutputState", Value::integer(x.output_state));
+            if (j.inc_events) {
+                ea.put("CanonicalInputState", Value::integer(rep[x.input_state]));
+                ea.put("CanonicalOutputState", Value::integer(rep[x.output_state]));
+                std::vector<int64_t> cons, prod;
+                for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k) cons.push_back(R.cpos[k]);
+                const uint64_t out_ne = R.soff[x.output_state + 1] - R.soff[x.output_state];
+                const size_t nrhs = j.rules[x.rule].second.size() -
+                                    (size_t)std::count_if(j.rules[x.rule].second.begin(), j.rules[x.rule].second.end(),
+                                                          [](const std::vector<int64_t>& ed) { return ed.empty(); });
+                for (uint64_t k = out_ne - nrhs; k < out_ne; ++k) prod.push_back((int64_t)k);
+                ea.put("ConsumedEdges", ilist(cons));
+                ea.put("ProducedEdges", ilist(prod));
+            }
+            evs.put((int64_t)e, std::move(ea));
+        }
+        out.put("Events", std::move(evs));
+    }
+    int64_t n_causal = 0;
+    {
+        Value cl = Value::list();
+        for (size_t k = 0; k < R.causal.size(); ++k) {
+            if (j.tr && !R.causal_tr[k]) continue;
+            ++n_causal;
+            if (!j.inc_causal) continue;
+            Value ed = Value::assoc();
+            ed.put("From", Value::integer(R.causal[k].a));
+            ed.put("To", Value::integer(R.causal[k].b));
+            ed.put("RawFrom", Value::integer(R.causal[k].a));
+            ed.put("RawTo", Value::integer(R.causal[k].b));
+            cl.push(std::move(ed));
+        }
+        if (j.inc_causal) out.put("CausalEdges", std::move(cl));
+    }
+    if (j.inc_branchial) {
+        Value bl = Value::list();
+        for (const auto& p : R.branchial) {
+            Value ed = Value::assoc();
+            ed.put("From", Value::integer(p.a));
+            ed.put("To", Value::integer(p.b));
+            bl.push(std::move(ed));
+        }
+        out.put("BranchialEdges", std::move(bl));
+    }
+    if (j.inc_nstates) out.put("NumStates", Value::integer((int64_t)R.c.canonical_states));
+    if (j.inc_nevents) out.put("NumEvents", Value::integer((int64_t)R.c.events));
+    if (j.inc_ncausal) out.put("NumCausalEdges", Value::integer(n_causal));
+    if (j.inc_nbranchial) out.put("NumBranchialEdges", Value::integer((int64_t)R.c.branchial_pairs));
+    if (!j.warnings.empty()) {
+        Value wl = Value::list();
+        for (auto& w : j.warnings) {
+            Value wa = Value::assoc();
+            wa.put("Kind", Value::string(w.first));
+            wa.put("Count", Value::integer(1));
+            wa.put("Context", Value::string(w.second));
+            wl.push(std::move(wa));
+        }
+        out.put("Warnings", std::move(wl));
+    }
+    return out;
+}
+
+static uint8_t* to_buffer(const std::vector<uint8_t>& bytes, size_t* n) {
+    uint8_t* p = (uint8_t*)std::malloc(bytes.size() ? bytes.size() : 1);
+    if (!p) throw JobError("out of host memory");
+    if (!bytes.empty()) std::memcpy(p, bytes.data(), bytes.size());
+    *n = bytes.size();
+    return p;
+}
+
+}  // namespace job
+}  // namespace hgx
+
+using namespace hgx::job;
+
+extern "C" int hgx_wxf_validate(const uint8_t* job, size_t n, char* err, size_t err_len) {
+    try {
+        if (!job) throw JobError("null job");
+        const Job j = parse(hgx::wxf::parse(job, n));
+        (void)program(j);
+        std::vector<uint32_t> eoff, verts, roff;
+        roots(j, eoff, verts, roff);
+        if (roff.size() < 2) throw JobError("the job has no non-empty initial state");
+        return HGX_OK;
+    } catch (const std::exception& e) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", e.what());
+        return HGX_ERR_INVALID;
+    }
+}
+
+extern "C" int hgx_run_wxf(hgx_ctx* ctx, const uint8_t* job, size_t n, uint8_t** out, size_t* out_n) {
+    if (!ctx || !job || !out || !out_n) return HGX_ERR_INVALID;
+    *out = nullptr;
+    *out_n = 0;
+    try {
+        const Job j = parse(hgx::wxf::parse(job, n));
+        const Value res = evolve(ctx, j);
+        *out = to_buffer(hgx::wxf::serialize(res), out_n);
+        return HGX_OK;
+    } catch (const std::exception& e) {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        ctx->err = e.what();
+        return HGX_ERR_INVALID;
+    }
+}
+
+extern "C" void hgx_buffer_free(uint8_t* p) { std::free(p); }

@@ -178,6 +178,21 @@ int  hgx_batch_prepare(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_b
 int  hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* batch, hgx_result** results);
 void hgx_batch_release(hgx_batch* batch);
 
+/* ---- the paclet's job envelope (WXF in, WXF out) -------------------------------------- */
+
+/* run_rewriting_core (paclet_source/hg_core.hpp:21-22) on this library: job is a WXF
+ * Association with the FFI's keys (InitialStates or InitialEdges, Rules as name ->
+ * Rule[lhs, rhs], Steps, Options; hypergraph_ffi.cpp:136-376). On HGX_OK *out holds a WXF
+ * Association shaped as the GPU backend's result (States, Events, CausalEdges, BranchialEdges,
+ * NumStates, NumEvents, NumCausalEdges, NumBranchialEdges, Warnings; hg_gpu_backend.cpp:362-640)
+ * of *out_n bytes, to be freed with hgx_buffer_free. Options that would change what is
+ * explored (sampling, caps, canonical exploration, sessions) give HGX_ERR_INVALID with the
+ * reason in hgx_last_error. */
+int  hgx_run_wxf(hgx_ctx* ctx, const uint8_t* job, size_t n, uint8_t** out, size_t* out_n);
+void hgx_buffer_free(uint8_t* p);
+/* Parse and check a WXF job without a device: HGX_OK, or HGX_ERR_INVALID with the reason. */
+int  hgx_wxf_validate(const uint8_t* job, size_t n, char* err, size_t err_len);
+
 /* ---- served path: continuous batching ------------------------------------------------- */
 
 /* Queue a request. It is admitted when its HBM footprint estimate fits what running
@@ -207,6 +222,14 @@ int hgx_result_counts_many(const hgx_result* const* rs, uint32_t n, hgx_counts* 
 int hgx_result_states(const hgx_result* r, uint64_t* hash, uint32_t* step,
                       uint32_t* creator_event, uint64_t cap);
 int hgx_result_events(const hgx_result* r, hgx_event* out, uint64_t cap);
+/* Consumed edges of every event (with HGX_REC_STATES): event e consumed the edges at
+ * positions[offsets[e] .. offsets[e+1]) of its input state's edge list (hgx_result_state_edges
+ * order), one per LHS edge in rule order; its produced edges are the last n_rhs edges of its
+ * output state. offsets has events + 1 entries; positions may be NULL to learn the total
+ * (offsets[events]). The device counterpart of Event::consumed_edges / produced_edges
+ * (hypergraph/src/hypergraph.cpp:432-520), as positions rather than global edge ids. */
+int hgx_result_consumed(const hgx_result* r, uint32_t* offsets, uint16_t* positions, uint64_t cap);
+
 /* Edge lists of every raw state (needs HGX_REC_STATES): state_offsets[raw_states+1] in
  * edges, edge_offsets[n_edges_total+1] in vertices, vertices[]. Call with NULL arrays to
  * learn the totals. Vertex labels are per-lineage: each root renumbered from 0, fresh
