@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = [
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
 OPTION_DELTA_CAP = 2   # HGX_OPTION_DELTA_CAP
+OPTION_EXPAND_TEAM = 3  # HGX_OPTION_EXPAND_TEAM
 
 
 class HGXError(RuntimeError):
@@ -260,6 +261,13 @@ class Context:
         """Delta matches one forwarded state may stage on chip (1..4096); beyond it the state
         takes the full join. Results never depend on it."""
         rc = self._lib.hgx_ctx_set_option(self._h, OPTION_DELTA_CAP, int(n))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def set_expand_team(self, waves: int):
+        """Waves that rewrite and hash one child: 1, 2, 4 or 8 for every child, 0 automatic.
+        Results never depend on it."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_EXPAND_TEAM, int(waves))
         if rc != abi.HGX_OK:
             raise self._err(rc)
 

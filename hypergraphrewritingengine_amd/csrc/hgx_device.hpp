@@ -156,4 +156,120 @@ __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
     return v;
 }
 
+// Exclusive prefix sum of a[0..n) in LDS, in place, by one wave; returns the total.
+__device__ __forceinline__ uint32_t wave_excl_scan_lds(uint32_t* a, uint32_t n) {
+    const uint32_t l = lane_id();
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += WAVE) {
+        const uint32_t i = base + l;
+        const uint32_t v = i < n ? a[i] : 0u;
+        const uint32_t inc = wave_incl_scan_u32(v);
+        if (i < n) a[i] = carry + inc - v;
+        carry += __shfl(inc, WAVE - 1, WAVE);
+    }
+    wave_sync();
+    return carry;
+}
+
+// ---------------- teams: the lanes that share one state ----------------
+// A team is one wave (W = 1: wave barriers and cross-lane reductions, nothing in LDS) or the
+// W waves of a workgroup (block barriers; the cross-wave step of a reduction goes through
+// `red`, W words of LDS). Every member must make every call: each one is a barrier. Results
+// are returned to every member.
+template <uint32_t W>
+struct Team {
+    static constexpr uint32_t N = WAVE * W;
+    uint32_t r;             // rank in the team
+    uint64_t* red;          // [W] LDS, used when W > 1
+
+    __device__ explicit Team(uint64_t* red_) : r(W == 1 ? lane_id() : (uint32_t)threadIdx.x), red(red_) {}
+    __device__ bool leader() const { return r == 0; }
+    __device__ void sync() const {
+        if constexpr (W == 1) wave_sync();
+        else __syncthreads();
+    }
+    // Sum of one value per wave (already reduced inside the wave) over the team's waves.
+    __device__ uint64_t across(uint64_t wave_value) const {
+        if (lane_id() == 0) red[threadIdx.x / WAVE] = wave_value;
+        __syncthreads();
+        uint64_t t = 0;
+        for (uint32_t k = 0; k < W; ++k) t += red[k];
+        __syncthreads();
+        return t;
+    }
+    __device__ uint32_t sum(uint32_t v) const {
+        if constexpr (W == 1) return wave_sum_u32(v);
+        else return (uint32_t)across(wave_sum_u32(v));
+    }
+    __device__ uint64_t sum64(uint64_t v) const {
+        if constexpr (W == 1) return wave_sum_u64(v);
+        else return across(wave_sum_u64(v));
+    }
+    __device__ uint32_t max(uint32_t v) const {
+        v = wave_max_u32(v);
+        if constexpr (W == 1) return v;
+        else {
+            if (lane_id() == 0) red[threadIdx.x / WAVE] = v;
+            __syncthreads();
+            uint32_t m = 0;
+            for (uint32_t k = 0; k < W; ++k) m = (uint32_t)red[k] > m ? (uint32_t)red[k] : m;
+            __syncthreads();
+            return m;
+        }
+    }
+    __device__ bool any(bool b) const {
+        const bool w = __ballot(b) != 0;
+        if constexpr (W == 1) return w;
+        else return across(w && lane_id() == 0 ? 1u : 0u) != 0;
+    }
+    // Stream compaction of one round of N items, item r kept when `keep`: returns the item's
+    // place among the round's kept items and their number in `total`.
+    __device__ uint32_t compact(bool keep, uint32_t& total) const {
+        const uint64_t bal = __ballot(keep);
+        const uint32_t below = lanes_below(bal);
+        if constexpr (W == 1) {
+            total = (uint32_t)__popcll(bal);
+            return below;
+        } else {
+            const uint32_t w = threadIdx.x / WAVE;
+            if (lane_id() == 0) red[w] = (uint64_t)__popcll(bal);
+            __syncthreads();
+            uint32_t pre = 0, t = 0;
+            for (uint32_t k = 0; k < W; ++k) {
+                const uint32_t c = (uint32_t)red[k];
+                pre += k < w ? c : 0u;
+                t += c;
+            }
+            __syncthreads();
+            total = t;
+            return pre + below;
+        }
+    }
+    // Exclusive prefix sum of a[0..n) in LDS, in place; returns the total.
+    __device__ uint32_t excl_scan_lds(uint32_t* a, uint32_t n) const {
+        if constexpr (W == 1) return wave_excl_scan_lds(a, n);
+        else {
+            const uint32_t w = threadIdx.x / WAVE;
+            uint32_t carry = 0;
+            for (uint32_t base = 0; base < n; base += N) {
+                const uint32_t i = base + r;
+                const uint32_t v = i < n ? a[i] : 0u;
+                const uint32_t inc = wave_incl_scan_u32(v);
+                if (lane_id() == WAVE - 1) red[w] = inc;
+                __syncthreads();
+                uint32_t pre = 0, t = 0;
+                for (uint32_t k = 0; k < W; ++k) {
+                    const uint32_t c = (uint32_t)red[k];
+                    pre += k < w ? c : 0u;
+                    t += c;
+                }
+                if (i < n) a[i] = carry + pre + inc - v;
+                carry += t;
+                __syncthreads();
+            }
+            return carry;
+        }
+    }
+};
+
 }  // namespace hgx

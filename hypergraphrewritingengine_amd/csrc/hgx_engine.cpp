@@ -168,10 +168,14 @@ namespace {
 // Children of the requests whose child-edge bound falls in one size class.
 struct ExpandClass {
     uint32_t ne = 0, nextv = 0;
+    bool uniform = true;                      // every request of the class has one arity 2..4
     uint64_t n = 0;
     std::vector<uint32_t> rb, rpre{0};
 };
 constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
+// Per-child workspace (bytes) up to which one wave rewrites and hashes a child, and up to which
+// four waves share one; above, eight.
+constexpr uint32_t kTeamBytes[2] = {24u << 10, 48u << 10};
 }  // namespace
 
 uint32_t Engine::step() {
@@ -496,6 +500,7 @@ uint32_t Engine::step() {
                 ExpandClass& x = cls[k];
                 x.ne = std::max(x.ne, cne);
                 x.nextv = std::max(x.nextv, cnv);
+                x.uniform = x.uniform && r.hp->uni_arity != 0;
                 x.rb.push_back(cb[i]);
                 x.n += ce[i] - cb[i];
                 x.rpre.push_back((uint32_t)x.n);
@@ -509,7 +514,8 @@ uint32_t Engine::step() {
                 ea.max_child_ne = std::max(1u, std::min(x.ne, max_ne));
                 ea.max_nextv = std::max(1u, std::min(x.nextv, max_nextv));
                 ea.max_nv = std::max(1u, std::min(ea.max_nextv, ea.max_child_ne * A));
-                ea.lds_per_wave = expand_lds_per_wave(ea.max_child_ne, ea.max_nv, ea.max_nextv, A);
+                ea.lean = x.uniform && ea.max_child_ne < 65536 ? 1u : 0u;
+                ea.lds_per_wave = expand_lds_per_wave(ea.max_child_ne, ea.max_nv, ea.max_nextv, A, ea.lean != 0);
                 ea.n_items = (uint32_t)x.n;
                 if (x.n != M) {
                     uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
@@ -517,10 +523,20 @@ uint32_t Engine::step() {
                     h2d(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
                     ea.rb = d_r; ea.rpre = d_r + x.rb.size(); ea.nrange = (uint32_t)x.rb.size();
                 }
-                const uint32_t wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds,
-                                                       ctx_->num_cu * 8, &ea.scratch);
+                // Team: one wave per child while a CU still holds several workspaces, else the
+                // workgroup's waves share one child (a CU holds 160 KiB of LDS).
+                uint32_t team = ctx_->expand_team;
+                if (!team) team = ea.lds_per_wave <= kTeamBytes[0] ? 1u : ea.lds_per_wave <= kTeamBytes[1] ? 4u : 8u;
+                uint32_t wpb_e = 1;
+                if (team == 1) {
+                    wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds, ctx_->num_cu * 8,
+                                            &ea.scratch);
+                } else if (ea.lds_per_wave > ctx_->max_lds) {
+                    const uint32_t grid = expand_grid(ea.n_items, team, 1, ctx_->num_cu * 8);
+                    ea.scratch = (uint8_t*)tmp.alloc((size_t)grid * ea.lds_per_wave);
+                }
                 timed(ctx_, st, "k_expand", (uint64_t)((double)bytes * x.n / M), x.n,
-                      [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, st); });
+                      [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, team, st); });
             }
             timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });

@@ -642,26 +642,32 @@ __device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
 }
 
 // GWS selects the workspace at compile time (global scratch for states too large for LDS), so
-// the LDS variant addresses its workspace with LDS instructions rather than flat ones.
-template <bool GWS>
-__global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
+// the LDS variant addresses its workspace with LDS instructions rather than flat ones. W is the
+// team: W = 1, one wave per child (up to four per workgroup, each with its own workspace);
+// W > 1, the workgroup's W waves share one child and one workspace, for large children whose
+// workspace would otherwise leave a CU one or two waves.
+template <bool GWS, uint32_t W>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_expand(ExpandArgs a) {
     extern __shared__ uint8_t lds[];
-    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    const uint32_t wpb_ = blockDim.x / WAVE;
+    const uint32_t l = lane_id();
+    const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;    // teams per workgroup
+    const uint32_t w = W == 1 ? threadIdx.x / WAVE : 0;     // team within the workgroup
     uint8_t* my;
-    if constexpr (GWS) my = a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave;
+    if constexpr (GWS) my = a.scratch + ((size_t)blockIdx.x * tpb + w) * a.lds_per_wave;
     else my = lds + (size_t)w * a.lds_per_wave;
+    const Team<W> tm((uint64_t*)(my + a.lds_per_wave - 64));   // the workspace's last 64 bytes
     const uint32_t PA = a.P.A, A = a.C.A;                  // parent / child vertex strides
-    const WLSizes z = WLSizes::make(a.max_child_ne, a.max_nv, A);
+    const WLSizes z = expand_sizes(a.max_child_ne, a.max_nv, a.max_nextv, A, a.lean != 0);
     WLSpace s;
     uint8_t* p = my;
     wl_bind(s, p, z, A);
-    uint32_t* gv = (uint32_t*)carve(p, z.NE * A * 4);     // child vertices (labels)
-    uint32_t* vmap = (uint32_t*)carve(p, (a.max_nextv + 1) * 4);
+    // Child vertex labels and the label -> local id map live in the hash's srt and nbr regions,
+    // which the hash only writes after they are read.
+    uint32_t* gv = (uint32_t*)s.srt;                      // [NE * A]
+    uint32_t* vmap = (uint32_t*)s.nbr;                    // [max_nextv + 1]
 
-    // Contiguous chunk of children per wave; consecutive chunks on one XCD.
-    const uint32_t wpb = blockDim.x / WAVE;
-    const uint32_t gw = xcd_block() * wpb + w, nw = gridDim.x * wpb;
+    // Contiguous chunk of children per team; consecutive chunks on one XCD.
+    const uint32_t gw = xcd_block() * tpb + w, nw = gridDim.x * tpb;
     const uint32_t per = (a.n_items + nw - 1) / nw;
     const uint32_t c0 = gw * per, c1 = min(a.n_items, c0 + per);
     uint32_t rr = 0;                                       // range of logical index c (subsets)
@@ -674,7 +680,7 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
     uint32_t acc_run = INV;
     uint64_t acc[CTR_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     auto flush = [&]() {
-        if (l == 0 && acc_run != INV)
+        if (tm.leader() && acc_run != INV)
             for (int k = 0; k < CTR_N; ++k)
                 if (acc[k]) atomicAdd((unsigned long long*)&a.ctr[(size_t)acc_run * CTR_N + k],
                                       (unsigned long long)acc[k]);
@@ -713,13 +719,14 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
 
         // Parent edges minus the consumed ones, order kept (SparseBitset::derive).
         uint32_t kept = 0;
-        for (uint32_t base = 0; base < pne; base += WAVE) {
-            const uint32_t i = base + l;
+        for (uint32_t base = 0; base < pne; base += Team<W>::N) {
+            const uint32_t i = base + tm.r;
             bool keep = i < pne;
             if (keep) for (uint32_t t = 0; t < nl; ++t) keep &= (q[t] != i);
-            const uint64_t bal = __ballot(keep);
+            uint32_t n_round;
+            const uint32_t at = tm.compact(keep, n_round);
             if (keep) {
-                const uint32_t pos = kept + lanes_below(bal);
+                const uint32_t pos = kept + at;
                 const uint8_t ar = a.P.ar[peo + i];
                 for (uint32_t k = 0; k < A; ++k) {
                     const uint32_t v = k < PA ? a.P.ev[(peo + i) * PA + k] : 0u;
@@ -730,18 +737,18 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
                 a.C.ps[ceo + pos] = a.P.ps[peo + i];
                 s.e_ar[pos] = ar;
             }
-            kept += __popcll(bal);
+            kept += n_round;
         }
         // Produced edges in RHS order; fresh vertices ascending by variable
         // (assign_fresh_consecutive / resolve_rhs_vertices, rewrite_core.hpp:53-87).
         const uint32_t fresh_base = a.P.nextv[par];
-        if (l < nr) {
-            const uint32_t pos = kept + l;
-            const uint32_t ra = R.ra[l];
+        if (tm.r < nr) {
+            const uint32_t pos = kept + tm.r;
+            const uint32_t ra = R.ra[tm.r];
             for (uint32_t k = 0; k < A; ++k) {
                 uint32_t v = 0;
                 if (k < ra) {
-                    const uint32_t var = R.rv[l][k];
+                    const uint32_t var = R.rv[tm.r][k];
                     if (R.lhs_mask & (1u << var)) {
                         const uint32_t se = q[R.var_src_edge[var]];
                         v = a.P.ev[(peo + se) * PA + R.var_src_pos[var]];
@@ -756,21 +763,21 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
             a.C.ps[ceo + pos] = (uint8_t)step;
             s.e_ar[pos] = (uint8_t)ra;
         }
-        wave_sync();
+        tm.sync();
         // Local vertex numbering: ascending label, as the reference's sorted vertex list.
         const uint32_t nextv = fresh_base + R.n_new;
-        for (uint32_t v = l; v <= nextv; v += WAVE) vmap[v] = 0;
-        wave_sync();
-        for (uint32_t e = l; e < cne; e += WAVE)
+        for (uint32_t v = tm.r; v <= nextv; v += Team<W>::N) vmap[v] = 0;
+        tm.sync();
+        for (uint32_t e = tm.r; e < cne; e += Team<W>::N)
             for (uint32_t k = 0; k < s.e_ar[e]; ++k) vmap[gv[e * A + k]] = 1;
-        wave_sync();
-        const uint32_t nv = wave_excl_scan_lds(vmap, nextv + 1);
-        for (uint32_t e = l; e < cne; e += WAVE)
+        tm.sync();
+        const uint32_t nv = tm.excl_scan_lds(vmap, nextv + 1);
+        for (uint32_t e = tm.r; e < cne; e += Team<W>::N)
             for (uint32_t k = 0; k < A; ++k) s.e_lv[e * A + k] = (uint16_t)(k < s.e_ar[e] ? vmap[gv[e * A + k]] : 0);
         s.nE = cne;
         s.nV = nv;
-        wave_sync();
-        const uint64_t h = wl_hash_wave(s);
+        tm.sync();
+        const uint64_t h = wl_hash_team(s, tm);
 
         // Causal: one triple per consumed edge with a producer; the producers are events of
         // this lineage, named by their generation (rewriter.cpp:121-191,
@@ -800,10 +807,10 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         const uint32_t triples_tr = (uint32_t)__popcll(__ballot(l < nl && ps_t != 0 && ((tr >> ps_t) & 1ull)));
         if (!rec_causal) { prod = 0; tr = 0; anc_c = 0; }
 
-        // Identity and bookkeeping (lane 0): the child's record and its event.
+        // Identity and bookkeeping (the team's first lane): the child's record and its event.
         const uint32_t ev_id = RN->ev_base + (c - RN->child_begin);
         const uint32_t child_sid = RN->n_roots + ev_id;
-        if (l == 0) {
+        if (tm.leader()) {
             a.C.run[c] = run;
             a.C.ne[c] = cne;
             a.C.nextv[c] = nextv;
@@ -826,9 +833,9 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
         {
             const uint64_t* plan = a.P.lanc + (size_t)par * a.P.lstride;
             uint64_t* clan = a.C.lanc + (size_t)c * a.C.lstride;
-            for (uint32_t t = l; t < step; t += WAVE) clan[t] = (t + 1 == step) ? anc_c : plan[t];
+            for (uint32_t t = tm.r; t < step; t += Team<W>::N) clan[t] = (t + 1 == step) ? anc_c : plan[t];
         }
-        wave_sync();
+        tm.sync();
     }
     flush();
 }
@@ -1220,17 +1227,34 @@ hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32
     return hipGetLastError();
 }
 
-uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A) {
-    const WLSizes z = WLSizes::make(max_child_ne, max_nv, A);
-    return ((z.bytes(A) + z.NE * A * 4 + (max_nextv + 1) * 4 + 64u) + 15u) & ~15u;
+uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean) {
+    const WLSizes z = expand_sizes(max_child_ne, max_nv, max_nextv, A, lean);
+    // + 64: carving slack; + 64: the team's reduction words at the end
+    return ((z.bytes(A) + 128u) + 15u) & ~15u;
 }
 
-hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
+template <uint32_t W>
+static void launch_expand_team(const ExpandArgs& a, uint32_t grid, uint32_t wpb, hipStream_t st) {
+    const uint32_t threads = W == 1 ? 64 * wpb : 64 * W;
+    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * (W == 1 ? wpb : 1);
+    if (a.scratch) hipLaunchKernelGGL((k_expand<true, W>), dim3(grid), dim3(threads), lds, st, a);
+    else hipLaunchKernelGGL((k_expand<false, W>), dim3(grid), dim3(threads), lds, st, a);
+}
+
+uint32_t expand_grid(uint32_t n_items, uint32_t team, uint32_t wpb, uint32_t grid_cap) {
+    return grid_for(n_items, team == 1 ? wpb : 1, grid_cap);
+}
+
+hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st) {
     if (!a.n_items) return hipSuccess;
-    uint32_t grid = grid_for(a.n_items, wpb, grid_cap);
-    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
-    if (a.scratch) hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
-    else hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(64 * wpb), lds, st, a);
+    const uint32_t grid = expand_grid(a.n_items, team, wpb, grid_cap);
+    switch (team) {
+        case 1: launch_expand_team<1>(a, grid, wpb, st); break;
+        case 2: launch_expand_team<2>(a, grid, wpb, st); break;
+        case 4: launch_expand_team<4>(a, grid, wpb, st); break;
+        case 8: launch_expand_team<8>(a, grid, wpb, st); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

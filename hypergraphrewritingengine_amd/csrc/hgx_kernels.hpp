@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "hgx_device.hpp"
+#include "hgx_wl.hpp"
 
 namespace hgx {
 
@@ -54,15 +55,24 @@ struct ExpandArgs {
     uint32_t max_nv;           // bound on distinct vertices of a child
     uint32_t max_nextv;        // bound on vertex labels of a child
     uint32_t lds_per_wave;
-    uint8_t* scratch;          // when set: per-wave workspace in global memory instead of LDS
-                               //   (states too large for LDS), lds_per_wave bytes per wave
+    uint8_t* scratch;          // when set: per-team workspace in global memory instead of LDS
+                               //   (states too large for LDS), lds_per_wave bytes per team
     // Optional subset of the children: ranges [rb[r], rb[r] + rpre[r+1] - rpre[r]) for
     // r < nrange (rpre = exclusive prefix of the lengths). nrange = 0: all of M.
     const uint32_t* rb;
     const uint32_t* rpre;
     uint32_t nrange;
     uint32_t n_items;          // children this launch covers
+    uint32_t lean;             // every child has one arity 2..4 and < 2^16 edges: lean workspace
 };
+
+// Workspace of one child in k_expand: the hash's, with room in its nbr region for the
+// label -> local id map and in its srt region for the child's vertex labels.
+__host__ __device__ inline WLSizes expand_sizes(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv,
+                                                uint32_t A, bool lean) {
+    const uint32_t ne = max_child_ne ? max_child_ne : 1;
+    return WLSizes::make(ne, max_nv, A, lean, (max_nextv + 2) / 2, (ne * A + 1) / 2);
+}
 
 struct BranchialArgs {
     GenView P;
@@ -142,8 +152,10 @@ struct RootCopy {
 hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v, hipStream_t st);
 
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
-uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A);
-hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean);
+// team: waves per child (1, 2, 4 or 8); with team 1, wpb children per workgroup.
+hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st);
+uint32_t expand_grid(uint32_t n_items, uint32_t team, uint32_t wpb, uint32_t grid_cap);
 hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr, hipStream_t st);
 // grid used by the launchers for `items` work items (for sizing global scratch)
 uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap);

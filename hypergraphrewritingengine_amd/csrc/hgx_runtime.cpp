@@ -292,6 +292,12 @@ RunHost parse_run(const hgx_run_desc* d) {
         h.max_new = std::max<uint32_t>(h.max_new, R.n_new);
     }
     for (auto& st : h.roots) h.root_edges += st.size();
+    uint32_t amin = 0xFFu, amax = 0;
+    for (auto& st : h.roots)
+        for (auto& ed : st) { amin = std::min<uint32_t>(amin, (uint32_t)ed.size()); amax = std::max<uint32_t>(amax, (uint32_t)ed.size()); }
+    for (auto& R : h.rules)
+        for (int e = 0; e < R.n_rhs; ++e) { amin = std::min<uint32_t>(amin, R.ra[e]); amax = std::max<uint32_t>(amax, R.ra[e]); }
+    h.uni_arity = (amin == amax && amax >= 2 && amax <= 4) ? amax : 0u;
     return h;
 }
 
@@ -436,6 +442,10 @@ extern "C" int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value) 
     }
     if (option == HGX_OPTION_DELTA_CAP && value >= 1 && value <= 4096) {
         ctx->delta_cap = (uint32_t)value;
+        return HGX_OK;
+    }
+    if (option == HGX_OPTION_EXPAND_TEAM && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8)) {
+        ctx->expand_team = (uint32_t)value;
         return HGX_OK;
     }
     ctx->err = "unknown option or value";

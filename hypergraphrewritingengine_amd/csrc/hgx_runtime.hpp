@@ -125,6 +125,8 @@ struct RunHost {
     std::vector<uint32_t> root_nv;
     uint32_t steps = 0, cap = 0, flags = 0;
     uint32_t max_arity = 1, max_lhs = 1, max_gain = 0, max_new = 0;
+    uint32_t uni_arity = 0;      // k when every root edge and every RHS edge has arity k (2..4),
+                                 // so every state of the run does; else 0
     uint64_t root_edges = 0;
 };
 RunHost parse_run(const hgx_run_desc* d);
@@ -165,6 +167,7 @@ struct hgx_ctx {
     uint64_t next_request_id = 1;
     int forward = -1;                            // HGX_OPTION_FORWARD
     uint32_t delta_cap = 256;                    // HGX_OPTION_DELTA_CAP
+    uint32_t expand_team = 0;                    // HGX_OPTION_EXPAND_TEAM
 };
 
 namespace hgx {

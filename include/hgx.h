@@ -156,9 +156,13 @@ size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
  *                       1 on, 0 off (full join for every state), -1 automatic (default).
  *   HGX_OPTION_DELTA_CAP delta matches one forwarded state may stage on chip, 1..4096
  *                       (default 256); a state with more takes the full join.
+ *   HGX_OPTION_EXPAND_TEAM waves that rewrite and hash one child: 1, 2, 4 or 8 for every
+ *                       child, 0 automatic (default: one wave for children whose workspace is
+ *                       small, a workgroup of 4 or 8 waves sharing one workspace for large ones).
  * Returns HGX_ERR_INVALID for an unknown option or value. */
 #define HGX_OPTION_FORWARD 1
 #define HGX_OPTION_DELTA_CAP 2
+#define HGX_OPTION_EXPAND_TEAM 3
 int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value);
 
 /* ---- synchronous batch: the plain and the batched path -------------------------------- */

@@ -1,0 +1,125 @@
+"""GPU parity of the expansion team size (HGX_OPTION_EXPAND_TEAM): one child rewritten and
+hashed by one wave, or by the 2, 4 or 8 waves of a workgroup sharing one workspace. Every
+event, state hash and relation must be the same whichever team size runs it, and the same as
+the oracle's. The cases cover the WL hash's uniform-arity front end (binary and ternary states
+above 256 edges included), its general front end (mixed and unary arities), states whose
+workspace exceeds the LDS (global workspace) and the automatic choice.
+"""
+import random
+
+import pytest
+
+from hypergraphrewritingengine_amd import workloads as W
+from hypergraphrewritingengine_amd.rules import Request, rule_from_lists
+from oracle import pyoracle as po
+from tests.parity import assert_exact
+from tests.test_gpu_forward import REL, TR, _random_request, _same
+
+pytestmark = pytest.mark.gpu
+
+TEAMS = (1, 2, 4, 8)
+
+
+def _by_team(ctx, reqs, teams=TEAMS):
+    out = {}
+    try:
+        for t in (0,) + tuple(teams):
+            ctx.set_expand_team(t)
+            out[t] = ctx.evolve_batch(reqs)
+    finally:
+        ctx.set_expand_team(0)
+    return out
+
+
+def _ternary_state(seed, m):
+    rnd = random.Random(seed)
+    nv = 1
+    out = []
+    for _ in range(m):
+        e = [rnd.randrange(nv)]
+        for _ in range(2):
+            if rnd.random() < 0.5:
+                e.append(nv)
+                nv += 1
+            else:
+                e.append(rnd.randrange(nv))
+        rnd.shuffle(e)
+        out.append(e)
+    return out
+
+
+def _mixed_state(seed, m):
+    rnd = random.Random(seed)
+    nv = 2
+    out = []
+    for _ in range(m):
+        a = rnd.choice([1, 2, 2, 3])
+        e = [rnd.randrange(nv) for _ in range(a)]
+        if rnd.random() < 0.4:
+            e[rnd.randrange(a)] = nv
+            nv += 1
+        out.append(e)
+    return out
+
+
+def test_team_random_rules_exact(ctx):
+    reqs = [_random_request(1000 + s) for s in range(48)]
+    r = _by_team(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for t in TEAMS:
+            _same(r[t][i], r[1][i])
+        _same(r[0][i], r[1][i])
+        assert_exact(r[8][i], po.evolve(req))
+
+
+def test_team_large_binary_states(ctx):
+    """Served-like runs on 400-1500 edge binary states: the uniform-arity front end above 256
+    edges, and the workgroup teams the automatic choice takes for them (the first: a second
+    generation over the cap, so the run stops truncated after the first)."""
+    rb = rule_from_lists([[0, 1], [1, 2]], [[0, 2], [2, 3], [3, 1]])
+    reqs = [Request([rb], W._connected_init(W.Stream(5), 400), 2, state_cap=3000, flags=REL | TR)]
+    for k, m, steps in ((1, 700, 2), (2, 1500, 1)):
+        st = W.Stream(W.splitmix64(0xAB00 + k))
+        rule = W._random_rule(st)
+        reqs.append(Request([rule], W._connected_init(st, m), steps, state_cap=20000, flags=REL | TR))
+    r = _by_team(ctx, reqs)
+    assert r[1][0].counts["truncated"] and r[1][1].counts["causal_pairs"]
+    for i, req in enumerate(reqs):
+        for t in (0,) + TEAMS:
+            _same(r[t][i], r[1][i])
+        assert_exact(r[0][i], po.evolve(req))
+
+
+def test_team_large_ternary_and_mixed_states(ctx):
+    rt = rule_from_lists([[0, 1, 2], [2, 3, 4]], [[0, 3, 5], [1, 4, 5], [2, 5, 5]])
+    rb = rule_from_lists([[0, 1], [1, 2]], [[0, 2], [2, 3], [3, 1]])
+    ru = rule_from_lists([[0], [0, 1]], [[1], [0, 1], [1, 2]])
+    reqs = [Request([rt], _ternary_state(7, 400), 2, state_cap=3000, flags=REL | TR),
+            Request([rb, ru], _mixed_state(8, 450), 2, state_cap=3000, flags=REL | TR),
+            Request([ru], _mixed_state(9, 300), 3, state_cap=3000, flags=REL | TR)]
+    r = _by_team(ctx, reqs)
+    for i, req in enumerate(reqs):
+        assert r[1][i].counts["events"] > 100
+        for t in (0,) + TEAMS:
+            _same(r[t][i], r[1][i])
+        assert_exact(r[4][i], po.evolve(req))
+
+
+def test_team_global_workspace(ctx):
+    """A 3000-edge state: its child's workspace is larger than a CU's LDS, so every team size
+    runs on the global workspace."""
+    st = W.Stream(W.splitmix64(0xC0FFEE))
+    rule = W._random_rule(st)
+    req = Request([rule], W._connected_init(st, 3000), 1, state_cap=200, flags=REL | TR)
+    r = _by_team(ctx, [req], teams=(1, 8))
+    _same(r[8][0], r[1][0])
+    _same(r[0][0], r[1][0])
+    assert_exact(r[1][0], po.evolve(req))
+
+
+def test_team_option_rejects_bad_values(ctx):
+    import hypergraphrewritingengine_amd as hgx
+    for bad in (3, 16, -1):
+        with pytest.raises(hgx.HGXError):
+            ctx.set_expand_team(bad)
+    ctx.set_expand_team(0)
