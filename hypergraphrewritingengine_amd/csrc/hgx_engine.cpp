@@ -101,7 +101,7 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     const uint32_t slot = take_slot();
     r->slot = slot;
     r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->hp->rules.size());
-    uint64_t want = r->hp->cap ? (uint64_t)r->hp->cap + r->hp->roots.size() : std::max<uint64_t>(512, r->hp->roots.size());
+    uint64_t want = r->hp->cap ? (uint64_t)r->hp->cap + r->hp->n_roots() : std::max<uint64_t>(512, r->hp->n_roots());
     uint64_t mc = 64;
     while (mc < 2 * want) mc <<= 1;
     r->map_cap = (uint32_t)mc;
@@ -111,17 +111,13 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     std::memset(&d, 0, sizeof(d));
     d.rules = r->d_rules;
     d.n_rules = (uint32_t)r->hp->rules.size();
-    d.n_roots = (uint32_t)r->hp->roots.size();
+    d.n_roots = r->hp->n_roots();
     d.flags = r->hp->flags;
     d.map_keys = r->map_keys;
     d.map_vals = r->map_vals;
     d.map_mask = (uint32_t)(mc - 1);
-    r->ne_bound = 0;
-    r->nv_bound = 0;
-    for (size_t k = 0; k < r->hp->roots.size(); ++k) {
-        r->ne_bound = std::max<uint32_t>(r->ne_bound, (uint32_t)r->hp->roots[k].size());
-        r->nv_bound = std::max(r->nv_bound, r->hp->root_nv[k]);
-    }
+    r->ne_bound = r->hp->max_root_ne;
+    r->nv_bound = r->hp->max_root_nv;
     r->admitted = true;
     r->t_admit = Clock::now();
     live_[r->id] = r;
@@ -373,8 +369,8 @@ uint32_t Engine::step() {
     uint32_t NR = 0, NRh = 0;
     uint64_t NER = 0, NERh = 0;
     for (auto& r : admitted) {
-        NR += (uint32_t)r->hp->roots.size(); NER += r->hp->root_edges;
-        if (!r->hp->staged.rules) { NRh += (uint32_t)r->hp->roots.size(); NERh += r->hp->root_edges; }
+        NR += r->hp->n_roots(); NER += r->hp->root_edges;
+        if (!r->hp->staged.rules) { NRh += r->hp->n_roots(); NERh += r->hp->root_edges; }
     }
     std::vector<DRule> a_rules;                 // staging; outlives the synchronisation below
     std::vector<AdmitItem> a_items;
@@ -437,10 +433,8 @@ uint32_t Engine::step() {
             A = std::max(A, r->hp->max_arity);
             lstride = std::max(lstride, r->hp->steps);
             keep |= (r->hp->flags & HGX_REC_STATES) != 0;
-            for (size_t k = 0; k < r->hp->roots.size(); ++k) {
-                max_ne = std::max<uint32_t>(max_ne, (uint32_t)r->hp->roots[k].size());
-                max_nextv = std::max(max_nextv, r->hp->root_nv[k]);
-            }
+            max_ne = std::max(max_ne, r->hp->max_root_ne);
+            max_nextv = std::max(max_nextv, r->hp->max_root_nv);
         }
         C->keep_edges = keep;
         C->max_ne = max_ne;
@@ -552,16 +546,17 @@ uint32_t Engine::step() {
                 uint64_t e = 0;
                 for (auto& r : admitted) {
                     if (r->hp->staged.rules) continue;
-                    for (size_t k = 0; k < r->hp->roots.size(); ++k, ++s) {
+                    const RunHost& h = *r->hp;
+                    for (uint32_t k = 0; k < h.n_roots(); ++k, ++s) {
                         r_run[s] = r->slot;
-                        r_ne[s] = (uint32_t)r->hp->roots[k].size();
-                        r_nextv[s] = r->hp->root_nv[k];
-                        r_sid[s] = (uint32_t)k;
+                        r_ne[s] = h.root_ne(k);
+                        r_nextv[s] = h.root_nv[k];
+                        r_sid[s] = k;
                         r_eoff[s] = EC + e;
-                        for (auto& ed : r->hp->roots[k]) {
-                            r_ar[e] = (uint8_t)ed.size();
-                            for (size_t x = 0; x < ed.size(); ++x) r_ev[e * A + x] = ed[x];
-                            ++e;
+                        for (uint32_t ed = h.root_eb[k]; ed < h.root_eb[k + 1]; ++ed, ++e) {
+                            const uint32_t a = h.arity(ed);
+                            r_ar[e] = (uint8_t)a;
+                            std::memcpy(&r_ev[e * A], &h.verts[h.edge_vb[ed]], 4 * a);
                         }
                     }
                 }
@@ -584,10 +579,9 @@ uint32_t Engine::step() {
                 for (auto& r : admitted) {
                     const StagedSrc& g = r->hp->staged;
                     if (!g.rules) continue;
-                    RootCopy c{g.ne, g.nextv, g.eoff, g.ar, g.ev, g.A, (uint32_t)r->hp->roots.size(),
-                               r->slot, s, e};
+                    RootCopy c{g.ne, g.nextv, g.eoff, g.ar, g.ev, g.A, r->hp->n_roots(), r->slot, s, e};
                     r_copy.push_back(c);
-                    s += (uint32_t)r->hp->roots.size();
+                    s += r->hp->n_roots();
                     e += r->hp->root_edges;
                 }
                 RootCopy* d_copy = dalloc<RootCopy>(tmp, r_copy.size());
@@ -595,11 +589,10 @@ uint32_t Engine::step() {
                 HIP_TRY(launch_copy_roots(d_copy, (uint32_t)r_copy.size(), v, st));
             }
             uint32_t rmax_ne = 1, rmax_nv = 1;
-            for (auto& r : admitted)
-                for (size_t k = 0; k < r->hp->roots.size(); ++k) {
-                    rmax_ne = std::max<uint32_t>(rmax_ne, (uint32_t)r->hp->roots[k].size());
-                    rmax_nv = std::max(rmax_nv, r->hp->root_nv[k]);
-                }
+            for (auto& r : admitted) {
+                rmax_ne = std::max(rmax_ne, r->hp->max_root_ne);
+                rmax_nv = std::max(rmax_nv, r->hp->max_root_nv);
+            }
             WLStatesArgs w{};
             w.n = NR; w.A = A; w.state_off = v.eoff + M; w.ar = v.ar; w.ev = v.ev; w.nv = v.nextv + M;
             w.out = v.hash + M; w.run_of = v.run + M; w.sid_of = v.sid + M; w.runs = d_runs_; w.ctr = d_ctr_;
@@ -645,31 +638,35 @@ uint32_t Engine::step() {
                 ba.pcnt = pcnt;
                 timed(ctx_, st, "k_branchial", M * (6ull + 2ull * K) + n * 12ull, n,
                       [&] { return launch_branchial(ba, ctx_->num_cu * 8, 2, st); });
-                // parents above 64 edges: per-edge lists of consuming matches
+                // parents above 64 edges: (edge slot, match) keys sorted into per-edge lists
                 BrListArgs bl{};
                 const bool large = F_->max_ne > 64;
                 if (large) {
-                    const uint64_t E = F_->n_edges;
-                    if (E >= 0xFFFFFFFFull || M * K >= 0xFFFFFFFFull)
-                        throw Error{HGX_ERR_UNSUPPORTED, "branchial lists above 2^32 entries"};
+                    const uint64_t E = F_->n_edges, NK = M * K;
+                    if (E >= 0xFFFFFFFFull) throw Error{HGX_ERR_UNSUPPORTED, "branchial lists above 2^32 edges"};
                     bl.P = F_->v; bl.M = mv; bl.runs = d_runs_; bl.cnt = bcnt; bl.moff = moff;
                     bl.ctr = d_ctr_; bl.pcnt = pcnt;
-                    bl.ecnt = dalloc<uint32_t>(tmp, E + 1);
-                    bl.ecur = dalloc<uint32_t>(tmp, E + 1);
-                    uint64_t* eoff = dalloc<uint64_t>(tmp, E + 1);
-                    bl.eoff = eoff;
-                    bl.elist = dalloc<uint32_t>(tmp, std::max<uint64_t>(1, M * K));
+                    bl.n_keys = NK;
+                    bl.sentinel = E << 32;
+                    bl.key_in = dalloc<uint64_t>(tmp, std::max<uint64_t>(1, NK));
+                    bl.val_in = dalloc<uint32_t>(tmp, std::max<uint64_t>(1, NK));
+                    uint64_t* key = dalloc<uint64_t>(tmp, std::max<uint64_t>(1, NK));
+                    uint32_t* val = dalloc<uint32_t>(tmp, std::max<uint64_t>(1, NK));
+                    bl.key = key; bl.val = val;
+                    bl.pos = dalloc<uint32_t>(tmp, std::max<uint64_t>(1, NK));
                     bl.bc = dalloc<uint32_t>(tmp, M + 1);
-                    HIP_TRY(hipMemsetAsync(bl.ecnt, 0, sizeof(uint32_t) * (E + 1), st));
-                    HIP_TRY(hipMemsetAsync(bl.ecur, 0, sizeof(uint32_t) * (E + 1), st));
                     HIP_TRY(hipMemsetAsync(bl.bc, 0, sizeof(uint32_t) * (M + 1), st));
-                    const uint64_t lbytes = M * (12ull + 2ull * K) * 4 + E * 20ull;
+                    int end_bit = 32;
+                    while (end_bit < 64 && (E >> (end_bit - 32)) != 0) ++end_bit;   // keys <= E << 32
+                    const uint64_t lbytes = NK * (8ull + 4ull) * 4 + NK * 4ull + M * (12ull + 2ull * K);
                     timed(ctx_, st, "k_branchial_lists", lbytes, M, [&] {
-                        hipError_t e = launch_brl(bl, 0, E, st);
-                        if (e == hipSuccess) { scan_u32_to_u64(tmp, bl.ecnt, eoff, (uint32_t)E, st); e = hipGetLastError(); }
-                        if (e == hipSuccess) e = launch_brl(bl, 1, E, st);
-                        if (e == hipSuccess) e = launch_brl(bl, 2, E, st);
-                        if (e == hipSuccess) e = launch_brl(bl, 3, E, st);
+                        hipError_t e = launch_brl(bl, 0, st);
+                        if (e == hipSuccess) {
+                            sort_pairs_u64(tmp, bl.key_in, key, bl.val_in, val, NK, end_bit, st);
+                            e = hipGetLastError();
+                        }
+                        if (e == hipSuccess) e = launch_brl(bl, 1, st);
+                        if (e == hipSuccess) e = launch_brl(bl, 2, st);
                         return e;
                     });
                 }
@@ -688,7 +685,7 @@ uint32_t Engine::step() {
                             uint64_t* bo = dalloc<uint64_t>(tmp, M + 1);
                             scan_u32_to_u64(tmp, bl.bc, bo, (uint32_t)M, st);
                             bl.bo = bo; bl.poff = poff; bl.pairs = C->br_pairs;
-                            HIP_TRY(launch_brl(bl, 4, F_->n_edges, st));
+                            HIP_TRY(launch_brl(bl, 3, st));
                         }
                     }
                 }
@@ -715,7 +712,7 @@ uint32_t Engine::step() {
         }
         uint32_t s = (uint32_t)M;
         for (auto& r : admitted) {
-            const uint32_t nr = (uint32_t)r->hp->roots.size();
+            const uint32_t nr = r->hp->n_roots();
             r->segs.push_back(Segment{C, s, s + nr, 0, 0});
             r->states = nr;
             if (r->hp->steps == 0) r->done = true;
@@ -785,7 +782,7 @@ struct RunTables {
 static RunTables run_tables(const RunRec& R) {
     RunTables T;
     const uint64_t ns = R.states;
-    const uint32_t nroots = (uint32_t)R.hp->roots.size();
+    const uint32_t nroots = R.hp->n_roots();
     const uint64_t ne = ns >= nroots ? ns - nroots : 0;
     T.hash.resize(ns); T.step.resize(ns); T.creator.assign(ns, 0xFFFFFFFFu);
     T.ev_in.resize(ne); T.ev_rule.resize(ne); T.ev_step.resize(ne); T.ev_prod.resize(ne); T.ev_tr.resize(ne);
@@ -878,8 +875,8 @@ extern "C" int hgx_batch_prepare(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_
         uint64_t nrules = 0, nroots = 0, nedges = 0;
         uint32_t A = 1;
         for (auto& h : hs) {
-            nrules += h.rules.size(); nroots += h.roots.size(); nedges += h.root_edges;
-            for (auto& st : h.roots) for (auto& e : st) A = std::max<uint32_t>(A, (uint32_t)e.size());
+            nrules += h.rules.size(); nroots += h.n_roots(); nedges += h.root_edges;
+            for (uint64_t e = 0; e < h.root_edges; ++e) A = std::max(A, h.arity(e));
         }
         std::vector<DRule> rules;
         rules.reserve(nrules);
@@ -892,14 +889,14 @@ extern "C" int hgx_batch_prepare(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_
             rule_b.push_back(rules.size());
             root_b.push_back(ne.size());
             rules.insert(rules.end(), h.rules.begin(), h.rules.end());
-            for (size_t k = 0; k < h.roots.size(); ++k) {
-                ne.push_back((uint32_t)h.roots[k].size());
+            for (uint32_t k = 0; k < h.n_roots(); ++k) {
+                ne.push_back(h.root_ne(k));
                 nv.push_back(h.root_nv[k]);
                 eoff.push_back(e);
-                for (auto& ed : h.roots[k]) {
-                    ar[e] = (uint8_t)ed.size();
-                    for (size_t x = 0; x < ed.size(); ++x) ev[e * A + x] = ed[x];
-                    ++e;
+                for (uint32_t ed = h.root_eb[k]; ed < h.root_eb[k + 1]; ++ed, ++e) {
+                    const uint32_t a = h.arity(ed);
+                    ar[e] = (uint8_t)a;
+                    std::memcpy(&ev[e * A], &h.verts[h.edge_vb[ed]], 4 * a);
                 }
             }
         }
@@ -1078,7 +1075,7 @@ extern "C" int hgx_result_events(const hgx_result* r, hgx_event* out, uint64_t c
     if (!r || (!out && r->run->events) || cap < r->run->events) return HGX_ERR_INVALID;
     try {
         RunTables T = run_tables(*r->run);
-        const uint32_t nroots = (uint32_t)r->run->hp->roots.size();
+        const uint32_t nroots = r->run->hp->n_roots();
         for (uint64_t e = 0; e < r->run->events; ++e)
             out[e] = hgx_event{T.ev_in[e], (uint32_t)(e + nroots), T.ev_rule[e], T.ev_step[e]};
         return HGX_OK;

@@ -841,11 +841,12 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_expand(ExpandArgs a) 
 }
 
 // ------------------------------------------------------------------------------------------
-// Branchial pairs of parents above 64 edges. Every consumed edge slot (the parent's edge in
-// the generation's edge pool) gets the ascending list of the matches consuming it; match i
-// meets each partner j > i through the list of the first of its own edges that j also
-// consumes. The work is the sum over edges of (list length)^2, not (matches per parent)^2.
-// One thread per match throughout.
+// Branchial pairs of parents above 64 edges. Every (consumed edge slot, match) of such a parent
+// becomes one 64-bit key (slot << 32 | match); sorted, the keys of one slot are its consuming
+// matches in ascending order. Match i meets each partner j > i through the first of its own
+// edges that j also consumes: from its key for that edge, the keys after it with the same slot.
+// The work is the entries plus the sum over edges of (matches consuming it)^2 -- not the
+// generation's edge count, nor (matches per parent)^2. One thread per match or entry.
 // ------------------------------------------------------------------------------------------
 
 __device__ __forceinline__ bool brl_large(const BrListArgs& a, uint32_t m, uint32_t& par) {
@@ -853,38 +854,27 @@ __device__ __forceinline__ bool brl_large(const BrListArgs& a, uint32_t m, uint3
     return a.P.ne[par] > 64 && a.cnt[par] >= 2;
 }
 
-__global__ void k_brl_count(BrListArgs a) {
+// Keys and payloads of match m at entries [m*K, m*K + K): (slot << 32 | m, m*K + k) for its
+// consumed edges, the sentinel (past every slot) for the rest.
+__global__ void k_brl_keys(BrListArgs a) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t par;
-    if (m >= a.M.n || !brl_large(a, m, par)) return;
-    const uint32_t nl = a.runs[a.P.run[par]].rules[a.M.rule[m]].n_lhs;
-    const uint16_t* q = a.M.q + (size_t)m * a.M.K;
-    for (uint32_t k = 0; k < nl; ++k) atomicAdd(&a.ecnt[a.P.eoff[par] + q[k]], 1u);
-}
-
-__global__ void k_brl_scatter(BrListArgs a) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t par;
-    if (m >= a.M.n || !brl_large(a, m, par)) return;
-    const uint32_t nl = a.runs[a.P.run[par]].rules[a.M.rule[m]].n_lhs;
-    const uint16_t* q = a.M.q + (size_t)m * a.M.K;
-    for (uint32_t k = 0; k < nl; ++k) {
-        const uint64_t e = a.P.eoff[par] + q[k];
-        a.elist[a.eoff[e] + atomicAdd(&a.ecur[e], 1u)] = m;
+    if (m >= a.M.n) return;
+    const uint32_t K = a.M.K;
+    uint32_t par, nl = 0;
+    if (brl_large(a, m, par)) nl = a.runs[a.P.run[par]].rules[a.M.rule[m]].n_lhs;
+    const uint16_t* q = a.M.q + (size_t)m * K;
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint64_t x = (uint64_t)m * K + k;
+        a.key_in[x] = k < nl ? ((a.P.eoff[par] + q[k]) << 32) | m : a.sentinel;
+        a.val_in[x] = (uint32_t)x;
     }
 }
 
-__global__ void k_brl_sort(BrListArgs a, uint64_t nslots) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nslots) return;
-    const uint64_t b = a.eoff[e], n = a.eoff[e + 1] - b;
-    uint32_t* L = a.elist + b;
-    for (uint64_t i = 1; i < n; ++i) {
-        const uint32_t key = L[i];
-        uint64_t j = i;
-        while (j > 0 && L[j - 1] > key) { L[j] = L[j - 1]; --j; }
-        L[j] = key;
-    }
+// Where each (match, k) entry landed in the sorted keys.
+__global__ void k_brl_pos(BrListArgs a, uint64_t n) {
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n || a.key[x] == a.sentinel) return;
+    a.pos[a.val[x]] = (uint32_t)x;
 }
 
 template <bool EMIT>
@@ -894,9 +884,9 @@ __global__ void k_brl_pairs(BrListArgs a) {
     if (i >= a.M.n || !brl_large(a, i, par)) return;
     const uint32_t run = a.P.run[par];
     const DRun& RN = a.runs[run];
+    const uint32_t K = a.M.K;
     const uint32_t nl = RN.rules[a.M.rule[i]].n_lhs;
-    const uint16_t* qi = a.M.q + (size_t)i * a.M.K;
-    const uint64_t e0 = a.P.eoff[par];
+    const uint16_t* qi = a.M.q + (size_t)i * K;
     uint32_t c = 0;
     uint64_t o = 0;
     if (EMIT) {
@@ -904,12 +894,12 @@ __global__ void k_brl_pairs(BrListArgs a) {
         o = a.poff[par] + (a.bo[i] - a.bo[mb]);
     }
     for (uint32_t k = 0; k < nl; ++k) {
-        const uint64_t e = e0 + qi[k];
-        for (uint64_t x = a.eoff[e]; x < a.eoff[e + 1]; ++x) {
-            const uint32_t j = a.elist[x];
-            if (j <= i) continue;
+        const uint64_t x0 = a.pos[(uint64_t)i * K + k];
+        const uint64_t slot = a.key[x0] >> 32;
+        for (uint64_t x = x0 + 1; x < a.n_keys && (a.key[x] >> 32) == slot; ++x) {
+            const uint32_t j = (uint32_t)a.key[x];
             // count j here only if this is the first of i's edges that j consumes
-            const uint16_t* qj = a.M.q + (size_t)j * a.M.K;
+            const uint16_t* qj = a.M.q + (size_t)j * K;
             const uint32_t nj = RN.rules[a.M.rule[j]].n_lhs;
             bool earlier = false;
             for (uint32_t k2 = 0; k2 < k && !earlier; ++k2)
@@ -932,15 +922,14 @@ __global__ void k_brl_pairs(BrListArgs a) {
     }
 }
 
-hipError_t launch_brl(const BrListArgs& a, int phase, uint64_t nslots, hipStream_t st) {
+hipError_t launch_brl(const BrListArgs& a, int phase, hipStream_t st) {
     const uint32_t n = a.M.n;
     if (!n) return hipSuccess;
     const dim3 g((n + 255) / 256), b(256);
     switch (phase) {
-        case 0: hipLaunchKernelGGL(k_brl_count, g, b, 0, st, a); break;
-        case 1: hipLaunchKernelGGL(k_brl_scatter, g, b, 0, st, a); break;
-        case 2: hipLaunchKernelGGL(k_brl_sort, dim3((uint32_t)((nslots + 255) / 256)), b, 0, st, a, nslots); break;
-        case 3: hipLaunchKernelGGL(k_brl_pairs<false>, g, b, 0, st, a); break;
+        case 0: hipLaunchKernelGGL(k_brl_keys, g, b, 0, st, a); break;
+        case 1: hipLaunchKernelGGL(k_brl_pos, dim3((uint32_t)((a.n_keys + 255) / 256)), b, 0, st, a, a.n_keys); break;
+        case 2: hipLaunchKernelGGL(k_brl_pairs<false>, g, b, 0, st, a); break;
         default: hipLaunchKernelGGL(k_brl_pairs<true>, g, b, 0, st, a); break;
     }
     return hipGetLastError();

@@ -95,10 +95,13 @@ struct BrListArgs {
     const DRun* runs;
     const uint32_t* cnt;       // [P.n] matches per parent (zero for runs without branchial)
     const uint64_t* moff;      // [P.n + 1]
-    uint32_t* ecnt;            // [edge slots] list lengths
-    const uint64_t* eoff;      // [edge slots + 1] list offsets (scan of ecnt)
-    uint32_t* ecur;            // [edge slots] scatter cursors (zeroed)
-    uint32_t* elist;           // [entries] consuming matches, ascending per slot after the sort
+    uint64_t* key_in;          // [M.n * K] (slot << 32 | match), or the sentinel
+    uint32_t* val_in;          // [M.n * K] entry index (match * K + k)
+    const uint64_t* key;       // [n_keys] the keys, sorted
+    const uint32_t* val;       // [n_keys] their entry indices
+    uint32_t* pos;             // [M.n * K] sorted position of each entry
+    uint64_t n_keys;           // M.n * K
+    uint64_t sentinel;         // above every key: (edge slots) << 32
     uint32_t* bc;              // [M.n] pairs found per match (count phase)
     const uint64_t* bo;        // [M.n + 1] exclusive scan of bc (emit phase)
     uint64_t* ctr;
@@ -106,8 +109,9 @@ struct BrListArgs {
     const uint64_t* poff;      // [P.n + 1] (emit)
     uint32_t* pairs;           // (emit)
 };
-// phase 0 count, 1 scatter, 2 sort lists, 3 pairs (count), 4 pairs (emit)
-hipError_t launch_brl(const BrListArgs& a, int phase, uint64_t nslots, hipStream_t st);
+// phase 0 keys, 1 sorted positions, 2 pairs (count), 3 pairs (emit); the sort between 0 and 1
+// is the caller's (sort_keys_u64).
+hipError_t launch_brl(const BrListArgs& a, int phase, hipStream_t st);
 
 struct WLStatesArgs {
     uint32_t n;

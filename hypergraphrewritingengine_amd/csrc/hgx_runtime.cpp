@@ -263,26 +263,54 @@ RunHost parse_run(const hgx_run_desc* d) {
     const uint32_t n_roots = d->root_offsets ? d->n_roots : 1;
     if (d->root_offsets && (d->n_roots == 0 || d->root_offsets[0] != 0 || d->root_offsets[d->n_roots] != d->n_edges))
         throw Error{HGX_ERR_INVALID, "root_offsets must span the edges"};
+    uint32_t amin = 0xFFu, amax = 0, lmax = 0;
+    for (uint32_t e = 0; e < d->n_edges; ++e) {
+        if (d->edge_offsets[e + 1] < d->edge_offsets[e] || d->edge_offsets[e + 1] - d->edge_offsets[e] < 1 ||
+            d->edge_offsets[e + 1] - d->edge_offsets[e] > 16)
+            throw Error{HGX_ERR_INVALID, "state edge arity must be 1..16"};
+        const uint32_t a = d->edge_offsets[e + 1] - d->edge_offsets[e];
+        amin = std::min(amin, a);
+        amax = std::max(amax, a);
+    }
+    const uint32_t base = d->n_edges ? d->edge_offsets[0] : 0;        // offsets may start anywhere
+    const uint32_t nvert = d->n_edges ? d->edge_offsets[d->n_edges] - base : 0;
+    const uint32_t* vin = d->n_edges ? d->vertices + base : nullptr;
+    for (uint32_t i = 0; i < nvert; ++i) lmax = std::max(lmax, vin[i]);
+    h.max_arity = std::max(h.max_arity, amax);
+    h.edge_vb.resize((size_t)d->n_edges + 1);
+    h.edge_vb[0] = 0;
+    for (uint32_t e = 0; e < d->n_edges; ++e) h.edge_vb[e + 1] = d->edge_offsets[e + 1] - base;
+    h.verts.resize(nvert);
+    h.root_eb.resize(n_roots + 1);
+    h.root_eb[0] = 0;
+    // Renumbering by first appearance within each root (FFI :599-636): a dense table when the
+    // labels are small, a hash map otherwise.
+    const bool dense = (uint64_t)lmax < 8ull * nvert + 4096;
+    std::vector<uint32_t> tab;
+    std::unordered_map<uint32_t, uint32_t> lab;
+    if (dense) tab.assign((size_t)lmax + 1, INV);
     for (uint32_t r = 0; r < n_roots; ++r) {
         const uint32_t e0 = d->root_offsets ? d->root_offsets[r] : 0;
         const uint32_t e1 = d->root_offsets ? d->root_offsets[r + 1] : d->n_edges;
-        if (e1 < e0) throw Error{HGX_ERR_INVALID, "root_offsets not ascending"};
-        std::unordered_map<uint32_t, uint32_t> lab;   // dense by first appearance (FFI :599-636)
-        std::vector<std::vector<uint32_t>> st;
-        for (uint32_t e = e0; e < e1; ++e) {
-            const uint32_t a = d->edge_offsets[e + 1] - d->edge_offsets[e];
-            if (d->edge_offsets[e + 1] < d->edge_offsets[e] || a < 1 || a > 16)
-                throw Error{HGX_ERR_INVALID, "state edge arity must be 1..16"};
-            std::vector<uint32_t> ed(a);
-            for (uint32_t i = 0; i < a; ++i) {
-                auto it = lab.emplace(d->vertices[d->edge_offsets[e] + i], (uint32_t)lab.size()).first;
-                ed[i] = it->second;
+        if (e1 < e0 || e1 > d->n_edges) throw Error{HGX_ERR_INVALID, "root_offsets not ascending"};
+        const uint32_t v0 = h.edge_vb[e0], v1 = h.edge_vb[e1];
+        uint32_t nv = 0;
+        if (dense) {
+            for (uint32_t i = v0; i < v1; ++i) {
+                uint32_t& t = tab[vin[i]];
+                if (t == INV) t = nv++;
+                h.verts[i] = t;
             }
-            h.max_arity = std::max(h.max_arity, a);
-            st.push_back(std::move(ed));
+            for (uint32_t i = v0; i < v1; ++i) tab[vin[i]] = INV;
+        } else {
+            lab.clear();
+            for (uint32_t i = v0; i < v1; ++i) h.verts[i] = lab.emplace(vin[i], (uint32_t)lab.size()).first->second;
+            nv = (uint32_t)lab.size();
         }
-        h.roots.push_back(std::move(st));
-        h.root_nv.push_back((uint32_t)lab.size());
+        h.root_eb[r + 1] = e1;
+        h.root_nv.push_back(nv);
+        h.max_root_ne = std::max(h.max_root_ne, e1 - e0);
+        h.max_root_nv = std::max(h.max_root_nv, nv);
     }
     for (auto& R : h.rules) {
         for (int e = 0; e < R.n_rhs; ++e) h.max_arity = std::max<uint32_t>(h.max_arity, R.ra[e]);
@@ -291,10 +319,7 @@ RunHost parse_run(const hgx_run_desc* d) {
         h.max_gain = std::max<uint32_t>(h.max_gain, R.n_rhs > R.n_lhs ? R.n_rhs - R.n_lhs : 0);
         h.max_new = std::max<uint32_t>(h.max_new, R.n_new);
     }
-    for (auto& st : h.roots) h.root_edges += st.size();
-    uint32_t amin = 0xFFu, amax = 0;
-    for (auto& st : h.roots)
-        for (auto& ed : st) { amin = std::min<uint32_t>(amin, (uint32_t)ed.size()); amax = std::max<uint32_t>(amax, (uint32_t)ed.size()); }
+    h.root_edges = d->n_edges;
     for (auto& R : h.rules)
         for (int e = 0; e < R.n_rhs; ++e) { amin = std::min<uint32_t>(amin, R.ra[e]); amax = std::max<uint32_t>(amax, R.ra[e]); }
     h.uni_arity = (amin == amax && amax >= 2 && amax <= 4) ? amax : 0u;
@@ -306,8 +331,7 @@ RunHost parse_run(const hgx_run_desc* d) {
 // its size when uncapped) at its largest edge count, in the generation that holds it and the
 // one built from it, plus its state map.
 uint64_t footprint_bytes(const RunHost& h) {
-    uint64_t max_ne = 0;
-    for (auto& st : h.roots) max_ne = std::max<uint64_t>(max_ne, st.size());
+    uint64_t max_ne = h.max_root_ne;
     max_ne += (uint64_t)h.steps * h.max_gain;
     const uint64_t states = h.cap ? (uint64_t)h.cap : (uint64_t)1 << 20;
     const uint64_t per_state = 2 * max_ne * (4ull * h.max_arity + 2) + 96 + 8ull * h.steps;
@@ -328,6 +352,16 @@ void scan_u32_to_u64(Arena& ar, const uint32_t* in, uint64_t* out, uint32_t n, h
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, it, out, n + 1, st));
     void* t = ar.alloc(tmp);
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(t, tmp, it, out, n + 1, st));
+}
+
+// Sorts n (key, value) pairs by key bits [0, end_bit), stable.
+void sort_pairs_u64(Arena& ar, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
+                    uint64_t n, int end_bit, hipStream_t st) {
+    if (n >= 0x7FFFFFFFull) throw Error{HGX_ERR_UNSUPPORTED, "sort above 2^31 entries"};
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, (int)n, 0, end_bit, st));
+    void* t = ar.alloc(tmp);
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(t, tmp, kin, kout, vin, vout, (int)n, 0, end_bit, st));
 }
 
 uint32_t waves_per_block(uint32_t lds_per_wave, uint32_t max_lds) {

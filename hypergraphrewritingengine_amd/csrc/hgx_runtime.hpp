@@ -105,6 +105,8 @@ void d2h(T* dst, const T* src, size_t n, hipStream_t st) {
 
 // Exclusive prefix sum of n uint32 values (in[n] must read 0) into n+1 uint64 offsets.
 void scan_u32_to_u64(Arena& ar, const uint32_t* in, uint64_t* out, uint32_t n, hipStream_t st);
+void sort_pairs_u64(Arena& ar, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
+                    uint64_t n, int end_bit, hipStream_t st);
 
 // One parsed request.
 // Device copies of a prepared request's rules and initial states (hgx_batch_prepare).
@@ -121,8 +123,16 @@ struct StagedSrc {
 struct RunHost {
     StagedSrc staged;
     std::vector<DRule> rules;
-    std::vector<std::vector<std::vector<uint32_t>>> roots;   // renumbered per root
+    // Initial states, flat: root k holds edges [root_eb[k], root_eb[k+1]); edge e holds
+    // verts[edge_vb[e] .. edge_vb[e+1]), vertices renumbered per root by first appearance.
+    std::vector<uint32_t> root_eb{0};
+    std::vector<uint32_t> edge_vb{0};
+    std::vector<uint32_t> verts;
     std::vector<uint32_t> root_nv;
+    uint32_t max_root_ne = 0, max_root_nv = 0;
+    uint32_t n_roots() const { return (uint32_t)root_nv.size(); }
+    uint32_t root_ne(size_t k) const { return root_eb[k + 1] - root_eb[k]; }
+    uint32_t arity(size_t e) const { return edge_vb[e + 1] - edge_vb[e]; }
     uint32_t steps = 0, cap = 0, flags = 0;
     uint32_t max_arity = 1, max_lhs = 1, max_gain = 0, max_new = 0;
     uint32_t uni_arity = 0;      // k when every root edge and every RHS edge has arity k (2..4),

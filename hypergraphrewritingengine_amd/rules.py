@@ -6,6 +6,7 @@ Mirrors the reference's rule builder -- ``make_rule(i).lhs({0,1}).rhs({0,2}).bui
 """
 from __future__ import annotations
 
+import itertools
 from dataclasses import dataclass, field
 from typing import Iterable, Sequence
 
@@ -100,14 +101,16 @@ def encode_rules(rules: Sequence[Rule]) -> bytes:
 
 def encode_state(edges: Sequence[Sequence[int]]):
     """(edge_offsets[n+1] uint32, vertices uint32) of one edge list."""
-    offs = np.zeros(len(edges) + 1, dtype=np.uint32)
-    flat: list = []
-    for i, e in enumerate(edges):
-        if not (1 <= len(e) <= MAX_ARITY):
-            raise ValueError("state: edge arity must be 1..16")
-        flat.extend(int(v) for v in e)
-        offs[i + 1] = len(flat)
-    return offs, np.asarray(flat, dtype=np.uint32)
+    n = len(edges)
+    lens = np.fromiter(map(len, edges), dtype=np.int64, count=n)
+    if n and (lens.min() < 1 or lens.max() > MAX_ARITY):
+        raise ValueError("state: edge arity must be 1..16")
+    offs = np.zeros(n + 1, dtype=np.uint32)
+    offs[1:] = np.cumsum(lens)
+    flat = np.fromiter(itertools.chain.from_iterable(edges), dtype=np.int64, count=int(offs[-1]))
+    if flat.size and (flat.min() < 0 or flat.max() > 0xFFFFFFFF):
+        raise ValueError("state: vertex labels must be 0..2^32-1")
+    return offs, flat.astype(np.uint32)
 
 
 def encode_roots(roots: Sequence[Sequence[Sequence[int]]]):
