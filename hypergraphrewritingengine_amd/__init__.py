@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = [
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
 OPTION_DELTA_CAP = 2   # HGX_OPTION_DELTA_CAP
 OPTION_EXPAND_TEAM = 3  # HGX_OPTION_EXPAND_TEAM
+OPTION_JOIN_TEAM = 4    # HGX_OPTION_JOIN_TEAM
 
 
 class HGXError(RuntimeError):
@@ -268,6 +269,13 @@ class Context:
         """Waves that rewrite and hash one child: 1, 2, 4 or 8 for every child, 0 automatic.
         Results never depend on it."""
         rc = self._lib.hgx_ctx_set_option(self._h, OPTION_EXPAND_TEAM, int(waves))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def set_join_team(self, waves: int):
+        """Waves that join one parent state: 1, 4 or 8 for every state, 0 automatic. Results
+        never depend on it."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_JOIN_TEAM, int(waves))
         if rc != abi.HGX_OK:
             raise self._err(rc)
 

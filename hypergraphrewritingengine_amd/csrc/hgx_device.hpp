@@ -222,6 +222,40 @@ struct Team {
         if constexpr (W == 1) return w;
         else return across(w && lane_id() == 0 ? 1u : 0u) != 0;
     }
+    // Inclusive prefix sum of one value per member in rank order; the team's total in `total`.
+    __device__ uint32_t incl_scan(uint32_t v, uint32_t& total) const {
+        const uint32_t inc = wave_incl_scan_u32(v);
+        if constexpr (W == 1) {
+            total = __shfl(inc, WAVE - 1, WAVE);
+            return inc;
+        } else {
+            const uint32_t w = threadIdx.x / WAVE;
+            if (lane_id() == WAVE - 1) red[w] = inc;
+            __syncthreads();
+            uint32_t pre = 0, t = 0;
+            for (uint32_t k = 0; k < W; ++k) {
+                const uint32_t c = (uint32_t)red[k];
+                pre += k < w ? c : 0u;
+                t += c;
+            }
+            __syncthreads();
+            total = t;
+            return pre + inc;
+        }
+    }
+    // The first member's value, to every member.
+    __device__ uint64_t bcast(uint64_t v) const {
+        if constexpr (W == 1) {
+            const uint32_t lo = __shfl((uint32_t)v, 0, WAVE), hi = __shfl((uint32_t)(v >> 32), 0, WAVE);
+            return ((uint64_t)hi << 32) | lo;
+        } else {
+            if (r == 0) red[0] = v;
+            __syncthreads();
+            const uint64_t x = red[0];
+            __syncthreads();
+            return x;
+        }
+    }
     // Stream compaction of one round of N items, item r kept when `keep`: returns the item's
     // place among the round's kept items and their number in `total`.
     __device__ uint32_t compact(bool keep, uint32_t& total) const {

@@ -229,9 +229,9 @@ uint32_t Engine::step() {
             ma.P = F_->v; ma.runs = d_runs_; ma.active = d_active_; ma.cnt = cnt; ma.moff = moff;
             ma.max_ne = std::max(1u, F_->max_ne);
             ma.max_nextv = std::max(1u, F_->max_nextv);
-            // forwarding: on when asked; automatic forwards the count pass of states above 128
-            // edges (measured cheaper than their full join) and joins everything else in full
-            // (cheaper for small states, and for the emit pass, where the merge costs more)
+            // forwarding: on when asked; automatic forwards the states of the size classes above
+            // 512 edges (count and emit measured up to 2x cheaper than their full join there) and
+            // joins smaller states in full (cheaper up to a few hundred edges)
             const bool fwd = prev_.n > 0 && prev_.n <= n && ctx_->forward != 0;
             ma.fwd_n = fwd ? prev_.n : 0;
             ma.pm_parent = prev_.parent; ma.pm_rule = prev_.rule; ma.pm_q = prev_.q; ma.pm_K = prev_.K;
@@ -266,11 +266,17 @@ uint32_t Engine::step() {
                 for (auto& x : cls) {
                     if (!x.n) continue;
                     MatchArgs mc = ma;
-                    if (ctx_->forward == -1 && x.ne <= kClassEdges[0]) { mc.fwd_n = 0; mc.dmax = 0; }
+                    if (ctx_->forward == -1 && x.ne <= kClassEdges[1]) { mc.fwd_n = 0; mc.dmax = 0; }
                     mc.max_ne = std::max(1u, std::min(x.ne, F_->max_ne));
                     mc.max_nextv = std::max(1u, std::min(x.nextv, F_->max_nextv));
-                    mc.lds_per_wave = (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax,
-                                                          mc.small != 0) + 15u) & ~15u;
+                    // Team: as for the expansion, by the one-wave workspace
+                    auto lpw = [&](uint32_t t) {
+                        return (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax, mc.small != 0,
+                                                   t) + 15u) & ~15u;
+                    };
+                    uint32_t team = ctx_->join_team;
+                    if (!team) { const uint32_t b1 = lpw(1); team = b1 <= kTeamBytes[0] ? 1u : b1 <= kTeamBytes[1] ? 4u : 8u; }
+                    mc.lds_per_wave = lpw(team);
                     if (x.n != n) {
                         uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
                         h2d(d_r, x.rb.data(), x.rb.size(), st);
@@ -278,9 +284,9 @@ uint32_t Engine::step() {
                         mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
                         mc.n_items = (uint32_t)x.n;
                     }
-                    const uint32_t wpb_m = waves_per_block(mc.lds_per_wave, ctx_->max_lds);
+                    const uint32_t wpb_m = waves_per_block(mc.lds_per_wave, ctx_->max_lds);   // checks the fit
                     timed(ctx_, st, name, (uint64_t)((double)bytes * x.n / n), x.n,
-                          [&] { return launch_match(mc, emit, ctx_->num_cu * 8, wpb_m, st); });
+                          [&] { return launch_match(mc, emit, ctx_->num_cu * 8, team == 1 ? wpb_m : 1, team, st); });
                 }
             };
             HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
@@ -343,10 +349,6 @@ uint32_t Engine::step() {
                 HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
                 ma.M = mv;
                 ma.parent_edges = nullptr;
-                if (ma.fwd_n && ctx_->forward != 1) {
-                    ma.fwd_n = 0;
-                    ma.dmax = 0;
-                }
                 launch_classes(true, mcls_emit, "k_match_emit", match_bytes + M * (12ull + 2ull * K));
             }
         }

@@ -327,20 +327,26 @@ __device__ __forceinline__ uint32_t reg_join_any(const DRule& R, const uint32_t*
 }
 
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
-                            bool small) {
+                            bool small, uint32_t team) {
     auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
+    // + 64: carving slack; + 64: the team's reduction words at the end
     uint32_t b = al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
-                 al(max_ne * A * 2) * 2 + al(max_ne * A * 4) + 64;
-    if (small) b += JoinLaneLds::WORDS * WAVE * 4 + 2 * (uint32_t)sizeof(DRule);
+                 al(max_ne * A * 2) * 2 + al(max_ne * A * 4) + 128;
+    if (small) b += JoinLaneLds::WORDS * WAVE * 4 * team + 2 * (uint32_t)sizeof(DRule);
     if (dmax) b += al(dmax * (KL + 1) * 2) + al(dmax * 2) + al((dmax + 1) * 4);
     return b;
 }
 
-template <bool EMIT, class JoinLane>
-__global__ __launch_bounds__(256) void k_match(MatchArgs a) {
+// W is the team: one wave per state (W = 1, up to four per workgroup, each with its own
+// workspace) or the W waves of a workgroup sharing one state and one workspace (large states,
+// whose workspace would otherwise leave a CU one or two waves).
+template <bool EMIT, class JoinLane, uint32_t W>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
     extern __shared__ uint8_t lds[];
+    constexpr uint32_t N = Team<W>::N;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    uint8_t* p = lds + (size_t)w * a.lds_per_wave;
+    uint8_t* p = lds + (W == 1 ? (size_t)w * a.lds_per_wave : 0);
+    const Team<W> tm((uint64_t*)(p + a.lds_per_wave - 64));      // the workspace's last 64 bytes
     const uint32_t A = a.P.A;
     uint32_t* sv = (uint32_t*)carve(p, a.max_ne * A * 4);      // vertices
     uint8_t* sa = carve(p, a.max_ne);                           // arity
@@ -359,13 +365,15 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     constexpr bool SMALL = std::is_same<JoinLane, JoinLaneLds>::value;
     DRule* lr = nullptr;
     if constexpr (SMALL) {
-        L.at((uint32_t*)carve(p, JoinLaneLds::WORDS * WAVE * 4), l);
+        // one lane-interleaved block per wave
+        L.at((uint32_t*)carve(p, JoinLaneLds::WORDS * WAVE * 4 * W) + (W == 1 ? 0 : w * JoinLaneLds::WORDS * WAVE), l);
         lr = (DRule*)carve(p, 2 * sizeof(DRule));
     }
 
-    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
+    const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;      // teams per workgroup
+    const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
-    for (uint32_t ls = blockIdx.x * (blockDim.x / WAVE) + w; ls < n_items; ls += waves) {
+    for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? w : 0); ls < n_items; ls += teams) {
         uint32_t s = ls;
         if (a.nrange) {                                         // subset: logical -> state
             uint32_t lo = 0, hi = a.nrange;
@@ -373,41 +381,45 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             s = a.rb[lo] + (ls - a.rpre[lo]);
         }
         const uint32_t run = a.P.run[s];
-        if (!a.active[run]) { if (!EMIT && l == 0) a.cnt[s] = 0; continue; }
+        if (!a.active[run]) { if (!EMIT && tm.leader()) a.cnt[s] = 0; continue; }
         const uint64_t budget = EMIT ? 0 : (uint64_t)a.budget[run];
-        if (!EMIT && a.run_acc[run] > budget) { if (l == 0) a.cnt[s] = 0; continue; }
+        // the run's tally moves under other teams: one read, shared, so the team agrees
+        if (!EMIT && tm.bcast(tm.leader() ? (uint64_t)a.run_acc[run] : 0ull) > budget) {
+            if (tm.leader()) a.cnt[s] = 0;
+            continue;
+        }
         const uint32_t ne = a.P.ne[s];
         const uint64_t eo = a.P.eoff[s];
         const uint32_t nvid = a.P.nextv[s];                     // labels are < nvid
-        for (uint32_t i = l; i < ne * A; i += WAVE) sv[i] = a.P.ev[eo * A + i];
-        for (uint32_t i = l; i < ne; i += WAVE) sa[i] = a.P.ar[eo + i];
-        for (uint32_t v = l; v <= nvid; v += WAVE) iend[v] = 0;
-        wave_sync();
+        for (uint32_t i = tm.r; i < ne * A; i += N) sv[i] = a.P.ev[eo * A + i];
+        for (uint32_t i = tm.r; i < ne; i += N) sa[i] = a.P.ar[eo + i];
+        for (uint32_t v = tm.r; v <= nvid; v += N) iend[v] = 0;
+        tm.sync();
         // Incidence index (vertex -> ascending edges, one entry per (vertex, edge)): counting
         // sort with LDS atomics, then each list put in edge order.
-        for (uint32_t e = l; e < ne; e += WAVE)
+        for (uint32_t e = tm.r; e < ne; e += N)
             for (uint32_t k = 0; k < sa[e]; ++k) {
                 const uint32_t x = sv[e * A + k];
                 bool dup = false;
                 for (uint32_t k2 = 0; k2 < k; ++k2) dup |= (sv[e * A + k2] == x);
                 if (!dup) atomicAdd(&iend[x], 1u);
             }
-        wave_sync();
-        const uint32_t nslot = wave_excl_scan_lds(iend, nvid + 1);   // iend[x] = start of x's list
-        for (uint32_t x = l; x < nvid; x += WAVE)
+        tm.sync();
+        const uint32_t nslot = tm.excl_scan_lds(iend, nvid + 1);    // iend[x] = start of x's list
+        for (uint32_t x = tm.r; x < nvid; x += N)
             for (uint32_t i = iend[x]; i < iend[x + 1]; ++i) iown[i] = x;
-        wave_sync();
-        for (uint32_t e = l; e < ne; e += WAVE)
+        tm.sync();
+        for (uint32_t e = tm.r; e < ne; e += N)
             for (uint32_t k = 0; k < sa[e]; ++k) {
                 const uint32_t x = sv[e * A + k];
                 bool dup = false;
                 for (uint32_t k2 = 0; k2 < k; ++k2) dup |= (sv[e * A + k2] == x);
                 if (!dup) ie2[atomicAdd(&iend[x], 1u)] = (uint16_t)e;   // -> iend[x] = end
             }
-        wave_sync();
+        tm.sync();
         // each list into ascending edge order: an entry's place is the number of smaller entries
         // of its list, so a hub vertex's list is placed by as many lanes as it has entries
-        for (uint32_t j = l; j < nslot; j += WAVE) {
+        for (uint32_t j = tm.r; j < nslot; j += N) {
             const uint32_t x = iown[j];
             const uint32_t b = x ? iend[x - 1] : 0u, n = iend[x] - b;
             const uint16_t key = ie2[j];
@@ -416,7 +428,7 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             for (uint32_t i = 0; i < n; ++i) r += ie2[b + i] < key ? 1u : 0u;
             ie[b + r] = key;
         }
-        wave_sync();
+        tm.sync();
 
         const DRun& RN = a.runs[run];
         const DRule* rules;
@@ -424,8 +436,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             const uint32_t* src = (const uint32_t*)RN.rules;
             uint32_t* dst = (uint32_t*)lr;
             const uint32_t words = RN.n_rules * (uint32_t)(sizeof(DRule) / 4);
-            for (uint32_t i = l; i < words; i += WAVE) dst[i] = src[i];
-            wave_sync();
+            for (uint32_t i = tm.r; i < words; i += N) dst[i] = src[i];
+            tm.sync();
             rules = lr;
         } else {
             rules = RN.rules;
@@ -455,8 +467,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             };
             uint32_t nd = 0;
             uint64_t de = 0;                                    // child edges of the delta matches
-            for (uint32_t base = 0; base < ncombo; base += WAVE) {
-                const uint32_t i = base + l;
+            for (uint32_t base = 0; base < ncombo; base += N) {
+                const uint32_t i = base + tm.r;
                 uint32_t c = 0;
                 if (i < ncombo) {
                     uint32_t r, pp, j;
@@ -464,27 +476,28 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                     c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
                     de += (uint64_t)c * (ne - rules[r].n_lhs + rules[r].n_rhs);
                 }
-                nd += wave_sum_u32(c);
+                nd += tm.sum(c);
             }
             if (!EMIT || (a.dmax && nd <= a.dmax)) {
                 full = false;
                 if (EMIT) {
                     uint32_t o = 0;
-                    for (uint32_t base = 0; base < ncombo; base += WAVE) {
-                        const uint32_t i = base + l;
+                    for (uint32_t base = 0; base < ncombo; base += N) {
+                        const uint32_t i = base + tm.r;
                         uint32_t c = 0, r = 0, pp = 0, j = 0;
                         if (i < ncombo) {
                             combo(i, r, pp, j);
                             c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
                         }
-                        const uint32_t inc = wave_incl_scan_u32(c);
+                        uint32_t round;
+                        const uint32_t inc = tm.incl_scan(c, round);
                         if (c)
                             join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L,
                                         dlist + (size_t)(o + inc - c) * dstride, dstride, r);
-                        o += __shfl(inc, WAVE - 1, WAVE);
+                        o += round;
                     }
-                    wave_sync();
-                    for (uint32_t i = l; i < nd; i += WAVE) {        // rank = place in order
+                    tm.sync();
+                    for (uint32_t i = tm.r; i < nd; i += N) {        // rank = place in order
                         const uint16_t* di = dlist + (size_t)i * dstride;
                         uint32_t rank = 0;
                         for (uint32_t j = 0; j < nd; ++j) {
@@ -493,12 +506,12 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                         }
                         dperm[rank] = (uint16_t)i;
                     }
-                    for (uint32_t i = l; i <= nd; i += WAVE) dhist[i] = 0;
-                    wave_sync();
+                    for (uint32_t i = tm.r; i <= nd; i += N) dhist[i] = 0;
+                    tm.sync();
                 }
                 uint32_t nf = 0;
-                for (uint64_t base = lb; base < le; base += WAVE) {
-                    const uint64_t m = base + l;
+                for (uint64_t base = lb; base < le; base += N) {
+                    const uint64_t m = base + tm.r;
                     bool fw = false;
                     uint32_t rm = 0, nm = 0;
                     uint16_t qm[16];
@@ -517,7 +530,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                             qm[t] = (uint16_t)(x - below);   // position among the kept edges
                         }
                     }
-                    const uint64_t bal = __ballot(fw);
+                    uint32_t nround;
+                    const uint32_t at = tm.compact(fw, nround);
                     if (!EMIT && fw) de += ne - nm + rules[rm].n_rhs;
                     if (EMIT && fw) {
                         uint32_t lo = 0, hi = nd;                   // delta matches before it
@@ -526,19 +540,19 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                             const uint16_t* dm = dlist + (size_t)dperm[mid] * dstride;
                             if (match_less(rules, dm[0], dm + 1, rm, qm)) lo = mid + 1; else hi = mid;
                         }
-                        const uint32_t o2 = out + nf + lanes_below(bal) + lo;
+                        const uint32_t o2 = out + nf + at + lo;
                         for (uint32_t t = 0; t < nm; ++t) a.M.q[(size_t)o2 * a.M.K + t] = qm[t];
                         a.M.parent[o2] = s;
                         a.M.rule[o2] = rm;
                         a.M.cne[o2] = ne - nm + rules[rm].n_rhs;
                         atomicAdd(&dhist[lo], 1u);
                     }
-                    nf += __popcll(bal);
+                    nf += nround;
                 }
                 if (EMIT) {
-                    wave_sync();
-                    wave_excl_scan_lds(dhist, nd + 1);        // dhist[k + 1] = forwarded before delta k
-                    for (uint32_t k = l; k < nd; k += WAVE) {
+                    tm.sync();
+                    tm.excl_scan_lds(dhist, nd + 1);          // dhist[k + 1] = forwarded before delta k
+                    for (uint32_t k = tm.r; k < nd; k += N) {
                         const uint16_t* dm = dlist + (size_t)dperm[k] * dstride;
                         const uint32_t rr = dm[0], nm = rules[rr].n_lhs;
                         const uint32_t o2 = out + dhist[k + 1] + k;
@@ -547,11 +561,11 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                         a.M.rule[o2] = rr;
                         a.M.cne[o2] = ne - nm + rules[rr].n_rhs;
                     }
-                    wave_sync();
+                    tm.sync();
                 } else {
                     total = nf + nd;
-                    edges_sum = wave_sum_u64(de);
-                    if (l == 0 && total) atomicAdd(&a.run_acc[run], (unsigned long long)total);
+                    edges_sum = tm.sum64(de);
+                    if (tm.leader() && total) atomicAdd(&a.run_acc[run], (unsigned long long)total);
                 }
             }
         }
@@ -561,21 +575,22 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
             const uint32_t child_ne = ne - R.n_lhs + R.n_rhs;
             // SCAN: compact the seeds of position order[0] in edge order.
             uint32_t nseed = 0;
-            for (uint32_t base = 0; base < ne; base += WAVE) {
-                const uint32_t e = base + l;
+            for (uint32_t base = 0; base < ne; base += N) {
+                const uint32_t e = base + tm.r;
                 bool ok = false;
                 if (e < ne && sa[e] == R.la[p0]) {
                     uint32_t m = 0;
                     ok = bind_edge(sv + e * A, sa[e], R.lv[p0], R.la[p0], L.bind, m);
                 }
-                const uint64_t bal = __ballot(ok);
-                if (ok) seeds[nseed + lanes_below(bal)] = (uint16_t)e;
-                nseed += __popcll(bal);
+                uint32_t nround;
+                const uint32_t at = tm.compact(ok, nround);
+                if (ok) seeds[nseed + at] = (uint16_t)e;
+                nseed += nround;
             }
-            wave_sync();
+            tm.sync();
             // EXPAND + SINK, 64 seeds per round.
-            for (uint32_t base = 0; base < nseed; base += WAVE) {
-                const uint32_t i = base + l;
+            for (uint32_t base = 0; base < nseed; base += N) {
+                const uint32_t i = base + tm.r;
                 uint32_t c = 0;
                 if (i < nseed) {
                     if constexpr (SMALL) c = reg_join_any<false>(R, sv, sa, ne, A, iend, ie, seeds[i], RegJoinOut{});
@@ -583,7 +598,8 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                                        nullptr, 0, 0, 0);
                 }
                 if (EMIT) {
-                    const uint32_t inc = wave_incl_scan_u32(c);
+                    uint32_t round;
+                    const uint32_t inc = tm.incl_scan(c, round);
                     if (i < nseed && c) {
                         if constexpr (SMALL) {
                             const RegJoinOut ro{a.M.q, a.M.K, out + inc - c, a.M.parent, a.M.rule, a.M.cne,
@@ -594,23 +610,23 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
                                       a.M.parent, a.M.rule, a.M.cne, s, r, child_ne);
                         }
                     }
-                    out += __shfl(inc, WAVE - 1, WAVE);
+                    out += round;
                 } else {
                     // The run's shared tally: once it passes what the run may still take, the
                     // generation is over the cap whatever the other states hold, so stop. Every
                     // state's count equals what it added, so the totals stay over the cap.
-                    const uint32_t round = wave_sum_u32(c);
+                    const uint32_t round = tm.sum(c);
                     total += round;
                     edges_sum += (uint64_t)round * child_ne;
                     unsigned long long before = 0;
-                    if (l == 0 && round) before = atomicAdd(&a.run_acc[run], (unsigned long long)round);
-                    before = __shfl(before, 0, WAVE);
+                    if (tm.leader() && round) before = atomicAdd(&a.run_acc[run], (unsigned long long)round);
+                    before = tm.bcast(before);
                     if (before + round > budget) { stop = true; break; }
                 }
             }
-            wave_sync();
+            tm.sync();
         }
-        if (!EMIT && l == 0) {
+        if (!EMIT && tm.leader()) {
             a.cnt[s] = total;
             if (total && a.parent_edges) atomicAdd(a.parent_edges, (unsigned long long)ne);
             if (edges_sum) atomicAdd(&a.run_edges[run], (unsigned long long)edges_sum);
@@ -618,10 +634,6 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
     }
 }
 
-template __global__ void k_match<false, JoinLaneBig>(MatchArgs);
-template __global__ void k_match<true, JoinLaneBig>(MatchArgs);
-template __global__ void k_match<false, JoinLaneLds>(MatchArgs);
-template __global__ void k_match<true, JoinLaneLds>(MatchArgs);
 
 // ------------------------------------------------------------------------------------------
 // Rewrite + hash + dedup + event + causal, one wave per child (Rewriter::apply,
@@ -1201,17 +1213,29 @@ static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
     return (g + 7u) & ~7u;
 }
 
-hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
+template <uint32_t W>
+static void launch_match_team(const MatchArgs& a, bool emit, uint32_t grid, uint32_t wpb, hipStream_t st) {
+    const uint32_t threads = W == 1 ? 64 * wpb : 64 * W;
+    const size_t lds = (size_t)a.lds_per_wave * (W == 1 ? wpb : 1);
+    if (a.small) {
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneLds, W>), dim3(grid), dim3(threads), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneLds, W>), dim3(grid), dim3(threads), lds, st, a);
+    } else {
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneBig, W>), dim3(grid), dim3(threads), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneBig, W>), dim3(grid), dim3(threads), lds, st, a);
+    }
+}
+
+hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, uint32_t team,
+                        hipStream_t st) {
     const uint32_t items = a.nrange ? a.n_items : a.P.n;
     if (!items) return hipSuccess;
-    const uint32_t grid = grid_for(items, wpb, grid_cap);
-    const size_t lds = (size_t)a.lds_per_wave * wpb;
-    if (a.small) {
-        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneLds>), dim3(grid), dim3(64 * wpb), lds, st, a);
-        else hipLaunchKernelGGL((k_match<false, JoinLaneLds>), dim3(grid), dim3(64 * wpb), lds, st, a);
-    } else {
-        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneBig>), dim3(grid), dim3(64 * wpb), lds, st, a);
-        else hipLaunchKernelGGL((k_match<false, JoinLaneBig>), dim3(grid), dim3(64 * wpb), lds, st, a);
+    const uint32_t grid = grid_for(items, team == 1 ? wpb : 1, grid_cap);
+    switch (team) {
+        case 1: launch_match_team<1>(a, emit, grid, wpb, st); break;
+        case 4: launch_match_team<4>(a, emit, grid, wpb, st); break;
+        case 8: launch_match_team<8>(a, emit, grid, wpb, st); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

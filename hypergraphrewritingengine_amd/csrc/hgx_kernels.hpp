@@ -131,7 +131,7 @@ struct WLStatesArgs {
 };
 
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
-                            bool small);
+                            bool small, uint32_t team);
 // One admitted request: its rules (copied from a packed staging array), its state map (zeroed)
 // and its counter row (zeroed) -- every admission of a step in one launch.
 struct AdmitItem {
@@ -155,7 +155,9 @@ struct RootCopy {
 };
 hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v, hipStream_t st);
 
-hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+// team: waves per state (1, 4 or 8); with team 1, wpb states per workgroup.
+hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, uint32_t team,
+                        hipStream_t st);
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean);
 // team: waves per child (1, 2, 4 or 8); with team 1, wpb children per workgroup.
 hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st);

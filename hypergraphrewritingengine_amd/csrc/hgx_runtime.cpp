@@ -482,6 +482,10 @@ extern "C" int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value) 
         ctx->expand_team = (uint32_t)value;
         return HGX_OK;
     }
+    if (option == HGX_OPTION_JOIN_TEAM && (value == 0 || value == 1 || value == 4 || value == 8)) {
+        ctx->join_team = (uint32_t)value;
+        return HGX_OK;
+    }
     ctx->err = "unknown option or value";
     return HGX_ERR_INVALID;
 }

@@ -178,6 +178,7 @@ struct hgx_ctx {
     int forward = -1;                            // HGX_OPTION_FORWARD
     uint32_t delta_cap = 256;                    // HGX_OPTION_DELTA_CAP
     uint32_t expand_team = 0;                    // HGX_OPTION_EXPAND_TEAM
+    uint32_t join_team = 0;                      // HGX_OPTION_JOIN_TEAM
 };
 
 namespace hgx {

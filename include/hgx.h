@@ -159,10 +159,13 @@ size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
  *   HGX_OPTION_EXPAND_TEAM waves that rewrite and hash one child: 1, 2, 4 or 8 for every
  *                       child, 0 automatic (default: one wave for children whose workspace is
  *                       small, a workgroup of 4 or 8 waves sharing one workspace for large ones).
+ *   HGX_OPTION_JOIN_TEAM waves that join one parent state: 1, 4 or 8 for every state, 0 automatic
+ *                       (chosen the same way).
  * Returns HGX_ERR_INVALID for an unknown option or value. */
 #define HGX_OPTION_FORWARD 1
 #define HGX_OPTION_DELTA_CAP 2
 #define HGX_OPTION_EXPAND_TEAM 3
+#define HGX_OPTION_JOIN_TEAM 4
 int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value);
 
 /* ---- synchronous batch: the plain and the batched path -------------------------------- */

@@ -1,7 +1,7 @@
-"""GPU parity of the expansion team size (HGX_OPTION_EXPAND_TEAM): one child rewritten and
-hashed by one wave, or by the 2, 4 or 8 waves of a workgroup sharing one workspace. Every
-event, state hash and relation must be the same whichever team size runs it, and the same as
-the oracle's. The cases cover the WL hash's uniform-arity front end (binary and ternary states
+"""GPU parity of the team sizes: one child rewritten and hashed (HGX_OPTION_EXPAND_TEAM), or
+one parent state joined (HGX_OPTION_JOIN_TEAM), by one wave or by the waves of a workgroup
+sharing one workspace. Every event, state hash and relation must be the same whichever team
+size runs it, and the same as the oracle's. The cases cover the WL hash's uniform-arity front end (binary and ternary states
 above 256 edges included), its general front end (mixed and unary arities), states whose
 workspace exceeds the LDS (global workspace) and the automatic choice.
 """
@@ -28,6 +28,21 @@ def _by_team(ctx, reqs, teams=TEAMS):
             out[t] = ctx.evolve_batch(reqs)
     finally:
         ctx.set_expand_team(0)
+    return out
+
+
+def _by_join_team(ctx, reqs):
+    """Join team sizes 1, 4, 8 and automatic, each with forwarding on and off."""
+    out = {}
+    try:
+        for fwd in (1, 0):
+            ctx.set_forwarding(fwd)
+            for t in (0, 1, 4, 8):
+                ctx.set_join_team(t)
+                out[(fwd, t)] = ctx.evolve_batch(reqs)
+    finally:
+        ctx.set_join_team(0)
+        ctx.set_forwarding(-1)
     return out
 
 
@@ -117,9 +132,40 @@ def test_team_global_workspace(ctx):
     assert_exact(r[1][0], po.evolve(req))
 
 
+def test_join_team_random_rules_exact(ctx):
+    """Join teams on random rule sets (forwarding's delta joins and merge included)."""
+    reqs = [_random_request(2000 + s) for s in range(40)]
+    r = _by_join_team(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for key in r:
+            _same(r[key][i], r[(0, 1)][i])
+        assert_exact(r[(1, 8)][i], po.evolve(req))
+
+
+def test_join_team_large_states(ctx):
+    """Join teams on 400-1500 edge binary, ternary and mixed-arity states."""
+    rb = rule_from_lists([[0, 1], [1, 2]], [[0, 2], [2, 3], [3, 1]])
+    rt = rule_from_lists([[0, 1, 2], [2, 3, 4]], [[0, 3, 5], [1, 4, 5], [2, 5, 5]])
+    ru = rule_from_lists([[0], [0, 1]], [[1], [0, 1], [1, 2]])
+    reqs = [Request([rb], W._connected_init(W.Stream(5), 400), 2, state_cap=3000, flags=REL | TR),
+            Request([rt], _ternary_state(7, 400), 2, state_cap=3000, flags=REL | TR),
+            Request([rb, ru], _mixed_state(8, 450), 2, state_cap=3000, flags=REL | TR)]
+    st = W.Stream(W.splitmix64(0xAB01))
+    reqs.append(Request([W._random_rule(st)], W._connected_init(st, 700), 2, state_cap=20000, flags=REL | TR))
+    r = _by_join_team(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for key in r:
+            _same(r[key][i], r[(0, 1)][i])
+        assert_exact(r[(1, 0)][i], po.evolve(req))
+
+
 def test_team_option_rejects_bad_values(ctx):
     import hypergraphrewritingengine_amd as hgx
     for bad in (3, 16, -1):
         with pytest.raises(hgx.HGXError):
             ctx.set_expand_team(bad)
+    for bad in (2, 3, 16, -1):
+        with pytest.raises(hgx.HGXError):
+            ctx.set_join_team(bad)
     ctx.set_expand_team(0)
+    ctx.set_join_team(0)
