@@ -169,9 +169,6 @@ struct ExpandClass {
     std::vector<uint32_t> rb, rpre{0};
 };
 constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
-// Per-child workspace (bytes) up to which one wave rewrites and hashes a child, and up to which
-// four waves share one; above, eight.
-constexpr uint32_t kTeamBytes[2] = {24u << 10, 48u << 10};
 }  // namespace
 
 uint32_t Engine::step() {
@@ -274,8 +271,7 @@ uint32_t Engine::step() {
                         return (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax, mc.small != 0,
                                                    t) + 15u) & ~15u;
                     };
-                    uint32_t team = ctx_->join_team;
-                    if (!team) { const uint32_t b1 = lpw(1); team = b1 <= kTeamBytes[0] ? 1u : b1 <= kTeamBytes[1] ? 4u : 8u; }
+                    const uint32_t team = pick_team(lpw(1), ctx_->join_team);
                     mc.lds_per_wave = lpw(team);
                     if (x.n != n) {
                         uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
@@ -521,8 +517,7 @@ uint32_t Engine::step() {
                 }
                 // Team: one wave per child while a CU still holds several workspaces, else the
                 // workgroup's waves share one child (a CU holds 160 KiB of LDS).
-                uint32_t team = ctx_->expand_team;
-                if (!team) team = ea.lds_per_wave <= kTeamBytes[0] ? 1u : ea.lds_per_wave <= kTeamBytes[1] ? 4u : 8u;
+                const uint32_t team = pick_team(ea.lds_per_wave, ctx_->expand_team);
                 uint32_t wpb_e = 1;
                 if (team == 1) {
                     wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds, ctx_->num_cu * 8,
@@ -599,11 +594,13 @@ uint32_t Engine::step() {
             w.n = NR; w.A = A; w.state_off = v.eoff + M; w.ar = v.ar; w.ev = v.ev; w.nv = v.nextv + M;
             w.out = v.hash + M; w.run_of = v.run + M; w.sid_of = v.sid + M; w.runs = d_runs_; w.ctr = d_ctr_;
             w.max_ne = rmax_ne; w.max_nv = rmax_nv;
-            w.lds_per_wave = wl_states_lds_per_wave(w.max_ne, w.max_nv, A);
-            const uint32_t wpb = place_workspace(tmp, w.lds_per_wave, NR, ctx_->max_lds, ctx_->num_cu * 8,
-                                                 &w.scratch);
+            bool roots_uniform = true;
+            for (auto& r : admitted) roots_uniform = roots_uniform && r->hp->root_uni;
+            uint32_t wpb = 1;
+            const uint32_t team = place_wl_states(tmp, w, roots_uniform, ctx_->max_lds, ctx_->num_cu * 8,
+                                                  ctx_->expand_team, &wpb);
             timed(ctx_, st, "k_wl_states", NER * (4ull * A + 1) + NR * 36ull, NR,
-                  [&] { return launch_wl_states(w, ctx_->num_cu * 8, wpb, st); });
+                  [&] { return launch_wl_states(w, ctx_->num_cu * 8, wpb, team, st); });
         }
 
         // branchial relation of this generation's events

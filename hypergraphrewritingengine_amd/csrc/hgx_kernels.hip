@@ -1086,32 +1086,33 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
 // each into its run's state map (roots).
 // ------------------------------------------------------------------------------------------
 
-template <bool GWS>
-__global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
+template <bool GWS, uint32_t W>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_wl_states(WLStatesArgs a) {
     extern __shared__ uint8_t lds[];
-    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    const uint32_t wpb_ = blockDim.x / WAVE;
+    const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;    // teams per workgroup
+    const uint32_t w = W == 1 ? threadIdx.x / WAVE : 0;     // team within the workgroup
     uint8_t* p;
-    if constexpr (GWS) p = a.scratch + ((size_t)blockIdx.x * wpb_ + w) * a.lds_per_wave;
+    if constexpr (GWS) p = a.scratch + ((size_t)blockIdx.x * tpb + w) * a.lds_per_wave;
     else p = lds + (size_t)w * a.lds_per_wave;
+    const Team<W> tm((uint64_t*)(p + a.lds_per_wave - 64));   // the workspace's last 64 bytes
     const uint32_t A = a.A;
-    const WLSizes z = WLSizes::make(a.max_ne, a.max_nv, A);
+    const WLSizes z = WLSizes::make(a.max_ne, a.max_nv, A, a.lean != 0);
     WLSpace s;
     wl_bind(s, p, z, A);
-    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
-    for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + w; i < a.n; i += waves) {
+    const uint32_t teams = gridDim.x * tpb;
+    for (uint32_t i = blockIdx.x * tpb + w; i < a.n; i += teams) {
         const uint64_t e0 = a.state_off[i];
         const uint32_t ne = (uint32_t)(a.state_off[i + 1] - e0);
-        for (uint32_t e = l; e < ne; e += WAVE) {
+        for (uint32_t e = tm.r; e < ne; e += Team<W>::N) {
             const uint8_t ar = a.ar[e0 + e];
             s.e_ar[e] = ar;
             for (uint32_t k = 0; k < A; ++k) s.e_lv[e * A + k] = (uint16_t)(k < ar ? a.ev[(e0 + e) * A + k] : 0u);
         }
         s.nE = ne;
         s.nV = a.nv[i];
-        wave_sync();
-        const uint64_t h = wl_hash_wave(s);
-        if (l == 0) {
+        tm.sync();
+        const uint64_t h = wl_hash_team(s, tm);
+        if (tm.leader()) {
             a.out[i] = h;
             if (a.run_of) {
                 DRun* RN = &a.runs[a.run_of[i]];
@@ -1122,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_wl_states(WLStatesArgs a) {
                 }
             }
         }
-        wave_sync();
+        tm.sync();
     }
 }
 
@@ -1278,16 +1279,34 @@ hipError_t launch_branchial(const BranchialArgs& a, uint32_t grid_cap, uint32_t 
     return hipGetLastError();
 }
 
-uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A) {
-    const WLSizes z = WLSizes::make(max_ne, max_nv, A);
-    return (z.bytes(A) + 64u + 15u) & ~15u;
+uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A, bool lean) {
+    const WLSizes z = WLSizes::make(max_ne, max_nv, A, lean);
+    // + 64: carving slack; + 64: the team's reduction words at the end
+    return (z.bytes(A) + 128u + 15u) & ~15u;
 }
 
-hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st) {
-    const uint32_t grid = grid_for(a.n, wpb, grid_cap);
-    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * wpb;
-    if (a.scratch) hipLaunchKernelGGL(k_wl_states<true>, dim3(grid), dim3(64 * wpb), lds, st, a);
-    else hipLaunchKernelGGL(k_wl_states<false>, dim3(grid), dim3(64 * wpb), lds, st, a);
+template <uint32_t W>
+static void launch_wl_states_team(const WLStatesArgs& a, uint32_t grid, uint32_t wpb, hipStream_t st) {
+    const uint32_t threads = W == 1 ? 64 * wpb : 64 * W;
+    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * (W == 1 ? wpb : 1);
+    if (a.scratch) hipLaunchKernelGGL((k_wl_states<true, W>), dim3(grid), dim3(threads), lds, st, a);
+    else hipLaunchKernelGGL((k_wl_states<false, W>), dim3(grid), dim3(threads), lds, st, a);
+}
+
+uint32_t wl_states_grid(uint32_t n, uint32_t team, uint32_t wpb, uint32_t grid_cap) {
+    return grid_for(n, team == 1 ? wpb : 1, grid_cap);
+}
+
+hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st) {
+    if (!a.n) return hipSuccess;
+    const uint32_t grid = wl_states_grid(a.n, team, wpb, grid_cap);
+    switch (team) {
+        case 1: launch_wl_states_team<1>(a, grid, wpb, st); break;
+        case 2: launch_wl_states_team<2>(a, grid, wpb, st); break;
+        case 4: launch_wl_states_team<4>(a, grid, wpb, st); break;
+        case 8: launch_wl_states_team<8>(a, grid, wpb, st); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

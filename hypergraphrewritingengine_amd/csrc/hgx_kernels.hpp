@@ -128,6 +128,7 @@ struct WLStatesArgs {
     uint32_t max_ne, max_nv;
     uint32_t lds_per_wave;
     uint8_t* scratch;          // as ExpandArgs::scratch
+    uint32_t lean;             // every state has one arity 2..4 and < 2^16 edges: lean workspace
 };
 
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
@@ -167,8 +168,10 @@ hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr,
 uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap);
 uint32_t branchial_lds_per_wave();
 hipError_t launch_branchial(const BranchialArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
-uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A);
-hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
+uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A, bool lean);
+// team: waves per state (1, 2, 4 or 8); with team 1, wpb states per workgroup.
+hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st);
+uint32_t wl_states_grid(uint32_t n, uint32_t team, uint32_t wpb, uint32_t grid_cap);
 hipError_t launch_run_totals(const uint64_t* moff, const uint32_t* b, const uint32_t* e, uint32_t nruns,
                              uint64_t* tot, hipStream_t st);
 hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32_t n, uint32_t* cnt,

@@ -320,6 +320,7 @@ RunHost parse_run(const hgx_run_desc* d) {
         h.max_new = std::max<uint32_t>(h.max_new, R.n_new);
     }
     h.root_edges = d->n_edges;
+    h.root_uni = d->n_edges == 0 || (amin == amax && amax >= 2 && amax <= 4);
     for (auto& R : h.rules)
         for (int e = 0; e < R.n_rhs; ++e) { amin = std::min<uint32_t>(amin, R.ra[e]); amax = std::max<uint32_t>(amax, R.ra[e]); }
     h.uni_arity = (amin == amax && amax >= 2 && amax <= 4) ? amax : 0u;
@@ -362,6 +363,22 @@ void sort_pairs_u64(Arena& ar, const uint64_t* kin, uint64_t* kout, const uint32
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, (int)n, 0, end_bit, st));
     void* t = ar.alloc(tmp);
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(t, tmp, kin, kout, vin, vout, (int)n, 0, end_bit, st));
+}
+
+uint32_t place_wl_states(Arena& ar, WLStatesArgs& w, bool lean, uint32_t max_lds, uint32_t grid_cap,
+                         uint32_t forced_team, uint32_t* wpb) {
+    w.lean = lean && w.max_ne < 65536 ? 1u : 0u;
+    w.lds_per_wave = wl_states_lds_per_wave(w.max_ne, w.max_nv, w.A, w.lean != 0);
+    w.scratch = nullptr;
+    const uint32_t team = pick_team(w.lds_per_wave, forced_team);
+    if (team == 1) {
+        *wpb = place_workspace(ar, w.lds_per_wave, w.n, max_lds, grid_cap, &w.scratch);
+    } else {
+        *wpb = 1;
+        if (w.lds_per_wave > max_lds)
+            w.scratch = (uint8_t*)ar.alloc((size_t)wl_states_grid(w.n, team, 1, grid_cap) * w.lds_per_wave);
+    }
+    return team;
 }
 
 uint32_t waves_per_block(uint32_t lds_per_wave, uint32_t max_lds) {
@@ -499,11 +516,13 @@ extern "C" int hgx_wl_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, ui
     return guarded(ctx, [&]() -> int {
         HIP_TRY(hipSetDevice(ctx->device));
         const uint64_t E = state_offsets[n_states] - state_offsets[0];
-        uint32_t A = 1, max_ne = 1, max_nv = 1;
+        uint32_t A = 1, max_ne = 1, max_nv = 1, amin = 0xFFu, amax = 0;
         for (uint64_t e = state_offsets[0]; e < state_offsets[n_states]; ++e) {
             const uint32_t a = edge_offsets[e + 1] - edge_offsets[e];
             if (a < 1 || a > 16) throw Error{HGX_ERR_INVALID, "edge arity must be 1..16"};
             A = std::max(A, a);
+            amin = std::min(amin, a);
+            amax = std::max(amax, a);
         }
         std::vector<uint32_t> ev(E * A, 0), nv(n_states);
         std::vector<uint8_t> ar(E);
@@ -541,10 +560,10 @@ extern "C" int hgx_wl_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, ui
         h2d(d_nv, nv.data(), n_states, st);
         w.state_off = d_so; w.ar = d_ar; w.ev = d_ev; w.nv = d_nv; w.out = d_out;
         w.max_ne = max_ne; w.max_nv = max_nv;
-        w.lds_per_wave = wl_states_lds_per_wave(max_ne, max_nv, A);
-        const uint32_t wpb = place_workspace(ar_, w.lds_per_wave, n_states, ctx->max_lds, ctx->num_cu * 8,
-                                             &w.scratch);
-        HIP_TRY(launch_wl_states(w, ctx->num_cu * 8, wpb, st));
+        uint32_t wpb = 1;
+        const uint32_t team = place_wl_states(ar_, w, amin == amax && amax >= 2 && amax <= 4, ctx->max_lds,
+                                              ctx->num_cu * 8, ctx->expand_team, &wpb);
+        HIP_TRY(launch_wl_states(w, ctx->num_cu * 8, wpb, team, st));
         d2h(out_hash, d_out, n_states, st);
         HIP_TRY(hipStreamSynchronize(st));
         return HGX_OK;

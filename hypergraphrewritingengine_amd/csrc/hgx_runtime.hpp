@@ -137,10 +137,23 @@ struct RunHost {
     uint32_t max_arity = 1, max_lhs = 1, max_gain = 0, max_new = 0;
     uint32_t uni_arity = 0;      // k when every root edge and every RHS edge has arity k (2..4),
                                  // so every state of the run does; else 0
+    bool root_uni = false;       // every root edge has one arity 2..4 (the roots alone)
     uint64_t root_edges = 0;
 };
 RunHost parse_run(const hgx_run_desc* d);
 uint64_t footprint_bytes(const RunHost& h);
+
+// Per-state workspace (bytes) up to which one wave handles a state, and up to which four waves
+// share one; above, eight (a CU holds 160 KiB of LDS).
+constexpr uint32_t kTeamBytes[2] = {24u << 10, 48u << 10};
+inline uint32_t pick_team(uint32_t bytes, uint32_t forced) {
+    return forced ? forced : bytes <= kTeamBytes[0] ? 1u : bytes <= kTeamBytes[1] ? 4u : 8u;
+}
+struct WLStatesArgs;
+// Workspace of a k_wl_states launch (lean when `lean`): sets lds_per_wave and scratch; returns
+// the team size, and the waves per workgroup in *wpb.
+uint32_t place_wl_states(Arena& ar, WLStatesArgs& w, bool lean, uint32_t max_lds, uint32_t grid_cap,
+                         uint32_t forced_team, uint32_t* wpb);
 
 uint32_t waves_per_block(uint32_t lds_per_wave, uint32_t max_lds);
 // Workspace placement for a wave-per-state kernel: LDS when it fits, else a global-memory slice
