@@ -17,7 +17,7 @@ from .rules import (Request, Rule, RuleBuilder, encode_roots, encode_rules, enco
 
 __all__ = ["Context", "Result", "Request", "Rule", "RuleBuilder", "make_rule", "rule_from_lists",
            "encode_rules", "encode_state", "encode_roots", "library_path", "load_library",
-           "HGXError", "EXPORTED_SYMBOLS", "wxf_validate"]
+           "HGXError", "EXPORTED_SYMBOLS", "wxf_validate", "ir_hash", "ir_canonical_form"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = [
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
+    "hgx_ir_hash_batch", "hgx_ir_canonical_form",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -85,6 +86,8 @@ def load_library():
     L.hgx_result_release.argtypes = [vp]
     L.hgx_result_release.restype = None
     L.hgx_wl_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
+    L.hgx_ir_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
+    L.hgx_ir_canonical_form.argtypes = [vp, vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint64)]
     L.hgx_match_count.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
                                   C.POINTER(C.c_uint64), vp, C.c_uint64, C.POINTER(C.c_float)]
     L.hgx_result_counts_many.argtypes = [vp, C.c_uint32, C.POINTER(abi.Counts)]
@@ -194,6 +197,48 @@ def wxf_validate(job: bytes):
     buf = C.create_string_buffer(1024)
     rc = L.hgx_wxf_validate(job, len(job), buf, 1024)
     return rc == abi.HGX_OK, buf.value.decode()
+
+
+def _flatten_states(states):
+    offs = [0]
+    edges: list = []
+    for st in states:
+        edges.extend(st)
+        offs.append(len(edges))
+    eo, vv = encode_state(edges) if edges else (np.zeros(1, np.uint32), np.zeros(1, np.uint32))
+    if vv.size == 0:
+        vv = np.zeros(1, np.uint32)
+    return np.asarray(offs, np.uint64), eo, vv
+
+
+def ir_hash(states) -> np.ndarray:
+    """Exact (individualization-refinement) canonical hashes of a list of states (each a list
+    of edges), computed on the host -- the identity of CanonicalizeStates -> Full. No device
+    needed."""
+    L = load_library()
+    so, eo, vv = _flatten_states(states)
+    out = np.zeros(max(1, len(states)), np.uint64)
+    rc = L.hgx_ir_hash_batch(None, so.ctypes.data, len(states), eo.ctypes.data, vv.ctypes.data,
+                             out.ctypes.data)
+    if rc != abi.HGX_OK:
+        raise HGXError(rc, "invalid state")
+    return out[:len(states)]
+
+
+def ir_canonical_form(edges):
+    """(hash, canonical edges): one state relabelled 0..n-1, its edges in canonical order."""
+    L = load_library()
+    eo, vv = encode_state(edges) if len(edges) else (np.zeros(1, np.uint32), np.zeros(1, np.uint32))
+    if vv.size == 0:
+        vv = np.zeros(1, np.uint32)
+    oe = np.zeros(len(edges) + 1, np.uint32)
+    ov = np.zeros(max(1, int(eo[-1])), np.uint32)
+    h = C.c_uint64()
+    rc = L.hgx_ir_canonical_form(eo.ctypes.data, vv.ctypes.data, len(edges), oe.ctypes.data,
+                                 ov.ctypes.data, C.byref(h))
+    if rc != abi.HGX_OK:
+        raise HGXError(rc, "invalid state" if rc == abi.HGX_ERR_INVALID else "search depth exceeded")
+    return int(h.value), [ov[oe[k]:oe[k + 1]].tolist() for k in range(len(edges))]
 
 
 class PreparedBatch:

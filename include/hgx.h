@@ -262,6 +262,22 @@ void hgx_result_release(hgx_result* r);
 int hgx_wl_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
                       const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash);
 
+/* Exact canonical hash of n states given like hgx_wl_hash_batch's input: the reference's
+ * IRCanonicalizer::compute_canonical_hash (hypergraph/src/ir_canonicalization.cpp:180-198,
+ * common/include/hgcommon/ir_core.hpp:596-955), the identity of CanonicalizeStates -> Full.
+ * Host computation on worker threads (as the reference's GPU backend does it after the
+ * device run, paclet_source/hg_gpu_backend.cpp:269-310); ctx may be NULL. An empty state, and
+ * one whose search would need more than 63 individualization levels, hash to 0 (the
+ * reference's values). */
+int hgx_ir_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
+                      const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash);
+/* The canonical form of one state (IRCanonicalizer::canonicalize_edges,
+ * ir_canonicalization.cpp:107-170): its edges relabelled 0..n-1 and in canonical order,
+ * out_edge_offsets[n_edges+1] and out_vertices[edge_offsets[n_edges] - edge_offsets[0]];
+ * *out_hash its canonical hash. Host only. */
+int hgx_ir_canonical_form(const uint32_t* edge_offsets, const uint32_t* vertices, uint32_t n_edges,
+                          uint32_t* out_edge_offsets, uint32_t* out_vertices, uint64_t* out_hash);
+
 /* Number of matches of rule `rule` (index into the program) in one state, computed on the
  * device with the large-state path (CSR incidence index + seeded join). If
  * match_edges is non-NULL it receives up to cap matches, n_lhs edge indices each in
