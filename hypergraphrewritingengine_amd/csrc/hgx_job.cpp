@@ -6,8 +6,11 @@
 // BranchialEdges, Num*). It is written over the library's own C ABI (hgx_evolve_batch and the
 // hgx_result_* readbacks).
 //
-// Served: full multiway evolution (CanonicalizeStates -> None, CanonicalizeEvents -> None), any
-// rule set, one or more initial states, causal edges with or without transitive reduction,
+// Served: full multiway evolution with CanonicalizeStates -> None or Full (the GPU backend's
+// grouping, hg_gpu_backend.cpp:269-310: None gives every state its own class, Full groups by the
+// exact IR hash computed on the host and emits each class's representative relabelled to its
+// canonical form; CanonicalHash is the IR hash in both), CanonicalizeEvents -> None, any rule
+// set, one or more initial states, causal edges with or without transitive reduction,
 // branchial edges, RequestedData selection. Options that change what is explored (sampling,
 // caps, canonical exploration, sessions, genesis events, graph properties) are refused with an
 // error rather than ignored, so a caller never receives a different evolution than it asked
@@ -39,6 +42,7 @@ struct Job {
     bool inc_states = true, inc_events = true, inc_events_min = false, inc_causal = true,
          inc_branchial = true, inc_nstates = true, inc_nevents = true, inc_ncausal = true,
          inc_nbranchial = true, inc_hashes = false;
+    bool full = false;                       // CanonicalizeStates -> Full
     std::vector<std::pair<std::string, std::string>> warnings;   // (kind, context)
 };
 
@@ -113,7 +117,9 @@ static Job parse(const Value& root) {
                 };
                 if (ok == "CanonicalizeStates") {
                     const std::string m = as_name(ov);
-                    if (m != "None" && m != "False") refuse("only None is served");
+                    if (m == "Full" || m == "True") j.full = true;
+                    else if (m == "None" || m == "False") j.full = false;
+                    else refuse("only None and Full are served");
                 } else if (ok == "CanonicalizeEvents") {
                     if (!(ov.kind == Value::Symbol && ov.s == "None")) refuse("only None is served");
                 } else if (ok == "CausalTransitiveReduction") {
@@ -272,7 +278,9 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (n_roots == 0) throw JobError("the job has no non-empty initial state");
     if (j.steps > HGX_MAX_STEPS) throw JobError("Steps above " + std::to_string(HGX_MAX_STEPS));
     uint32_t flags = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL | HGX_REC_RELATIONS;
-    if (j.inc_states || j.inc_events) flags |= HGX_REC_STATES;
+    // state edges: for the output, and for the IR hash (Full's grouping, CanonicalHash)
+    const bool need_ir = j.full || (j.inc_hashes && j.inc_states);
+    if (j.inc_states || j.inc_events || need_ir) flags |= HGX_REC_STATES;
     if (j.tr) flags |= HGX_OPT_TR;
     hgx_run_desc d{};
     d.rules = prog.data(); d.rules_len = (uint32_t)prog.size(); d.n_rules = (uint32_t)j.rules.size();
@@ -306,34 +314,59 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
         check(hgx_result_consumed(res, R.coff.data(), R.cpos.data(), R.cpos.size()), ctx, "consumed edges");
     }
 
-    // The representative of each WL class: its lowest state id (the first writer in id order)
-    std::unordered_map<uint64_t, uint32_t> rep_of_hash;
+    // Exact (IR) hash of every state, on the host, when Full groups by it or CanonicalHash is
+    // asked for. Classes: None -> every state its own; Full -> equal IR hash, represented by the
+    // first state in id order (hg_gpu_backend.cpp:278-310).
+    std::vector<uint64_t> ir(need_ir ? ns : 0);
+    if (need_ir && ns)
+        check(hgx_ir_hash_batch(ctx, R.soff.data(), (uint32_t)ns, R.eoff.data(), R.verts.empty() ? nullptr : R.verts.data(),
+                                ir.data()), ctx, "canonical hashes");
     std::vector<int64_t> rep(ns);
-    for (uint64_t s = 0; s < ns; ++s) {
-        auto it = rep_of_hash.emplace(R.hash[s], (uint32_t)s).first;
-        rep[s] = it->second;
+    std::vector<uint64_t> reps;                               // class representatives, in id order
+    if (j.full) {
+        std::unordered_map<uint64_t, uint32_t> rep_of_hash;
+        for (uint64_t s = 0; s < ns; ++s) {
+            auto it = rep_of_hash.emplace(ir[s], (uint32_t)s);
+            rep[s] = it.first->second;
+            if (it.second) reps.push_back(s);
+        }
+    } else {
+        for (uint64_t s = 0; s < ns; ++s) { rep[s] = (int64_t)s; reps.push_back(s); }
     }
 
     Value out = Value::assoc();
     if (j.inc_states) {
         Value sts = Value::assoc();
-        sts.rules.reserve(ns);
-        for (uint64_t s = 0; s < ns; ++s) {
+        sts.rules.reserve(reps.size());
+        std::vector<uint32_t> coff, cv;
+        for (uint64_t s : reps) {
             Value sa = Value::assoc();
             sa.put("Id", Value::integer((int64_t)s));
             sa.put("CanonicalId", Value::integer(rep[s]));
             sa.put("ContentStateId", Value::integer(rep[s]));
             sa.put("Step", Value::integer(R.step[s]));
+            // Full relabels to the canonical form; None keeps the state's own labels
+            const uint32_t e0 = (uint32_t)R.soff[s], ne_s = (uint32_t)(R.soff[s + 1] - R.soff[s]);
+            const uint32_t* eo = R.eoff.data() + e0;
+            const uint32_t* vv = R.verts.data();
+            if (j.full && ne_s) {
+                coff.assign(ne_s + 1, 0);
+                cv.assign(R.eoff[e0 + ne_s] - R.eoff[e0], 0);
+                uint64_t h = 0;
+                check(hgx_ir_canonical_form(eo, vv, ne_s, coff.data(), cv.data(), &h), ctx, "canonical form");
+                eo = coff.data();
+                vv = cv.data();
+            }
             Value edges = Value::list();
-            for (uint64_t e = R.soff[s], k = 0; e < R.soff[s + 1]; ++e, ++k) {
+            for (uint32_t k = 0; k < ne_s; ++k) {
                 Value ed = Value::list();
                 ed.push(Value::integer((int64_t)k));
-                for (uint32_t x = R.eoff[e]; x < R.eoff[e + 1]; ++x) ed.push(Value::integer(R.verts[x]));
+                for (uint32_t x = eo[k]; x < eo[k + 1]; ++x) ed.push(Value::integer(vv[x]));
                 edges.push(std::move(ed));
             }
             sa.put("Edges", std::move(edges));
             sa.put("IsInitial", Value::boolean(s < n_roots));
-            if (j.inc_hashes) sa.put("CanonicalHash", Value::integer((int64_t)R.hash[s]));
+            if (j.inc_hashes) sa.put("CanonicalHash", Value::integer((int64_t)ir[s]));
             sts.put((int64_t)s, std::move(sa));
         }
         out.put("States", std::move(sts));
@@ -394,7 +427,7 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
         }
         out.put("BranchialEdges", std::move(bl));
     }
-    if (j.inc_nstates) out.put("NumStates", Value::integer((int64_t)R.c.canonical_states));
+    if (j.inc_nstates) out.put("NumStates", Value::integer((int64_t)reps.size()));
     if (j.inc_nevents) out.put("NumEvents", Value::integer((int64_t)R.c.events));
     if (j.inc_ncausal) out.put("NumCausalEdges", Value::integer(n_causal));
     if (j.inc_nbranchial) out.put("NumBranchialEdges", Value::integer((int64_t)R.c.branchial_pairs));

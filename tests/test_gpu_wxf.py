@@ -4,8 +4,10 @@ A WXF job is evolved through the envelope and its decoded result is checked agai
 request evolved through the plain API and the oracle: counts, every state's edges, events,
 consumed/produced edges (applying an event's consumed positions to its input state and appending
 its produced edges gives its output state), causal pairs (with and without transitive reduction)
-and branchial pairs. The worker binary answers one-shot and framed (--serve) jobs with the same
-bytes, and a failing job with a zero-length frame.
+and branchial pairs. With CanonicalizeStates -> Full the classes are the exact (IR) ones: the golden matrix's Full
+rows fix their number, each representative is emitted in its canonical form, and every event
+names its states' representatives. The worker binary answers one-shot and framed (--serve)
+jobs with the same bytes, and a failing job with a zero-length frame.
 """
 import os
 import struct
@@ -49,7 +51,8 @@ def test_wxf_envelope_matches_direct_evolution(ctx, tr):
     res = ctx.evolve(req)
     c = res.counts
     o = po.evolve(req)
-    assert out["NumStates"] == c["canonical_states"] == o.counts["canonical_states"]
+    # CanonicalizeStates -> None: every state its own class (hg_gpu_backend.cpp:278-305)
+    assert out["NumStates"] == c["raw_states"] == o.counts["raw_states"]
     assert out["NumEvents"] == c["events"]
     assert out["NumBranchialEdges"] == c["branchial_pairs"]
     assert out["NumCausalEdges"] == (c["causal_pairs_tr"] if tr else c["causal_pairs"])
@@ -67,8 +70,7 @@ def test_wxf_envelope_matches_direct_evolution(ctx, tr):
         got = [ed[1:] for ed in st["Edges"]]
         assert [ed[0] for ed in st["Edges"]] == list(range(len(want)))
         assert got == want
-        rep = min(k for k in range(len(hashes)) if hashes[k] == hashes[sid])
-        assert st["CanonicalId"] == rep
+        assert st["CanonicalId"] == st["ContentStateId"] == sid
         edges_of[sid] = got
     # events, with consumed/produced edges reproducing each output state
     ev = res.events()
@@ -89,6 +91,42 @@ def test_wxf_envelope_matches_direct_evolution(ctx, tr):
     assert len(out["CausalEdges"]) == len(want_c)
     want_b = sorted((int(a), int(b)) for a, b in res.branchial())
     assert sorted((x["From"], x["To"]) for x in out["BranchialEdges"]) == want_b
+
+
+GOLD_NONE = __import__("json").load(open(os.path.join(HERE, "golden", "golden_matrix_none.json")))["cases"]
+GOLD_FULL = __import__("json").load(open(os.path.join(HERE, "golden", "golden_matrix_full.json")))["cases"]
+
+
+@pytest.mark.parametrize("case", [c for c in GOLD_NONE if c["name"] in GOLD_FULL], ids=lambda c: c["name"])
+def test_wxf_full_canonicalization(ctx, case):
+    job = dumps({"InitialStates": [case["init"]],
+                 "Rules": {f"R{k}": Rule(r["lhs"], r["rhs"]) for k, r in enumerate(case["rules"])},
+                 "Steps": case["steps"],
+                 "Options": {"CanonicalizeStates": Symbol("Full"), "IncludeCanonicalHashes": Symbol("True")}})
+    out = loads(ctx.run_wxf(job))
+    want = GOLD_FULL[case["name"]]
+    assert out["NumStates"] == want["ir_states"]
+    assert out["NumEvents"] == want["events"]
+    # the same evolution through the plain API: the raw states and their exact hashes
+    rules = [rule_from_lists(r["lhs"], r["rhs"], k) for k, r in enumerate(case["rules"])]
+    res = ctx.evolve(Request(rules, [case["init"]], case["steps"], flags=0x1 | 0x2 | 0x4 | 0x8 | 0x100,
+                             multi_root=True))
+    so, eo, vv = res.state_edges()
+    edges = [[list(map(int, vv[eo[e]:eo[e + 1]])) for e in range(int(so[s]), int(so[s + 1]))]
+             for s in range(len(so) - 1)]
+    ir = [int(h) for h in hgx.ir_hash(edges)]
+    reps = {}
+    for s, h in enumerate(ir):
+        reps.setdefault(h, s)
+    assert sorted(out["States"]) == sorted(reps.values())
+    for sid, st in out["States"].items():
+        h, form = hgx.ir_canonical_form(edges[sid])
+        # written as a signed 64-bit integer, as the reference writes it
+        assert st["CanonicalHash"] % (1 << 64) == h and st["CanonicalId"] == sid
+        assert [e[1:] for e in st["Edges"]] == form
+    for e in out["Events"].values():
+        assert e["CanonicalInputState"] == reps[ir[e["InputState"]]]
+        assert e["CanonicalOutputState"] == reps[ir[e["OutputState"]]]
 
 
 def test_wxf_requested_data_and_errors(ctx):

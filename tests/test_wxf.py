@@ -53,7 +53,7 @@ def test_encoder_roundtrip_and_valid_job():
     (_job(Rules={"R": Rule([[0, -1]], [[0]])}), "negative pattern variable"),
     (_job(Rules={"R": Rule([[]], [[0]])}), "empty left-hand side"),
     (_job(Rules={"R": [[0, 1], [1, 2]]}), "Rule[lhs, rhs]"),
-    (_job(Options={"CanonicalizeStates": Symbol("Full")}), "only None"),
+    (_job(Options={"CanonicalizeStates": Symbol("Automatic")}), "only None and Full"),
     (_job(Options={"MaxStatesPerStep": 10}), "caps are not served"),
     (_job(Options={"ExplorationProbability": 0.5}), "sampling"),
     (_job(Op="Open"), "sessions"),
