@@ -16,6 +16,10 @@ Reported beside the value:
   cpu       the reference CPU engine (oracle/_ref/hgref, built from the reference sources)
             timed on this host on a bounded sample, each run to the same generation as the GPU
             (so the same states), with its counts cross-checked against the GPU's.
+  others    config 2 (one large evolution), configs 3a/3b (large-state joins), and ir_audit:
+            the host exact (IR) canonicalizer over every raw state of 256 batched runs -- its
+            throughput and the WL audit (distinct WL vs distinct IR hashes per run) -- with the
+            reference IRCanonicalizer (oracle/_ref/irref) on 20,000 of them, hashes compared.
 
     python bench.py [--steps K] [--warmup W] [--runs N] [--gpus 1]
 """
@@ -184,6 +188,56 @@ def config3_line(ctx, name, rule, edges, no_cpu):
         t = time.perf_counter()
         cn = po.match([rule], 0, edges)
         out["cpu_port"] = {"s": round(time.perf_counter() - t, 4), "threads": 1, "count_match": cn == n}
+    return out
+
+
+def ir_audit_line(ctx, runs, no_cpu):
+    """The host exact (IR) canonicalizer over every raw state of a sample of the batched
+    workload: its throughput, and the WL audit -- per run, the number of distinct WL hashes
+    against the number of distinct IR hashes (equal: WL merged no non-isomorphic states)."""
+    import hypergraphrewritingengine_amd as hgx
+    from hypergraphrewritingengine_amd import workloads as W
+    reqs = [W.batched_run(i, flags=0x3 | 0x8) for i in range(runs)]
+    res = ctx.evolve_batch(reqs)
+    states, wl_runs = [], []
+    for r in res:
+        so, eo, vv = r.state_edges()
+        st = [[vv[eo[e]:eo[e + 1]].tolist() for e in range(int(so[s]), int(so[s + 1]))] for s in range(len(so) - 1)]
+        states.append(st)
+        wl_runs.append(r.states()[0])
+    flat = [st for run in states for st in run]
+    hgx.ir_hash(flat[:64])
+    t = time.perf_counter()
+    ir = hgx.ir_hash(flat)
+    dt = time.perf_counter() - t
+    wl_classes = ir_classes = 0
+    k = 0
+    for run, wl in zip(states, wl_runs):
+        wl_classes += len(set(int(h) for h in wl))
+        ir_classes += len(set(int(h) for h in ir[k:k + len(run)]))
+        k += len(run)
+    out = {"runs": runs, "states": len(flat), "host_threads": min(os.cpu_count() or 1, 16),
+           "ir_s": round(dt, 4), "ir_states_per_s": round(len(flat) / dt, 1),
+           "wl_classes": wl_classes, "ir_classes": ir_classes, "wl_merged_nothing": wl_classes == ir_classes}
+    irref = os.path.join(ROOT, "oracle", "_ref", "irref")
+    if not no_cpu and os.path.exists(irref):
+        # the reference canonicalizer (one thread; it also builds each canonical form) on a sample
+        sample = flat[:20000]
+        with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+            f.write(f"{len(sample)}\n")
+            for st in sample:
+                f.write(f"{len(st)} " + " ".join(f"{len(e)} " + " ".join(map(str, e)) for e in st) + "\n")
+            path = f.name
+        try:
+            t = time.perf_counter()
+            lines = subprocess.run([irref, path], capture_output=True, text=True, check=True).stdout.splitlines()
+            rt = time.perf_counter() - t
+        finally:
+            os.unlink(path)
+        ref = [int(x.split()[0]) for x in lines]
+        out["cpu_reference"] = {"states": len(sample), "s": round(rt, 3), "threads": 1,
+                                "states_per_s": round(len(sample) / rt, 1),
+                                "hash_mismatches": sum(int(a) != b for a, b in zip(ir[:len(sample)], ref))}
     return out
 
 
@@ -394,6 +448,8 @@ def main():
             rule, edges = mk()
             others[name] = config3_line(ctx, name, rule, edges, args.no_cpu)
             print(f"[bench] {name}: {others[name]}", file=sys.stderr, flush=True)
+        others["ir_audit"] = ir_audit_line(ctx, 256, args.no_cpu)
+        print(f"[bench] ir_audit: {others['ir_audit']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
