@@ -17,7 +17,7 @@ from .rules import (Request, Rule, RuleBuilder, encode_roots, encode_rules, enco
 
 __all__ = ["Context", "Result", "Request", "Rule", "RuleBuilder", "make_rule", "rule_from_lists",
            "encode_rules", "encode_state", "encode_roots", "library_path", "load_library",
-           "HGXError", "EXPORTED_SYMBOLS", "wxf_validate", "ir_hash", "ir_canonical_form"]
+           "HGXError", "EXPORTED_SYMBOLS", "wxf_validate", "ir_hash", "ir_rank", "ir_canonical_form"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -32,7 +32,7 @@ EXPORTED_SYMBOLS = [
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
-    "hgx_ir_hash_batch", "hgx_ir_canonical_form",
+    "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -87,6 +87,7 @@ def load_library():
     L.hgx_result_release.restype = None
     L.hgx_wl_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
     L.hgx_ir_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
+    L.hgx_ir_rank_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, vp]
     L.hgx_ir_canonical_form.argtypes = [vp, vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint64)]
     L.hgx_match_count.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
                                   C.POINTER(C.c_uint64), vp, C.c_uint64, C.POINTER(C.c_float)]
@@ -174,6 +175,16 @@ class Result:
                                                      vv.ctypes.data, None, None))
         return so, eo, vv
 
+    def consumed(self):
+        """(offsets, positions): event e consumed the edges at positions[offsets[e]:offsets[e+1]]
+        of its input state's edge list, in pattern order (needs HGX_REC_STATES)."""
+        n = self.counts["events"]
+        off = np.zeros(n + 1, np.uint32)
+        self._check(self._lib.hgx_result_consumed(self._h, off.ctypes.data, None, 0))
+        pos = np.zeros(max(1, int(off[-1])), np.uint16)
+        self._check(self._lib.hgx_result_consumed(self._h, off.ctypes.data, pos.ctypes.data, int(off[-1])))
+        return off, pos[:int(off[-1])]
+
     def timing(self):
         d, l = C.c_uint64(), C.c_uint64()
         self._check(self._lib.hgx_result_timing(self._h, C.byref(d), C.byref(l)))
@@ -223,6 +234,20 @@ def ir_hash(states) -> np.ndarray:
     if rc != abi.HGX_OK:
         raise HGXError(rc, "invalid state")
     return out[:len(states)]
+
+
+def ir_rank(states):
+    """(hashes, ranks): the exact canonical hash of each state and, per state, each edge's
+    canonical rank (its place in the state's canonical edge order)."""
+    L = load_library()
+    so, eo, vv = _flatten_states(states)
+    out = np.zeros(max(1, len(states)), np.uint64)
+    rk = np.zeros(max(1, int(so[-1]) if len(so) else 1), np.uint32)
+    rc = L.hgx_ir_rank_batch(None, so.ctypes.data, len(states), eo.ctypes.data, vv.ctypes.data,
+                             out.ctypes.data, rk.ctypes.data)
+    if rc != abi.HGX_OK:
+        raise HGXError(rc, "invalid state")
+    return out[:len(states)], [rk[int(so[i]):int(so[i + 1])].tolist() for i in range(len(states))]
 
 
 def ir_canonical_form(edges):

@@ -60,7 +60,7 @@ public:
     }
 
     // The hash; false when a path needs more individualizations than the reference allows.
-    bool run(uint64_t& hash, std::vector<uint32_t>* form_out) {
+    bool run(uint64_t& hash, std::vector<uint32_t>* form_out, uint32_t* rank_out) {
         Part root;
         initial_partition(root);
         refine(root);
@@ -73,6 +73,8 @@ public:
             hash = fnv(hash, 0xDEADBEEFull);
         }
         if (form_out) *form_out = best_;
+        if (rank_out)
+            for (uint32_t i = 0; i < ne_; ++i) rank_out[best_order_[i]] = i;
         return true;
     }
 
@@ -86,7 +88,7 @@ private:
     uint32_t epoch_ = 0;
     std::vector<uint8_t> mark_;
     std::vector<std::vector<uint64_t>> sig_;
-    std::vector<uint32_t> best_, first_, first_lab_;
+    std::vector<uint32_t> best_, best_order_, first_, first_lab_;
     bool have_best_ = false, have_first_ = false;
     std::vector<std::vector<uint32_t>> gens_;
 
@@ -276,7 +278,7 @@ private:
             form.push_back(ar_[e]);
             for (uint32_t k = 0; k < ar_[e]; ++k) form.push_back(lab[vtx(e, k)]);
         }
-        if (!have_best_ || form < best_) { best_ = form; have_best_ = true; }
+        if (!have_best_ || form < best_) { best_ = form; best_order_ = order; have_best_ = true; }
         if (!have_first_) {
             first_ = form;
             first_lab_ = lab;
@@ -335,8 +337,9 @@ private:
 }  // namespace
 
 uint64_t state_hash(const uint32_t* edge_off, const uint32_t* verts, uint32_t n_edges,
-                    std::vector<uint32_t>* form) {
+                    std::vector<uint32_t>* form, uint32_t* rank) {
     if (form) form->clear();
+    if (rank) std::fill(rank, rank + n_edges, 0u);
     if (n_edges == 0) return 0;
     // local numbering: distinct labels ascending (ir_canonicalization.cpp:42-58)
     const uint32_t base = edge_off[0], occ = edge_off[n_edges] - base;
@@ -353,8 +356,9 @@ uint64_t state_hash(const uint32_t* edge_off, const uint32_t* verts, uint32_t n_
         ev[i] = (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), verts[base + i]) - sorted.begin());
     Canon c((uint32_t)sorted.size(), n_edges, ar.data(), eoff.data(), ev.data());
     uint64_t h = 0;
-    if (!c.run(h, form)) {
+    if (!c.run(h, form, rank)) {
         if (form) form->clear();
+        if (rank) std::fill(rank, rank + n_edges, 0u);
         return 0;
     }
     return h;
@@ -383,6 +387,12 @@ bool state_ok(const uint32_t* edge_off, uint32_t e0, uint32_t e1) {
 
 extern "C" int hgx_ir_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
                                  const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash) {
+    return hgx_ir_rank_batch(ctx, state_offsets, n_states, edge_offsets, vertices, out_hash, nullptr);
+}
+
+extern "C" int hgx_ir_rank_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
+                                 const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash,
+                                 uint32_t* out_rank) {
     (void)ctx;
     if (n_states == 0) return HGX_OK;
     if (!state_offsets || !out_hash) return HGX_ERR_INVALID;
@@ -395,7 +405,9 @@ extern "C" int hgx_ir_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, ui
     auto work = [&]() {
         for (uint32_t s; (s = next.fetch_add(1)) < n_states;) {
             const uint32_t e0 = (uint32_t)state_offsets[s], ne = (uint32_t)(state_offsets[s + 1] - e0);
-            out_hash[s] = ne ? hgx::ir::state_hash(edge_offsets + e0, vertices, ne) : 0;
+            out_hash[s] = ne ? hgx::ir::state_hash(edge_offsets + e0, vertices, ne, nullptr,
+                                                   out_rank ? out_rank + (e0 - (uint32_t)state_offsets[0]) : nullptr)
+                             : 0;
         }
     };
     const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));

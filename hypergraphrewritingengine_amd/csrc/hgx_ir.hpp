@@ -41,7 +41,10 @@ namespace ir {
 // verts[edge_off[e] .. edge_off[e+1]). When `form` is given it receives the canonical form,
 // flat [arity, labels...] per edge in canonical order (empty when the hash is 0).
 uint64_t state_hash(const uint32_t* edge_off, const uint32_t* verts, uint32_t n_edges,
-                    std::vector<uint32_t>* form = nullptr);
+                    std::vector<uint32_t>* form = nullptr, uint32_t* rank = nullptr);
+// rank (when given, n_edges entries): each input edge's canonical RANK, its place in the
+// winning leaf's edge order (canonical content, then input index) -- the per-edge identity
+// events are canonicalized by (ir_core.hpp:589-595, :697-700); all zero when the hash is 0.
 
 }  // namespace ir
 }  // namespace hgx

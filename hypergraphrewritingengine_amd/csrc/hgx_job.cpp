@@ -9,7 +9,10 @@
 // Served: full multiway evolution with CanonicalizeStates -> None or Full (the GPU backend's
 // grouping, hg_gpu_backend.cpp:269-310: None gives every state its own class, Full groups by the
 // exact IR hash computed on the host and emits each class's representative relabelled to its
-// canonical form; CanonicalHash is the IR hash in both), CanonicalizeEvents -> None, any rule
+// canonical form; CanonicalHash is the IR hash in both), CanonicalizeEvents -> None, Full,
+// Automatic, Positional or a list of signature keys (events identified by the signature of
+// common/include/hgcommon/event_core.hpp:60-84 over exact state hashes and canonical edge
+// ranks, as Hypergraph::create_event does, hypergraph/src/hypergraph.cpp:448-499), any rule
 // set, one or more initial states, causal edges with or without transitive reduction,
 // branchial edges, RequestedData selection. Options that change what is explored (sampling,
 // caps, canonical exploration, sessions, genesis events, graph properties) are refused with an
@@ -43,6 +46,7 @@ struct Job {
          inc_branchial = true, inc_nstates = true, inc_nevents = true, inc_ncausal = true,
          inc_nbranchial = true, inc_hashes = false;
     bool full = false;                       // CanonicalizeStates -> Full
+    uint32_t ev_keys = 0;                    // CanonicalizeEvents: event signature keys
     std::vector<std::pair<std::string, std::string>> warnings;   // (kind, context)
 };
 
@@ -121,7 +125,26 @@ static Job parse(const Value& root) {
                     else if (m == "None" || m == "False") j.full = false;
                     else refuse("only None and Full are served");
                 } else if (ok == "CanonicalizeEvents") {
-                    if (!(ov.kind == Value::Symbol && ov.s == "None")) refuse("only None is served");
+                    // event_core.hpp:26-45; the FFI's names (hypergraph_ffi.cpp:236-262)
+                    constexpr uint32_t In = 1, Out = 2, Step = 4, Rule = 8, Cons = 16, Prod = 32;
+                    if (ov.is_head("List")) {
+                        j.ev_keys = 0;
+                        for (size_t k = 0; k < ov.nargs(); ++k) {
+                            const std::string key = as_name(ov.arg(k));
+                            if (key == "InputState") j.ev_keys |= In;
+                            else if (key == "OutputState") j.ev_keys |= Out;
+                            else if (key == "Step") j.ev_keys |= Step;
+                            else if (key == "Rule") j.ev_keys |= Rule;
+                            else if (key == "ConsumedEdges") j.ev_keys |= Cons;
+                            else if (key == "ProducedEdges") j.ev_keys |= Prod;
+                        }
+                    } else {
+                        const std::string m = as_name(ov);
+                        if (m == "None") j.ev_keys = 0;
+                        else if (m == "Full") j.ev_keys = In | Out;
+                        else if (m == "Automatic" || m == "Positional") j.ev_keys = In | Out | Step | Cons | Prod;
+                        else refuse("unknown event canonicalization " + m);
+                    }
                 } else if (ok == "CausalTransitiveReduction") {
                     j.tr = as_bool(ov, ok.c_str());
                 } else if (ok == "IncludeCanonicalHashes") {
@@ -251,6 +274,12 @@ static Value ilist(const std::vector<int64_t>& xs) {
     return v;
 }
 
+// Edges a rule's right-hand side produces (empty pattern edges make none).
+static size_t rhs_edges(const Job& j, uint32_t rule) {
+    const auto& rhs = j.rules[rule].second;
+    return rhs.size() - (size_t)std::count_if(rhs.begin(), rhs.end(), [](const std::vector<int64_t>& ed) { return ed.empty(); });
+}
+
 struct Run {
     hgx_counts c{};
     std::vector<uint64_t> hash;
@@ -279,7 +308,8 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (j.steps > HGX_MAX_STEPS) throw JobError("Steps above " + std::to_string(HGX_MAX_STEPS));
     uint32_t flags = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL | HGX_REC_RELATIONS;
     // state edges: for the output, and for the IR hash (Full's grouping, CanonicalHash)
-    const bool need_ir = j.full || (j.inc_hashes && j.inc_states);
+    const bool need_ir = j.full || (j.inc_hashes && j.inc_states) || j.ev_keys;
+    const bool need_rank = (j.ev_keys & (16 | 32)) != 0;
     if (j.inc_states || j.inc_events || need_ir) flags |= HGX_REC_STATES;
     if (j.tr) flags |= HGX_OPT_TR;
     hgx_run_desc d{};
@@ -318,9 +348,10 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     // asked for. Classes: None -> every state its own; Full -> equal IR hash, represented by the
     // first state in id order (hg_gpu_backend.cpp:278-310).
     std::vector<uint64_t> ir(need_ir ? ns : 0);
+    std::vector<uint32_t> rank(need_rank ? R.soff[ns] : 0);
     if (need_ir && ns)
-        check(hgx_ir_hash_batch(ctx, R.soff.data(), (uint32_t)ns, R.eoff.data(), R.verts.empty() ? nullptr : R.verts.data(),
-                                ir.data()), ctx, "canonical hashes");
+        check(hgx_ir_rank_batch(ctx, R.soff.data(), (uint32_t)ns, R.eoff.data(), R.verts.empty() ? nullptr : R.verts.data(),
+                                ir.data(), need_rank ? rank.data() : nullptr), ctx, "canonical hashes");
     std::vector<int64_t> rep(ns);
     std::vector<uint64_t> reps;                               // class representatives, in id order
     if (j.full) {
@@ -332,6 +363,41 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
         }
     } else {
         for (uint64_t s = 0; s < ns; ++s) { rep[s] = (int64_t)s; reps.push_back(s); }
+    }
+
+    // Event identity (Hypergraph::create_event, hypergraph.cpp:448-499): the signature of
+    // event_core.hpp:60-84 over the exact hashes of the input and output states, the step of
+    // the output state's class (its first state), the rule, and the canonical ranks of the
+    // consumed edges (pattern order) and the produced edges (RHS order); the first event in id
+    // order with a signature is its class's id, and NumEvents counts the classes.
+    std::vector<int64_t> ev_rep(ne);
+    uint64_t n_ev_classes = ne;
+    if (j.ev_keys) {
+        std::unordered_map<uint64_t, uint32_t> first_of_class;     // exact hash -> first state
+        for (uint64_t s = 0; s < ns; ++s) first_of_class.emplace(ir[s], (uint32_t)s);
+        std::unordered_map<uint64_t, uint32_t> first_of_sig;
+        for (uint64_t e = 0; e < ne; ++e) {
+            const hgx_event& x = R.events[e];
+            uint64_t sig = FNV_OFFSET;
+            if (j.ev_keys & 1) sig = fnv(sig, ir[x.input_state]);
+            if (j.ev_keys & 2) sig = fnv(sig, ir[x.output_state]);
+            if (j.ev_keys & 4) sig = fnv(sig, R.step[first_of_class[ir[x.output_state]]]);
+            if (j.ev_keys & 8) sig = fnv(sig, (uint64_t)(uint16_t)x.rule);
+            if (j.ev_keys & 16)
+                for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k)
+                    sig = fnv(sig, rank[R.soff[x.input_state] + R.cpos[k]]);
+            if (j.ev_keys & 32) {
+                const uint64_t o0 = R.soff[x.output_state], o1 = R.soff[x.output_state + 1];
+                const uint64_t nrhs = rhs_edges(j, x.rule);
+                for (uint64_t k = o1 - nrhs; k < o1; ++k) sig = fnv(sig, rank[k]);
+                (void)o0;
+            }
+            if (sig == 0 || sig == FNV_OFFSET) sig = 1;
+            ev_rep[e] = first_of_sig.emplace(sig, (uint32_t)e).first->second;
+        }
+        n_ev_classes = first_of_sig.size();
+    } else {
+        for (uint64_t e = 0; e < ne; ++e) ev_rep[e] = (int64_t)e;
     }
 
     Value out = Value::assoc();
@@ -379,7 +445,7 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
             Value ea = Value::assoc();
             ea.put("Id", Value::integer((int64_t)e));
             if (j.inc_events) {
-                ea.put("CanonicalId", Value::integer((int64_t)e));
+                ea.put("CanonicalId", Value::integer(ev_rep[e]));
                 ea.put("RuleIndex", Value::integer(x.rule));
             }
             ea.put("InputState", Value::integer(x.input_state));
@@ -390,9 +456,7 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
                 std::vector<int64_t> cons, prod;
                 for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k) cons.push_back(R.cpos[k]);
                 const uint64_t out_ne = R.soff[x.output_state + 1] - R.soff[x.output_state];
-                const size_t nrhs = j.rules[x.rule].second.size() -
-                                    (size_t)std::count_if(j.rules[x.rule].second.begin(), j.rules[x.rule].second.end(),
-                                                          [](const std::vector<int64_t>& ed) { return ed.empty(); });
+                const size_t nrhs = rhs_edges(j, x.rule);
                 for (uint64_t k = out_ne - nrhs; k < out_ne; ++k) prod.push_back((int64_t)k);
                 ea.put("ConsumedEdges", ilist(cons));
                 ea.put("ProducedEdges", ilist(prod));
@@ -428,7 +492,7 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
         out.put("BranchialEdges", std::move(bl));
     }
     if (j.inc_nstates) out.put("NumStates", Value::integer((int64_t)reps.size()));
-    if (j.inc_nevents) out.put("NumEvents", Value::integer((int64_t)R.c.events));
+    if (j.inc_nevents) out.put("NumEvents", Value::integer((int64_t)n_ev_classes));
     if (j.inc_ncausal) out.put("NumCausalEdges", Value::integer(n_causal));
     if (j.inc_nbranchial) out.put("NumBranchialEdges", Value::integer((int64_t)R.c.branchial_pairs));
     if (!j.warnings.empty()) {

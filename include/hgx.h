@@ -271,6 +271,13 @@ int hgx_wl_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_st
  * reference's values). */
 int hgx_ir_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
                       const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash);
+/* The same, and each edge's canonical rank: out_rank[e - state_offsets[0]] for every edge e of
+ * the states, its place in its state's canonical edge order (canonical content, then input
+ * index; the ranks of compute_canonical_hash_with_edge_rank, ir_canonicalization.cpp:227-243)
+ * -- the per-edge identity of event canonicalization (common/include/hgcommon/event_core.hpp). */
+int hgx_ir_rank_batch(hgx_ctx* ctx, const uint64_t* state_offsets, uint32_t n_states,
+                      const uint32_t* edge_offsets, const uint32_t* vertices, uint64_t* out_hash,
+                      uint32_t* out_rank);
 /* The canonical form of one state (IRCanonicalizer::canonicalize_edges,
  * ir_canonicalization.cpp:107-170): its edges relabelled 0..n-1 and in canonical order,
  * out_edge_offsets[n_edges+1] and out_vertices[edge_offsets[n_edges] - edge_offsets[0]];

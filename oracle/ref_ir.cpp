@@ -8,7 +8,8 @@
 // Input (whitespace-separated integers): n_states, then per state: n_edges, then per edge:
 // arity v0 .. v_{arity-1}. Output: one line per state,
 //   <compute_canonical_hash> <vertex_count> <n_edges> then each canonical edge as
-//   <arity> <labels...> (canonicalize_edges().canonical_form, in canonical order).
+//   <arity> <labels...> (canonicalize_edges().canonical_form, in canonical order), then each
+//   input edge's rank (compute_canonical_hash_with_edge_rank).
 
 #include "hypergraph/ir_canonicalization.hpp"
 
@@ -42,6 +43,16 @@ int main(int argc, char** argv) {
             std::printf(" %zu", e.size());
             for (auto v : e) std::printf(" %llu", (unsigned long long)v);
         }
+        std::vector<uint32_t> rank;
+        {
+            auto mk = worker_scratch().mark();
+            SVec<SVec<VertexId>> se;
+            se.reserve(edges.size());
+            for (const auto& e : edges) se.emplace_back(e.begin(), e.end());
+            ir.compute_canonical_hash_with_edge_rank(se, rank);
+            worker_scratch().release(mk);
+        }
+        for (size_t e = 0; e < edges.size(); ++e) std::printf(" %u", e < rank.size() ? rank[e] : 0u);
         std::printf("\n");
     }
     return 0;

@@ -1,7 +1,8 @@
 """Writes tests/golden/golden_matrix_full.json: for every corpus case of golden_matrix_none.json,
 the reference's 'Full None 0' row of reference/golden_matrix.txt -- the number of exact (IR)
 canonical states and the sum of mix64 over their distinct IR hashes (the fingerprint of
-tools/generate_golden_matrix.cpp:42-59). Full multiway expands every raw state in every mode,
+tools/generate_golden_matrix.cpp:42-59) -- and the event counts of its 'None Full 0' and
+'None Automatic 0' rows (event canonicalization by signature). Full multiway expands every raw state in every mode,
 so these are the distinct IR hashes over the None-mode run's raw states. Run in the build
 container: python tests/golden/make_golden_full.py
 """
@@ -21,9 +22,11 @@ def main():
             continue
         case, smode, emode, quotient = t[0], t[1], t[2], t[3]
         if smode == "Full" and emode == "None" and quotient == "0":
-            rows[case] = {"ir_states": int(t[5]), "events": int(t[6]), "ir_fingerprint": int(t[10])}
+            rows.setdefault(case, {}).update(ir_states=int(t[5]), events=int(t[6]), ir_fingerprint=int(t[10]))
+        if smode == "None" and emode in ("Full", "Automatic") and quotient == "0":
+            rows.setdefault(case, {})["events_" + emode.lower()] = int(t[6])
     doc = {"source": "reference/golden_matrix.txt rows '<case> Full None 0': states = distinct IR hashes, "
-                     "fingerprint = sum of mix64 over them",
+                     "fingerprint = sum of mix64 over them; rows '<case> None Full|Automatic 0': events",
            "cases": {n: rows[n] for n in names if n in rows}}
     with open(os.path.join(HERE, "golden_matrix_full.json"), "w") as f:
         json.dump(doc, f, indent=1)

@@ -4,7 +4,8 @@ built from the reference sources by oracle/Makefile). Run in the build container
     make -C oracle ref && python tests/golden/make_ir_vectors.py
 
 Each vector is a state (edge lists over arbitrary vertex labels) with the reference's
-IRCanonicalizer::compute_canonical_hash and canonicalize_edges() canonical form. The states mix
+IRCanonicalizer::compute_canonical_hash, canonicalize_edges() canonical form and
+compute_canonical_hash_with_edge_rank() per-edge ranks. The states mix
 random hypergraphs (arity 1-4, repeated vertices, unary edges) with symmetric ones (cycles,
 circulants, disjoint copies, complete digraphs, stars, permutation hypergraphs), where the
 search and its automorphism pruning run.
@@ -68,7 +69,8 @@ def main():
             a = rest[w]
             form.append(rest[w + 1:w + 1 + a])
             w += 1 + a
-        vectors.append({"edges": st, "hash": h, "vertex_count": nv, "form": form})
+        rank = rest[w:w + len(st)]
+        vectors.append({"edges": st, "hash": h, "vertex_count": nv, "form": form, "rank": rank})
     doc = {"source": "oracle/_ref/irref: IRCanonicalizer::compute_canonical_hash and canonicalize_edges "
                      "(hypergraph/src/ir_canonicalization.cpp over common/include/hgcommon/ir_core.hpp)",
            "vectors": vectors}

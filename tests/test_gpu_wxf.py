@@ -129,6 +129,37 @@ def test_wxf_full_canonicalization(ctx, case):
         assert e["CanonicalOutputState"] == reps[ir[e["OutputState"]]]
 
 
+@pytest.mark.parametrize("mode", ["Full", "Automatic"])
+@pytest.mark.parametrize("case", [c for c in GOLD_NONE if c["name"] in GOLD_FULL], ids=lambda c: c["name"])
+def test_wxf_event_canonicalization(ctx, case, mode):
+    """CanonicalizeEvents: events identified by their signature over exact state hashes (and,
+    for Automatic, the step and the canonical ranks of the consumed and produced edges); the
+    golden matrix's 'None Full' / 'None Automatic' rows fix the event counts, and the relations
+    are the raw events' (the same counts as without event canonicalization)."""
+    base = {"InitialStates": [case["init"]],
+            "Rules": {f"R{k}": Rule(r["lhs"], r["rhs"]) for k, r in enumerate(case["rules"])},
+            "Steps": case["steps"]}
+    plain = loads(ctx.run_wxf(dumps({**base, "Options": {}})))
+    out = loads(ctx.run_wxf(dumps({**base, "Options": {"CanonicalizeEvents": Symbol(mode)}})))
+    assert out["NumEvents"] == GOLD_FULL[case["name"]]["events_" + mode.lower()]
+    assert plain["NumEvents"] == GOLD_FULL[case["name"]]["events"] == len(out["Events"])
+    for k in ("NumCausalEdges", "NumBranchialEdges", "NumStates"):
+        assert out[k] == plain[k]
+    ids = {e["CanonicalId"] for e in out["Events"].values()}
+    assert len(ids) == out["NumEvents"]
+    for eid, e in out["Events"].items():
+        assert e["CanonicalId"] <= eid and out["Events"][e["CanonicalId"]]["CanonicalId"] == e["CanonicalId"]
+
+
+def test_wxf_event_signature_keys(ctx):
+    """A key list: {"InputState", "OutputState"} is Full's signature."""
+    c = next(c for c in GOLD_NONE if c["name"] == "cycle4-automorphic")
+    base = {"InitialStates": [c["init"]],
+            "Rules": {f"R{k}": Rule(r["lhs"], r["rhs"]) for k, r in enumerate(c["rules"])}, "Steps": c["steps"]}
+    out = loads(ctx.run_wxf(dumps({**base, "Options": {"CanonicalizeEvents": ["InputState", "OutputState"]}})))
+    assert out["NumEvents"] == GOLD_FULL[c["name"]]["events_full"]
+
+
 def test_wxf_requested_data_and_errors(ctx):
     out = loads(ctx.run_wxf(_job(requested=["NumStates", "EventsMinimal"])))
     assert set(out) == {"Events", "NumStates", "Warnings"}

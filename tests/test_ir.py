@@ -1,7 +1,8 @@
 """The host exact canonicalizer (hgx_ir_hash_batch / hgx_ir_canonical_form), pinned three ways:
 
 * against the reference's own IRCanonicalizer on 600 states (tests/golden/ir_vectors.json,
-  written by tests/golden/make_ir_vectors.py from oracle/_ref/irref): hash and canonical form;
+  written by tests/golden/make_ir_vectors.py from oracle/_ref/irref): hash, canonical form and
+  per-edge canonical ranks;
 * against the reference's golden matrix 'Full' rows: every raw state of each corpus case's
   multiway evolution (from the oracle) hashed here, the number of distinct hashes and the sum of
   mix64 over them equal the reference's exact-canonical state count and fingerprint;
@@ -39,6 +40,12 @@ def test_reference_vectors_hash():
     got = hgx.ir_hash([v["edges"] for v in VEC])
     bad = [i for i, v in enumerate(VEC) if int(got[i]) != v["hash"]]
     assert not bad, f"{len(bad)} hashes differ, first {bad[:5]}"
+
+
+def test_reference_vectors_rank():
+    _, ranks = hgx.ir_rank([v["edges"] for v in VEC])
+    bad = [i for i, v in enumerate(VEC) if ranks[i] != v["rank"]]
+    assert not bad, f"{len(bad)} rank vectors differ, first {bad[:5]}"
 
 
 def test_reference_vectors_form():
