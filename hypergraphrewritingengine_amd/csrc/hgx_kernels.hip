@@ -659,7 +659,7 @@ __device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
 // W > 1, the workgroup's W waves share one child and one workspace, for large children whose
 // workspace would otherwise leave a CU one or two waves.
 template <bool GWS, uint32_t W>
-__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_expand(ExpandArgs a) {
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expand(ExpandArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t l = lane_id();
     const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;    // teams per workgroup
@@ -689,14 +689,21 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_expand(ExpandArgs a) 
         rr = lo;
     }
 
+    // The run's counters, summed over the team's consecutive children of one run and added to
+    // the run's row once per run: kept by the team's first lane in the workspace (64 bytes
+    // before the reduction words), not in registers every lane would carry.
     uint32_t acc_run = INV;
-    uint64_t acc[CTR_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-    auto flush = [&]() {
-        if (tm.leader() && acc_run != INV)
-            for (int k = 0; k < CTR_N; ++k)
-                if (acc[k]) atomicAdd((unsigned long long*)&a.ctr[(size_t)acc_run * CTR_N + k],
-                                      (unsigned long long)acc[k]);
+    uint64_t* acc = (uint64_t*)(my + a.lds_per_wave - 128);
+    if (tm.leader())
         for (int k = 0; k < CTR_N; ++k) acc[k] = 0;
+    auto flush = [&]() {
+        if (tm.leader()) {
+            if (acc_run != INV)
+                for (int k = 0; k < CTR_N; ++k)
+                    if (acc[k]) atomicAdd((unsigned long long*)&a.ctr[(size_t)acc_run * CTR_N + k],
+                                          (unsigned long long)acc[k]);
+            for (int k = 0; k < CTR_N; ++k) acc[k] = 0;
+        }
     };
 
     for (uint32_t lc = c0; lc < c1; ++lc) {
@@ -1243,8 +1250,8 @@ hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32
 
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean) {
     const WLSizes z = expand_sizes(max_child_ne, max_nv, max_nextv, A, lean);
-    // + 64: carving slack; + 64: the team's reduction words at the end
-    return ((z.bytes(A) + 128u) + 15u) & ~15u;
+    // + 64: carving slack; + 64: the run counters; + 64: the team's reduction words at the end
+    return ((z.bytes(A) + 192u) + 15u) & ~15u;
 }
 
 template <uint32_t W>
