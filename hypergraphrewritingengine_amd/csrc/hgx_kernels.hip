@@ -370,6 +370,8 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
         lr = (DRule*)carve(p, 2 * sizeof(DRule));
     }
 
+    // States strided over the teams: neighbouring states (often one request's, of similar size)
+    // go to different teams, which balances the launch better than contiguous chunks do.
     const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;      // teams per workgroup
     const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
@@ -618,10 +620,15 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
                     const uint32_t round = tm.sum(c);
                     total += round;
                     edges_sum += (uint64_t)round * child_ne;
-                    unsigned long long before = 0;
-                    if (tm.leader() && round) before = atomicAdd(&a.run_acc[run], (unsigned long long)round);
-                    before = tm.bcast(before);
-                    if (before + round > budget) { stop = true; break; }
+                    if (base + N >= nseed && r + 1 >= RN.n_rules) {
+                        // the state's last round: nothing is left to stop, so add without waiting
+                        if (tm.leader() && round) atomicAdd(&a.run_acc[run], (unsigned long long)round);
+                    } else {
+                        unsigned long long before = 0;
+                        if (tm.leader() && round) before = atomicAdd(&a.run_acc[run], (unsigned long long)round);
+                        before = tm.bcast(before);
+                        if (before + round > budget) { stop = true; break; }
+                    }
                 }
             }
             tm.sync();
@@ -678,16 +685,13 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
     uint32_t* gv = (uint32_t*)s.srt;                      // [NE * A]
     uint32_t* vmap = (uint32_t*)s.nbr;                    // [max_nextv + 1]
 
-    // Contiguous chunk of children per team; consecutive chunks on one XCD.
+    // Children in chunks of kChunk consecutive ones, dealt round-robin to the teams (2 measured
+    // best of 1, 2, 4 and contiguous per-team blocks on the batched workload): a chunk
+    // keeps neighbouring children (one parent, one request) together for the counters, and the
+    // dealing spreads large and small requests' children over all teams.
+    constexpr uint32_t kChunk = 2;
     const uint32_t gw = xcd_block() * tpb + w, nw = gridDim.x * tpb;
-    const uint32_t per = (a.n_items + nw - 1) / nw;
-    const uint32_t c0 = gw * per, c1 = min(a.n_items, c0 + per);
     uint32_t rr = 0;                                       // range of logical index c (subsets)
-    if (a.nrange) {
-        uint32_t lo = 0, hi = a.nrange;                    // last range starting at or before c0
-        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= c0) lo = mid; else hi = mid; }
-        rr = lo;
-    }
 
     // The run's counters, summed over the team's consecutive children of one run and added to
     // the run's row once per run: kept by the team's first lane in the workspace (64 bytes
@@ -706,9 +710,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         }
     };
 
-    for (uint32_t lc = c0; lc < c1; ++lc) {
+    for (uint32_t lc = gw * kChunk; lc < a.n_items; lc += (lc % kChunk == kChunk - 1) ? (nw - 1) * kChunk + 1 : 1) {
         uint32_t c = lc;
         if (a.nrange) {
+            if (lc % kChunk == 0 || lc == gw * kChunk) {       // a new chunk: find its range
+                uint32_t lo = 0, hi = a.nrange;
+                while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= lc) lo = mid; else hi = mid; }
+                rr = lo;
+            }
             while (lc >= a.rpre[rr + 1]) ++rr;
             c = a.rb[rr] + (lc - a.rpre[rr]);
         }
