@@ -269,7 +269,7 @@ uint32_t Engine::step() {
                     // Team: as for the expansion, by the one-wave workspace
                     auto lpw = [&](uint32_t t) {
                         return (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax, mc.small != 0,
-                                                   t) + 15u) & ~15u;
+                                                   t, mc.fwd_n != 0) + 15u) & ~15u;
                     };
                     const uint32_t team = pick_team(lpw(1), ctx_->join_team);
                     mc.lds_per_wave = lpw(team);

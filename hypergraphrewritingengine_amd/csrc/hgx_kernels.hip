@@ -327,12 +327,12 @@ __device__ __forceinline__ uint32_t reg_join_any(const DRule& R, const uint32_t*
 }
 
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
-                            bool small, uint32_t team) {
+                            bool small, uint32_t team, bool lanes) {
     auto al = [](uint32_t b) { return (b + 7u) & ~7u; };
     // + 64: carving slack; + 64: the team's reduction words at the end
     uint32_t b = al(max_ne * A * 4) + al(max_ne) + al(max_ne * 2) + al((max_nextv + 1) * 4) +
                  al(max_ne * A * 2) * 2 + al(max_ne * A * 4) + 128;
-    if (small) b += JoinLaneLds::WORDS * WAVE * 4 * team + 2 * (uint32_t)sizeof(DRule);
+    if (small) b += (lanes ? JoinLaneLds::WORDS * WAVE * 4 * team : 0u) + 2 * (uint32_t)sizeof(DRule);
     if (dmax) b += al(dmax * (KL + 1) * 2) + al(dmax * 2) + al((dmax + 1) * 4);
     return b;
 }
@@ -365,8 +365,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
     constexpr bool SMALL = std::is_same<JoinLane, JoinLaneLds>::value;
     DRule* lr = nullptr;
     if constexpr (SMALL) {
-        // one lane-interleaved block per wave
-        L.at((uint32_t*)carve(p, JoinLaneLds::WORDS * WAVE * 4 * W) + (W == 1 ? 0 : w * JoinLaneLds::WORDS * WAVE), l);
+        // one lane-interleaved block per wave; only the forwarding path's anchored joins use it
+        // (the full join keeps its search in registers and checks seeds by first-binding slot)
+        const uint32_t lane_words = a.fwd_n ? JoinLaneLds::WORDS * WAVE * W : 0u;
+        L.at((uint32_t*)carve(p, lane_words * 4) + (W == 1 || !lane_words ? 0 : w * JoinLaneLds::WORDS * WAVE), l);
         lr = (DRule*)carve(p, 2 * sizeof(DRule));
     }
 
@@ -581,8 +583,17 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
                 const uint32_t e = base + tm.r;
                 bool ok = false;
                 if (e < ne && sa[e] == R.la[p0]) {
-                    uint32_t m = 0;
-                    ok = bind_edge(sv + e * A, sa[e], R.lv[p0], R.la[p0], L.bind, m);
+                    if constexpr (SMALL) {
+                        // repeated pattern variables: each position equals its first binding's
+                        ok = true;
+                        for (uint32_t i = 0; i < R.la[p0]; ++i) {
+                            const uint32_t src = R.jsrc[0][i];
+                            if (src != 0xFFu) ok &= sv[e * A + i] == sv[e * A + (src & 15u)];
+                        }
+                    } else {
+                        uint32_t m = 0;
+                        ok = bind_edge(sv + e * A, sa[e], R.lv[p0], R.la[p0], L.bind, m);
+                    }
                 }
                 uint32_t nround;
                 const uint32_t at = tm.compact(ok, nround);

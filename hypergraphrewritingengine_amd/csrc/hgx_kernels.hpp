@@ -131,8 +131,9 @@ struct WLStatesArgs {
     uint32_t lean;             // every state has one arity 2..4 and < 2^16 edges: lean workspace
 };
 
+// lanes: the launch may forward (fwd_n != 0), so the small-rule path needs its per-lane LDS
 uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uint32_t KL, uint32_t dmax,
-                            bool small, uint32_t team);
+                            bool small, uint32_t team, bool lanes);
 // One admitted request: its rules (copied from a packed staging array), its state map (zeroed)
 // and its counter row (zeroed) -- every admission of a step in one launch.
 struct AdmitItem {
