@@ -6,6 +6,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace hgx {
@@ -36,25 +37,28 @@ __host__ __device__ inline uint64_t splitmix(uint64_t z) {   // core.hpp:82-87
 // (hypergraph/src/pattern.cpp:43-75), computed on the host; var_src_* name the first LHS
 // (edge, position) of each LHS variable so a rewrite resolves a bound vertex in O(1).
 struct DRule {
+    // The fields the small-rule join reads come first, so it can stage just this prefix
+    // (through jsrc) in LDS.
     uint8_t n_lhs, n_rhs, n_new, pad0;
     uint8_t la[16];
+    uint8_t order[16];
+    uint8_t piv[16];          // per join depth: a position sharing a bound variable, or 0xFF
+    // Binding sources in the join order: jsrc[d][i] = (d' << 4) | i' when the variable at slot i
+    // of pattern edge order[d] first appeared at slot i' of depth d' (an earlier depth, or an
+    // earlier slot of the same edge), 0xFF where it appears first.
+    uint8_t jsrc[16][16];
     uint8_t lv[16][16];
     uint8_t ra[16];
     uint8_t rv[16][16];
-    uint8_t order[16];
     uint8_t var_src_edge[32];
     uint8_t var_src_pos[32];
-    uint8_t piv[16];          // per join depth: a position sharing a bound variable, or 0xFF
     uint32_t lhs_mask, rhs_mask, new_mask, pad1;
     // Anchored join orders for delta joins: aord[p] starts at LHS position p and adds the
     // remaining positions greedily by shared variables; apiv[p] are its pivots.
     uint8_t aord[16][16];
     uint8_t apiv[16][16];
-    // Binding sources in the join order: jsrc[d][i] = (d' << 4) | i' when the variable at slot i
-    // of pattern edge order[d] first appeared at slot i' of depth d' (an earlier depth, or an
-    // earlier slot of the same edge), 0xFF where it appears first.
-    uint8_t jsrc[16][16];
 };
+static_assert(offsetof(DRule, lv) % 4 == 0 && sizeof(DRule) % 4 == 0, "DRule is staged in 32-bit words");
 
 // Per request (run) state the kernels read and update.
 struct DRun {

@@ -13,6 +13,7 @@
 // MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <type_traits>
 
 #include "hgx_device.hpp"
@@ -384,17 +385,19 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
             while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= ls) lo = mid; else hi = mid; }
             s = a.rb[lo] + (ls - a.rpre[lo]);
         }
+        // the state's record and its request's flags in two rounds of independent loads
         const uint32_t run = a.P.run[s];
-        if (!a.active[run]) { if (!EMIT && tm.leader()) a.cnt[s] = 0; continue; }
-        const uint64_t budget = EMIT ? 0 : (uint64_t)a.budget[run];
-        // the run's tally moves under other teams: one read, shared, so the team agrees
-        if (!EMIT && tm.bcast(tm.leader() ? (uint64_t)a.run_acc[run] : 0ull) > budget) {
-            if (tm.leader()) a.cnt[s] = 0;
-            continue;
-        }
         const uint32_t ne = a.P.ne[s];
         const uint64_t eo = a.P.eoff[s];
         const uint32_t nvid = a.P.nextv[s];                     // labels are < nvid
+        const uint8_t active = a.active[run];
+        const uint64_t budget = EMIT ? 0 : (uint64_t)a.budget[run];
+        // the run's tally moves under other teams: one read, shared, so the team agrees
+        const uint64_t tally = EMIT ? 0 : tm.bcast(tm.leader() ? (uint64_t)a.run_acc[run] : 0ull);
+        if (!active || (!EMIT && tally > budget)) {
+            if (!EMIT && tm.leader()) a.cnt[s] = 0;
+            continue;
+        }
         for (uint32_t i = tm.r; i < ne * A; i += N) sv[i] = a.P.ev[eo * A + i];
         for (uint32_t i = tm.r; i < ne; i += N) sa[i] = a.P.ar[eo + i];
         for (uint32_t v = tm.r; v <= nvid; v += N) iend[v] = 0;
@@ -437,10 +440,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
         const DRun& RN = a.runs[run];
         const DRule* rules;
         if constexpr (SMALL) {
-            const uint32_t* src = (const uint32_t*)RN.rules;
-            uint32_t* dst = (uint32_t*)lr;
-            const uint32_t words = RN.n_rules * (uint32_t)(sizeof(DRule) / 4);
-            for (uint32_t i = tm.r; i < words; i += N) dst[i] = src[i];
+            // the whole rules when forwarding (anchored orders), else the prefix the full join
+            // reads (DRule puts those fields first)
+            const uint32_t pw = a.fwd_n ? (uint32_t)(sizeof(DRule) / 4) : (uint32_t)(offsetof(DRule, lv) / 4);
+            const uint32_t words = RN.n_rules * pw;
+            for (uint32_t i = tm.r; i < words; i += N) {
+                const uint32_t r = i / pw, k = i - r * pw;
+                ((uint32_t*)&lr[r])[k] = ((const uint32_t*)&RN.rules[r])[k];
+            }
             tm.sync();
             rules = lr;
         } else {
