@@ -6,6 +6,7 @@ or without a gfx950 device every entry point raises.
 """
 from __future__ import annotations
 
+import collections.abc
 import ctypes as C
 import os
 
@@ -15,7 +16,7 @@ from . import abi
 from .rules import (Request, Rule, RuleBuilder, encode_roots, encode_rules, encode_state,
                     make_rule, rule_from_lists)
 
-__all__ = ["Context", "Result", "Request", "Rule", "RuleBuilder", "make_rule", "rule_from_lists",
+__all__ = ["Context", "Result", "ResultList", "Request", "Rule", "RuleBuilder", "make_rule", "rule_from_lists",
            "encode_rules", "encode_state", "encode_roots", "library_path", "load_library",
            "HGXError", "EXPORTED_SYMBOLS", "wxf_validate", "ir_hash", "ir_rank", "ir_canonical_form"]
 
@@ -28,7 +29,8 @@ EXPORTED_SYMBOLS = [
     "hgx_evolve_batch", "hgx_request_submit", "hgx_serve_step", "hgx_request_poll",
     "hgx_request_wait", "hgx_request_footprint", "hgx_result_counts", "hgx_result_states",
     "hgx_result_events", "hgx_result_state_edges", "hgx_result_causal", "hgx_result_branchial",
-    "hgx_result_timing", "hgx_result_release", "hgx_wl_hash_batch", "hgx_match_count",
+    "hgx_result_timing", "hgx_result_release", "hgx_result_release_many", "hgx_wl_hash_batch",
+    "hgx_match_count",
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
@@ -85,6 +87,8 @@ def load_library():
     L.hgx_result_timing.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.hgx_result_release.argtypes = [vp]
     L.hgx_result_release.restype = None
+    L.hgx_result_release_many.argtypes = [vp, C.c_uint32]
+    L.hgx_result_release_many.restype = None
     L.hgx_wl_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
     L.hgx_ir_hash_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
     L.hgx_ir_rank_batch.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, vp]
@@ -109,15 +113,34 @@ def load_library():
     return L
 
 
+_COUNTS_DTYPE = np.dtype([(k, np.uint64 if t is C.c_uint64 else np.uint32) for k, t in abi.Counts._fields_])
+
+
+class _BatchHandles:
+    """The hgx_result* handles of one batch, released together when the last holder goes."""
+
+    def __init__(self, lib, outs, n, ctx):
+        self._lib, self.outs, self.n, self._ctx = lib, outs, n, ctx
+
+    def __del__(self):
+        try:
+            if self.outs is not None:
+                self._lib.hgx_result_release_many(self.outs, self.n)
+                self.outs = None
+        except Exception:
+            pass
+
+
 class Result:
     """One request's result (hgx_result*). Arrays are read back lazily."""
 
-    def __init__(self, lib, handle, ctx, counts=None, row=None):
+    def __init__(self, lib, handle, ctx, counts=None, row=None, owner=None):
         self._lib = lib
         self._h = handle
         self._ctx = ctx           # keep the context alive while results exist
         self._row = row           # (structured array, index) of a batch's counts, read lazily
         self._counts = counts
+        self._owner = owner       # a batch's handles: released with the batch, not one by one
         if counts is None and row is None:
             c = abi.Counts()
             self._check(lib.hgx_result_counts(handle, C.byref(c)))
@@ -192,14 +215,48 @@ class Result:
 
     def close(self):
         if self._h:
-            self._lib.hgx_result_release(self._h)
+            if self._owner is None:
+                self._lib.hgx_result_release(self._h)
             self._h = None
+            self._owner = None
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+
+class ResultList(collections.abc.Sequence):
+    """The results of one batch, in request order. Each Result is wrapped on first access; the
+    batch's handles are released together once the list and every Result taken from it are gone."""
+
+    def __init__(self, handles: _BatchHandles, counts):
+        self._hd = handles
+        self._arr = counts
+        self._items = [None] * handles.n
+
+    def __len__(self):
+        return self._hd.n
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(self._hd.n))]
+        n = self._hd.n
+        if i < 0:
+            i += n
+        if not 0 <= i < n:
+            raise IndexError("result index out of range")
+        r = self._items[i]
+        if r is None:
+            hd = self._hd
+            r = Result(hd._lib, C.c_void_p(hd.outs[i]), hd._ctx, row=(self._arr, i), owner=hd)
+            self._items[i] = r
+        return r
+
+    def counts(self):
+        """Every result's counts as one numpy structured array (one row per request)."""
+        return self._arr
 
 
 def wxf_validate(job: bytes):
@@ -355,7 +412,8 @@ class Context:
         return PreparedBatch(requests, self)
 
     def evolve_batch(self, requests):
-        """Evolve requests (a list or a PreparedBatch) together on the device; one Result each."""
+        """Evolve requests (a list or a PreparedBatch) together on the device: a ResultList,
+        one Result per request in request order."""
         import time as _t
         t0 = _t.perf_counter()
         pb = requests if isinstance(requests, PreparedBatch) else PreparedBatch(requests)
@@ -369,13 +427,12 @@ class Context:
         t2 = _t.perf_counter()
         if rc != abi.HGX_OK:
             raise self._err(rc)
+        handles = _BatchHandles(self._lib, outs, n, self)
         cnts = (abi.Counts * max(n, 1))()
         rc = self._lib.hgx_result_counts_many(outs, n, cnts)
         if rc != abi.HGX_OK:
             raise self._err(rc)
-        arr = np.frombuffer(cnts, dtype=np.dtype([(k, np.uint64 if t is C.c_uint64 else np.uint32)
-                                                   for k, t in abi.Counts._fields_]), count=n)
-        out = [Result(self._lib, C.c_void_p(outs[i]), self, row=(arr, i)) for i in range(n)]
+        out = ResultList(handles, np.frombuffer(cnts, dtype=_COUNTS_DTYPE, count=n))
         self.last_timing = {"prepare_s": t1 - t0, "library_s": t2 - t1,
                             "results_s": _t.perf_counter() - t2}
         return out

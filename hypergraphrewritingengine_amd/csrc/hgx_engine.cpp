@@ -40,18 +40,22 @@ void Engine::grow_slots(uint32_t need) {
     auto na = std::make_unique<Arena>(pool_.get());
     DRun* runs = dalloc<DRun>(*na, cap);
     uint64_t* ctr = dalloc<uint64_t>(*na, (size_t)cap * CTR_N);
-    uint8_t* active = dalloc<uint8_t>(*na, cap);
-    uint32_t* budget = dalloc<uint32_t>(*na, cap);
-    uint64_t* acc = dalloc<uint64_t>(*na, cap);
-    uint64_t* redges = dalloc<uint64_t>(*na, cap);
-    uint32_t* sb = dalloc<uint32_t>(*na, cap);
-    uint32_t* se = dalloc<uint32_t>(*na, cap);
-    uint64_t* tot = dalloc<uint64_t>(*na, cap + 1);
+    // per-slot tables uploaded together each generation: budget, range begin, range end (u32),
+    // active (u8); and the ones zeroed together: accumulated matches, child edges, totals
+    uint8_t* tab = dalloc<uint8_t>(*na, 13ull * cap);
+    uint32_t* budget = (uint32_t*)tab;
+    uint32_t* sb = budget + cap;
+    uint32_t* se = sb + cap;
+    uint8_t* active = (uint8_t*)(se + cap);
+    uint64_t* acc = dalloc<uint64_t>(*na, 3ull * cap + 1);
+    uint64_t* redges = acc + cap;
+    uint64_t* tot = redges + cap;                     // [cap + 1]
     HIP_TRY(hipMemsetAsync(ctr, 0, sizeof(uint64_t) * cap * CTR_N, st_));
     if (cap_slots_)
         HIP_TRY(hipMemcpyAsync(ctr, d_ctr_, sizeof(uint64_t) * cap_slots_ * CTR_N, hipMemcpyDeviceToDevice, st_));
     HIP_TRY(hipStreamSynchronize(st_));
     h_runs_.resize(cap);
+    h_tab_.assign(13ull * cap, 0);
     for (uint32_t s = cap; s-- > cap_slots_;) {
         std::memset(&h_runs_[s], 0, sizeof(DRun));
         free_slots_.push_back(s);
@@ -153,9 +157,9 @@ void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
         live_.erase(r->id);
         live_footprint_ -= r->footprint;
         release_next_.push_back(r->slot);
-        slab_.free(r->d_rules);
-        slab_.free(r->map_keys);
-        slab_.free(r->map_vals);
+        slab_.free(r->d_rules, sizeof(DRule) * r->hp->rules.size());
+        slab_.free(r->map_keys, 8ull * r->map_cap);
+        slab_.free(r->map_vals, 4ull * r->map_cap);
         r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
     }
 }
@@ -173,7 +177,7 @@ constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
 
 uint32_t Engine::step() {
     const auto t_step = Clock::now();
-    double t_count = 0, t_emit = 0, t_child = 0;
+    double t_count = 0, t_emit = 0, t_child = 0, t_launch = 0, t_fin = 0;
     hipStream_t st = st_;
     Arena tmp(pool_.get(), (size_t)4 << 20);
     auto keep = std::make_unique<Arena>(pool_.get(), (size_t)4 << 20);   // this step's match list
@@ -191,7 +195,10 @@ uint32_t Engine::step() {
 
     // ---- 1. the join over the frontier, with the state-cap decision ----
     if (n) {
-        std::vector<uint32_t> h_budget(cap_slots_, 0), h_b(cap_slots_, 0), h_e(cap_slots_, 0);
+        std::fill(h_tab_.begin(), h_tab_.end(), 0);
+        uint32_t* h_budget = (uint32_t*)h_tab_.data();
+        uint32_t* h_b = h_budget + cap_slots_;
+        uint32_t* h_e = h_b + cap_slots_;
         bool any = false;
         uint32_t KL = 1;
         bool small = true;
@@ -210,16 +217,11 @@ uint32_t Engine::step() {
             any = true;
         }
         if (any) {
-            h2d(d_active_, h_active.data(), cap_slots_, st);
-            h2d(d_budget_, h_budget.data(), cap_slots_, st);
-            h2d(d_sbeg_, h_b.data(), cap_slots_, st);
-            h2d(d_send_, h_e.data(), cap_slots_, st);
-            HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * cap_slots_, st));
-            HIP_TRY(hipMemsetAsync(d_run_edges_, 0, sizeof(uint64_t) * cap_slots_, st));
-            HIP_TRY(hipMemsetAsync(d_tot_ + cap_slots_, 0, sizeof(uint64_t), st));
+            std::memcpy(h_tab_.data() + 12ull * cap_slots_, h_active.data(), cap_slots_);
+            h2d((uint8_t*)d_budget_, h_tab_.data(), h_tab_.size(), st);    // budget, sbeg, send, active
+            HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * (3ull * cap_slots_ + 1), st));
             cnt = dalloc<uint32_t>(tmp, n + 1);
             moff = dalloc<uint64_t>(*keep, n + 1);
-            HIP_TRY(hipMemsetAsync(cnt + n, 0, sizeof(uint32_t), st));
             if (F_->max_ne > 65535)
                 throw Error{HGX_ERR_UNSUPPORTED, "states above 65535 edges need the large-state join"};
             MatchArgs ma{};
@@ -293,6 +295,7 @@ uint32_t Engine::step() {
             h_edges.assign(cap_slots_, 0);
             d2h(htot.data(), d_tot_, cap_slots_ + 1, st);
             d2h(h_edges.data(), d_run_edges_, cap_slots_, st);
+            t_launch = ms_since(t_step);
             HIP_TRY(hipStreamSynchronize(st));
             t_count = ms_since(t_step);
             parent_edges = htot[cap_slots_];
@@ -331,8 +334,8 @@ uint32_t Engine::step() {
                     uint32_t* vals = (uint32_t*)slab_.alloc(4 * c);
                     HIP_TRY(hipMemsetAsync(keys, 0, 8 * c, st));
                     HIP_TRY(launch_rehash(r.map_keys, r.map_vals, r.map_cap, keys, vals, (uint32_t)(c - 1), st));
-                    slab_.free(r.map_keys);
-                    slab_.free(r.map_vals);
+                    slab_.free(r.map_keys, 8ull * r.map_cap);
+                    slab_.free(r.map_vals, 4ull * r.map_cap);
                     r.map_keys = keys; r.map_vals = vals; r.map_cap = (uint32_t)c;
                     DRun& d = h_runs_[r.slot];
                     d.map_keys = keys; d.map_vals = vals; d.map_mask = (uint32_t)(c - 1);
@@ -724,16 +727,17 @@ uint32_t Engine::step() {
 
     t_child = ms_since(t_step);
     // ---- 4. finished runs, slot recycling, the new frontier ----
-    for (auto& r : F_runs_)
-        if (r->done && live_.count(r->id)) done_now.push_back(r);
-    for (auto& r : next_runs)
-        if (r->done && live_.count(r->id) &&
-            std::find(done_now.begin(), done_now.end(), r) == done_now.end())
-            done_now.push_back(r);
+    for (auto* list : {&F_runs_, &next_runs})
+        for (auto& r : *list)
+            if (r->done && !r->finishing && live_.count(r->id)) {
+                r->finishing = true;
+                done_now.push_back(r);
+            }
     std::vector<uint32_t> releasable;
     releasable.swap(release_next_);
     finalize(done_now);                        // synchronises the stream
     if (done_now.empty()) HIP_TRY(hipStreamSynchronize(st));
+    t_fin = ms_since(t_step);
     prof_resolve(ctx_);
     for (uint32_t s : releasable) free_slots_.push_back(s);
     if (F_ && !F_->keep_edges) F_->edges.release();
@@ -761,9 +765,10 @@ uint32_t Engine::step() {
     }
     if (trace_on())
         std::fprintf(stderr, "[hgx] step: frontier %u states, %llu children, %u admitted, %zu finished | "
-                             "count+sync %.3f ms, +emit %.3f, +child gen %.3f, total %.3f ms\n",
-                     n, (unsigned long long)M, (unsigned)admitted.size(), done_now.size(), t_count, t_emit,
-                     t_child, ms_since(t_step));
+                             "count launched %.3f ms, +sync %.3f, +emit %.3f, +child gen %.3f, +finish %.3f, "
+                             "total %.3f ms\n",
+                     n, (unsigned long long)M, (unsigned)admitted.size(), done_now.size(), t_launch, t_count,
+                     t_emit, t_child, t_fin, ms_since(t_step));
     return live_or_queued();
 }
 
@@ -938,12 +943,22 @@ extern "C" int hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* b, hgx_result** r
     return guarded(ctx, [&]() -> int {
         HIP_TRY(hipSetDevice(ctx->device));
         const uint32_t n = (uint32_t)b->runs.size();
-        Engine E(ctx);
-        E.soft_budget_frac = 0.8;
-        E.reserve_slots(n + 1);
-        for (uint32_t i = 0; i < n; ++i) E.submit(b->runs[i], i + 1);
-        while (E.live_or_queued()) E.step();
-        for (uint32_t i = 0; i < n; ++i) results[i] = make_result(E.take(i + 1));
+        const auto t0 = Clock::now();
+        double t_setup = 0, t_steps = 0, t_results = 0;
+        {
+            Engine E(ctx);
+            E.soft_budget_frac = 0.8;
+            E.reserve_slots(n + 1);
+            for (uint32_t i = 0; i < n; ++i) E.submit(b->runs[i], i + 1);
+            t_setup = ms_since(t0);
+            while (E.live_or_queued()) E.step();
+            t_steps = ms_since(t0);
+            for (uint32_t i = 0; i < n; ++i) results[i] = make_result(E.take(i + 1));
+            t_results = ms_since(t0);
+        }
+        if (trace_on())
+            std::fprintf(stderr, "[hgx] batch: %u requests | setup %.3f ms, +steps %.3f, +results %.3f, "
+                                 "+engine release %.3f\n", n, t_setup, t_steps, t_results, ms_since(t0));
         return HGX_OK;
     });
 }
@@ -1053,6 +1068,11 @@ extern "C" int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint6
 }
 
 extern "C" void hgx_result_release(hgx_result* r) { delete r; }
+
+extern "C" void hgx_result_release_many(hgx_result* const* rs, uint32_t n) {
+    if (!rs) return;
+    for (uint32_t i = 0; i < n; ++i) delete rs[i];
+}
 
 extern "C" int hgx_result_states(const hgx_result* r, uint64_t* hash, uint32_t* step, uint32_t* creator,
                                  uint64_t cap) {

@@ -10,7 +10,7 @@
 
 #include <chrono>
 #include <deque>
-#include <map>
+#include <unordered_map>
 #include <memory>
 #include <vector>
 
@@ -57,6 +57,7 @@ struct RunRec {
     uint32_t depth = 0, steps_done = 0;
     uint32_t ne_bound = 0, nv_bound = 0;   // bounds on edges / vertex labels of its frontier states
     bool truncated = false, done = false, admitted = false;
+    bool finishing = false;          // listed for finalize in the current step
     uint64_t footprint = 0;
     std::vector<Segment> segs;       // every state of the run, in state-id order
     hgx_counts counts{};
@@ -73,14 +74,14 @@ public:
     Engine& operator=(const Engine&) = delete;
 
     uint64_t submit(std::shared_ptr<const RunHost> h, uint64_t id);
-    void reserve_slots(uint32_t n) { grow_slots(n); }
+    void reserve_slots(uint32_t n) { grow_slots(n); live_.reserve(n); finished.reserve(n); }
     // One generation of every admitted request, plus admissions. Returns live + queued runs.
     uint32_t step();
     uint32_t live_or_queued() const { return (uint32_t)(live_.size() + pending_.size()); }
     // A finished run (removed from the finished set), or null.
     std::shared_ptr<RunRec> take(uint64_t id);
     bool known(uint64_t id) const;
-    std::map<uint64_t, std::shared_ptr<RunRec>> finished;
+    std::unordered_map<uint64_t, std::shared_ptr<RunRec>> finished;
     double soft_budget_frac = 0.6;
 
 private:
@@ -100,9 +101,10 @@ private:
     uint32_t* d_send_ = nullptr;
     uint64_t* d_tot_ = nullptr;               // [cap_slots + 1]
     std::vector<DRun> h_runs_;
+    std::vector<uint8_t> h_tab_;              // host side of budget, sbeg, send, active
     std::vector<uint32_t> free_slots_;
     std::vector<uint32_t> release_next_;      // slots whose states are still in the frontier
-    std::map<uint64_t, std::shared_ptr<RunRec>> live_;
+    std::unordered_map<uint64_t, std::shared_ptr<RunRec>> live_;
     std::deque<std::shared_ptr<RunRec>> pending_;
     std::shared_ptr<DevGen> F_;               // current frontier
     std::vector<std::shared_ptr<RunRec>> F_runs_;   // runs with states in F, in F order

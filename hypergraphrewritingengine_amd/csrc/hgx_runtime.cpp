@@ -88,15 +88,15 @@ SlabAlloc::~SlabAlloc() {
     for (auto& kv : big_) pool_->put(kv.second);
 }
 
-static uint32_t size_class(size_t bytes) {
-    uint32_t c = 12;                                  // 4 KiB minimum
+static uint32_t size_class(size_t bytes, uint32_t min_class) {
+    uint32_t c = min_class;
     while (((size_t)1 << c) < bytes) ++c;
     return c;
 }
 
 void* SlabAlloc::alloc(size_t bytes) {
-    const uint32_t c = size_class(bytes);
-    if (c > 24) {                                     // above 16 MiB: a chunk of its own
+    const uint32_t c = size_class(bytes, kMinClass);
+    if (c > kMaxClass) {                              // above 16 MiB: a chunk of its own
         Chunk ch = pool_->get(bytes);
         big_[ch.p] = ch;
         return ch.p;
@@ -113,18 +113,18 @@ void* SlabAlloc::alloc(size_t bytes) {
     }
     void* p = fl.back();
     fl.pop_back();
-    live_[p] = c;
     return p;
 }
 
-void SlabAlloc::free(void* p) {
+void SlabAlloc::free(void* p, size_t bytes) {
     if (!p) return;
-    auto b = big_.find(p);
-    if (b != big_.end()) { pool_->put(b->second); big_.erase(b); return; }
-    auto it = live_.find(p);
-    if (it == live_.end()) return;
-    free_[it->second].push_back(p);
-    live_.erase(it);
+    const uint32_t c = size_class(bytes, kMinClass);
+    if (c > kMaxClass) {
+        auto b = big_.find(p);
+        if (b != big_.end()) { pool_->put(b->second); big_.erase(b); }
+        return;
+    }
+    free_[c].push_back(p);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hgx.h"
@@ -79,14 +80,14 @@ public:
     explicit SlabAlloc(ChunkPool* pool) : pool_(pool) {}
     ~SlabAlloc();
     void* alloc(size_t bytes);
-    void free(void* p);
+    void free(void* p, size_t bytes);                 // bytes: as passed to alloc
 
 private:
+    static constexpr uint32_t kMinClass = 12, kMaxClass = 24;   // 4 KiB .. 16 MiB blocks
     ChunkPool* pool_;
     std::vector<Chunk> slabs_;
-    std::map<uint32_t, std::vector<void*>> free_;     // class -> free blocks
-    std::map<void*, uint32_t> live_;                  // block -> class
-    std::map<void*, Chunk> big_;                      // blocks above the largest class
+    std::vector<void*> free_[kMaxClass + 1];          // class -> free blocks
+    std::unordered_map<void*, Chunk> big_;            // blocks above the largest class
     size_t slab_used_ = 0;
 };
 

@@ -252,6 +252,8 @@ int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64_t cap);
  * (completion - submission) on the served path, in nanoseconds. */
 int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint64_t* latency_ns);
 void hgx_result_release(hgx_result* r);
+/* The same for n results at once (a whole batch; null entries are skipped). */
+void hgx_result_release_many(hgx_result* const* rs, uint32_t n);
 
 /* ---- building blocks, exposed for tests and the match-join workload -------------------- */
 

@@ -218,3 +218,23 @@ def test_branchial_large_parents_exact(ctx, seed, steps, cap):
     res = ctx.evolve(req)
     assert res.counts["branchial_pairs"] > 0
     assert_exact(res, po.evolve(req))
+
+
+def test_result_list_views(ctx):
+    """A batch's ResultList: length, negative and slice indexing, one wrapper per request, and
+    results taken from it stay readable after the list itself is gone."""
+    import gc
+    reqs = [W.batched_run(i, cap=512, flags=REL) for i in range(6)]
+    res = ctx.evolve_batch(reqs)
+    assert len(res) == 6
+    assert res[-1] is res[5] and res[0] is res[0]
+    assert [r.counts for r in res[1:4]] == [res[k].counts for k in (1, 2, 3)]
+    with pytest.raises(IndexError):
+        res[6]
+    assert [int(x) for x in res.counts()["raw_states"]] == [r.counts["raw_states"] for r in res]
+    keep = res[2]
+    want = ctx.evolve(reqs[2])
+    del res
+    gc.collect()
+    assert keep.counts == want.counts
+    assert np.array_equal(keep.states()[0], want.states()[0])
