@@ -340,9 +340,12 @@ uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uin
 
 // W is the team: one wave per state (W = 1, up to four per workgroup, each with its own
 // workspace) or the W waves of a workgroup sharing one state and one workspace (large states,
-// whose workspace would otherwise leave a CU one or two waves).
+// whose workspace would otherwise leave a CU one or two waves). The one-wave emit pass is held to
+// five waves per SIMD (96 registers; measured 2.56 -> 2.35 ms per batched pass, where its
+// unbounded 107 allow four); the count pass already fits five, and a bound of six there spills
+// in the join and measured slower.
 template <bool EMIT, class JoinLane, uint32_t W>
-__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_match(MatchArgs a) {
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 && EMIT ? 5 : 1) void k_match(MatchArgs a) {
     extern __shared__ uint8_t lds[];
     constexpr uint32_t N = Team<W>::N;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
