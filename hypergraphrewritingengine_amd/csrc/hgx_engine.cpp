@@ -70,6 +70,7 @@ void Engine::grow_slots(uint32_t need) {
 uint32_t Engine::take_slot() {
     if (free_slots_.empty()) grow_slots(cap_slots_ * 2);
     const uint32_t s = free_slots_.back();
+    hw_slots_ = std::max(hw_slots_, s + 1);
     free_slots_.pop_back();
     return s;
 }
@@ -130,7 +131,7 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
 
 void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
     if (runs.empty()) return;
-    std::vector<uint64_t> ctr((size_t)cap_slots_ * CTR_N);
+    std::vector<uint64_t> ctr((size_t)hw_slots_ * CTR_N);       // slots ever taken
     d2h(ctr.data(), d_ctr_, ctr.size(), st_);
     HIP_TRY(hipStreamSynchronize(st_));
     const auto now = Clock::now();
@@ -291,13 +292,14 @@ uint32_t Engine::step() {
             launch_classes(false, mcls_count, "k_match_count", match_bytes + n * 4ull);
             scan_u32_to_u64(tmp, cnt, moff, n, st);
             HIP_TRY(launch_run_totals(moff, d_sbeg_, d_send_, cap_slots_, d_tot_, st));
-            std::vector<uint64_t> htot(cap_slots_ + 1);
-            h_edges.assign(cap_slots_, 0);
-            d2h(htot.data(), d_tot_, cap_slots_ + 1, st);
-            d2h(h_edges.data(), d_run_edges_, cap_slots_, st);
+            // per-slot child edges and totals are adjacent ([edges cap][totals cap + 1]): one read
+            std::vector<uint64_t> rb(2ull * cap_slots_ + 1);
+            d2h(rb.data(), d_run_edges_, rb.size(), st);
             t_launch = ms_since(t_step);
             HIP_TRY(hipStreamSynchronize(st));
             t_count = ms_since(t_step);
+            const uint64_t* htot = rb.data() + cap_slots_;
+            h_edges.assign(rb.begin(), rb.begin() + cap_slots_);
             parent_edges = htot[cap_slots_];
             bool masked = false;
             for (uint32_t i = 0; i < nF; ++i) {
@@ -480,7 +482,7 @@ uint32_t Engine::step() {
             h_runs_[r.slot].ev_base = (uint32_t)r.events;
             h_runs_[r.slot].child_begin = cb[i];
         }
-        h2d(d_runs_, h_runs_.data(), cap_slots_, st);
+        h2d(d_runs_, h_runs_.data(), hw_slots_, st);              // slots ever taken
 
         if (M) {
             // Size classes: children grouped by their request's bound on child edges, each class

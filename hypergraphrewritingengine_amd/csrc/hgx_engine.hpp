@@ -91,6 +91,7 @@ private:
     SlabAlloc slab_;
     std::unique_ptr<Arena> slot_arena_;
     uint32_t cap_slots_ = 0;
+    uint32_t hw_slots_ = 0;                   // one past the highest slot ever taken
     DRun* d_runs_ = nullptr;
     uint64_t* d_ctr_ = nullptr;
     uint8_t* d_active_ = nullptr;
