@@ -151,6 +151,12 @@ def config2_line(ctx, steps, cpu_steps, threads, no_cpu):
            "raw_states_per_s": round(c["raw_states"] / wall, 1),
            "canonical_states_per_s": round(c["canonical_states"] / wall, 1),
            "roofline": dominant(stats)}
+    # atomic throughput of the state map: one 64-bit compare-and-swap insert (plus probes) per
+    # child state (one per event), all in k_dedup
+    dd = stats.get("k_dedup")
+    if dd and dd["total_ms"] > 0:
+        out["state_map"] = {"inserts": c["events"], "k_dedup_ms": round(dd["total_ms"], 4),
+                            "inserts_per_s": round(c["events"] / (dd["total_ms"] * 1e-3), 1)}
     if not no_cpu:
         from oracle import pyoracle as po
         if po.ref_available():
@@ -455,6 +461,9 @@ def main():
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(reqs, counts, args.cpu_seconds, threads)
+        # the same engine on one thread, on a shorter sample
+        one = cpu_baseline(reqs, counts, args.cpu_seconds / 3, 1)
+        cpu["single_thread"] = {k: one[k] for k in ("value", "raw_states_per_s", "sample", "count_mismatches")}
         print(f"[bench] cpu: {cpu}", file=sys.stderr, flush=True)
 
     line = {
