@@ -642,9 +642,9 @@ uint32_t Engine::step() {
                 ba.pcnt = pcnt;
                 timed(ctx_, st, "k_branchial", M * (6ull + 2ull * K) + n * 12ull, n,
                       [&] { return launch_branchial(ba, ctx_->num_cu * 8, 2, st); });
-                // parents above 64 edges: (edge slot, match) keys sorted into per-edge lists
+                // parents above BR_MAX_EDGES edges: (edge slot, match) keys sorted into per-edge lists
                 BrListArgs bl{};
-                const bool large = F_->max_ne > 64;
+                const bool large = F_->max_ne > BR_MAX_EDGES;
                 if (large) {
                     const uint64_t E = F_->n_edges, NK = M * K;
                     if (E >= 0xFFFFFFFFull) throw Error{HGX_ERR_UNSUPPORTED, "branchial lists above 2^32 edges"};

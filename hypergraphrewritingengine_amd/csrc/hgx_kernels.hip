@@ -890,8 +890,8 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
 }
 
 // ------------------------------------------------------------------------------------------
-// Branchial pairs of parents above 64 edges. Every (consumed edge slot, match) of such a parent
-// becomes one 64-bit key (slot << 32 | match); sorted, the keys of one slot are its consuming
+// Branchial pairs of parents above BR_MAX_EDGES edges. Every (consumed edge slot, match) of such
+// a parent becomes one 64-bit key (slot << 32 | match); sorted, the keys of one slot are its consuming
 // matches in ascending order. Match i meets each partner j > i through the first of its own
 // edges that j also consumes: from its key for that edge, the keys after it with the same slot.
 // The work is the entries plus the sum over edges of (matches consuming it)^2 -- not the
@@ -900,7 +900,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
 
 __device__ __forceinline__ bool brl_large(const BrListArgs& a, uint32_t m, uint32_t& par) {
     par = a.M.parent[m];
-    return a.P.ne[par] > 64 && a.cnt[par] >= 2;
+    return a.P.ne[par] > BR_MAX_EDGES && a.cnt[par] >= 2;
 }
 
 // Keys and payloads of match m at entries [m*K, m*K + K): (slot << 32 | m, m*K + k) for its
@@ -1009,103 +1009,140 @@ hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr,
 // Branchial pairs: two events of the SAME input state whose consumed edge sets intersect,
 // one pair each (causal_graph.cpp:324-351). One wave per parent.
 //
-// Parents of at most 64 edges: the parent's matches are taken in chunks of BR_CHUNK; for each
-// chunk an LDS bitset per edge position marks the chunk's matches that consume it. Match i
-// then meets every later match j of the chunk through the OR of the bitsets of its own edges:
-// one popcount per 64 candidates instead of one test per pair. Larger parents take the
+// Parents of at most 128 edges: the parent's matches are taken in chunks; for each chunk an
+// LDS bitset per edge position marks the chunk's matches that consume it. Match i then meets
+// every later match j of the chunk through the OR of the bitsets of its own edges: one popcount
+// per 64 candidates instead of one test per pair. A consumed set is NW 64-bit words (one up to
+// 64 edges, two up to 128); the bitsets and the cached consumed sets take the same 16 KiB of
+// LDS either way, so a two-word parent works in half-size chunks. Larger parents take the
 // per-edge match lists above (k_brl_*).
 // ------------------------------------------------------------------------------------------
 
-constexpr uint32_t BR_WORDS = 16;                  // 64-bit words per edge bitset
-constexpr uint32_t BR_CHUNK = BR_WORDS * 64;        // matches per chunk
-constexpr uint32_t BR_MASKS = 1024;                 // consumed masks cached in LDS
+constexpr uint32_t BR_LDS_WORDS = 1024;             // per table: 8 KiB of bitsets, 8 KiB of masks
 
-uint32_t branchial_lds_per_wave() { return 64 * BR_WORDS * 8 + BR_MASKS * 8 + 64; }
+uint32_t branchial_lds_per_wave() { return 2 * BR_LDS_WORDS * 8 + 64; }
 
-__device__ __forceinline__ uint64_t consumed_mask(const BranchialArgs& a, const DRun& RN, uint32_t m) {
+template <uint32_t NW>
+__device__ __forceinline__ void consumed_mask(const BranchialArgs& a, const DRun& RN, uint32_t m,
+                                              uint64_t (&msk)[NW]) {
     const uint32_t nl = RN.rules[a.M.rule[m]].n_lhs;
     const uint16_t* q = a.M.q + (size_t)m * a.M.K;
-    uint64_t msk = 0;
-    for (uint32_t t = 0; t < nl; ++t) msk |= 1ull << q[t];
-    return msk;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) msk[w] = 0;
+    for (uint32_t t = 0; t < nl; ++t) {
+        const uint32_t e = q[t];
+#pragma unroll
+        for (uint32_t w = 0; w < NW; ++w)
+            if ((e >> 6) == w) msk[w] |= 1ull << (e & 63);
+    }
+}
+
+// One parent (cnt >= 2 matches from m0): its pairs counted (emit 0) or written (emit 1).
+template <uint32_t NW>
+__device__ __forceinline__ uint32_t branchial_parent(const BranchialArgs& a, const DRun& RN, uint32_t m0,
+                                                     uint32_t cnt, uint32_t ev0, uint64_t out, uint8_t* my,
+                                                     uint32_t l) {
+    constexpr uint32_t WORDS = 16 / NW;              // 64-bit words per edge bitset
+    constexpr uint32_t CHUNK = WORDS * 64;           // matches per chunk
+    constexpr uint32_t MASKS = BR_LDS_WORDS / NW;    // consumed sets cached in LDS
+    uint64_t* bits = (uint64_t*)my;                  // [64 * NW edges][WORDS]
+    uint64_t* mcache = bits + BR_LDS_WORDS;          // [MASKS][NW]
+    uint32_t total = 0;
+    const uint32_t ncache = cnt < MASKS ? cnt : MASKS;
+    for (uint32_t i = l; i < ncache; i += WAVE) {
+        uint64_t mk[NW];
+        consumed_mask<NW>(a, RN, m0 + i, mk);
+#pragma unroll
+        for (uint32_t w = 0; w < NW; ++w) mcache[i * NW + w] = mk[w];
+    }
+    wave_sync();
+    auto mask_of = [&](uint32_t i, uint64_t (&mk)[NW]) {
+        if (i < MASKS) {
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) mk[w] = mcache[i * NW + w];
+        } else {
+            consumed_mask<NW>(a, RN, m0 + i, mk);
+        }
+    };
+    for (uint32_t cb = 0; cb < cnt; cb += CHUNK) {
+        const uint32_t ce = min(cnt, cb + CHUNK);
+        for (uint32_t x = l; x < BR_LDS_WORDS; x += WAVE) bits[x] = 0;
+        wave_sync();
+        for (uint32_t j = cb + l; j < ce; j += WAVE) {
+            const uint32_t r = j - cb;
+            uint64_t mj[NW];
+            mask_of(j, mj);
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w)
+                for (uint64_t mk = mj[w]; mk; mk &= mk - 1)
+                    atomicOr((unsigned long long*)&bits[(w * 64 + __builtin_ctzll(mk)) * WORDS + (r >> 6)],
+                             1ull << (r & 63));
+        }
+        wave_sync();
+        const uint32_t words = (ce - cb + 63) >> 6;
+        for (uint32_t base = 0; base < ce; base += WAVE) {
+            const uint32_t i = base + l;
+            uint32_t c = 0;
+            uint64_t mi[NW];
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) mi[w] = 0;
+            auto partners = [&](uint32_t wd) {         // later matches of word wd sharing an edge with i
+                const uint32_t jb = cb + wd * 64;
+                uint64_t word = 0;
+#pragma unroll
+                for (uint32_t w = 0; w < NW; ++w)
+                    for (uint64_t mk = mi[w]; mk; mk &= mk - 1)
+                        word |= bits[(w * 64 + __builtin_ctzll(mk)) * WORDS + wd];
+                if (jb <= i) word &= ~((2ull << (i - jb)) - 1ull);
+                return word;
+            };
+            if (i < ce) {
+                mask_of(i, mi);
+                for (uint32_t wd = 0; wd < words; ++wd) {
+                    if (cb + wd * 64 + 63 <= i) continue;
+                    c += __popcll(partners(wd));
+                }
+            }
+            if (a.emit) {
+                const uint32_t inc = wave_incl_scan_u32(c);
+                uint64_t o = out + inc - c;
+                if (c) {
+                    for (uint32_t wd = 0; wd < words; ++wd) {
+                        const uint32_t jb = cb + wd * 64;
+                        if (jb + 63 <= i) continue;
+                        for (uint64_t word = partners(wd); word; word &= word - 1) {
+                            a.pairs[2 * o] = ev0 + i;
+                            a.pairs[2 * o + 1] = ev0 + jb + __builtin_ctzll(word);
+                            ++o;
+                        }
+                    }
+                }
+                out += __shfl(inc, WAVE - 1, WAVE);
+            } else {
+                total += c;
+            }
+        }
+        wave_sync();
+    }
+    return total;
 }
 
 __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
     extern __shared__ uint8_t lds[];
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
     uint8_t* my = lds + (size_t)w * a.lds_per_wave;
-    uint64_t* bits = (uint64_t*)my;                      // [64][BR_WORDS]
-    uint64_t* mcache = bits + 64 * BR_WORDS;             // [BR_MASKS]
     const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
     for (uint32_t s = blockIdx.x * (blockDim.x / WAVE) + w; s < a.P.n; s += waves) {
         const uint32_t cnt = a.cnt[s];
-        if (cnt < 2 || a.P.ne[s] > 64) { if (a.emit == 0 && a.pcnt && l == 0) a.pcnt[s] = 0; continue; }
+        const uint32_t ne = a.P.ne[s];
+        if (cnt < 2 || ne > BR_MAX_EDGES) { if (a.emit == 0 && a.pcnt && l == 0) a.pcnt[s] = 0; continue; }
         const uint32_t run = a.P.run[s];
         const uint32_t m0 = (uint32_t)a.moff[s];
-        const uint32_t ne = a.P.ne[s];
         const DRun& RN = a.runs[run];
         const uint32_t ev0 = RN.ev_base + (m0 - RN.child_begin);
-        uint64_t out = a.emit ? a.poff[s] : 0;
-        uint32_t total = 0;
-        {
-            const uint32_t ncache = cnt < BR_MASKS ? cnt : BR_MASKS;
-            for (uint32_t i = l; i < ncache; i += WAVE) mcache[i] = consumed_mask(a, RN, m0 + i);
-            wave_sync();
-            auto mask_of = [&](uint32_t i) -> uint64_t {
-                return i < BR_MASKS ? mcache[i] : consumed_mask(a, RN, m0 + i);
-            };
-            for (uint32_t cb = 0; cb < cnt; cb += BR_CHUNK) {
-                const uint32_t ce = min(cnt, cb + BR_CHUNK);
-                for (uint32_t x = l; x < 64 * BR_WORDS; x += WAVE) bits[x] = 0;
-                wave_sync();
-                for (uint32_t j = cb + l; j < ce; j += WAVE) {
-                    const uint32_t r = j - cb;
-                    for (uint64_t mk = mask_of(j); mk; mk &= mk - 1)
-                        atomicOr((unsigned long long*)&bits[__builtin_ctzll(mk) * BR_WORDS + (r >> 6)],
-                                 1ull << (r & 63));
-                }
-                wave_sync();
-                const uint32_t words = (ce - cb + 63) >> 6;
-                for (uint32_t base = 0; base < ce; base += WAVE) {
-                    const uint32_t i = base + l;
-                    uint32_t c = 0;
-                    uint64_t mi = 0;
-                    if (i < ce) {
-                        mi = mask_of(i);
-                        for (uint32_t wd = 0; wd < words; ++wd) {
-                            const uint32_t jb = cb + wd * 64;
-                            if (jb + 63 <= i) continue;
-                            uint64_t word = 0;
-                            for (uint64_t mk = mi; mk; mk &= mk - 1) word |= bits[__builtin_ctzll(mk) * BR_WORDS + wd];
-                            if (jb <= i) word &= ~((2ull << (i - jb)) - 1ull);
-                            c += __popcll(word);
-                        }
-                    }
-                    if (a.emit) {
-                        const uint32_t inc = wave_incl_scan_u32(c);
-                        uint64_t o = out + inc - c;
-                        if (c) {
-                            for (uint32_t wd = 0; wd < words; ++wd) {
-                                const uint32_t jb = cb + wd * 64;
-                                if (jb + 63 <= i) continue;
-                                uint64_t word = 0;
-                                for (uint64_t mk = mi; mk; mk &= mk - 1) word |= bits[__builtin_ctzll(mk) * BR_WORDS + wd];
-                                if (jb <= i) word &= ~((2ull << (i - jb)) - 1ull);
-                                for (; word; word &= word - 1) {
-                                    a.pairs[2 * o] = ev0 + i;
-                                    a.pairs[2 * o + 1] = ev0 + jb + __builtin_ctzll(word);
-                                    ++o;
-                                }
-                            }
-                        }
-                        out += __shfl(inc, WAVE - 1, WAVE);
-                    } else {
-                        total += c;
-                    }
-                }
-                wave_sync();
-            }
-        }
+        const uint64_t out = a.emit ? a.poff[s] : 0;
+        const uint32_t total = ne <= 64 ? branchial_parent<1>(a, RN, m0, cnt, ev0, out, my, l)
+                                        : branchial_parent<2>(a, RN, m0, cnt, ev0, out, my, l);
         if (!a.emit) {
             const uint32_t tot = wave_sum_u32(total);
             if (l == 0) {

@@ -168,6 +168,9 @@ hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr,
 // grid used by the launchers for `items` work items (for sizing global scratch)
 uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap);
 uint32_t branchial_lds_per_wave();
+// Parents above this many edges have their branchial pairs found through sorted per-edge match
+// lists (launch_brl); smaller ones through LDS bitsets (launch_branchial).
+constexpr uint32_t BR_MAX_EDGES = 128;
 hipError_t launch_branchial(const BranchialArgs& a, uint32_t grid_cap, uint32_t wpb, hipStream_t st);
 uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A, bool lean);
 // team: waves per state (1, 2, 4 or 8); with team 1, wpb states per workgroup.

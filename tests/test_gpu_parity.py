@@ -203,14 +203,15 @@ def test_prepared_batch_equals_host_batch(ctx):
     prep.close()
 
 
-@pytest.mark.parametrize("seed,steps,cap", [(0, 2, 0), (1, 2, 4000), (2, 1, 0)])
-def test_branchial_large_parents_exact(ctx, seed, steps, cap):
-    """Parents above 64 edges take the per-edge match-list path for branchial pairs: exact
-    pairs against the oracle, with hyperedges, repeated edges and a two-edge rule whose
-    matches share both edges in reverse order."""
+@pytest.mark.parametrize("seed,n,nv,steps,cap", [(0, 70, 40, 2, 0), (1, 110, 40, 2, 4000), (2, 150, 40, 1, 0),
+                                                (3, 58, 80, 3, 0), (4, 125, 40, 2, 0)])
+def test_branchial_large_parents_exact(ctx, seed, n, nv, steps, cap):
+    """Branchial pairs of parents past one 64-bit consumed set: LDS bitsets with two-word sets up
+    to 128 edges, the per-edge match-list path above, and runs whose states cross 64 and 128
+    edges between generations. Exact pairs against the oracle, with hyperedges, repeated edges
+    and a two-edge rule whose matches share both edges in reverse order."""
     rnd = random.Random(seed)
-    n = 70 + 40 * seed
-    init = [[rnd.randrange(40), rnd.randrange(40)] for _ in range(n)]
+    init = [[rnd.randrange(nv), rnd.randrange(nv)] for _ in range(n)]
     init += [[3, 4, 5], [5, 4, 3], [7, 7]]
     rules = [rule_from_lists([[0, 1], [0, 2]], [[0, 2], [0, 3], [1, 3], [2, 3]], 0),
              rule_from_lists([[0, 1, 2]], [[0, 1], [1, 2]], 1)]
