@@ -1040,8 +1040,8 @@ __device__ __forceinline__ void consumed_mask(const BranchialArgs& a, const DRun
 // One parent (cnt >= 2 matches from m0): its pairs counted (emit 0) or written (emit 1).
 template <uint32_t NW>
 __device__ __forceinline__ uint32_t branchial_parent(const BranchialArgs& a, const DRun& RN, uint32_t m0,
-                                                     uint32_t cnt, uint32_t ev0, uint64_t out, uint8_t* my,
-                                                     uint32_t l) {
+                                                     uint32_t cnt, uint32_t ne, uint32_t ev0, uint64_t out,
+                                                     uint8_t* my, uint32_t l) {
     constexpr uint32_t WORDS = 16 / NW;              // 64-bit words per edge bitset
     constexpr uint32_t CHUNK = WORDS * 64;           // matches per chunk
     constexpr uint32_t MASKS = BR_LDS_WORDS / NW;    // consumed sets cached in LDS
@@ -1066,7 +1066,12 @@ __device__ __forceinline__ uint32_t branchial_parent(const BranchialArgs& a, con
     };
     for (uint32_t cb = 0; cb < cnt; cb += CHUNK) {
         const uint32_t ce = min(cnt, cb + CHUNK);
-        for (uint32_t x = l; x < BR_LDS_WORDS; x += WAVE) bits[x] = 0;
+        const uint32_t words = (ce - cb + 63) >> 6;
+        // only the words this chunk reads: the parent's edges x the chunk's 64-match words
+        for (uint32_t x = l; x < ne * words; x += WAVE) {
+            const uint32_t e = x / words;
+            bits[e * WORDS + (x - e * words)] = 0;
+        }
         wave_sync();
         for (uint32_t j = cb + l; j < ce; j += WAVE) {
             const uint32_t r = j - cb;
@@ -1079,7 +1084,6 @@ __device__ __forceinline__ uint32_t branchial_parent(const BranchialArgs& a, con
                              1ull << (r & 63));
         }
         wave_sync();
-        const uint32_t words = (ce - cb + 63) >> 6;
         for (uint32_t base = 0; base < ce; base += WAVE) {
             const uint32_t i = base + l;
             uint32_t c = 0;
@@ -1141,8 +1145,8 @@ __global__ __launch_bounds__(256) void k_branchial(BranchialArgs a) {
         const DRun& RN = a.runs[run];
         const uint32_t ev0 = RN.ev_base + (m0 - RN.child_begin);
         const uint64_t out = a.emit ? a.poff[s] : 0;
-        const uint32_t total = ne <= 64 ? branchial_parent<1>(a, RN, m0, cnt, ev0, out, my, l)
-                                        : branchial_parent<2>(a, RN, m0, cnt, ev0, out, my, l);
+        const uint32_t total = ne <= 64 ? branchial_parent<1>(a, RN, m0, cnt, ne, ev0, out, my, l)
+                                        : branchial_parent<2>(a, RN, m0, cnt, ne, ev0, out, my, l);
         if (!a.emit) {
             const uint32_t tot = wave_sum_u32(total);
             if (l == 0) {
