@@ -238,6 +238,8 @@ uint32_t Engine::step() {
             ma.pm_moff = prev_.moff;
             ma.KL = KL;
             ma.small = small ? 1u : 0u;
+            // small rules: the count pass keeps each seed's match count for the emit pass
+            ma.seedcnt = small ? dalloc<uint16_t>(tmp, 2 * std::max<uint64_t>(1, F_->n_edges)) : nullptr;
             ma.dmax = fwd ? std::min(ctx_->delta_cap, 16384u / (2u * KL + 8u)) : 0u;   // <= 16 KiB LDS
             ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
             ma.budget = d_budget_;

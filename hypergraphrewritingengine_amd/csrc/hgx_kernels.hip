@@ -340,12 +340,11 @@ uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uin
 
 // W is the team: one wave per state (W = 1, up to four per workgroup, each with its own
 // workspace) or the W waves of a workgroup sharing one state and one workspace (large states,
-// whose workspace would otherwise leave a CU one or two waves). The one-wave emit pass is held to
-// five waves per SIMD (96 registers; measured 2.56 -> 2.35 ms per batched pass, where its
-// unbounded 107 allow four); the count pass already fits five, and a bound of six there spills
-// in the join and measured slower.
+// whose workspace would otherwise leave a CU one or two waves). The one-wave passes are held to
+// five waves per SIMD (96 registers; the emit pass measured 2.56 -> 2.35 ms per batched pass,
+// where its unbounded 107 allow four); a bound of six spills in the join and measured slower.
 template <bool EMIT, class JoinLane, uint32_t W>
-__global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 && EMIT ? 5 : 1) void k_match(MatchArgs a) {
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match(MatchArgs a) {
     extern __shared__ uint8_t lds[];
     constexpr uint32_t N = Team<W>::N;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
@@ -612,13 +611,20 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 && EMIT ? 5 : 1) void
             }
             tm.sync();
             // EXPAND + SINK, 64 seeds per round.
+            uint16_t* sc = (SMALL && a.seedcnt && s >= a.fwd_n) ? a.seedcnt + 2 * eo + (size_t)r * ne : nullptr;
             for (uint32_t base = 0; base < nseed; base += N) {
                 const uint32_t i = base + tm.r;
                 uint32_t c = 0;
                 if (i < nseed) {
-                    if constexpr (SMALL) c = reg_join_any<false>(R, sv, sa, ne, A, iend, ie, seeds[i], RegJoinOut{});
-                    else c = join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
-                                       nullptr, 0, 0, 0);
+                    if constexpr (SMALL) {
+                        const uint32_t kept = (EMIT && sc) ? sc[i] : 0xFFFFu;
+                        c = kept != 0xFFFFu ? kept
+                                            : reg_join_any<false>(R, sv, sa, ne, A, iend, ie, seeds[i], RegJoinOut{});
+                        if (!EMIT && sc) sc[i] = (uint16_t)(c < 0xFFFFu ? c : 0xFFFFu);
+                    } else {
+                        c = join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
+                                      nullptr, 0, 0, 0);
+                    }
                 }
                 if (EMIT) {
                     uint32_t round;

@@ -42,6 +42,11 @@ struct MatchArgs {
     const uint32_t* rpre;
     uint32_t nrange;
     uint32_t n_items;
+    // Small rules: the count pass's matches per seed of each fully joined state, kept for the
+    // emit pass (which then joins each seed once, not twice): [2 * P's edges], state s's seeds
+    // of rule r at 2 * eoff[s] + r * ne[s]; 0xFFFF means "at least that many, join again".
+    // Null: every seed is counted in the emit pass too.
+    uint16_t* seedcnt;
 };
 
 struct ExpandArgs {
