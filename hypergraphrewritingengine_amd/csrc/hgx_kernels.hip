@@ -380,6 +380,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
     const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;      // teams per workgroup
     const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
+    unsigned long long parent_edges = 0;                       // count pass: edges of parents with matches
     for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? w : 0); ls < n_items; ls += teams) {
         uint32_t s = ls;
         if (a.nrange) {                                         // subset: logical -> state
@@ -662,10 +663,12 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
         }
         if (!EMIT && tm.leader()) {
             a.cnt[s] = total;
-            if (total && a.parent_edges) atomicAdd(a.parent_edges, (unsigned long long)ne);
             if (edges_sum) atomicAdd(&a.run_edges[run], (unsigned long long)edges_sum);
         }
+        if (!EMIT && total) parent_edges += ne;
     }
+    // one add per team, not one per state: every team would otherwise queue on this one address
+    if (!EMIT && tm.leader() && parent_edges && a.parent_edges) atomicAdd(a.parent_edges, parent_edges);
 }
 
 
