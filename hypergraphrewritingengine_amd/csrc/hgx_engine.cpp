@@ -244,7 +244,7 @@ uint32_t Engine::step() {
             ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
             ma.budget = d_budget_;
             ma.run_acc = (unsigned long long*)d_run_acc_;
-            ma.run_edges = (unsigned long long*)d_run_edges_;
+            ma.state_edges = dalloc<uint64_t>(tmp, n);
             const uint64_t match_bytes = F_->n_edges * (4ull * F_->v.A + 1) + (uint64_t)n * 24;
             // One launch per size class of the requests taking part (by their bound on state
             // edges), each with LDS staging sized for its class; states outside the launches
@@ -293,7 +293,7 @@ uint32_t Engine::step() {
             HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
             launch_classes(false, mcls_count, "k_match_count", match_bytes + n * 4ull);
             scan_u32_to_u64(tmp, cnt, moff, n, st);
-            HIP_TRY(launch_run_totals(moff, d_sbeg_, d_send_, cap_slots_, d_tot_, st));
+            HIP_TRY(launch_run_totals(moff, ma.state_edges, d_sbeg_, d_send_, cap_slots_, d_tot_, d_run_edges_, st));
             // per-slot child edges and totals are adjacent ([edges cap][totals cap + 1]): one read
             std::vector<uint64_t> rb(2ull * cap_slots_ + 1);
             d2h(rb.data(), d_run_edges_, rb.size(), st);

@@ -398,7 +398,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
         // the run's tally moves under other teams: one read, shared, so the team agrees
         const uint64_t tally = EMIT ? 0 : tm.bcast(tm.leader() ? (uint64_t)a.run_acc[run] : 0ull);
         if (!active || (!EMIT && tally > budget)) {
-            if (!EMIT && tm.leader()) a.cnt[s] = 0;
+            if (!EMIT && tm.leader()) { a.cnt[s] = 0; a.state_edges[s] = 0; }
             continue;
         }
         for (uint32_t i = tm.r; i < ne * A; i += N) sv[i] = a.P.ev[eo * A + i];
@@ -663,7 +663,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
         }
         if (!EMIT && tm.leader()) {
             a.cnt[s] = total;
-            if (edges_sum) atomicAdd(&a.run_edges[run], (unsigned long long)edges_sum);
+            a.state_edges[s] = edges_sum;
         }
         if (!EMIT && total) parent_edges += ne;
     }
@@ -1260,10 +1260,20 @@ hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v
 }
 
 // Per-run match totals of a generation: runs' states are contiguous ranges.
-__global__ void k_run_totals(const uint64_t* moff, const uint32_t* run_sbeg, const uint32_t* run_send,
-                             uint32_t nruns, uint64_t* tot) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nruns) tot[r] = moff[run_send[r]] - moff[run_sbeg[r]];
+// One wave per run: its matches from the offsets, its children's edges summed over its states
+// (a per-state store and this sum instead of every state adding to the run's counter).
+__global__ void k_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* run_sbeg,
+                             const uint32_t* run_send, uint32_t nruns, uint64_t* tot, uint64_t* edges) {
+    const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, l = lane_id();
+    if (r >= nruns) return;
+    const uint32_t b = run_sbeg[r], e = run_send[r];
+    uint64_t sum = 0;
+    for (uint32_t s = b + l; s < e; s += WAVE) sum += state_edges[s];
+    sum = wave_sum_u64(sum);
+    if (l == 0) {
+        tot[r] = moff[e] - moff[b];
+        edges[r] = sum;
+    }
 }
 
 // Zero the counts of states whose run does not advance this generation.
@@ -1397,9 +1407,11 @@ hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t w
     return hipGetLastError();
 }
 
-hipError_t launch_run_totals(const uint64_t* moff, const uint32_t* b, const uint32_t* e, uint32_t nruns,
-                             uint64_t* tot, hipStream_t st) {
-    hipLaunchKernelGGL(k_run_totals, dim3((nruns + 255) / 256), dim3(256), 0, st, moff, b, e, nruns, tot);
+hipError_t launch_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* b, const uint32_t* e,
+                             uint32_t nruns, uint64_t* tot, uint64_t* edges, hipStream_t st) {
+    if (nruns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_run_totals, dim3((nruns + 3) / 4), dim3(256), 0, st, moff, state_edges, b, e, nruns, tot,
+                       edges);
     return hipGetLastError();
 }
 
