@@ -41,15 +41,17 @@ void Engine::grow_slots(uint32_t need) {
     DRun* runs = dalloc<DRun>(*na, cap);
     uint64_t* ctr = dalloc<uint64_t>(*na, (size_t)cap * CTR_N);
     // per-slot tables uploaded together each generation: budget, range begin, range end (u32),
-    // active (u8); and the ones zeroed together: accumulated matches, child edges, totals
+    // active (u8); and the ones zeroed together: accumulated matches, child edges, totals, the
+    // edges of the states with matches
     uint8_t* tab = dalloc<uint8_t>(*na, 13ull * cap);
     uint32_t* budget = (uint32_t*)tab;
     uint32_t* sb = budget + cap;
     uint32_t* se = sb + cap;
     uint8_t* active = (uint8_t*)(se + cap);
-    uint64_t* acc = dalloc<uint64_t>(*na, 3ull * cap + 1);
+    uint64_t* acc = dalloc<uint64_t>(*na, 4ull * cap + 1);
     uint64_t* redges = acc + cap;
     uint64_t* tot = redges + cap;                     // [cap + 1]
+    uint64_t* rpe = tot + cap + 1;
     HIP_TRY(hipMemsetAsync(ctr, 0, sizeof(uint64_t) * cap * CTR_N, st_));
     if (cap_slots_)
         HIP_TRY(hipMemcpyAsync(ctr, d_ctr_, sizeof(uint64_t) * cap_slots_ * CTR_N, hipMemcpyDeviceToDevice, st_));
@@ -63,7 +65,7 @@ void Engine::grow_slots(uint32_t need) {
     slot_arena_ = std::move(na);
     d_runs_ = runs; d_ctr_ = ctr; d_active_ = active; d_budget_ = budget; d_run_acc_ = acc;
     d_run_edges_ = redges;
-    d_sbeg_ = sb; d_send_ = se; d_tot_ = tot;
+    d_sbeg_ = sb; d_send_ = se; d_tot_ = tot; d_run_pe_ = rpe;
     cap_slots_ = cap;
 }
 
@@ -220,7 +222,7 @@ uint32_t Engine::step() {
         if (any) {
             std::memcpy(h_tab_.data() + 12ull * cap_slots_, h_active.data(), cap_slots_);
             h2d((uint8_t*)d_budget_, h_tab_.data(), h_tab_.size(), st);    // budget, sbeg, send, active
-            HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * (3ull * cap_slots_ + 1), st));
+            HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * (4ull * cap_slots_ + 1), st));
             cnt = dalloc<uint32_t>(tmp, n + 1);
             moff = dalloc<uint64_t>(*keep, n + 1);
             if (F_->max_ne > 65535)
@@ -241,7 +243,6 @@ uint32_t Engine::step() {
             // small rules: the count pass keeps each seed's match count for the emit pass
             ma.seedcnt = small ? dalloc<uint16_t>(tmp, 2 * std::max<uint64_t>(1, F_->n_edges)) : nullptr;
             ma.dmax = fwd ? std::min(ctx_->delta_cap, 16384u / (2u * KL + 8u)) : 0u;   // <= 16 KiB LDS
-            ma.parent_edges = (unsigned long long*)(d_tot_ + cap_slots_);
             ma.budget = d_budget_;
             ma.run_acc = (unsigned long long*)d_run_acc_;
             ma.state_edges = dalloc<uint64_t>(tmp, n);
@@ -295,16 +296,18 @@ uint32_t Engine::step() {
             HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
             launch_classes(false, mcls_count, "k_match_count", match_bytes + n * 4ull);
             scan_u32_to_u64(tmp, cnt, moff, n, st);
-            HIP_TRY(launch_run_totals(moff, ma.state_edges, d_sbeg_, d_send_, cap_slots_, d_tot_, d_run_edges_, st));
-            // per-slot child edges and totals are adjacent ([edges cap][totals cap + 1]): one read
-            std::vector<uint64_t> rb(2ull * cap_slots_ + 1);
+            HIP_TRY(launch_run_totals(moff, ma.state_edges, cnt, F_->v.ne, d_sbeg_, d_send_, cap_slots_, d_tot_,
+                                      d_run_edges_, d_run_pe_, st));
+            // per-slot child edges, totals and parent edges are adjacent
+            // ([edges cap][totals cap + 1][parent edges cap]): one read
+            std::vector<uint64_t> rb(3ull * cap_slots_ + 1);
             d2h(rb.data(), d_run_edges_, rb.size(), st);
             t_launch = ms_since(t_step);
             HIP_TRY(hipStreamSynchronize(st));
             t_count = ms_since(t_step);
             const uint64_t* htot = rb.data() + cap_slots_;
+            const uint64_t* hpe = htot + cap_slots_ + 1;
             h_edges.assign(rb.begin(), rb.begin() + cap_slots_);
-            parent_edges = htot[cap_slots_];
             bool masked = false;
             for (uint32_t i = 0; i < nF; ++i) {
                 RunRec& r = *F_runs_[i];
@@ -316,6 +319,7 @@ uint32_t Engine::step() {
                     r.done = true; h_active[r.slot] = 0;
                 } else {
                     tot[i] = t;
+                    parent_edges += hpe[r.slot];          // parents the expansion reads
                     for (auto& R : r.hp->rules) K = std::max<uint32_t>(K, R.n_lhs);
                 }
             }
@@ -353,7 +357,6 @@ uint32_t Engine::step() {
                 mv.cne = dalloc<uint32_t>(tmp, M + 1);
                 HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
                 ma.M = mv;
-                ma.parent_edges = nullptr;
                 launch_classes(true, mcls_emit, "k_match_emit", match_bytes + M * (12ull + 2ull * K));
             }
         }

@@ -101,6 +101,7 @@ private:
     uint32_t* d_sbeg_ = nullptr;
     uint32_t* d_send_ = nullptr;
     uint64_t* d_tot_ = nullptr;               // [cap_slots + 1]
+    uint64_t* d_run_pe_ = nullptr;            // [cap_slots] edges of each run's states with matches
     std::vector<DRun> h_runs_;
     std::vector<uint8_t> h_tab_;              // host side of budget, sbeg, send, active
     std::vector<uint32_t> free_slots_;

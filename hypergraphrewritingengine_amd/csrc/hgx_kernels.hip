@@ -380,7 +380,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
     const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;      // teams per workgroup
     const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
-    unsigned long long parent_edges = 0;                       // count pass: edges of parents with matches
     for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? w : 0); ls < n_items; ls += teams) {
         uint32_t s = ls;
         if (a.nrange) {                                         // subset: logical -> state
@@ -665,10 +664,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
             a.cnt[s] = total;
             a.state_edges[s] = edges_sum;
         }
-        if (!EMIT && total) parent_edges += ne;
     }
-    // one add per team, not one per state: every team would otherwise queue on this one address
-    if (!EMIT && tm.leader() && parent_edges && a.parent_edges) atomicAdd(a.parent_edges, parent_edges);
 }
 
 
@@ -1260,19 +1256,26 @@ hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v
 }
 
 // Per-run match totals of a generation: runs' states are contiguous ranges.
-// One wave per run: its matches from the offsets, its children's edges summed over its states
-// (a per-state store and this sum instead of every state adding to the run's counter).
-__global__ void k_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* run_sbeg,
-                             const uint32_t* run_send, uint32_t nruns, uint64_t* tot, uint64_t* edges) {
+// One wave per run: its matches from the offsets; its children's edges and the edges of its
+// states that have matches (the parents its expansion reads), summed over its states -- a
+// per-state store and these sums instead of every state adding to shared counters.
+__global__ void k_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* cnt,
+                             const uint32_t* ne, const uint32_t* run_sbeg, const uint32_t* run_send,
+                             uint32_t nruns, uint64_t* tot, uint64_t* edges, uint64_t* parent_edges) {
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, l = lane_id();
     if (r >= nruns) return;
     const uint32_t b = run_sbeg[r], e = run_send[r];
-    uint64_t sum = 0;
-    for (uint32_t s = b + l; s < e; s += WAVE) sum += state_edges[s];
+    uint64_t sum = 0, pe = 0;
+    for (uint32_t s = b + l; s < e; s += WAVE) {
+        sum += state_edges[s];
+        pe += cnt[s] ? ne[s] : 0u;
+    }
     sum = wave_sum_u64(sum);
+    pe = wave_sum_u64(pe);
     if (l == 0) {
         tot[r] = moff[e] - moff[b];
         edges[r] = sum;
+        parent_edges[r] = pe;
     }
 }
 
@@ -1407,11 +1410,12 @@ hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t w
     return hipGetLastError();
 }
 
-hipError_t launch_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* b, const uint32_t* e,
-                             uint32_t nruns, uint64_t* tot, uint64_t* edges, hipStream_t st) {
+hipError_t launch_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* cnt,
+                             const uint32_t* ne, const uint32_t* b, const uint32_t* e, uint32_t nruns, uint64_t* tot,
+                             uint64_t* edges, uint64_t* parent_edges, hipStream_t st) {
     if (nruns == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_run_totals, dim3((nruns + 3) / 4), dim3(256), 0, st, moff, state_edges, b, e, nruns, tot,
-                       edges);
+    hipLaunchKernelGGL(k_run_totals, dim3((nruns + 3) / 4), dim3(256), 0, st, moff, state_edges, cnt, ne, b, e, nruns,
+                       tot, edges, parent_edges);
     return hipGetLastError();
 }
 

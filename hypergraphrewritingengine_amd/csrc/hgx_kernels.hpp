@@ -18,7 +18,6 @@ struct MatchArgs {
     MatchView M;               // emit pass: outputs
     uint32_t max_ne;           // largest state in P (LDS staging)
     uint32_t lds_per_wave;
-    unsigned long long* parent_edges;   // count pass: += edges of each state with a match
     const uint32_t* budget;    // count pass: [runs] matches a run may still take
     unsigned long long* run_acc;   // count pass: [runs] running tally (zeroed per generation)
     uint64_t* state_edges;     // count pass: [P.n] edges of each state's children (summed per run after)
@@ -181,9 +180,11 @@ uint32_t wl_states_lds_per_wave(uint32_t max_ne, uint32_t max_nv, uint32_t A, bo
 // team: waves per state (1, 2, 4 or 8); with team 1, wpb states per workgroup.
 hipError_t launch_wl_states(const WLStatesArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st);
 uint32_t wl_states_grid(uint32_t n, uint32_t team, uint32_t wpb, uint32_t grid_cap);
-// Per run (slot r, states [b[r], e[r])): tot[r] = its matches, edges[r] = its children's edges.
-hipError_t launch_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* b, const uint32_t* e,
-                             uint32_t nruns, uint64_t* tot, uint64_t* edges, hipStream_t st);
+// Per run (slot r, states [b[r], e[r])): tot[r] = its matches, edges[r] = its children's edges,
+// parent_edges[r] = the edges of its states with at least one match.
+hipError_t launch_run_totals(const uint64_t* moff, const uint64_t* state_edges, const uint32_t* cnt,
+                             const uint32_t* ne, const uint32_t* b, const uint32_t* e, uint32_t nruns, uint64_t* tot,
+                             uint64_t* edges, uint64_t* parent_edges, hipStream_t st);
 hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32_t n, uint32_t* cnt,
                               hipStream_t st);
 hipError_t launch_rehash(const uint64_t* ok, const uint32_t* ov, uint32_t old_cap, uint64_t* keys,
