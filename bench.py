@@ -355,7 +355,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the batched path shards whole "
-                                   "requests across GPUs with no exchange (see DESIGN.md)",
+                                   "requests across GPUs with no exchange",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

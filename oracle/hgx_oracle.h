@@ -6,8 +6,8 @@
  *
  * Every function in hgx_oracle.c cites the reference file:line it restates. Parity of
  * each piece is pinned by tests/test_oracle.py against reference/golden_matrix.txt,
- * BASELINE's brute-force event counts, the reference-built golden vectors under
- * tests/golden/, and closed-form/brute-force checks.
+ * the brute-force event counts of the 5-step binary rule (3/15/117/1,329/20,739), the
+ * reference-built golden vectors under tests/golden/, and closed-form/brute-force checks.
  */
 #ifndef HGX_ORACLE_H
 #define HGX_ORACLE_H
