@@ -420,6 +420,7 @@ uint32_t Engine::step() {
     std::vector<uint8_t> r_depth, r_ar;
     std::vector<RootCopy> r_copy;
     ExpandClass cls[4];                         // host range lists of the expand launches
+    std::vector<RunEvents> rev;                 // per-run child ranges for k_run_events
     const uint32_t nC = (uint32_t)M + NR;
     if (nC) {
         C = std::make_shared<DevGen>(pool_);
@@ -511,10 +512,12 @@ uint32_t Engine::step() {
             }
             const uint64_t bytes = M * (8ull + 2ull * K) + parent_edges * (4ull * F_->v.A + 2) +
                                    EC * (4ull * A + 2) + M * (28ull + 8ull * lstride) + M * 24ull;
+            uint16_t* tcnt = dalloc<uint16_t>(tmp, M);   // per child: causal triples, for k_run_events
             for (auto& x : cls) {
                 if (!x.n) continue;
                 ExpandArgs ea{};
                 ea.P = F_->v; ea.C = v; ea.M = mv; ea.E = C->ev; ea.runs = d_runs_; ea.ctr = d_ctr_;
+                ea.tcnt = tcnt;
                 ea.max_child_ne = std::max(1u, std::min(x.ne, max_ne));
                 ea.max_nextv = std::max(1u, std::min(x.nextv, max_nextv));
                 ea.max_nv = std::max(1u, std::min(ea.max_nextv, ea.max_child_ne * A));
@@ -543,6 +546,12 @@ uint32_t Engine::step() {
             }
             timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });
+            // each run's event counters from its children's records: one add per run
+            for (uint32_t i = 0; i < nF; ++i)
+                if (tot[i]) rev.push_back(RunEvents{F_runs_[i]->slot, cb[i], ce[i]});
+            RunEvents* d_rev = dalloc<RunEvents>(tmp, rev.size());
+            h2d(d_rev, rev.data(), rev.size(), st);
+            HIP_TRY(launch_run_events(d_rev, (uint32_t)rev.size(), C->ev, tcnt, d_ctr_, st));
         }
 
         // new roots: uploaded (host inputs) or copied on the device (prepared inputs), then

@@ -719,22 +719,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
     const uint32_t gw = xcd_block() * tpb + w, nw = gridDim.x * tpb;
     uint32_t rr = 0;                                       // range of logical index c (subsets)
 
-    // The run's counters, summed over the team's consecutive children of one run and added to
-    // the run's row once per run: kept by the team's first lane in the workspace (64 bytes
-    // before the reduction words), not in registers every lane would carry.
-    uint32_t acc_run = INV;
-    uint64_t* acc = (uint64_t*)(my + a.lds_per_wave - 128);
-    if (tm.leader())
-        for (int k = 0; k < CTR_N; ++k) acc[k] = 0;
-    auto flush = [&]() {
-        if (tm.leader()) {
-            if (acc_run != INV)
-                for (int k = 0; k < CTR_N; ++k)
-                    if (acc[k]) atomicAdd((unsigned long long*)&a.ctr[(size_t)acc_run * CTR_N + k],
-                                          (unsigned long long)acc[k]);
-            for (int k = 0; k < CTR_N; ++k) acc[k] = 0;
-        }
-    };
 
     for (uint32_t lc = gw * kChunk; lc < a.n_items; lc += (lc % kChunk == kChunk - 1) ? (nw - 1) * kChunk + 1 : 1) {
         uint32_t c = lc;
@@ -750,7 +734,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         const uint32_t par = a.M.parent[c];
         const uint32_t rid = a.M.rule[c];
         const uint32_t run = a.P.run[par];
-        if (run != acc_run) { flush(); acc_run = run; }
         DRun* RN = &a.runs[run];
         const DRule& R = RN->rules[rid];
         const uint32_t nl = R.n_lhs, nr = R.n_rhs;
@@ -873,15 +856,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
             a.C.depth[c] = (uint8_t)step;
             a.E.in_state[c] = a.P.sid[par];
             a.E.rule[c] = rid;
-            acc[CTR_EVENTS] += 1;
-            if (rec_causal) {
-                acc[CTR_TRIPLES] += triples;
-                acc[CTR_PAIRS] += __popcll(prod);
-                acc[CTR_PAIRS_TR] += __popcll(tr);
-                acc[CTR_TRIPLES_TR] += triples_tr;
-            }
             a.E.prod[c] = prod;
             a.E.tr[c] = tr;
+            // the run's counters are summed from these by k_run_events, not added here
+            a.tcnt[c] = rec_causal ? (uint16_t)(triples | (triples_tr << 8)) : (uint16_t)0;
         }
         // Lineage table of the child: the parent's, then this event's ancestor mask.
         {
@@ -891,7 +869,43 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         }
         tm.sync();
     }
-    flush();
+}
+
+// A generation's event counters per run (the run's children [cb, ce), one wave per run): events,
+// causal triples and pairs with and without transitive reduction, from the per-child records
+// k_expand wrote -- one add per counter and run instead of adds from every team.
+__global__ void k_run_events(const RunEvents* items, uint32_t n, const uint64_t* prod, const uint64_t* tr,
+                             const uint16_t* tcnt, uint64_t* ctr) {
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, l = lane_id();
+    if (i >= n) return;
+    const RunEvents it = items[i];
+    uint64_t triples = 0, pairs = 0, pairs_tr = 0, triples_tr = 0;
+    for (uint32_t c = it.cb + l; c < it.ce; c += WAVE) {
+        const uint32_t t = tcnt[c];
+        triples += t & 0xFFu;
+        triples_tr += t >> 8;
+        pairs += (uint64_t)__popcll(prod[c]);
+        pairs_tr += (uint64_t)__popcll(tr[c]);
+    }
+    triples = wave_sum_u64(triples);
+    triples_tr = wave_sum_u64(triples_tr);
+    pairs = wave_sum_u64(pairs);
+    pairs_tr = wave_sum_u64(pairs_tr);
+    if (l == 0) {
+        uint64_t* row = ctr + (size_t)it.slot * CTR_N;
+        row[CTR_EVENTS] += it.ce - it.cb;
+        row[CTR_TRIPLES] += triples;
+        row[CTR_PAIRS] += pairs;
+        row[CTR_PAIRS_TR] += pairs_tr;
+        row[CTR_TRIPLES_TR] += triples_tr;
+    }
+}
+
+hipError_t launch_run_events(const RunEvents* items, uint32_t n, const EventView& E, const uint16_t* tcnt,
+                             uint64_t* ctr, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_run_events, dim3((n + 3) / 4), dim3(256), 0, st, items, n, E.prod, E.tr, tcnt, ctr);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------

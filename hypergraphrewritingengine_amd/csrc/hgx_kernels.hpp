@@ -68,7 +68,15 @@ struct ExpandArgs {
     uint32_t nrange;
     uint32_t n_items;          // children this launch covers
     uint32_t lean;             // every child has one arity 2..4 and < 2^16 edges: lean workspace
+    uint16_t* tcnt;            // [M] per child: causal triples | triples kept by TR << 8
 };
+
+// One run's children in a generation, for k_run_events.
+struct RunEvents {
+    uint32_t slot, cb, ce;
+};
+hipError_t launch_run_events(const RunEvents* items, uint32_t n, const EventView& E, const uint16_t* tcnt,
+                             uint64_t* ctr, hipStream_t st);
 
 // Workspace of one child in k_expand: the hash's, with room in its nbr region for the
 // label -> local id map and in its srt region for the child's vertex labels.
