@@ -547,8 +547,10 @@ uint32_t Engine::step() {
             timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });
             // each run's event counters from its children's records: one add per run
-            for (uint32_t i = 0; i < nF; ++i)
-                if (tot[i]) rev.push_back(RunEvents{F_runs_[i]->slot, cb[i], ce[i]});
+            for (uint32_t i = 0; i < nF; ++i)                // items of at most 4,096 children
+                for (uint64_t b = cb[i]; b < ce[i]; b += 4096)
+                    rev.push_back(RunEvents{F_runs_[i]->slot, (uint32_t)b,
+                                            (uint32_t)std::min<uint64_t>(ce[i], b + 4096)});
             RunEvents* d_rev = dalloc<RunEvents>(tmp, rev.size());
             h2d(d_rev, rev.data(), rev.size(), st);
             HIP_TRY(launch_run_events(d_rev, (uint32_t)rev.size(), C->ev, tcnt, d_ctr_, st));

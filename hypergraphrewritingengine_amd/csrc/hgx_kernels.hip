@@ -2,13 +2,16 @@
 //
 //   k_match<false>   SCAN -> EXPAND -> SINK join, counting matches per state
 //   k_match<true>    the same join, writing match records in the deterministic order
-//   k_expand         rewrite + WL hash + state-map dedup + event + causal emission, one wave
-//                    per child state
-//   k_branchial      sibling-event overlap pairs, one wave per parent state
+//   k_expand         rewrite + WL hash + event + causal emission, one team per child state
+//   k_dedup          each child's hash into its request's state map
+//   k_run_events     a generation's event and causal counters, summed per request
+//   k_branchial      sibling-event overlap pairs, one wave per parent state (k_brl_*: the
+//                    sorted per-edge lists for parents above 128 edges)
 //   k_wl_states      WL hash of given states (roots, and the hgx_wl_hash_batch entry point)
 //   small helpers    per-run totals, count masking, map rehash
 //
-// Every kernel is wave64-native: one wave owns one state (or one child), stages that state's
+// Every kernel is wave64-native: one team (a wave, or the waves of a workgroup for large
+// states) owns one state or one child, stages that state's
 // edges in LDS and cooperates through ballots, mbcnt prefix sums and LDS atomics. There is no
 // MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
@@ -871,9 +874,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
     }
 }
 
-// A generation's event counters per run (the run's children [cb, ce), one wave per run): events,
-// causal triples and pairs with and without transitive reduction, from the per-child records
-// k_expand wrote -- one add per counter and run instead of adds from every team.
+// A generation's event counters per run (items of at most a few thousand of a run's children
+// [cb, ce), one wave each): events, causal triples and pairs with and without transitive
+// reduction, from the per-child records k_expand wrote -- one add per counter and item instead
+// of adds from every team.
 __global__ void k_run_events(const RunEvents* items, uint32_t n, const uint64_t* prod, const uint64_t* tr,
                              const uint16_t* tcnt, uint64_t* ctr) {
     const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, l = lane_id();
@@ -891,13 +895,13 @@ __global__ void k_run_events(const RunEvents* items, uint32_t n, const uint64_t*
     triples_tr = wave_sum_u64(triples_tr);
     pairs = wave_sum_u64(pairs);
     pairs_tr = wave_sum_u64(pairs_tr);
-    if (l == 0) {
-        uint64_t* row = ctr + (size_t)it.slot * CTR_N;
-        row[CTR_EVENTS] += it.ce - it.cb;
-        row[CTR_TRIPLES] += triples;
-        row[CTR_PAIRS] += pairs;
-        row[CTR_PAIRS_TR] += pairs_tr;
-        row[CTR_TRIPLES_TR] += triples_tr;
+    if (l == 0) {                                    // a large run is split over several items
+        unsigned long long* row = (unsigned long long*)(ctr + (size_t)it.slot * CTR_N);
+        atomicAdd(&row[CTR_EVENTS], (unsigned long long)(it.ce - it.cb));
+        if (triples) atomicAdd(&row[CTR_TRIPLES], (unsigned long long)triples);
+        if (pairs) atomicAdd(&row[CTR_PAIRS], (unsigned long long)pairs);
+        if (pairs_tr) atomicAdd(&row[CTR_PAIRS_TR], (unsigned long long)pairs_tr);
+        if (triples_tr) atomicAdd(&row[CTR_TRIPLES_TR], (unsigned long long)triples_tr);
     }
 }
 
