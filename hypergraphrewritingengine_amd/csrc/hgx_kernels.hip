@@ -715,9 +715,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
     uint32_t* vmap = (uint32_t*)s.nbr;                    // [max_nextv + 1]
 
     // Children in chunks of kChunk consecutive ones, dealt round-robin to the teams (2 measured
-    // best of 1, 2, 4 and contiguous per-team blocks on the batched workload): a chunk
-    // keeps neighbouring children (one parent, one request) together for the counters, and the
-    // dealing spreads large and small requests' children over all teams.
+    // best of 1, 2, 4 and contiguous per-team blocks on the batched workload; 1 and 2 measure
+    // alike since the run counters left this kernel): a chunk keeps neighbouring children (one
+    // parent) on one team, and the dealing spreads large and small requests' children over all
+    // teams.
     constexpr uint32_t kChunk = 2;
     const uint32_t gw = xcd_block() * tpb + w, nw = gridDim.x * tpb;
     uint32_t rr = 0;                                       // range of logical index c (subsets)
