@@ -243,19 +243,23 @@ static std::vector<uint8_t> program(const Job& j) {
 }
 
 // Initial states as the FFI takes them: negative vertices and empty edges dropped, empty states
-// dropped (hypergraph_ffi.cpp:598-636); vertex renumbering happens in the library.
+// dropped (hypergraph_ffi.cpp:598-636). Each state's int64 labels are renumbered densely by
+// first appearance here, as the FFI does, so any non-negative int64 label is taken; the library
+// renumbers the same way, so the dense labels reach it unchanged.
 static void roots(const Job& j, std::vector<uint32_t>& eoff, std::vector<uint32_t>& verts,
                   std::vector<uint32_t>& roff) {
     eoff.assign(1, 0);
     roff.assign(1, 0);
+    std::unordered_map<int64_t, uint32_t> label;
     for (const auto& st : j.states) {
         uint32_t kept = 0;
+        label.clear();
         for (const auto& e : st) {
             uint32_t n = 0;
             for (int64_t v : e) {
                 if (v < 0) continue;
-                if (v > 0xFFFFFFFFll) throw JobError("vertex id above 2^32-1");
-                verts.push_back((uint32_t)v);
+                const auto it = label.emplace(v, (uint32_t)label.size()).first;
+                verts.push_back(it->second);
                 ++n;
             }
             if (!n) continue;

@@ -1362,8 +1362,8 @@ hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32
 
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean) {
     const WLSizes z = expand_sizes(max_child_ne, max_nv, max_nextv, A, lean);
-    // + 64: carving slack; + 64: the run counters; + 64: the team's reduction words at the end
-    return ((z.bytes(A) + 192u) + 15u) & ~15u;
+    // + 64: carving slack; + 64: the team's reduction words at the end
+    return ((z.bytes(A) + 128u) + 15u) & ~15u;
 }
 
 template <uint32_t W>

@@ -190,10 +190,15 @@ private:
         const uint64_t rank = varint();
         if (rank == 0 || rank > 16) throw Error("array rank out of range");
         std::vector<uint64_t> dims(rank);
+        // Every prefix product of the dims counts nodes the nested lists will hold (list nodes
+        // at the inner levels, elements at the last). Elements are bounded by the data bytes
+        // below; list nodes by the input left plus a fixed allowance of empty lists, so dims
+        // like {2^28, 0} (no data, 2^28 empty lists) are refused before anything is reserved.
+        const uint64_t room = (uint64_t)(n_ - k_) + 4096;
         uint64_t total = 1;
         for (auto& d : dims) {
             d = varint();
-            if (d && total > (uint64_t)(n_ - k_) / d + 1) throw Error("array larger than the input");
+            if (d && total > room / d) throw Error("array larger than the input");
             total *= d;
         }
         size_t w;

@@ -2,6 +2,8 @@
 normalisation (gpu/tests/test_gpu_vs_cpu_differential.cpp:64-160) and exact array checks."""
 from __future__ import annotations
 
+import hashlib
+
 import numpy as np
 
 from oracle import pyoracle as po
@@ -57,3 +59,36 @@ def assert_exact(res, o, relations=True, tr=False):
             gb = sorted_pairs(res.branchial())
             ob = sorted_pairs(o.branchial)
             assert np.array_equal(gb, ob), "branchial pairs differ"
+
+
+def _rows(a, cols):
+    a = np.asarray(a, dtype=np.uint64).reshape(-1, cols)
+    return a[np.lexsort(a.T[::-1])] if len(a) else a
+
+
+def arrays_digest(hashes, steps, creators, events, causal, causal_tr, branchial) -> str:
+    """sha256 over one result's arrays (16 hex digits): per-state hashes, steps and creators
+    and the event records in the deterministic numbering; the causal (producer, consumer,
+    reduction tag) and branchial (smaller, larger) pairs as sorted sets of rows. The fixtures of
+    tests/golden/make_config_fixtures.py hold the oracle's digests."""
+    h = hashlib.sha256()
+    for a, dt in ((hashes, np.uint64), (steps, np.uint32), (creators, np.uint32),
+                  (events, np.uint32)):
+        h.update(np.ascontiguousarray(np.asarray(a, dtype=dt)).tobytes())
+    cz = np.concatenate([np.asarray(causal, np.uint64).reshape(-1, 2),
+                         np.asarray(causal_tr, np.uint64).reshape(-1, 1)], axis=1)
+    h.update(_rows(cz, 3).tobytes())
+    h.update(_rows(branchial, 2).tobytes())
+    return h.hexdigest()[:16]
+
+
+def gpu_digest(res) -> str:
+    """arrays_digest of a device result (needs causal, branchial and relations recorded)."""
+    h, s, c = res.states()
+    cnt = res.counts
+    if cnt["causal_pairs"]:
+        cp, ct = res.causal()
+    else:
+        cp, ct = np.zeros((0, 2), np.uint32), np.zeros(0, np.uint8)
+    br = res.branchial() if cnt["branchial_pairs"] else np.zeros((0, 2), np.uint32)
+    return arrays_digest(h, s, c, res.events(), cp, ct, br)

@@ -218,3 +218,30 @@ def test_reference_vectors():
             req.flags |= 0x100
         got = digest_oracle(po.evolve(req), tr)
         assert got == entry["digest"], entry["name"]
+
+
+def test_oracle_reproduces_fullsize_fixtures_sample():
+    """The committed full-size fixtures (whose counts the reference engine confirmed) are the
+    oracle's: a sample of batched runs and the small served draws recomputed here."""
+    import json
+    import os
+
+    from hypergraphrewritingengine_amd import workloads as W
+    from tests.parity import arrays_digest
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, make, ids in (("batched4096.json", W.batched_run, range(0, 4096, 97)),
+                            ("served_mix.json", W.served_run, None)):
+        fx = json.load(open(os.path.join(gold, name)))
+        pairs = list(zip(fx["ids"], fx["runs"])) if ids is None else [(i, fx["runs"][i]) for i in ids]
+        for i, row in pairs:
+            req = make(i, cap=fx["cap"], flags=fx["flags"])
+            if ids is None and len(req.init) > 400:
+                continue                      # the large served draws take seconds each
+            r = po.evolve(req)
+            want = dict(zip(fx["columns"], row))
+            for k in fx["columns"][:-1]:
+                assert r.counts[k] == want[k], (name, i, k)
+            d = arrays_digest(r.state_hash, r.state_step, r.state_creator, r.events, r.causal,
+                              r.causal_tr, r.branchial)
+            assert d == want["digest"], (name, i)

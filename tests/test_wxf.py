@@ -71,3 +71,21 @@ def test_compressed_and_garbage_are_refused():
     assert not hgx.wxf_validate(dumps(_job()) + b"C\x00")[0]          # trailing bytes
     # a list length the input cannot hold is refused before anything is allocated
     assert not hgx.wxf_validate(b"8:f\xff\xff\xff\xff\x0fs\x04List")[0]
+
+
+def test_array_dims_with_no_data_are_refused_before_allocation():
+    # PackedArray int8 of dims {2^28, 0}: no element bytes, 2^28 empty lists. The reader bounds
+    # the list nodes by the input left, so it refuses this before reserving anything.
+    ok, msg = hgx.wxf_validate(bytes.fromhex("383ac10002808080800100"))
+    assert not ok and "larger than the input" in msg, msg
+    # a small array of empty rows ({{}, {}}: dims {2, 0}) still parses; it is no job
+    ok, msg = hgx.wxf_validate(bytes.fromhex("383ac100020200"))
+    assert not ok and "must be an Association" in msg, msg
+
+
+def test_int64_vertex_labels_are_renumbered():
+    # labels above 2^32-1 are renumbered by first appearance, as the FFI does
+    # (hypergraph_ffi.cpp:598-636), so the job is accepted
+    big = 1 << 40
+    ok, msg = hgx.wxf_validate(dumps(_job(InitialEdges=[[big, big + 1], [big + 1, 7], [7, big]])))
+    assert ok, msg
