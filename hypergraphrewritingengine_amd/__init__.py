@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = [
     "hgx_profile_enable", "hgx_profile_read", "hgx_result_counts_many", "hgx_serve_drain",
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
-    "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form",
+    "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -85,6 +85,7 @@ def load_library():
     L.hgx_result_causal.argtypes = [vp, vp, vp, C.c_uint64]
     L.hgx_result_branchial.argtypes = [vp, vp, C.c_uint64]
     L.hgx_result_timing.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.hgx_result_status.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.hgx_result_release.argtypes = [vp]
     L.hgx_result_release.restype = None
     L.hgx_result_release_many.argtypes = [vp, C.c_uint32]
@@ -212,6 +213,13 @@ class Result:
         d, l = C.c_uint64(), C.c_uint64()
         self._check(self._lib.hgx_result_timing(self._h, C.byref(d), C.byref(l)))
         return d.value, l.value
+
+    def status(self):
+        """(status, reason): HGX_OK for a complete result, HGX_ERR_CAPACITY for a partial one (a
+        generation did not fit the context's HBM budget; the result holds the ones before it)."""
+        buf = C.create_string_buffer(1024)
+        rc = self._lib.hgx_result_status(self._h, buf, 1024)
+        return rc, buf.value.decode()
 
     def close(self):
         if self._h:

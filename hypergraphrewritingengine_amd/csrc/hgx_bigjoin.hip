@@ -1,30 +1,43 @@
 // hgx_bigjoin.hip -- the join over one large state (tens of thousands of edges and up), where
 // the state does not fit LDS: the match-join stress workload, one rewriting step of one big
-// state (hgx_match_count).
+// state (hgx_match_count, and hgx_match_count_noalloc over a fixed device scratch).
 //
-// Index build (the role of SignatureIndex / InvertedVertexIndex, hypergraph/src/index.cpp:45-80):
-// a CSR keyed by (vertex, position) -> ascending edge ids, by counting sort: histogram with
-// atomics, exclusive scan, scatter, then each bucket sorted so the CSR is deterministic.
-// Keying by position as well as vertex makes a candidate list exactly the edges that carry the
-// bound vertex where the pattern needs it, so the join never re-tests the position.
+// Index (the role of SignatureIndex / InvertedVertexIndex, hypergraph/src/index.cpp:45-80): a CSR
+// keyed by (vertex, position) -> ascending edge ids. Offsets from a histogram and a scan; the
+// lists from a stable radix sort of (key, edge) pairs laid out in edge order, so each list is in
+// ascending edge order with no per-bucket sort. Keying by position as well as vertex makes a
+// candidate list exactly the edges carrying the bound vertex where the pattern needs it.
 //
-// Join (join_core.hpp:143-208): one lane per seed edge of the first join position (SCAN: the
-// arity and repetition test), depth-first completion over the CSR lists (EXPAND), per-seed
-// counts prefix-summed so matches are written in (seed, depth-first) order (SINK) -- the order
-// of a serial DFS over ascending candidates.
+// Join (join_core.hpp:143-208), breadth first: the partial matches of depth d (d edges in join
+// order) are a frontier held in HBM, in lexicographic order. Depth d+1 takes every (partial,
+// candidate) pair -- candidates from the CSR list of the pivot vertex, ascending -- as one item:
+// the items are numbered by a scan of the per-partial list lengths and dealt to workgroups in
+// tiles of 2,048, each tile finding its partials by a binary search over the scan staged in LDS.
+// A count pass sizes the next frontier per tile, a scan of the tile counts places every tile, and
+// an emit pass writes the survivors in item order (ballot + mbcnt within a wave, wave totals
+// across the workgroup). Item order is (partial, ascending candidate), so the final match list
+// is in the order of a serial depth-first join over ascending candidates -- the oracle's order.
+// The work per depth is spread over all its candidates, not one thread per seed.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
-#include <unordered_map>
 #include <vector>
 
+#include "hgx_bigjoin.hpp"
 #include "hgx_device.hpp"
 #include "hgx_kernels.hpp"
 #include "hgx_runtime.hpp"
 
 namespace hgx {
+
+constexpr uint32_t BJ_THREADS = 256;
+constexpr uint32_t BJ_IPT = 8;                            // items per thread per tile
+constexpr uint32_t BJ_TILE = BJ_THREADS * BJ_IPT;         // items per workgroup tile
+constexpr uint32_t BJ_LDS_PART = BJ_TILE + 2;             // scan entries a tile stages in LDS
+
+// ---- index -------------------------------------------------------------------------------
 
 __global__ void k_index_count(const uint32_t* ev, const uint8_t* ar, uint32_t m, uint32_t A, uint32_t* cnt) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -32,111 +45,349 @@ __global__ void k_index_count(const uint32_t* ev, const uint8_t* ar, uint32_t m,
     for (uint32_t p = 0; p < ar[e]; ++p) atomicAdd(&cnt[ev[(size_t)e * A + p] * A + p], 1u);
 }
 
-__global__ void k_off_to_u32(const uint64_t* off, uint32_t* cur, uint32_t n) {
+// One (key, edge) pair per vertex slot, in edge order; slots past an edge's arity get the key
+// nkeys, which sorts after every real one.
+__global__ void k_index_keys(const uint32_t* ev, const uint8_t* ar, uint32_t m, uint32_t A, uint32_t nkeys,
+                             uint32_t* key, uint32_t* val) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (uint64_t)m * A) return;
+    const uint32_t e = (uint32_t)(s / A), p = (uint32_t)(s % A);
+    key[s] = p < ar[e] ? ev[s] * A + p : nkeys;
+    val[s] = e;
+}
+
+__global__ void k_u64_to_u32(const uint64_t* in, uint32_t* out, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) cur[i] = (uint32_t)off[i];
+    if (i < n) out[i] = (uint32_t)in[i];
 }
 
-__global__ void k_index_scatter(const uint32_t* ev, const uint8_t* ar, uint32_t m, uint32_t A,
-                                uint32_t* cursor, uint32_t* idx) {
+// ---- join --------------------------------------------------------------------------------
+
+// Depth 0 (SCAN): the edges that fit pattern edge order[0] -- its arity, and its repeated
+// variables (match_core.hpp:32-49).
+__global__ void k_bj_seed(BJView v, uint8_t* flag) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m) return;
-    for (uint32_t p = 0; p < ar[e]; ++p) idx[atomicAdd(&cursor[ev[(size_t)e * A + p] * A + p], 1u)] = e;
-}
-
-__global__ void k_index_sort(const uint64_t* off, uint32_t nkeys, uint32_t* idx) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nkeys) return;
-    const uint64_t b = off[k], n = off[k + 1] - b;
-    for (uint64_t i = 1; i < n; ++i) {
-        const uint32_t key = idx[b + i];
-        uint64_t j = i;
-        while (j > 0 && idx[b + j - 1] > key) { idx[b + j] = idx[b + j - 1]; --j; }
-        idx[b + j] = key;
+    if (e >= v.m) return;
+    const DRule& R = v.R;
+    const uint32_t p = R.order[0], a = R.la[p];
+    bool ok = v.ar[e] == a;
+    const uint32_t* x = v.ev + (size_t)e * v.A;
+    for (uint32_t i = 0; ok && i < a; ++i) {
+        const uint32_t s = R.jsrc[0][i];
+        if (s != 0xFFu) ok = x[i] == x[s & 15u];
     }
+    flag[e] = ok ? 1 : 0;
 }
 
-struct BigJoinArgs {
-    DRule R;
-    const uint32_t* ev;
-    const uint8_t* ar;
-    uint32_t m, A;
-    const uint64_t* off;    // [nkeys + 1]
-    const uint32_t* idx;
-    uint32_t* seed_cnt;     // count pass: [m + 1]
-    const uint64_t* seed_off;   // emit pass
-    uint32_t* out;          // emit pass: n_lhs edge ids per match, pattern order
-    uint64_t cap;
-};
+// Candidate list of each partial of depth d: [lo, lo + cnt) of the pivot vertex's CSR list, or
+// every edge when pattern edge order[d] shares no bound variable.
+__global__ void k_bj_range(BJView v, uint32_t d, const uint32_t* F, uint64_t P, uint32_t* lo, uint32_t* cnt) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P) return;
+    const DRule& R = v.R;
+    const uint32_t pv = R.piv[d];
+    if (pv == 0xFFu) { lo[q] = 0; cnt[q] = v.m; return; }
+    const uint32_t s = R.jsrc[d][pv];                      // bound at an earlier depth
+    const uint32_t vert = v.ev[(size_t)F[q * d + (s >> 4)] * v.A + (s & 15u)];
+    const uint32_t key = vert * v.A + pv;
+    lo[q] = v.off[key];
+    cnt[q] = v.off[key + 1] - v.off[key];
+}
 
-__device__ inline bool bj_bind(const uint32_t* ev, uint32_t ea, const uint8_t* pv, uint32_t pa,
-                               uint32_t* bind, uint32_t& mask) {   // match_core.hpp:32-49
-    if (ea != pa) return false;
-    for (uint32_t i = 0; i < ea; ++i) {
-        const uint32_t var = pv[i], bit = 1u << var;
-        if (mask & bit) { if (bind[var] != ev[i]) return false; }
-        else { bind[var] = ev[i]; mask |= bit; }
+// Last q with start[q] <= t, over start[0..n] (start[0] = 0).
+__device__ __forceinline__ uint64_t bj_owner(const uint64_t* start, uint64_t n, uint64_t t) {
+    uint64_t a = 0, b = n;                                  // start[a] <= t < start[b]
+    while (b - a > 1) {
+        const uint64_t mid = (a + b) >> 1;
+        if (start[mid] <= t) a = mid; else b = mid;
+    }
+    return a;
+}
+
+// Item t of depth d: candidate e of partial q. Fits when it has the arity, is not already in the
+// partial (edge-injective) and agrees with every binding the partial and its own earlier slots
+// fixed (vertex-non-injective: distinct variables may share a vertex).
+__device__ __forceinline__ bool bj_fits(const BJView& v, uint32_t d, const uint32_t* part, uint32_t e) {
+    const DRule& R = v.R;
+    const uint32_t p = R.order[d], a = R.la[p];
+    if (v.ar[e] != a) return false;
+    for (uint32_t t = 0; t < d; ++t)
+        if (part[t] == e) return false;
+    const uint32_t* x = v.ev + (size_t)e * v.A;
+    for (uint32_t i = 0; i < a; ++i) {
+        const uint32_t s = R.jsrc[d][i];
+        if (s == 0xFFu) continue;
+        const uint32_t sd = s >> 4, si = s & 15u;
+        const uint32_t want = sd < d ? v.ev[(size_t)part[sd] * v.A + si] : x[si];
+        if (x[i] != want) return false;
     }
     return true;
 }
 
-template <bool EMIT>
-__global__ __launch_bounds__(256) void k_bigjoin(BigJoinArgs a) {
-    const DRule& R = a.R;
-    const uint32_t n = R.n_lhs;
-    uint32_t matched[16], smask[17], cidx[16], chi[16], bind[32];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t seed = blockIdx.x * blockDim.x + threadIdx.x; seed < a.m; seed += stride) {
+struct BJLevel {
+    uint32_t d;               // depth being added (1 .. n-1)
+    const uint32_t* F;        // [P * d] partials, join order
+    uint64_t P;
+    const uint32_t* lo;       // [P]
+    const uint64_t* start;    // [P + 1] scan of the list lengths
+    uint64_t T;               // items
+    uint64_t* tile_cnt;       // count pass: survivors per tile
+    const uint64_t* tile_off; // emit pass: scan of tile_cnt
+    uint32_t* Fn;             // emit, inner depth: [S * (d+1)]
+    uint32_t* out;            // emit, last depth: matches, n_lhs edges each in pattern order
+    uint64_t out_base, cap;   //   (matches found before this call; room for cap in all)
+};
+
+// One pass over the items of a depth, a tile of BJ_TILE items per workgroup iteration.
+template <bool EMIT, bool LAST>
+__global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
+    __shared__ uint64_t sst[BJ_LDS_PART];
+    __shared__ uint32_t wtot[BJ_THREADS / WAVE];
+    __shared__ uint64_t sq[2];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    const uint32_t d = L.d;
+    const uint32_t pv = v.R.piv[d];
+    const uint64_t ntiles = (L.T + BJ_TILE - 1) / BJ_TILE;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t t0 = tile * BJ_TILE;
+        const uint64_t t1 = t0 + BJ_TILE < L.T ? t0 + BJ_TILE : L.T;
+        if (tid == 0) {
+            sq[0] = bj_owner(L.start, L.P, t0);
+            sq[1] = bj_owner(L.start, L.P, t1 - 1);
+        }
+        __syncthreads();
+        const uint64_t q0 = sq[0], nq = sq[1] - sq[0] + 1;  // partials this tile touches
+        const bool staged = nq + 1 <= BJ_LDS_PART;
+        if (staged)
+            for (uint64_t i = tid; i <= nq; i += BJ_THREADS) sst[i] = L.start[q0 + i];
+        __syncthreads();
+        uint64_t rank = EMIT ? L.tile_off[tile] : 0;      // running place of this tile's survivors
         uint32_t found = 0;
-        uint64_t o = EMIT ? a.seed_off[seed] : 0;
-        uint32_t mask = 0;
-        const uint32_t p0 = R.order[0];
-        if (bj_bind(a.ev + (size_t)seed * a.A, a.ar[seed], R.lv[p0], R.la[p0], bind, mask)) {
-            matched[0] = seed;
-            if (n == 1) {
-                if (EMIT && o < a.cap) a.out[o * n + p0] = seed;
-                found = 1;
-            } else {
-                auto range = [&](uint32_t d, uint32_t& lo, uint32_t& hi) {
-                    const uint32_t pv = R.piv[d];
-                    if (pv == 0xFFu) { lo = 0; hi = a.m; return; }
-                    const uint64_t key = (uint64_t)bind[R.lv[R.order[d]][pv]] * a.A + pv;
-                    lo = (uint32_t)a.off[key];
-                    hi = (uint32_t)a.off[key + 1];
-                };
-                smask[1] = mask;
-                range(1, cidx[1], chi[1]);
-                uint32_t d = 1;
-                while (d >= 1) {
-                    const uint32_t p = R.order[d];
-                    const bool indexed = R.piv[d] != 0xFFu;
-                    bool adv = false;
-                    while (cidx[d] < chi[d]) {
-                        const uint32_t j = cidx[d]++;
-                        const uint32_t e = indexed ? a.idx[j] : j;
-                        bool taken = false;
-                        for (uint32_t t = 0; t < d; ++t) taken |= (matched[t] == e);
-                        if (taken) continue;
-                        uint32_t mm = smask[d];
-                        if (!bj_bind(a.ev + (size_t)e * a.A, a.ar[e], R.lv[p], R.la[p], bind, mm)) continue;
-                        matched[d] = e;
-                        if (d + 1 == n) {
-                            if (EMIT && o + found < a.cap)
-                                for (uint32_t t = 0; t < n; ++t) a.out[(o + found) * n + R.order[t]] = matched[t];
-                            ++found;
-                            continue;
-                        }
-                        smask[d + 1] = mm;
-                        range(d + 1, cidx[d + 1], chi[d + 1]);
-                        ++d;
-                        adv = true;
-                        break;
+        for (uint32_t k = 0; k < BJ_IPT; ++k) {
+            const uint64_t t = t0 + (uint64_t)k * BJ_THREADS + tid;
+            bool ok = false;
+            uint64_t q = 0;
+            uint32_t e = 0;
+            if (t < t1) {
+                q = staged ? q0 + bj_owner(sst, nq, t) : bj_owner(L.start, L.P, t);
+                const uint64_t j = t - (staged ? sst[q - q0] : L.start[q]);
+                e = pv == 0xFFu ? (uint32_t)j : v.idx[L.lo[q] + j];
+                ok = bj_fits(v, d, L.F + q * d, e);
+            }
+            if (!EMIT) { found += ok ? 1u : 0u; continue; }
+            // place among this round's survivors: within the wave, then across the waves
+            const uint64_t bal = __ballot(ok);
+            if (lane == 0) wtot[w] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, round = 0;
+            for (uint32_t x = 0; x < BJ_THREADS / WAVE; ++x) {
+                before += x < w ? wtot[x] : 0u;
+                round += wtot[x];
+            }
+            __syncthreads();
+            if (ok) {
+                const uint64_t o = rank + before + lanes_below(bal);
+                const uint32_t* part = L.F + q * d;
+                if (LAST) {
+                    const uint32_t n = v.R.n_lhs;
+                    const uint64_t g = L.out_base + o;
+                    if (L.out && g < L.cap) {
+                        for (uint32_t x = 0; x < d; ++x) L.out[g * n + v.R.order[x]] = part[x];
+                        L.out[g * n + v.R.order[d]] = e;
                     }
-                    if (!adv) --d;
+                } else {
+                    for (uint32_t x = 0; x < d; ++x) L.Fn[o * (d + 1) + x] = part[x];
+                    L.Fn[o * (d + 1) + d] = e;
                 }
             }
+            rank += round;
         }
-        if (!EMIT) a.seed_cnt[seed] = found;
+        if (!EMIT) {
+            const uint32_t ws = wave_sum_u32(found);
+            if (lane == 0) wtot[w] = ws;
+            __syncthreads();
+            if (tid == 0) {
+                uint64_t s = 0;
+                for (uint32_t x = 0; x < BJ_THREADS / WAVE; ++x) s += wtot[x];
+                L.tile_cnt[tile] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace hgx
+
+namespace hgx {
+
+struct BJCast64 {
+    __host__ __device__ uint64_t operator()(uint32_t x) const { return x; }
+};
+
+template <class T>
+static T* bget(BJMem& m, size_t n) { return (T*)m.get(sizeof(T) * (n ? n : 1)); }
+
+template <class T>
+static T read1(const T* d, hipStream_t st) {
+    T x{};
+    HIP_TRY(hipMemcpyAsync(&x, d, sizeof(T), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return x;
+}
+
+// exclusive sums over n + 1 entries (the last input zero): out[n] is the total
+static void bj_scan(BJMem& m, const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t st) {
+    if (n + 1 > 0x7FFFFFFFull) throw Error{HGX_ERR_UNSUPPORTED, "join frontier above 2^31 partials"};
+    hipcub::TransformInputIterator<uint64_t, BJCast64, const uint32_t*> it(in, BJCast64());
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, it, out, (int)(n + 1), st));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(m.get(tmp), tmp, it, out, (int)(n + 1), st));
+}
+static void bj_scan(BJMem& m, const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st) {
+    if (n + 1 > 0x7FFFFFFFull) throw Error{HGX_ERR_UNSUPPORTED, "join above 2^31 tiles"};
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)(n + 1), st));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(m.get(tmp), tmp, in, out, (int)(n + 1), st));
+}
+
+static uint32_t blocks_for(uint64_t n, uint32_t threads) { return (uint32_t)std::max<uint64_t>(1, (n + threads - 1) / threads); }
+
+// CSR (vertex, position) -> ascending edges of a state held on the device as ev [m * A], ar [m],
+// vertex ids below nv.
+static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_t* alg_bytes) {
+    const uint32_t m = v.m, A = v.A;
+    const uint64_t nkeys = (uint64_t)nv * A, slots = (uint64_t)m * A;
+    if (nkeys >= 0xFFFFFFF0ull || slots >= 0x7FFFFFF0ull)
+        throw Error{HGX_ERR_UNSUPPORTED, "index above 2^31 vertex slots"};
+    uint32_t* cnt = bget<uint32_t>(mem, nkeys + 1);
+    uint64_t* off64 = bget<uint64_t>(mem, nkeys + 1);
+    uint32_t* off = bget<uint32_t>(mem, nkeys + 1);
+    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (nkeys + 1), st));
+    if (m) hipLaunchKernelGGL(k_index_count, dim3(blocks_for(m, 256)), dim3(256), 0, st, v.ev, v.ar, m, A, cnt);
+    bj_scan(mem, cnt, off64, nkeys, st);
+    hipLaunchKernelGGL(k_u64_to_u32, dim3(blocks_for(nkeys + 1, 256)), dim3(256), 0, st, off64, off,
+                       (uint32_t)(nkeys + 1));
+    uint32_t* kin = bget<uint32_t>(mem, slots);
+    uint32_t* vin = bget<uint32_t>(mem, slots);
+    uint32_t* kout = bget<uint32_t>(mem, slots);
+    uint32_t* idx = bget<uint32_t>(mem, slots);
+    if (slots) {
+        hipLaunchKernelGGL(k_index_keys, dim3(blocks_for(slots, 256)), dim3(256), 0, st, v.ev, v.ar, m, A,
+                           (uint32_t)nkeys, kin, vin);
+        int end_bit = 1;
+        while (end_bit < 32 && (nkeys >> end_bit) != 0) ++end_bit;      // keys <= nkeys
+        size_t tmp = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, idx, (int)slots, 0, end_bit, st));
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(mem.get(tmp), tmp, kin, kout, vin, idx, (int)slots, 0, end_bit, st));
+    }
+    HIP_TRY(hipGetLastError());
+    v.off = off;
+    v.idx = idx;
+    // read the state once, write and read the pairs through the sort, write the offsets
+    *alg_bytes += (uint64_t)m * (4ull * A + 1) + slots * 8ull * 3 + (nkeys + 1) * 16ull;
+}
+
+// Extends the partials F (P of them, d edges each) by pattern edge order[d], and on to the last
+// depth; splits F in two, in order, when the next frontier would outgrow the memory at hand.
+static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
+    BJMem& mem = *J.mem;
+    const uint32_t n = J.v.R.n_lhs, a = J.v.R.la[J.v.R.order[d]];
+    const bool last = d + 1 == n;
+    const size_t mk = mem.mark();
+    uint32_t* lo = bget<uint32_t>(mem, P);
+    uint32_t* cnt = bget<uint32_t>(mem, P + 1);
+    uint64_t* start = bget<uint64_t>(mem, P + 1);
+    HIP_TRY(hipMemsetAsync(cnt + P, 0, sizeof(uint32_t), J.st));
+    hipLaunchKernelGGL(k_bj_range, dim3(blocks_for(P, 256)), dim3(256), 0, J.st, J.v, d, F, P, lo, cnt);
+    bj_scan(mem, cnt, start, P, J.st);
+    const uint64_t T = read1(start + P, J.st);
+    J.alg_bytes += P * (4ull * d + 12ull);
+    if (T == 0) { mem.rewind(mk); return; }
+    const uint64_t ntiles = (T + BJ_TILE - 1) / BJ_TILE;
+    uint64_t* tcnt = bget<uint64_t>(mem, ntiles + 1);
+    uint64_t* toff = bget<uint64_t>(mem, ntiles + 1);
+    HIP_TRY(hipMemsetAsync(tcnt + ntiles, 0, sizeof(uint64_t), J.st));
+    BJLevel L{};
+    L.d = d; L.F = F; L.P = P; L.lo = lo; L.start = start; L.T = T; L.tile_cnt = tcnt; L.tile_off = toff;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, J.grid);
+    if (last) hipLaunchKernelGGL((k_bj_items<false, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+    else hipLaunchKernelGGL((k_bj_items<false, false>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+    HIP_TRY(hipGetLastError());
+    bj_scan(mem, tcnt, toff, ntiles, J.st);
+    const uint64_t S = read1(toff + ntiles, J.st);
+    // every item reads its candidate id and the candidate's vertices
+    J.alg_bytes += T * 4ull * (1ull + a);
+    if (last) {
+        if (J.out && J.found < J.cap && S) {
+            L.out = J.out; L.out_base = J.found; L.cap = J.cap;
+            hipLaunchKernelGGL((k_bj_items<true, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+            HIP_TRY(hipGetLastError());
+            J.alg_bytes += std::min(S, J.cap - J.found) * 4ull * n;
+        }
+        J.found += S;
+        mem.rewind(mk);
+        return;
+    }
+    if (S == 0) { mem.rewind(mk); return; }
+    const uint64_t need = S * (4ull * (d + 1) + 16ull) + ((S * 4ull + BJ_TILE - 1) / BJ_TILE) * 16ull;
+    if (need > mem.room()) {
+        if (P == 1) throw Error{HGX_ERR_CAPACITY, "one partial match of the join has more extensions than memory holds"};
+        mem.rewind(mk);
+        bj_depth(J, d, F, P / 2);
+        bj_depth(J, d, F + (P / 2) * d, P - P / 2);
+        return;
+    }
+    uint32_t* Fn = bget<uint32_t>(mem, S * (d + 1));
+    L.Fn = Fn;
+    hipLaunchKernelGGL((k_bj_items<true, false>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+    HIP_TRY(hipGetLastError());
+    J.alg_bytes += S * 4ull * (d + 1);
+    bj_depth(J, d + 1, Fn, S);
+    mem.rewind(mk);
+}
+
+// All matches of rule R in the state (ev, ar) on the device: the index, the seeds, the depths.
+void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed) {
+    BJMem& mem = *J.mem;
+    hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;
+    if (timed) {
+        HIP_TRY(hipEventCreate(&t0)); HIP_TRY(hipEventCreate(&t1)); HIP_TRY(hipEventCreate(&t2));
+        HIP_TRY(hipEventRecord(t0, J.st));
+    }
+    bj_index(mem, J.v, nv, J.st, &J.index_bytes);
+    if (timed) HIP_TRY(hipEventRecord(t1, J.st));
+    const uint32_t m = J.v.m;
+    uint8_t* flag = bget<uint8_t>(mem, m);
+    uint32_t* seeds = bget<uint32_t>(mem, m);
+    uint32_t* nsel = bget<uint32_t>(mem, 1);
+    if (m) {
+        hipLaunchKernelGGL(k_bj_seed, dim3(blocks_for(m, 256)), dim3(256), 0, J.st, J.v, flag);
+        hipcub::CountingInputIterator<uint32_t> ids(0);
+        size_t tmp = 0;
+        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tmp, ids, flag, seeds, nsel, (int)m, J.st));
+        HIP_TRY(hipcub::DeviceSelect::Flagged(mem.get(tmp), tmp, ids, flag, seeds, nsel, (int)m, J.st));
+    } else {
+        HIP_TRY(hipMemsetAsync(nsel, 0, sizeof(uint32_t), J.st));
+    }
+    const uint64_t P0 = read1(nsel, J.st);
+    J.alg_bytes += (uint64_t)m * (4ull * J.v.A + 1) + P0 * 4ull;
+    if (J.v.R.n_lhs == 1) {
+        if (J.out && P0) {
+            const uint64_t k = std::min(P0, J.cap);
+            HIP_TRY(hipMemcpyAsync(J.out, seeds, sizeof(uint32_t) * k, hipMemcpyDeviceToDevice, J.st));
+        }
+        J.found = P0;
+    } else if (P0) {
+        bj_depth(J, 1, seeds, P0);
+    }
+    if (timed) {
+        HIP_TRY(hipEventRecord(t2, J.st));
+        HIP_TRY(hipStreamSynchronize(J.st));
+        HIP_TRY(hipEventElapsedTime(ms_index, t0, t1));
+        HIP_TRY(hipEventElapsedTime(ms_join, t1, t2));
+        (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(t2);
+    } else {
+        HIP_TRY(hipStreamSynchronize(J.st));
     }
 }
 
@@ -171,74 +422,46 @@ extern "C" int hgx_match_count(hgx_ctx* ctx, const uint8_t* rules, uint32_t rule
         }
         hipStream_t s = ctx->stream;
         Arena arena(ctx->pool.get());
-        uint32_t* d_ev = dalloc<uint32_t>(arena, (size_t)m * A);
-        uint8_t* d_ar = dalloc<uint8_t>(arena, m);
-        const uint64_t nkeys = (uint64_t)nv * A;
-        if (nkeys > 0xFFFFFFF0ull) throw Error{HGX_ERR_UNSUPPORTED, "index above 2^32 keys"};
-        uint32_t* d_cnt = dalloc<uint32_t>(arena, nkeys + 1);
-        uint64_t* d_off = dalloc<uint64_t>(arena, nkeys + 1);
-        uint32_t* d_cur = dalloc<uint32_t>(arena, nkeys + 1);
-        uint64_t occ = 0;
-        for (auto x : ar) occ += x;
-        uint32_t* d_idx = dalloc<uint32_t>(arena, occ);
-        uint32_t* d_seed = dalloc<uint32_t>(arena, (size_t)m + 1);
-        uint64_t* d_soff = dalloc<uint64_t>(arena, (size_t)m + 1);
+        BJMem mem;
+        mem.arena = &arena;
+        uint32_t* d_ev = bget<uint32_t>(mem, (size_t)m * A);
+        uint8_t* d_ar = bget<uint8_t>(mem, m);
         h2d(d_ev, ev.data(), ev.size(), s);
         h2d(d_ar, ar.data(), ar.size(), s);
-        hipEvent_t t0, t1, t2;
-        HIP_TRY(hipEventCreate(&t0)); HIP_TRY(hipEventCreate(&t1)); HIP_TRY(hipEventCreate(&t2));
-        HIP_TRY(hipEventRecord(t0, s));
-        // ---- index build: histogram, scan, scatter, bucket order ----
-        HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (nkeys + 1), s));
-        const uint32_t gb = (m + 255) / 256;
-        if (m) hipLaunchKernelGGL(k_index_count, dim3(gb), dim3(256), 0, s, d_ev, d_ar, m, A, d_cnt);
-        scan_u32_to_u64(arena, d_cnt, d_off, (uint32_t)nkeys, s);
-        // scatter cursors start at the bucket offsets (occurrences < 2^32)
-        hipLaunchKernelGGL(k_off_to_u32, dim3((uint32_t)((nkeys + 255) / 256)), dim3(256), 0, s, d_off, d_cur,
-                           (uint32_t)nkeys);
-        if (m) hipLaunchKernelGGL(k_index_scatter, dim3(gb), dim3(256), 0, s, d_ev, d_ar, m, A, d_cur, d_idx);
-        hipLaunchKernelGGL(k_index_sort, dim3((uint32_t)((nkeys + 255) / 256)), dim3(256), 0, s, d_off,
-                           (uint32_t)nkeys, d_idx);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(t1, s));
-        // ---- join: count, scan, emit ----
-        BigJoinArgs ja{};
-        ja.R = R; ja.ev = d_ev; ja.ar = d_ar; ja.m = m; ja.A = A; ja.off = d_off; ja.idx = d_idx;
-        ja.seed_cnt = d_seed;
-        HIP_TRY(hipMemsetAsync(d_seed + m, 0, sizeof(uint32_t), s));
-        const uint32_t grid = std::max(1u, std::min(gb, ctx->num_cu * 16));
-        if (m) hipLaunchKernelGGL(k_bigjoin<false>, dim3(grid), dim3(256), 0, s, ja);
-        scan_u32_to_u64(arena, d_seed, d_soff, m, s);
-        uint64_t total = 0;
-        d2h(&total, d_soff + m, 1, s);
+        BJRun J{};
+        J.st = s; J.mem = &mem; J.grid = ctx->num_cu * 8;
+        J.v.R = R; J.v.ev = d_ev; J.v.ar = d_ar; J.v.m = m; J.v.A = A;
         uint32_t* d_out = nullptr;
-        if (match_edges && cap && total) {
-            const uint64_t keep = std::min(cap, total);
-            d_out = dalloc<uint32_t>(arena, keep * R.n_lhs);
-            ja.seed_off = d_soff; ja.out = d_out; ja.cap = keep;
-            hipLaunchKernelGGL(k_bigjoin<true>, dim3(grid), dim3(256), 0, s, ja);
-            HIP_TRY(hipGetLastError());
-        }
-        HIP_TRY(hipEventRecord(t2, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (d_out) d2h(match_edges, d_out, std::min(cap, total) * R.n_lhs, s);
-        HIP_TRY(hipStreamSynchronize(s));
         float ms_index = 0, ms_join = 0;
-        HIP_TRY(hipEventElapsedTime(&ms_index, t0, t1));
-        HIP_TRY(hipEventElapsedTime(&ms_join, t1, t2));
-        (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(t2);
-        if (device_ms) { device_ms[0] = ms_index + ms_join; }
+        if (match_edges && cap && cap * R.n_lhs * 4ull > (256ull << 20)) {
+            // a large caller buffer: count first, then hold only the matches there are
+            BJRun C = J;
+            Arena counting(ctx->pool.get());
+            BJMem cm;
+            cm.arena = &counting;
+            C.mem = &cm;
+            bj_match(C, nv, &ms_index, &ms_join, false);
+            cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, C.found));
+        }
+        if (match_edges && cap) {
+            d_out = bget<uint32_t>(mem, cap * R.n_lhs);
+            J.out = d_out; J.cap = cap;
+        }
+        bj_match(J, nv, &ms_index, &ms_join, true);
+        const uint64_t total = J.found;
+        if (d_out && total) d2h(match_edges, d_out, std::min(cap, total) * R.n_lhs, s);
+        HIP_TRY(hipStreamSynchronize(s));
+        if (device_ms) device_ms[0] = ms_index + ms_join;
         if (ctx->prof.on) {
             auto& si = ctx->prof.acc["bigjoin_index"];
             std::snprintf(si.name, sizeof(si.name), "bigjoin_index");
-            si.launches += 1; si.total_ms += ms_index; si.items += m;
-            si.alg_bytes += (uint64_t)m * (4ull * A + 1) + 2ull * 4 * (occ + nkeys + 1);
+            si.launches += 1; si.total_ms += ms_index; si.items += m; si.alg_bytes += J.index_bytes;
             auto& sj = ctx->prof.acc["bigjoin_join"];
             std::snprintf(sj.name, sizeof(sj.name), "bigjoin_join");
-            sj.launches += 1; sj.total_ms += ms_join; sj.items += m;
-            sj.alg_bytes += (uint64_t)m * (4ull * A + 1) + total * 4ull * R.n_lhs;
+            sj.launches += 1; sj.total_ms += ms_join; sj.items += m; sj.alg_bytes += J.alg_bytes;
         }
         *count = total;
         return HGX_OK;
     });
 }
+

@@ -113,6 +113,8 @@ struct EventView {
     uint32_t* rule;           // [n] rule index inside the run
     uint64_t* prod;           // [n] generations of the distinct producer events
     uint64_t* tr;             // [n] the subset kept by transitive reduction
+    uint8_t* cps;             // [n * cK] or null: producer generation of each consumed edge
+    uint32_t cK;              //   (LHS order, 0 = an initial edge), for the triple readback
 };
 
 // ---------------- wave64 primitives ----------------

@@ -40,6 +40,7 @@ void Engine::grow_slots(uint32_t need) {
     auto na = std::make_unique<Arena>(pool_.get());
     DRun* runs = dalloc<DRun>(*na, cap);
     uint64_t* ctr = dalloc<uint64_t>(*na, (size_t)cap * CTR_N);
+    uint64_t* snap = dalloc<uint64_t>(*na, (size_t)cap * CTR_N);
     // per-slot tables uploaded together each generation: budget, range begin, range end (u32),
     // active (u8); and the ones zeroed together: accumulated matches, child edges, totals, the
     // edges of the states with matches
@@ -53,8 +54,13 @@ void Engine::grow_slots(uint32_t need) {
     uint64_t* tot = redges + cap;                     // [cap + 1]
     uint64_t* rpe = tot + cap + 1;
     HIP_TRY(hipMemsetAsync(ctr, 0, sizeof(uint64_t) * cap * CTR_N, st_));
-    if (cap_slots_)
+    HIP_TRY(hipMemsetAsync(snap, 0, sizeof(uint64_t) * cap * CTR_N, st_));
+    if (cap_slots_) {
+        // slots grow only at admission, before any kernel of the step has changed a counter, so
+        // the old counters are the step's snapshot as well
         HIP_TRY(hipMemcpyAsync(ctr, d_ctr_, sizeof(uint64_t) * cap_slots_ * CTR_N, hipMemcpyDeviceToDevice, st_));
+        HIP_TRY(hipMemcpyAsync(snap, d_ctr_, sizeof(uint64_t) * cap_slots_ * CTR_N, hipMemcpyDeviceToDevice, st_));
+    }
     HIP_TRY(hipStreamSynchronize(st_));
     h_runs_.resize(cap);
     h_tab_.assign(13ull * cap, 0);
@@ -63,7 +69,7 @@ void Engine::grow_slots(uint32_t need) {
         free_slots_.push_back(s);
     }
     slot_arena_ = std::move(na);
-    d_runs_ = runs; d_ctr_ = ctr; d_active_ = active; d_budget_ = budget; d_run_acc_ = acc;
+    d_runs_ = runs; d_ctr_ = ctr; d_ctr_snap_ = snap; d_active_ = active; d_budget_ = budget; d_run_acc_ = acc;
     d_run_edges_ = redges;
     d_sbeg_ = sb; d_send_ = se; d_tot_ = tot; d_run_pe_ = rpe;
     cap_slots_ = cap;
@@ -107,11 +113,11 @@ std::shared_ptr<RunRec> Engine::take(uint64_t id) {
 void Engine::admit(std::shared_ptr<RunRec>& r) {
     const uint32_t slot = take_slot();
     r->slot = slot;
-    r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->hp->rules.size());
     uint64_t want = r->hp->cap ? (uint64_t)r->hp->cap + r->hp->n_roots() : std::max<uint64_t>(512, r->hp->n_roots());
     uint64_t mc = 64;
     while (mc < 2 * want) mc <<= 1;
     r->map_cap = (uint32_t)mc;
+    r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->hp->rules.size());
     r->map_keys = (uint64_t*)slab_.alloc(8 * mc);
     r->map_vals = (uint32_t*)slab_.alloc(4 * mc);
     DRun& d = h_runs_[slot];
@@ -153,7 +159,7 @@ void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
         c.causal_triples_tr = tr ? x[CTR_TRIPLES_TR] : c.causal_triples;
         c.branchial_pairs = (r->hp->flags & HGX_REC_BRANCHIAL) ? x[CTR_BRANCHIAL] : 0;
         c.steps_done = r->steps_done;
-        c.truncated = r->truncated ? 1u : 0u;
+        c.truncated = r->status == HGX_ERR_CAPACITY ? 2u : (r->truncated ? 1u : 0u);
         r->t_done = now;
         r->device_ms = std::chrono::duration<double, std::milli>(now - r->t_admit).count();
         finished[r->id] = r;
@@ -179,6 +185,101 @@ constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
 }  // namespace
 
 uint32_t Engine::step() {
+    // runs of the frontier still evolving, and the counters before this generation touches them
+    bool frontier_live = false;
+    for (auto& r : F_runs_) frontier_live |= !r->done && live_.count(r->id) != 0;
+    if (frontier_live && hw_slots_)
+        HIP_TRY(hipMemcpyAsync(d_ctr_snap_, d_ctr_, sizeof(uint64_t) * hw_slots_ * CTR_N,
+                               hipMemcpyDeviceToDevice, st_));
+    std::vector<std::shared_ptr<RunRec>> admitted;
+    try {
+        const uint32_t left = step_impl(admitted);
+        admit_limit_ = 0xFFFFFFFFu;
+        return left;
+    } catch (const Error& e) {
+        if (e.code != HGX_ERR_CAPACITY) throw;
+        recover(e.msg, admitted, frontier_live);
+        return live_or_queued();
+    }
+}
+
+// A generation ran out of the HBM budget (the reference's capacity warning and partial result,
+// gpu/src/evolve.cu:573-700): the counters go back to the snapshot taken before the generation,
+// the frontier's requests finish with the generations they completed, and the requests admitted
+// into the failed generation return to the front of the queue. With no frontier, the admitted
+// requests failed by themselves: one alone finishes with no states; several are re-queued and
+// started one at a time.
+void Engine::recover(const std::string& why, std::vector<std::shared_ptr<RunRec>>& admitted, bool frontier_live) {
+    (void)hipStreamSynchronize(st_);
+    (void)hipGetLastError();
+    if (frontier_live && hw_slots_)
+        HIP_TRY(hipMemcpyAsync(d_ctr_, d_ctr_snap_, sizeof(uint64_t) * hw_slots_ * CTR_N,
+                               hipMemcpyDeviceToDevice, st_));
+    std::vector<std::shared_ptr<RunRec>> done_now, failed_alone;
+    if (!frontier_live && admitted.size() == 1) failed_alone.push_back(admitted[0]);
+    else if (!frontier_live) admit_limit_ = 1;
+    for (auto it = admitted.rbegin(); it != admitted.rend(); ++it) {
+        auto& r = *it;
+        if (!failed_alone.empty() && r == failed_alone[0]) continue;
+        if (r->slot != 0xFFFFFFFFu) free_slots_.push_back(r->slot);
+        slab_.free(r->d_rules, sizeof(DRule) * r->hp->rules.size());
+        slab_.free(r->map_keys, 8ull * r->map_cap);
+        slab_.free(r->map_vals, 4ull * r->map_cap);
+        r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
+        r->slot = 0xFFFFFFFFu;
+        if (r->admitted) {
+            live_.erase(r->id);
+            live_footprint_ -= r->footprint;
+        }
+        r->admitted = false;
+        pending_.push_front(r);
+    }
+    for (auto& r : failed_alone) {
+        // no generation of it completed: it finishes with no states, the capacity status and
+        // its reason, and whatever its admission took is given back
+        if (r->slot != 0xFFFFFFFFu) free_slots_.push_back(r->slot);
+        slab_.free(r->d_rules, sizeof(DRule) * r->hp->rules.size());
+        slab_.free(r->map_keys, 8ull * r->map_cap);
+        slab_.free(r->map_vals, 4ull * r->map_cap);
+        r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
+        if (r->admitted) {
+            live_.erase(r->id);
+            live_footprint_ -= r->footprint;
+        }
+        r->states = 0; r->events = 0; r->steps_done = 0;
+        r->segs.clear();
+        r->status = HGX_ERR_CAPACITY;
+        r->warning = "the request's initial states do not fit the context's HBM budget: " + why;
+        std::memset(&r->counts, 0, sizeof(r->counts));
+        r->counts.truncated = 2;
+        r->t_done = Clock::now();
+        r->finishing = true;
+        finished[r->id] = r;
+    }
+    for (auto& r : F_runs_) {
+        if (!live_.count(r->id) || r->finishing) continue;
+        if (!r->done) {
+            r->done = true;
+            r->truncated = true;
+            r->status = HGX_ERR_CAPACITY;
+            r->warning = "generation " + std::to_string(r->depth + 1) +
+                         " does not fit the context's HBM budget; the result holds generations 0.." +
+                         std::to_string(r->depth) + ": " + why;
+        }
+        r->finishing = true;
+        done_now.push_back(r);
+    }
+    finalize(done_now);
+    // the frontier is gone with its requests: every slot it held is free now
+    if (F_ && !F_->keep_edges) F_->edges.release();
+    F_.reset();
+    prev_ = PrevMatches{};
+    F_runs_.clear(); F_b_.clear(); F_e_.clear();
+    for (uint32_t s2 : release_next_) free_slots_.push_back(s2);
+    release_next_.clear();
+}
+
+uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
     const auto t_step = Clock::now();
     double t_count = 0, t_emit = 0, t_child = 0, t_launch = 0, t_fin = 0;
     hipStream_t st = st_;
@@ -321,6 +422,67 @@ uint32_t Engine::step() {
                     tot[i] = t;
                     parent_edges += hpe[r.slot];          // parents the expansion reads
                     for (auto& R : r.hp->rules) K = std::max<uint32_t>(K, R.n_lhs);
+                    if (!r.hp->cap) {
+                        // an uncapped request is charged what it holds once this generation
+                        // is built, when that exceeds its admission estimate, so a request that
+                        // grows keeps the queue from being admitted past the budget
+                        const uint64_t f = footprint_bytes(*r.hp, r.states + t);
+                        if (f > r.footprint) { live_footprint_ += f - r.footprint; r.footprint = f; }
+                    }
+                }
+            }
+            // HBM check before anything of the generation is written: what this generation will
+            // take (match list, child state and event records, child edges, the step's scratch)
+            // against what the budget still hands out. When it does not fit, the requests with
+            // the largest children stop here -- they finish with the generations they completed,
+            // status HGX_ERR_CAPACITY (the reference's capacity warning with a partial result,
+            // gpu/src/evolve.cu:573-700) -- and the others build this generation unchanged, so
+            // their results are what they would be alone.
+            {
+                // per run: (bytes of its children's records in new arenas, frontier index); the
+                // match list and scratch grow arenas that already hold a chunk
+                std::vector<std::pair<uint64_t, uint32_t>> need;
+                uint64_t fresh = 0, grow = 0;
+                for (uint32_t i = 0; i < nF; ++i) {
+                    if (!tot[i]) continue;
+                    const RunHost& h = *F_runs_[i]->hp;
+                    uint32_t k = 1;
+                    for (auto& R : h.rules) k = std::max<uint32_t>(k, R.n_lhs);
+                    const uint64_t rec = tot[i] * (57ull + 8ull * h.steps) +
+                                         h_edges[F_runs_[i]->slot] * (4ull * h.max_arity + 2ull);
+                    const uint64_t scr = tot[i] * (8ull + 2ull * k + 16ull);
+                    need.push_back({rec + scr, i});
+                    fresh += rec;
+                    grow += scr;
+                }
+                // arenas take chunks of at least 64 MiB: the child generation opens two (state
+                // and event records, edges); one more chunk covers what the rest adds
+                const uint64_t chunk = (uint64_t)64 << 20;
+                auto fits = [&] {
+                    const uint64_t want = std::max(fresh, 2 * chunk) + grow + chunk;
+                    return pool_->in_use() + want <= pool_->budget();
+                };
+                if (!fits()) {
+                    std::sort(need.begin(), need.end(), [](auto& x, auto& y) { return x.first > y.first; });
+                    for (auto& [b, i] : need) {
+                        if (fits()) break;
+                        RunRec& r = *F_runs_[i];
+                        r.done = true;
+                        r.truncated = true;
+                        r.status = HGX_ERR_CAPACITY;
+                        r.warning = "generation " + std::to_string(r.depth + 1) + " (" + std::to_string(tot[i]) +
+                                    " states) does not fit the context's HBM budget; the result holds generations 0.." +
+                                    std::to_string(r.depth);
+                        h_active[r.slot] = 0;
+                        masked = true;
+                        parent_edges -= hpe[r.slot];
+                        const RunHost& h = *r.hp;
+                        uint32_t k = 1;
+                        for (auto& R : h.rules) k = std::max<uint32_t>(k, R.n_lhs);
+                        fresh -= tot[i] * (57ull + 8ull * h.steps) + h_edges[r.slot] * (4ull * h.max_arity + 2ull);
+                        grow -= tot[i] * (8ull + 2ull * k + 16ull);
+                        tot[i] = 0;
+                    }
                 }
             }
             if (masked) {
@@ -364,14 +526,13 @@ uint32_t Engine::step() {
 
     t_emit = ms_since(t_step);
     // ---- 2. admissions: queued requests join the generation built now ----
-    std::vector<std::shared_ptr<RunRec>> admitted;
     const uint64_t soft = (uint64_t)(pool_->budget() * soft_budget_frac);
-    while (!pending_.empty()) {
+    while (!pending_.empty() && admitted.size() < admit_limit_) {
         auto r = pending_.front();
         if (!live_.empty() && live_footprint_ + r->footprint > soft) break;
         pending_.pop_front();
+        admitted.push_back(r);                  // listed first: a failed admission is rolled back
         admit(r);
-        admitted.push_back(r);
     }
     // requests with host inputs first, then prepared ones (device-resident inputs)
     std::stable_partition(admitted.begin(), admitted.end(),
@@ -465,6 +626,15 @@ uint32_t Engine::step() {
             C->ev.rule = dalloc<uint32_t>(C->meta, M);
             C->ev.prod = dalloc<uint64_t>(C->meta, M);
             C->ev.tr = dalloc<uint64_t>(C->meta, M);
+            // with causal relations materialised: the producer generation of every consumed
+            // edge, for the causal triple readback
+            bool triples = false;
+            for (uint32_t i = 0; i < nF; ++i)
+                if (tot[i] && (F_runs_[i]->hp->flags & (HGX_REC_CAUSAL | HGX_REC_RELATIONS)) ==
+                                  (HGX_REC_CAUSAL | HGX_REC_RELATIONS))
+                    triples = true;
+            C->ev.cps = triples ? dalloc<uint8_t>(C->meta, M * K) : nullptr;
+            C->ev.cK = K;
             scan_u32_to_u64(tmp, mv.cne, v.eoff, (uint32_t)M, st);
             if (keep) {                          // consumed positions, for event edge readback
                 C->cq = dalloc<uint16_t>(C->meta, M * K);
@@ -1032,10 +1202,17 @@ extern "C" int hgx_request_wait(hgx_ctx* ctx, uint64_t id, hgx_result** out) {
         *out = nullptr;
         if (!ctx->serve || !ctx->serve->known(id)) { ctx->err = "unknown request id"; return HGX_ERR_INVALID; }
         HIP_TRY(hipSetDevice(ctx->device));
+        // every step finishes or advances something (a generation that does not fit the budget
+        // finishes the requests it held), so this ends; the bound guards against a lost id
         while (true) {
             auto r = ctx->serve->take(id);
             if (r) { *out = make_result(r); return HGX_OK; }
-            ctx->serve->step();
+            if (!ctx->serve->known(id) || ctx->serve->step() == 0) {
+                auto late = ctx->serve->take(id);
+                if (late) { *out = make_result(late); return HGX_OK; }
+                ctx->err = "request " + std::to_string(id) + " is no longer queued, running or finished";
+                return HGX_ERR_INVALID;
+            }
         }
     });
 }
@@ -1085,6 +1262,17 @@ extern "C" int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint6
     if (device_ns) *device_ns = r->device_ns;
     if (latency_ns) *latency_ns = r->latency_ns;
     return HGX_OK;
+}
+
+extern "C" int hgx_result_status(const hgx_result* r, char* msg, size_t len) {
+    if (!r) return HGX_ERR_INVALID;
+    const std::string& w = r->run->warning;
+    if (msg && len) {
+        const size_t k = std::min(len - 1, w.size());
+        std::memcpy(msg, w.data(), k);
+        msg[k] = 0;
+    }
+    return r->run->status;
 }
 
 extern "C" void hgx_result_release(hgx_result* r) { delete r; }
@@ -1142,6 +1330,56 @@ extern "C" int hgx_result_causal(const hgx_result* r, hgx_pair* pairs, uint8_t* 
             }
         }
         return HGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    }
+}
+
+extern "C" int hgx_result_causal_triples(const hgx_result* r, hgx_triple* triples, uint8_t* in_tr, uint64_t cap) {
+    if (!r) return HGX_ERR_INVALID;
+    const RunRec& R = *r->run;
+    if ((R.hp->flags & (HGX_REC_CAUSAL | HGX_REC_RELATIONS)) != (HGX_REC_CAUSAL | HGX_REC_RELATIONS) ||
+        cap < R.counts.causal_triples)
+        return HGX_ERR_INVALID;
+    try {
+        RunTables T = run_tables(R);
+        const uint32_t nroots = R.hp->n_roots();
+        // producer generation of every consumed edge, per event
+        std::vector<uint8_t> cps;
+        uint32_t K = 0;
+        for (size_t k = 1; k < R.segs.size(); ++k) {          // segment 0 holds the roots
+            const Segment& sg = R.segs[k];
+            const uint32_t m = sg.e - sg.b;
+            if (!m) continue;
+            if (!sg.g->ev.cps) throw Error{HGX_ERR_INVALID, "causal triples were not recorded"};
+            const uint32_t gk = sg.g->ev.cK;
+            if (!K) { K = 16; cps.assign((size_t)R.events * K, 0); }
+            std::vector<uint8_t> rows((size_t)m * gk);
+            std::vector<uint32_t> sid(m);
+            d2h(rows.data(), sg.g->ev.cps + (size_t)sg.b * gk, rows.size(), R.st);
+            d2h(sid.data(), sg.g->v.sid + sg.b, m, R.st);
+            HIP_TRY(hipStreamSynchronize(R.st));
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t e = (uint64_t)sid[i] - nroots;
+                if (e >= R.events) throw Error{HGX_ERR_DEVICE, "event id out of range in readback"};
+                std::memcpy(&cps[e * K], &rows[(size_t)i * gk], std::min<uint32_t>(gk, K));
+            }
+        }
+        uint64_t o = 0;
+        for (uint64_t c = 0; c < R.events; ++c) {
+            const uint32_t nl = R.hp->rules.at(T.ev_rule[c]).n_lhs;
+            for (uint32_t t = 0; t < nl; ++t) {
+                const uint32_t ps = cps[c * K + t];
+                if (!ps) continue;
+                uint32_t x = T.ev_in[c];                 // up the lineage to generation ps
+                while (T.step[x] > ps) x = T.ev_in[T.creator[x]];
+                if (o >= cap) return HGX_ERR_INVALID;
+                if (triples) triples[o] = hgx_triple{T.creator[x], (uint32_t)c, t};
+                if (in_tr) in_tr[o] = (uint8_t)((T.ev_tr[c] >> ps) & 1u);
+                ++o;
+            }
+        }
+        return o == R.counts.causal_triples ? HGX_OK : HGX_ERR_DEVICE;
     } catch (const Error& e) {
         return e.code;
     }

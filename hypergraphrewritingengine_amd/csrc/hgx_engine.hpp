@@ -12,6 +12,7 @@
 #include <deque>
 #include <unordered_map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "hgx_runtime.hpp"
@@ -57,6 +58,11 @@ struct RunRec {
     uint32_t depth = 0, steps_done = 0;
     uint32_t ne_bound = 0, nv_bound = 0;   // bounds on edges / vertex labels of its frontier states
     bool truncated = false, done = false, admitted = false;
+    // Why the run stopped early: HGX_OK (it did not, or its state cap stopped it) or
+    // HGX_ERR_CAPACITY (a generation did not fit the context's HBM budget: the result holds the
+    // generations completed before it, with `warning` saying so).
+    int status = 0;
+    std::string warning;
     bool finishing = false;          // listed for finalize in the current step
     uint64_t footprint = 0;
     std::vector<Segment> segs;       // every state of the run, in state-id order
@@ -76,6 +82,10 @@ public:
     uint64_t submit(std::shared_ptr<const RunHost> h, uint64_t id);
     void reserve_slots(uint32_t n) { grow_slots(n); live_.reserve(n); finished.reserve(n); }
     // One generation of every admitted request, plus admissions. Returns live + queued runs.
+    // A generation that does not fit the HBM budget does not fail the engine: the requests in
+    // the frontier finish with the generations they completed (status HGX_ERR_CAPACITY), the
+    // requests admitted into that generation go back to the queue, and a request that fails
+    // alone finishes with no states -- so every step makes progress.
     uint32_t step();
     uint32_t live_or_queued() const { return (uint32_t)(live_.size() + pending_.size()); }
     // A finished run (removed from the finished set), or null.
@@ -94,6 +104,7 @@ private:
     uint32_t hw_slots_ = 0;                   // one past the highest slot ever taken
     DRun* d_runs_ = nullptr;
     uint64_t* d_ctr_ = nullptr;
+    uint64_t* d_ctr_snap_ = nullptr;          // the counters at the start of the step
     uint8_t* d_active_ = nullptr;
     uint32_t* d_budget_ = nullptr;
     uint64_t* d_run_acc_ = nullptr;
@@ -112,6 +123,7 @@ private:
     std::vector<std::shared_ptr<RunRec>> F_runs_;   // runs with states in F, in F order
     std::vector<uint32_t> F_b_, F_e_;         // their ranges, parallel to F_runs_
     uint64_t live_footprint_ = 0;
+    uint32_t admit_limit_ = 0xFFFFFFFFu;       // admissions per step (1 after a failed joint start)
     // The previous generation's match list, kept one generation longer: state s < prev_.n of
     // the frontier was made by match s, and its parent's matches seed its own (forwarding).
     struct PrevMatches {
@@ -123,6 +135,8 @@ private:
         uint64_t* moff = nullptr;
     } prev_;
 
+    uint32_t step_impl(std::vector<std::shared_ptr<RunRec>>& admitted);
+    void recover(const std::string& why, std::vector<std::shared_ptr<RunRec>>& admitted, bool frontier_live);
     void grow_slots(uint32_t need);
     uint32_t take_slot();
     void admit(std::shared_ptr<RunRec>& r);

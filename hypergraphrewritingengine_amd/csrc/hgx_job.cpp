@@ -29,6 +29,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/hgx_gpu_backend.hpp"
 #include "hgx_runtime.hpp"
 #include "hgx_wxf.hpp"
 
@@ -327,6 +328,12 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     Run R;
     struct Release { hgx_result* r; ~Release() { hgx_result_release(r); } } rel{res};
     check(hgx_result_counts(res, &R.c), ctx, "counts");
+    // a generation beyond the HBM budget: the result holds the ones before it, and says so
+    std::vector<std::pair<std::string, std::string>> warns = j.warnings;
+    {
+        char why[512] = {0};
+        if (hgx_result_status(res, why, sizeof(why)) == HGX_ERR_CAPACITY) warns.emplace_back("CapacityExceeded", why);
+    }
     const uint64_t ns = R.c.raw_states, ne = R.c.events;
     R.hash.resize(ns); R.step.resize(ns); R.creator.resize(ns);
     check(hgx_result_states(res, R.hash.data(), R.step.data(), R.creator.data(), ns), ctx, "states");
@@ -499,9 +506,9 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (j.inc_nevents) out.put("NumEvents", Value::integer((int64_t)n_ev_classes));
     if (j.inc_ncausal) out.put("NumCausalEdges", Value::integer(n_causal));
     if (j.inc_nbranchial) out.put("NumBranchialEdges", Value::integer((int64_t)R.c.branchial_pairs));
-    if (!j.warnings.empty()) {
+    if (!warns.empty()) {
         Value wl = Value::list();
-        for (auto& w : j.warnings) {
+        for (auto& w : warns) {
             Value wa = Value::assoc();
             wa.put("Kind", Value::string(w.first));
             wa.put("Count", Value::integer(1));
@@ -558,3 +565,70 @@ extern "C" int hgx_run_wxf(hgx_ctx* ctx, const uint8_t* job, size_t n, uint8_t**
 }
 
 extern "C" void hgx_buffer_free(uint8_t* p) { std::free(p); }
+
+// ---- run_gpu_evolution (paclet_source/hg_gpu_backend.hpp:119) ------------------------------
+
+namespace hgx {
+
+std::vector<uint8_t> run_gpu_evolution(const GpuJob& g, const HostBridge& host) {
+    auto refuse = [](const std::string& what) {
+        throw std::runtime_error(what + " is not served (this backend evolves the full multiway system)");
+    };
+    if (g.explore_from_canonical_states_only) refuse("ExploreFromCanonicalStatesOnly");
+    if (g.quotient_initial_states) refuse("QuotientInitialStates");
+    if (g.exploration_probability != 1.0 || g.transition_rate != 1.0 || !g.rule_weights.empty())
+        refuse("sampling");
+    if (g.max_states_per_step || g.max_successor_states_per_parent || g.matches_per_state_rule) refuse("a cap");
+    if (!g.graph_properties.empty()) refuse("GraphProperties");
+    if (g.branchial_step != 0) refuse("BranchialStep");
+    if (g.show_genesis_events) refuse("ShowGenesisEvents");
+    if (g.include_branchial_state_edges || g.include_branchial_state_edges_all_siblings ||
+        g.include_global_edges || g.include_state_bitvectors)
+        refuse("that RequestedData component");
+    if (!g.session_op.empty() && g.session_op != "Evolve") refuse("a session (" + g.session_op + ")");
+    if (g.state_canon_mode == GpuJob::StateCanonCode::kAutomatic) refuse("CanonicalizeStates -> Automatic");
+    if (g.steps < 0) throw std::runtime_error("Steps must be non-negative");
+
+    Job j;
+    for (const auto& r : g.rules) {
+        if (r.second.size() != 2) throw std::runtime_error("rule " + r.first + " must be {lhs, rhs}");
+        j.rules.emplace_back(r.second[0], r.second[1]);
+    }
+    if (j.rules.empty()) throw std::runtime_error("the job has no Rules");
+    j.states = g.initial_states;
+    j.steps = g.steps;
+    j.tr = g.transitive_reduction;
+    j.full = g.state_canon_mode == GpuJob::StateCanonCode::kFull;
+    constexpr uint32_t In = 1, Out = 2, Step = 4, Cons = 16, Prod = 32;
+    j.ev_keys = g.event_canon_mode == GpuJob::EventCanonCode::kFull ? (In | Out)
+              : g.event_canon_mode == GpuJob::EventCanonCode::kAutomatic ? (In | Out | Step | Cons | Prod) : 0u;
+    j.inc_states = g.include_states;
+    j.inc_events = g.include_events;
+    j.inc_events_min = g.include_events_minimal;
+    j.inc_causal = g.include_causal_edges;
+    j.inc_branchial = g.include_branchial_edges;
+    j.inc_hashes = g.include_canonical_hashes;
+    j.inc_nstates = g.include_num_states;
+    j.inc_nevents = g.include_num_events;
+    j.inc_ncausal = g.include_num_causal_edges;
+    j.inc_nbranchial = g.include_num_branchial_edges;
+
+    // one context for the process, created on first use with the first job's budget
+    static std::mutex mu;
+    static hgx_ctx* ctx = nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ctx) {
+        const int rc = hgx_ctx_create(0, g.max_device_memory_bytes, &ctx);
+        if (rc != HGX_OK) {
+            char buf[512] = {0};
+            hgx_last_error(nullptr, buf, sizeof(buf));
+            ctx = nullptr;
+            throw std::runtime_error(std::string("no device context: ") + buf);
+        }
+    }
+    if (host.progress) host.progress("hgx: evolving " + std::to_string(g.steps) + " steps on the device");
+    const Value res = job::evolve(ctx, j);
+    return wxf::serialize(res);
+}
+
+}  // namespace hgx

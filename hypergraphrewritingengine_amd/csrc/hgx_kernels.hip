@@ -758,6 +758,8 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
             if (ps_t) anc_t = a.P.lanc[(size_t)par * a.P.lstride + ps_t - 1];
         }
 
+        if (a.E.cps && l < nl) a.E.cps[(size_t)c * a.E.cK + l] = (uint8_t)ps_t;
+
         // Parent edges minus the consumed ones, order kept (SparseBitset::derive).
         uint32_t kept = 0;
         for (uint32_t base = 0; base < pne; base += Team<W>::N) {

@@ -41,7 +41,9 @@ Chunk ChunkPool::get(size_t min_bytes) {
     size_t sz = std::max(min_bytes, chunk_bytes_);
     sz = (sz + (2u << 20) - 1) & ~((size_t)(2u << 20) - 1);
     if (total_ + sz > budget_) {
-        // give back idle chunks before refusing
+        // give back idle chunks before refusing (after the device is idle: a chunk returned to
+        // the pool may still be read by work queued before it was returned)
+        if (!free_.empty()) (void)hipDeviceSynchronize();
         for (auto& c : free_) { (void)hipFree(c.p); total_ -= c.size; }
         free_.clear();
         if (total_ + sz > budget_)
@@ -49,7 +51,10 @@ Chunk ChunkPool::get(size_t min_bytes) {
                                               " MiB more, budget " + std::to_string(budget_ >> 20) + " MiB"};
     }
     void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, sz));
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{HGX_ERR_CAPACITY, "the device has no " + std::to_string(sz >> 20) + " MiB left"};
+    }
     total_ += sz;
     in_use_ += sz;
     return Chunk{p, sz};
@@ -167,7 +172,7 @@ static void join_order(DRule& R) {
     }
 }
 
-static void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<DRule>& out) {
+void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<DRule>& out) {
     uint32_t k = 0;
     for (uint32_t r = 0; r < n; ++r) {
         DRule R;
@@ -328,13 +333,15 @@ RunHost parse_run(const hgx_run_desc* d) {
 }
 
 
-// HBM a request is charged at admission: every state it may create (its cap, or a bound from
-// its size when uncapped) at its largest edge count, in the generation that holds it and the
-// one built from it, plus its state map.
-uint64_t footprint_bytes(const RunHost& h) {
+// HBM a request is charged at admission: every state it may create (its cap, or 2^20 when
+// uncapped) at its largest edge count, in the generation that holds it and the one built from
+// it, plus its state map. With `held` > 0: the same for that many states (an uncapped request is
+// re-charged as it grows).
+uint64_t footprint_bytes(const RunHost& h, uint64_t held) {
     uint64_t max_ne = h.max_root_ne;
     max_ne += (uint64_t)h.steps * h.max_gain;
-    const uint64_t states = h.cap ? (uint64_t)h.cap : (uint64_t)1 << 20;
+    uint64_t states = h.cap ? (uint64_t)h.cap : (uint64_t)1 << 20;
+    if (held) states = held;
     const uint64_t per_state = 2 * max_ne * (4ull * h.max_arity + 2) + 96 + 8ull * h.steps;
     uint64_t map = 64;
     while (map < 2 * states) map <<= 1;

@@ -142,7 +142,9 @@ struct RunHost {
     uint64_t root_edges = 0;
 };
 RunHost parse_run(const hgx_run_desc* d);
-uint64_t footprint_bytes(const RunHost& h);
+// The rule program (include/hgx.h) into device rules with the reference's join order.
+void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<DRule>& out);
+uint64_t footprint_bytes(const RunHost& h, uint64_t held = 0);
 
 // Per-state workspace (bytes) up to which one wave handles a state, and up to which four waves
 // share one; above, eight (a CU holds 160 KiB of LDS).

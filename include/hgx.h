@@ -117,7 +117,10 @@ typedef struct hgx_counts {
     uint64_t branchial_pairs;
     uint64_t fingerprint;
     uint32_t steps_done;       /* generations actually evolved                        */
-    uint32_t truncated;        /* 1 when state_cap stopped the run before `steps`      */
+    uint32_t truncated;        /* 1 when state_cap stopped the run before `steps`;
+                                  2 when a generation did not fit the context's HBM
+                                  budget: the counts are those of the generations
+                                  completed before it (hgx_result_status says why)    */
 } hgx_counts;
 
 /* Event record. States are numbered roots first (0..n_roots-1), then the child of event
@@ -246,11 +249,31 @@ int hgx_result_state_edges(const hgx_result* r, uint64_t* state_offsets, uint32_
 /* Causal pairs (needs HGX_REC_CAUSAL|HGX_REC_RELATIONS), deduplicated; in_tr[i] = 1 if the
  * pair survives transitive reduction. */
 int hgx_result_causal(const hgx_result* r, hgx_pair* pairs, uint8_t* in_tr, uint64_t cap);
+/* Causal triples (needs HGX_REC_CAUSAL|HGX_REC_RELATIONS): one per consumed edge that an event
+ * produced -- the entries of CausalGraph::get_causal_edges() (hypergraph/src/causal_graph.cpp:
+ * 187-234, CausalEdge{producer, consumer, edge}, hypergraph/include/hypergraph/types.hpp:239-248)
+ * -- with the consumed edge named by its LHS index in the consumer's match (the reference names
+ * it by its global EdgeId, kept there "for debugging/viz"). causal_triples of them, in consumer
+ * order, then LHS order; in_tr[i] = 1 if the triple's pair survives transitive reduction (the
+ * triples the reference keeps with reduction on). */
+typedef struct hgx_triple {
+    uint32_t producer;
+    uint32_t consumer;
+    uint32_t lhs_index;
+} hgx_triple;
+int hgx_result_causal_triples(const hgx_result* r, hgx_triple* triples, uint8_t* in_tr, uint64_t cap);
 /* Branchial pairs (needs HGX_REC_BRANCHIAL|HGX_REC_RELATIONS), (smaller, larger) event ids. */
 int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64_t cap);
 /* Device time spent on this request's generations, and its latency
  * (completion - submission) on the served path, in nanoseconds. */
 int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint64_t* latency_ns);
+/* HGX_OK for a complete result (or one its state cap stopped), HGX_ERR_CAPACITY for a partial one:
+ * a generation did not fit the context's HBM budget, and the result holds the generations before
+ * it (none, when the request's initial states did not fit). The reason goes to msg (at most len-1
+ * bytes plus NUL; msg may be NULL). The counterpart of EvolveResult::warnings and the partial
+ * result the reference returns on capacity overflow (gpu/src/evolve.cu:573-700,
+ * gpu/include/hg_gpu/errors.hpp:31-69). */
+int hgx_result_status(const hgx_result* r, char* msg, size_t len);
 void hgx_result_release(hgx_result* r);
 /* The same for n results at once (a whole batch; null entries are skipped). */
 void hgx_result_release_many(hgx_result* const* rs, uint32_t n);

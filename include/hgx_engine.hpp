@@ -1,12 +1,16 @@
 // hgx_engine.hpp -- header-only C++ face of the C ABI (include/hgx.h), shaped like the two C++
 // surfaces the reference's callers use for this path:
 //
-//   hgx::ParallelEvolutionEngine   add_rule() / evolve() / counters, mirroring
-//       hypergraph::ParallelEvolutionEngine (hypergraph/include/hypergraph/parallel_evolution.hpp:
-//       741-1041; add_rule :756, evolve :1009 and the multi-root overload :1041, setters
-//       :758-935). Results are read from the engine instead of a caller-owned Hypergraph: the
-//       raw-state/event/relation accessors below return what Hypergraph::get_state / get_event /
-//       causal_graph() expose for this path.
+//   hgx::Hypergraph + hgx::ParallelEvolutionEngine(Hypergraph*, num_threads, ExecutionMode)
+//       add_rule() / evolve(), mirroring hypergraph::ParallelEvolutionEngine
+//       (hypergraph/include/hypergraph/parallel_evolution.hpp:741-1041; the constructor :741-742,
+//       add_rule :756, evolve :1009 and the multi-root overload :1041, setters :758-935). The
+//       results land in the caller-owned Hypergraph and are read through its accessors
+//       (hypergraph/include/hypergraph/hypergraph.hpp:555-812: get_state, num_published_states,
+//       get_or_compute_canonical_hash, num_canonical_states, get_event, num_events,
+//       causal_graph()), so code that drives the reference engine compiles against this header
+//       with the include and the namespace changed. The engine also keeps the direct accessors of
+//       earlier versions (num_states(), state_hashes(), events(), ...).
 //   hgx::EvolveInput / EvolveResult / evolve()   mirroring hg_gpu::EvolveInput / EvolveResult
 //       (gpu/include/hg_gpu/evolve.hpp:26-259) for the fields of a full-multiway run, the
 //       struct run_gpu_evolution (paclet_source/hg_gpu_backend.cpp:174) builds and reads.
@@ -32,6 +36,9 @@ namespace hgx {
 using VertexId = uint32_t;
 using StateId = uint32_t;
 using EventId = uint32_t;
+using EdgeId = uint32_t;
+using RuleIndex = uint16_t;
+constexpr uint32_t INVALID_ID = 0xFFFFFFFFu;      // hypergraph/include/hypergraph/types.hpp
 
 class Error : public std::runtime_error {
 public:
@@ -118,25 +125,142 @@ private:
     std::unique_ptr<hgx_ctx, detail::CtxDeleter> ctx_;
 };
 
-struct Event {
-    EventId id;
-    StateId input_state, output_state;
-    uint32_t rule_index, step;
+// hypergraph::State / Event / CausalEdge / BranchialEdge (types.hpp:131-258) for this path.
+struct State {
+    StateId id = INVALID_ID;
+    uint32_t step = 0;
+    uint64_t canonical_hash = 0;            // the WL hash
+    EventId parent_event = INVALID_ID;      // the event that created it, INVALID_ID for a root
 };
-struct CausalEdge { EventId producer, consumer; bool in_reduction; };
-struct BranchialEdge { EventId event1, event2; };
+struct Event {
+    EventId id = INVALID_ID;
+    StateId input_state = INVALID_ID, output_state = INVALID_ID;
+    uint32_t rule_index = 0, step = 0;
+};
+// A causal edge: the reference lists one per (producer, consumer, consumed edge) and names the
+// edge by its global EdgeId; here the edge is its LHS index in the consumer's match. The
+// engine's causal_edges() lists deduplicated event pairs (edge = INVALID_ID there).
+struct CausalEdge { EventId producer, consumer; EdgeId edge; bool in_reduction; };
+struct BranchialEdge { EventId event1, event2; EdgeId shared_edge = INVALID_ID; };
+
+// The process-wide device context the Hypergraph-shaped engines share (the reference's GPU
+// backend likewise keeps one process-lifetime evolver, paclet_source/hg_gpu_backend.cpp:177-183).
+inline std::shared_ptr<Device> default_device() {
+    static std::shared_ptr<Device> dev = std::make_shared<Device>(0);
+    return dev;
+}
+
+// hypergraph::CausalGraph's counters and edge lists (causal_graph.cpp:187-386) over one result.
+class CausalGraph {
+public:
+    size_t num_causal_edges() const { return (size_t)(tr_ ? c_.causal_triples_tr : c_.causal_triples); }
+    size_t num_causal_event_pairs() const { return (size_t)(tr_ ? c_.causal_pairs_tr : c_.causal_pairs); }
+    size_t num_branchial_edges() const { return (size_t)c_.branchial_pairs; }
+    // One entry per causal triple (with reduction on: the triples whose pair it keeps). Needs the
+    // relations materialised (the engine's default).
+    std::vector<CausalEdge> get_causal_edges() const {
+        std::vector<CausalEdge> out;
+        if (!res_ || !c_.causal_triples) return out;
+        std::vector<hgx_triple> t(c_.causal_triples);
+        std::vector<uint8_t> k(c_.causal_triples);
+        detail::check(hgx_result_causal_triples(res_.get(), t.data(), k.data(), t.size()), nullptr);
+        out.reserve(t.size());
+        for (size_t i = 0; i < t.size(); ++i)
+            if (!tr_ || k[i]) out.push_back(CausalEdge{t[i].producer, t[i].consumer, t[i].lhs_index, k[i] != 0});
+        return out;
+    }
+    std::vector<BranchialEdge> get_branchial_edges() const {
+        std::vector<BranchialEdge> out;
+        if (!res_ || !c_.branchial_pairs) return out;
+        std::vector<hgx_pair> p(c_.branchial_pairs);
+        detail::check(hgx_result_branchial(res_.get(), p.data(), p.size()), nullptr);
+        out.reserve(p.size());
+        for (const auto& x : p) out.push_back(BranchialEdge{x.a, x.b});
+        return out;
+    }
+
+private:
+    friend class Hypergraph;
+    std::shared_ptr<hgx_result> res_;
+    hgx_counts c_{};
+    bool tr_ = false;
+};
+
+// hypergraph::Hypergraph as the results of an evolution are read from it (hypergraph.hpp:555-812).
+// Owned by the caller; an engine constructed over it fills it on evolve(). Raw states are numbered
+// roots first, then the output state of event e is n_roots + e; every raw state is published.
+class Hypergraph {
+public:
+    uint32_t num_states() const { return (uint32_t)c_.raw_states; }
+    uint32_t num_published_states() const { return (uint32_t)c_.raw_states; }
+    uint32_t num_events() const { return (uint32_t)c_.events; }
+    uint32_t num_published_events() const { return (uint32_t)c_.events; }
+    size_t num_canonical_states() const { return (size_t)c_.canonical_states; }
+    const State& get_state(StateId sid) const {
+        load_states();
+        static const State invalid{};
+        return sid < states_.size() ? states_[sid] : invalid;
+    }
+    uint64_t get_or_compute_canonical_hash(StateId sid) const { return get_state(sid).canonical_hash; }
+    const Event& get_event(EventId eid) const {
+        load_events();
+        static const Event invalid{};
+        return eid < events_.size() ? events_[eid] : invalid;
+    }
+    const CausalGraph& causal_graph() const { return cg_; }
+    const hgx_counts& counts() const { return c_; }
+
+    // Filled by ParallelEvolutionEngine::evolve.
+    void assign(std::shared_ptr<hgx_result> res, const hgx_counts& c, bool tr) {
+        res_ = res;
+        c_ = c;
+        cg_.res_ = res;
+        cg_.c_ = c;
+        cg_.tr_ = tr;
+        states_.clear();
+        events_.clear();
+    }
+
+private:
+    void load_states() const {
+        if (!res_ || states_.size() == c_.raw_states) return;
+        std::vector<uint64_t> h(c_.raw_states);
+        std::vector<uint32_t> st(c_.raw_states), cr(c_.raw_states);
+        detail::check(hgx_result_states(res_.get(), h.data(), st.data(), cr.data(), h.size()), nullptr);
+        states_.resize(h.size());
+        for (size_t s = 0; s < h.size(); ++s) states_[s] = State{(StateId)s, st[s], h[s], cr[s]};
+    }
+    void load_events() const {
+        if (!res_ || events_.size() == c_.events || !c_.events) return;
+        std::vector<hgx_event> raw(c_.events);
+        detail::check(hgx_result_events(res_.get(), raw.data(), raw.size()), nullptr);
+        events_.resize(raw.size());
+        for (size_t i = 0; i < raw.size(); ++i)
+            events_[i] = Event{(EventId)i, raw[i].input_state, raw[i].output_state, raw[i].rule, raw[i].step};
+    }
+    std::shared_ptr<hgx_result> res_;
+    hgx_counts c_{};
+    CausalGraph cg_;
+    mutable std::vector<State> states_;
+    mutable std::vector<Event> events_;
+};
 
 // hypergraph::ParallelEvolutionEngine for the full-multiway, WL-hashed path.
 class ParallelEvolutionEngine {
 public:
     enum class ExecutionMode { Parallel, Serial };   // both run on the device; kept for signature parity
 
-    explicit ParallelEvolutionEngine(std::shared_ptr<Device> dev = nullptr, size_t num_threads = 0,
+    // The reference's constructor (parallel_evolution.hpp:741-742): results go to *hg.
+    explicit ParallelEvolutionEngine(Hypergraph* hg, size_t num_threads = 0,
                                      ExecutionMode mode = ExecutionMode::Parallel)
-        : dev_(dev ? dev : std::make_shared<Device>(0)) {
+        : dev_(default_device()), hg_(hg) {
         (void)num_threads;
         (void)mode;
     }
+    // On a given device context (several engines may share one); hg may be null.
+    explicit ParallelEvolutionEngine(std::shared_ptr<Device> dev, Hypergraph* hg = nullptr)
+        : dev_(dev ? dev : default_device()), hg_(hg) {}
+    ParallelEvolutionEngine() : dev_(default_device()) {}
     ParallelEvolutionEngine(const ParallelEvolutionEngine&) = delete;
     ParallelEvolutionEngine& operator=(const ParallelEvolutionEngine&) = delete;
 
@@ -194,7 +318,7 @@ public:
         std::vector<uint8_t> t(counts_.causal_pairs);
         detail::check(hgx_result_causal(res_.get(), p.data(), t.data(), p.size()), dev_->get());
         std::vector<CausalEdge> out(p.size());
-        for (size_t i = 0; i < p.size(); ++i) out[i] = CausalEdge{p[i].a, p[i].b, t[i] != 0};
+        for (size_t i = 0; i < p.size(); ++i) out[i] = CausalEdge{p[i].a, p[i].b, INVALID_ID, t[i] != 0};
         return out;
     }
     std::vector<BranchialEdge> branchial_edges() const {
@@ -249,16 +373,18 @@ private:
                   (tr_ ? HGX_OPT_TR : 0u);
         hgx_result* r = nullptr;
         detail::check(hgx_evolve_batch(dev_->get(), &d, 1, &r), dev_->get());
-        res_.reset(r);
+        res_.reset(r, detail::ResDeleter());
         detail::check(hgx_result_counts(r, &counts_), dev_->get());
+        if (hg_) hg_->assign(res_, counts_, tr_);
     }
 
     std::shared_ptr<Device> dev_;
+    Hypergraph* hg_ = nullptr;
     std::vector<RewriteRule> rules_;
     bool tr_ = false, rec_causal_ = true, rec_branchial_ = true, relations_ = true, keep_edges_ = false;
     uint32_t cap_ = 0;
     hgx_counts counts_{};
-    std::unique_ptr<hgx_result, detail::ResDeleter> res_;
+    std::shared_ptr<hgx_result> res_;
 };
 
 // hg_gpu::EvolveInput / EvolveResult for a full-multiway run (the fields run_gpu_evolution sets

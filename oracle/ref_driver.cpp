@@ -22,9 +22,16 @@
 // raw relation for the LAST rep as text: per state "S hash step", per event
 // "E in out rule step", per causal triple "C producer consumer", per branchial "B e1 e2".
 
+// Built a second way, with -DHGX_DROPIN against include/hgx_engine.hpp (tests/cpp/ref_driver_hgx,
+// by __graft_entry__.build()): the same driving code over the device engine, with only the
+// include and the namespace below changed -- the check that the C++ surface is a drop-in.
+#ifdef HGX_DROPIN
+#include "hgx_engine.hpp"
+#else
 #include "hypergraph/hypergraph.hpp"
 #include "hypergraph/parallel_evolution.hpp"
 #include "hypergraph/pattern.hpp"
+#endif
 
 #include <algorithm>
 #include <chrono>
@@ -36,7 +43,11 @@
 #include <string>
 #include <vector>
 
+#ifdef HGX_DROPIN
+using namespace hgx;
+#else
 using namespace hypergraph;
+#endif
 
 namespace {
 

@@ -50,3 +50,35 @@ def test_random_rules_small_states(ctx, seed):
     n, m, _ = ctx.match([rule], 0, edges, want_edges=True, cap=10**6)
     on, om = po.match([rule], 0, edges, want_edges=True, cap=10**6)
     assert n == on and np.array_equal(m, om)
+
+
+def test_single_edge_rule_empty_state_and_disconnected_lhs(ctx):
+    rnd = random.Random(11)
+    edges = [[rnd.randrange(50), rnd.randrange(50)] for _ in range(3000)]
+    for lhs in ([[0, 1]], [[0, 0]], [[0, 1], [2, 3]], [[0, 1], [1, 1]]):
+        rule = rule_from_lists(lhs, [[0]])
+        n, m, _ = ctx.match([rule], 0, edges, want_edges=True, cap=10**7)
+        on, om = po.match([rule], 0, edges, want_edges=True, cap=10**7)
+        assert n == on and np.array_equal(m, om), lhs
+    rule = rule_from_lists([[0, 1], [1, 2]], [[0]])
+    n, m, _ = ctx.match([rule], 0, [], want_edges=True, cap=10)
+    assert n == 0
+
+
+def test_deep_rule_and_cap_prefix(ctx):
+    # a four-edge path over a random digraph: three breadth-first depths after the seeds
+    rnd = random.Random(5)
+    edges = [[rnd.randrange(400), rnd.randrange(400)] for _ in range(4000)]
+    rule = rule_from_lists([[0, 1], [1, 2], [2, 3], [3, 4]], [[0, 4]])
+    n, m, ms = ctx.match([rule], 0, edges, want_edges=True, cap=10**7)
+    on, om = po.match([rule], 0, edges, want_edges=True, cap=10**7)
+    assert n == on and np.array_equal(m, om)
+    k = n // 3
+    n2, m2, _ = ctx.match([rule], 0, edges, want_edges=True, cap=k)
+    assert n2 == n and np.array_equal(m2, om[:k])
+    # mixed arities with a repeated variable and a ternary pivot
+    edges3 = [[rnd.randrange(60) for _ in range(rnd.choice([2, 3]))] for _ in range(5000)]
+    rule3 = rule_from_lists([[0, 1, 1], [1, 2], [2, 0, 3]], [[0]])
+    n3, m3, _ = ctx.match([rule3], 0, edges3, want_edges=True, cap=10**7)
+    on3, om3 = po.match([rule3], 0, edges3, want_edges=True, cap=10**7)
+    assert n3 == on3 and np.array_equal(m3, om3)
