@@ -35,12 +35,14 @@ EXPORTED_SYMBOLS = [
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
     "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
+    "hgx_result_causal_triples",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
 OPTION_DELTA_CAP = 2   # HGX_OPTION_DELTA_CAP
 OPTION_EXPAND_TEAM = 3  # HGX_OPTION_EXPAND_TEAM
 OPTION_JOIN_TEAM = 4    # HGX_OPTION_JOIN_TEAM
+OPTION_EXPAND_LANES = 5 # HGX_OPTION_EXPAND_LANES
 
 
 class HGXError(RuntimeError):
@@ -86,6 +88,7 @@ def load_library():
     L.hgx_result_branchial.argtypes = [vp, vp, C.c_uint64]
     L.hgx_result_timing.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.hgx_result_status.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.hgx_result_causal_triples.argtypes = [vp, vp, vp, C.c_uint64]
     L.hgx_result_release.argtypes = [vp]
     L.hgx_result_release.restype = None
     L.hgx_result_release_many.argtypes = [vp, C.c_uint32]
@@ -404,6 +407,13 @@ class Context:
         """Waves that rewrite and hash one child: 1, 2, 4 or 8 for every child, 0 automatic.
         Results never depend on it."""
         rc = self._lib.hgx_ctx_set_option(self._h, OPTION_EXPAND_TEAM, int(waves))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def set_expand_lanes(self, lanes: int):
+        """Lanes that rewrite and hash one child when a wave is the team: 64, 32 or 16 (sub-wave
+        teams), 0 automatic. Results never depend on it."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_EXPAND_LANES, int(lanes))
         if rc != abi.HGX_OK:
             raise self._err(rc)
 

@@ -188,8 +188,17 @@ struct Team {
     uint32_t r;             // rank in the team
     uint64_t* red;          // [W] LDS, used when W > 1
 
+    static constexpr uint32_t LANES = WAVE;   // lanes of one wave that hold the same state
     __device__ explicit Team(uint64_t* red_) : r(W == 1 ? lane_id() : (uint32_t)threadIdx.x), red(red_) {}
     __device__ bool leader() const { return r == 0; }
+    // Per-wave helpers (every wave of a multi-wave team computes the same from its own lanes):
+    // this lane's place in the wave, lane u's value, and a ballot with bit u for lane u.
+    __device__ uint32_t lane() const { return lane_id(); }
+    __device__ uint32_t shfl(uint32_t v, uint32_t u) const { return __shfl(v, (int)u, WAVE); }
+    __device__ uint32_t read_lane(uint32_t v, uint32_t u) const {
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)u);
+    }
+    __device__ uint64_t ballot(bool b) const { return __ballot(b); }
     __device__ void sync() const {
         if constexpr (W == 1) wave_sync();
         else __syncthreads();
@@ -311,5 +320,83 @@ struct Team {
         }
     }
 };
+
+// A sub-wave team: G consecutive lanes of one wave (G = 16 or 32) share one state, so a wave
+// works on 64 / G small states at once, each with its own workspace. The groups of a wave may
+// run different trip counts: every cross-lane step reads only the group's own lanes (shuffles
+// within width G, ballots masked to the group), and a group's lanes are all active at each such
+// step because the group's control flow depends on team-uniform values only. Synchronisation is
+// the wave's (the groups share its program order).
+template <uint32_t G>
+struct SubTeam {
+    static_assert(G == 16 || G == 32, "sub-wave teams are 16 or 32 lanes");
+    static constexpr uint32_t N = G;
+    static constexpr uint32_t LANES = G;
+    uint32_t r;             // rank in the team
+    uint32_t base;          // the team's first lane in the wave
+
+    __device__ SubTeam() : r(lane_id() & (G - 1)), base(lane_id() & ~(G - 1)) {}
+    __device__ explicit SubTeam(uint64_t*) : SubTeam() {}
+    __device__ bool leader() const { return r == 0; }
+    __device__ void sync() const { wave_sync(); }
+    __device__ uint32_t lane() const { return r; }
+    __device__ uint32_t shfl(uint32_t v, uint32_t u) const { return __shfl(v, (int)u, (int)G); }
+    __device__ uint32_t read_lane(uint32_t v, uint32_t u) const { return __shfl(v, (int)u, (int)G); }
+    __device__ uint64_t ballot(bool b) const { return (__ballot(b) >> base) & ((1ull << G) - 1ull); }
+    __device__ uint32_t sum(uint32_t v) const {
+        for (uint32_t o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, (int)o, WAVE);
+        return v;
+    }
+    __device__ uint64_t sum64(uint64_t v) const {
+        for (uint32_t o = G / 2; o > 0; o >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)v, (int)o, WAVE);
+            const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), (int)o, WAVE);
+            v += ((uint64_t)hi << 32) | lo;
+        }
+        return v;
+    }
+    __device__ uint32_t max(uint32_t v) const {
+        for (uint32_t o = G / 2; o > 0; o >>= 1) { const uint32_t w = __shfl_xor(v, (int)o, WAVE); v = w > v ? w : v; }
+        return v;
+    }
+    __device__ bool any(bool b) const { return ballot(b) != 0; }
+    __device__ uint32_t incl_scan(uint32_t v, uint32_t& total) const {
+        for (uint32_t o = 1; o < G; o <<= 1) {
+            const uint32_t w = __shfl_up(v, o, (int)G);
+            if (r >= o) v += w;
+        }
+        total = __shfl(v, (int)(G - 1), (int)G);
+        return v;
+    }
+    __device__ uint64_t bcast(uint64_t v) const {
+        const uint32_t lo = __shfl((uint32_t)v, 0, (int)G), hi = __shfl((uint32_t)(v >> 32), 0, (int)G);
+        return ((uint64_t)hi << 32) | lo;
+    }
+    __device__ uint32_t compact(bool keep, uint32_t& total) const {
+        const uint64_t m = ballot(keep);
+        total = (uint32_t)__popcll(m);
+        return (uint32_t)__popcll(m & ((1ull << r) - 1ull));
+    }
+    __device__ uint32_t excl_scan_lds(uint32_t* a, uint32_t n) const {
+        uint32_t carry = 0;
+        for (uint32_t b = 0; b < n; b += G) {
+            const uint32_t i = b + r;
+            const uint32_t v = i < n ? a[i] : 0u;
+            uint32_t t;
+            const uint32_t inc = incl_scan(v, t);
+            if (i < n) a[i] = carry + inc - v;
+            carry += t;
+        }
+        wave_sync();
+        return carry;
+    }
+};
+
+// The team type of a launch: W waves sharing one state (G = 64), or one wave split into teams
+// of G lanes (W = 1, G = 16 or 32).
+template <uint32_t W, uint32_t G>
+struct TeamSel { using type = SubTeam<G>; };
+template <uint32_t W>
+struct TeamSel<W, 64> { using type = Team<W>; };
 
 }  // namespace hgx

@@ -182,6 +182,12 @@ struct ExpandClass {
     std::vector<uint32_t> rb, rpre{0};
 };
 constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
+// The expansion's size classes are finer at the small end, where a child's workspace is a few KiB
+// and several children share one wave (sub-wave teams of kExpandLanes lanes): a wave hashes four
+// children of up to 32 edges, or two of up to 64, each in its own workspace.
+constexpr int kExpandClasses = 6;
+constexpr uint32_t kExpandEdges[kExpandClasses] = {32, 64, 128, 512, 2048, 0xFFFFFFFFu};
+constexpr uint32_t kExpandLanes[kExpandClasses] = {16, 32, 64, 64, 64, 64};
 }  // namespace
 
 uint32_t Engine::step() {
@@ -580,7 +586,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
     std::vector<uint64_t> r_eoff;
     std::vector<uint8_t> r_depth, r_ar;
     std::vector<RootCopy> r_copy;
-    ExpandClass cls[4];                         // host range lists of the expand launches
+    ExpandClass cls[kExpandClasses];            // host range lists of the expand launches
     std::vector<RunEvents> rev;                 // per-run child ranges for k_run_events
     const uint32_t nC = (uint32_t)M + NR;
     if (nC) {
@@ -671,7 +677,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 const RunRec& r = *F_runs_[i];
                 const uint32_t cne = r.ne_bound + r.hp->max_gain, cnv = r.nv_bound + r.hp->max_new;
                 int k = 0;
-                while (cne > kClassEdges[k]) ++k;
+                while (cne > kExpandEdges[k]) ++k;
                 ExpandClass& x = cls[k];
                 x.ne = std::max(x.ne, cne);
                 x.nextv = std::max(x.nextv, cnv);
@@ -683,7 +689,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             const uint64_t bytes = M * (8ull + 2ull * K) + parent_edges * (4ull * F_->v.A + 2) +
                                    EC * (4ull * A + 2) + M * (28ull + 8ull * lstride) + M * 24ull;
             uint16_t* tcnt = dalloc<uint16_t>(tmp, M);   // per child: causal triples, for k_run_events
-            for (auto& x : cls) {
+            for (int k = 0; k < kExpandClasses; ++k) {
+                ExpandClass& x = cls[k];
                 if (!x.n) continue;
                 ExpandArgs ea{};
                 ea.P = F_->v; ea.C = v; ea.M = mv; ea.E = C->ev; ea.runs = d_runs_; ea.ctr = d_ctr_;
@@ -703,16 +710,23 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 // Team: one wave per child while a CU still holds several workspaces, else the
                 // workgroup's waves share one child (a CU holds 160 KiB of LDS).
                 const uint32_t team = pick_team(ea.lds_per_wave, ctx_->expand_team);
-                uint32_t wpb_e = 1;
+                uint32_t wpb_e = 1, lanes = 64;
                 if (team == 1) {
-                    wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds, ctx_->num_cu * 8,
-                                            &ea.scratch);
+                    // sub-wave teams for small children, when the wave's workspaces fit a workgroup
+                    const uint32_t want = ctx_->expand_lanes ? ctx_->expand_lanes : kExpandLanes[k];
+                    if (want < 64 && (uint64_t)ea.lds_per_wave * (64 / want) <= ctx_->max_lds) {
+                        lanes = want;
+                        wpb_e = waves_per_block(ea.lds_per_wave * (64 / lanes), ctx_->max_lds);
+                    } else {
+                        wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds,
+                                                ctx_->num_cu * 8, &ea.scratch);
+                    }
                 } else if (ea.lds_per_wave > ctx_->max_lds) {
                     const uint32_t grid = expand_grid(ea.n_items, team, 1, ctx_->num_cu * 8);
                     ea.scratch = (uint8_t*)tmp.alloc((size_t)grid * ea.lds_per_wave);
                 }
                 timed(ctx_, st, "k_expand", (uint64_t)((double)bytes * x.n / M), x.n,
-                      [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, team, st); });
+                      [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, team, st, lanes); });
             }
             timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });

@@ -694,16 +694,17 @@ __device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
 // team: W = 1, one wave per child (up to four per workgroup, each with its own workspace);
 // W > 1, the workgroup's W waves share one child and one workspace, for large children whose
 // workspace would otherwise leave a CU one or two waves.
-template <bool GWS, uint32_t W>
+template <bool GWS, uint32_t W, uint32_t G>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expand(ExpandArgs a) {
+    using TM = typename TeamSel<W, G>::type;
     extern __shared__ uint8_t lds[];
-    const uint32_t l = lane_id();
-    const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;    // teams per workgroup
-    const uint32_t w = W == 1 ? threadIdx.x / WAVE : 0;     // team within the workgroup
+    const uint32_t tpb = W == 1 ? (blockDim.x / WAVE) * (WAVE / G) : 1;   // teams per workgroup
+    const uint32_t w = W == 1 ? threadIdx.x / G : 0;                       // team within the workgroup
     uint8_t* my;
     if constexpr (GWS) my = a.scratch + ((size_t)blockIdx.x * tpb + w) * a.lds_per_wave;
     else my = lds + (size_t)w * a.lds_per_wave;
-    const Team<W> tm((uint64_t*)(my + a.lds_per_wave - 64));   // the workspace's last 64 bytes
+    const TM tm((uint64_t*)(my + a.lds_per_wave - 64));   // a wave team's reduction words: the last 64 bytes
+    const uint32_t l = tm.lane();
     const uint32_t PA = a.P.A, A = a.C.A;                  // parent / child vertex strides
     const WLSizes z = expand_sizes(a.max_child_ne, a.max_nv, a.max_nextv, A, a.lean != 0);
     WLSpace s;
@@ -762,7 +763,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
 
         // Parent edges minus the consumed ones, order kept (SparseBitset::derive).
         uint32_t kept = 0;
-        for (uint32_t base = 0; base < pne; base += Team<W>::N) {
+        for (uint32_t base = 0; base < pne; base += TM::N) {
             const uint32_t i = base + tm.r;
             bool keep = i < pne;
             if (keep) for (uint32_t t = 0; t < nl; ++t) keep &= (q[t] != i);
@@ -809,13 +810,13 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         tm.sync();
         // Local vertex numbering: ascending label, as the reference's sorted vertex list.
         const uint32_t nextv = fresh_base + R.n_new;
-        for (uint32_t v = tm.r; v <= nextv; v += Team<W>::N) vmap[v] = 0;
+        for (uint32_t v = tm.r; v <= nextv; v += TM::N) vmap[v] = 0;
         tm.sync();
-        for (uint32_t e = tm.r; e < cne; e += Team<W>::N)
+        for (uint32_t e = tm.r; e < cne; e += TM::N)
             for (uint32_t k = 0; k < s.e_ar[e]; ++k) vmap[gv[e * A + k]] = 1;
         tm.sync();
         const uint32_t nv = tm.excl_scan_lds(vmap, nextv + 1);
-        for (uint32_t e = tm.r; e < cne; e += Team<W>::N)
+        for (uint32_t e = tm.r; e < cne; e += TM::N)
             for (uint32_t k = 0; k < A; ++k) s.e_lv[e * A + k] = (uint16_t)(k < s.e_ar[e] ? vmap[gv[e * A + k]] : 0);
         s.nE = cne;
         s.nV = nv;
@@ -829,9 +830,9 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         // from the nl lanes holding the consumed edges' producers.
         uint64_t prod = 0, anc_c = 0, others = 0;
         for (uint32_t u = 0; u < nl; ++u) {
-            const uint32_t pu = __shfl(ps_t, u, WAVE);
-            const uint64_t au = ((uint64_t)(uint32_t)__shfl((uint32_t)(anc_t >> 32), u, WAVE) << 32) |
-                                (uint32_t)__shfl((uint32_t)anc_t, u, WAVE);
+            const uint32_t pu = tm.shfl(ps_t, u);
+            const uint64_t au = ((uint64_t)tm.shfl((uint32_t)(anc_t >> 32), u) << 32) |
+                                tm.shfl((uint32_t)anc_t, u);
             if (pu) {
                 prod |= 1ull << pu;
                 anc_c |= (1ull << pu) | au;
@@ -840,14 +841,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         }
         const bool rec_causal = (RN->flags & 0x1u) != 0;
         const bool drop = (RN->flags & 0x100u) && l < nl && ps_t && ((others >> ps_t) & 1ull);
-        const uint64_t dropped = __ballot(drop);
+        const uint64_t dropped = tm.ballot(drop);
         uint64_t tr = prod;
         for (uint32_t u = 0; u < nl; ++u) {
-            const uint32_t pu = __shfl(ps_t, u, WAVE);
+            const uint32_t pu = tm.shfl(ps_t, u);
             if ((dropped >> u) & 1ull) tr &= ~(1ull << pu);
         }
-        const uint32_t triples = (uint32_t)__popcll(__ballot(l < nl && ps_t != 0));
-        const uint32_t triples_tr = (uint32_t)__popcll(__ballot(l < nl && ps_t != 0 && ((tr >> ps_t) & 1ull)));
+        const uint32_t triples = (uint32_t)__popcll(tm.ballot(l < nl && ps_t != 0));
+        const uint32_t triples_tr = (uint32_t)__popcll(tm.ballot(l < nl && ps_t != 0 && ((tr >> ps_t) & 1ull)));
         if (!rec_causal) { prod = 0; tr = 0; anc_c = 0; }
 
         // Identity and bookkeeping (the team's first lane): the child's record and its event.
@@ -871,7 +872,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_expan
         {
             const uint64_t* plan = a.P.lanc + (size_t)par * a.P.lstride;
             uint64_t* clan = a.C.lanc + (size_t)c * a.C.lstride;
-            for (uint32_t t = tm.r; t < step; t += Team<W>::N) clan[t] = (t + 1 == step) ? anc_c : plan[t];
+            for (uint32_t t = tm.r; t < step; t += TM::N) clan[t] = (t + 1 == step) ? anc_c : plan[t];
         }
         tm.sync();
     }
@@ -1368,26 +1369,32 @@ uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t ma
     return ((z.bytes(A) + 128u) + 15u) & ~15u;
 }
 
-template <uint32_t W>
+template <uint32_t W, uint32_t G>
 static void launch_expand_team(const ExpandArgs& a, uint32_t grid, uint32_t wpb, hipStream_t st) {
     const uint32_t threads = W == 1 ? 64 * wpb : 64 * W;
-    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * (W == 1 ? wpb : 1);
-    if (a.scratch) hipLaunchKernelGGL((k_expand<true, W>), dim3(grid), dim3(threads), lds, st, a);
-    else hipLaunchKernelGGL((k_expand<false, W>), dim3(grid), dim3(threads), lds, st, a);
+    const size_t lds = a.scratch ? 0 : (size_t)a.lds_per_wave * (W == 1 ? wpb * (64 / G) : 1);
+    if constexpr (G == 64) {
+        if (a.scratch) { hipLaunchKernelGGL((k_expand<true, W, 64>), dim3(grid), dim3(threads), lds, st, a); return; }
+    }
+    hipLaunchKernelGGL((k_expand<false, W, G>), dim3(grid), dim3(threads), lds, st, a);
 }
 
-uint32_t expand_grid(uint32_t n_items, uint32_t team, uint32_t wpb, uint32_t grid_cap) {
-    return grid_for(n_items, team == 1 ? wpb : 1, grid_cap);
+uint32_t expand_grid(uint32_t n_items, uint32_t team, uint32_t wpb, uint32_t grid_cap, uint32_t lanes) {
+    return grid_for(n_items, team == 1 ? wpb * (64 / lanes) : 1, grid_cap);
 }
 
-hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st) {
+hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st,
+                         uint32_t lanes) {
     if (!a.n_items) return hipSuccess;
-    const uint32_t grid = expand_grid(a.n_items, team, wpb, grid_cap);
-    switch (team) {
-        case 1: launch_expand_team<1>(a, grid, wpb, st); break;
-        case 2: launch_expand_team<2>(a, grid, wpb, st); break;
-        case 4: launch_expand_team<4>(a, grid, wpb, st); break;
-        case 8: launch_expand_team<8>(a, grid, wpb, st); break;
+    if (lanes != 64 && (team != 1 || a.scratch || (lanes != 16 && lanes != 32))) return hipErrorInvalidValue;
+    const uint32_t grid = expand_grid(a.n_items, team, wpb, grid_cap, lanes);
+    switch (team * 100 + lanes) {
+        case 164: launch_expand_team<1, 64>(a, grid, wpb, st); break;
+        case 132: launch_expand_team<1, 32>(a, grid, wpb, st); break;
+        case 116: launch_expand_team<1, 16>(a, grid, wpb, st); break;
+        case 264: launch_expand_team<2, 64>(a, grid, wpb, st); break;
+        case 464: launch_expand_team<4, 64>(a, grid, wpb, st); break;
+        case 864: launch_expand_team<8, 64>(a, grid, wpb, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

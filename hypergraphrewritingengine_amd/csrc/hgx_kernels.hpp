@@ -174,8 +174,11 @@ hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32
                         hipStream_t st);
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean);
 // team: waves per child (1, 2, 4 or 8); with team 1, wpb children per workgroup.
-hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st);
-uint32_t expand_grid(uint32_t n_items, uint32_t team, uint32_t wpb, uint32_t grid_cap);
+// team: waves sharing one child (1, 2, 4, 8); lanes: with team 1, the lanes sharing one child
+// (64, or 32 / 16 for sub-wave teams, several small children per wave, LDS workspaces only).
+hipError_t launch_expand(const ExpandArgs& a, uint32_t grid_cap, uint32_t wpb, uint32_t team, hipStream_t st,
+                         uint32_t lanes = 64);
+uint32_t expand_grid(uint32_t n_items, uint32_t team, uint32_t wpb, uint32_t grid_cap, uint32_t lanes = 64);
 hipError_t launch_dedup(const GenView& C, uint32_t n, DRun* runs, uint64_t* ctr, hipStream_t st);
 // grid used by the launchers for `items` work items (for sizing global scratch)
 uint32_t launch_grid(uint32_t items, uint32_t wpb, uint32_t grid_cap);

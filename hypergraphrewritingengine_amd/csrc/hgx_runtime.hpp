@@ -195,6 +195,7 @@ struct hgx_ctx {
     uint32_t delta_cap = 256;                    // HGX_OPTION_DELTA_CAP
     uint32_t expand_team = 0;                    // HGX_OPTION_EXPAND_TEAM
     uint32_t join_team = 0;                      // HGX_OPTION_JOIN_TEAM
+    uint32_t expand_lanes = 0;                   // HGX_OPTION_EXPAND_LANES
 };
 
 namespace hgx {

@@ -157,18 +157,18 @@ __device__ __forceinline__ void seg_rank_sort(const TM& tm, const T* vals, T* ou
 // Exact number of distinct values in cur[0..nV).
 template <class TM>
 __device__ __forceinline__ uint32_t wl_ndistinct(const TM& tm, WLSpace& s, const uint64_t* cur) {
-    const uint32_t l = lane_id();
-    if (s.nV <= WAVE) {
+    const uint32_t l = tm.lane();
+    if (s.nV <= TM::LANES) {
         // one colour per lane: a lane counts when no lower lane holds its colour
         const uint64_t x = l < s.nV ? cur[l] : 0ull;
         const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)(x >> 32);
         bool first = l < s.nV;
         for (uint32_t u = 0; u < s.nV; ++u) {
-            const uint32_t ulo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, (int)u);
-            const uint32_t uhi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, (int)u);
+            const uint32_t ulo = tm.read_lane(xlo, u);
+            const uint32_t uhi = tm.read_lane(xhi, u);
             first &= !(u < l && ulo == xlo && uhi == xhi);
         }
-        return (uint32_t)__popcll(__ballot(first));
+        return (uint32_t)__popcll(tm.ballot(first));
     }
     for (uint32_t i = tm.r; i <= s.ht_mask; i += TM::N) s.hset[i] = 0;
     tm.sync();

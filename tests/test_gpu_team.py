@@ -169,3 +169,53 @@ def test_team_option_rejects_bad_values(ctx):
             ctx.set_join_team(bad)
     ctx.set_expand_team(0)
     ctx.set_join_team(0)
+
+
+def _by_lanes(ctx, reqs, lanes=(64, 32, 16)):
+    out = {}
+    try:
+        for t in (0,) + tuple(lanes):
+            ctx.set_expand_lanes(t)
+            out[t] = ctx.evolve_batch(reqs)
+    finally:
+        ctx.set_expand_lanes(0)
+    return out
+
+
+def test_subwave_teams_random_rules_exact(ctx):
+    """Sub-wave expansion teams: 16 and 32 lanes per child (four or two children per wave, each
+    with its own workspace), forced for every size class whose workspaces fit, against one wave
+    per child and the oracle -- random rule sets (unary to ternary arities, repeated variables,
+    general and uniform-arity hash front ends) and batched runs of 3-64 edges."""
+    reqs = [_random_request(3000 + s) for s in range(40)]
+    reqs += [W.batched_run(i, cap=2048, flags=REL | TR) for i in range(0, 400, 25)]
+    r = _by_lanes(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for t in (0, 32, 16):
+            _same(r[t][i], r[64][i])
+        assert_exact(r[16][i], po.evolve(req))
+
+
+def test_subwave_teams_mixed_class_batch(ctx):
+    """One batch whose children fall in every expansion size class (up to 32, 64, 128 and 512
+    edges), so sub-wave launches and wave launches run side by side in one generation; the
+    children of 65-300 edges take 16 or 32 lanes only where their workspace still fits."""
+    rb = rule_from_lists([[0, 1], [1, 2]], [[0, 2], [2, 3], [3, 1]])
+    reqs = []
+    for k, m in enumerate((5, 20, 40, 60, 90, 200, 300)):
+        reqs.append(Request([rb], W._connected_init(W.Stream(70 + k), m), 3, state_cap=1500, flags=REL | TR))
+    reqs.append(Request([rule_from_lists([[0], [0, 1]], [[1], [0, 1], [1, 2]])], _mixed_state(3, 40), 3,
+                        state_cap=1500, flags=REL | TR))
+    r = _by_lanes(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for t in (0, 32, 16):
+            _same(r[t][i], r[64][i])
+        assert_exact(r[0][i], po.evolve(req))
+
+
+def test_expand_lanes_option_rejects_bad_values(ctx):
+    import hypergraphrewritingengine_amd as hgx
+    for bad in (8, 24, 128, -1):
+        with pytest.raises(hgx.HGXError):
+            ctx.set_expand_lanes(bad)
+    ctx.set_expand_lanes(0)
