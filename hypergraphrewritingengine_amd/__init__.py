@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = [
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
     "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
-    "hgx_result_causal_triples",
+    "hgx_result_causal_triples", "hgx_match_count_noalloc",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -89,6 +89,8 @@ def load_library():
     L.hgx_result_timing.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.hgx_result_status.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.hgx_result_causal_triples.argtypes = [vp, vp, vp, C.c_uint64]
+    L.hgx_match_count_noalloc.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), vp, C.c_uint64]
     L.hgx_result_release.argtypes = [vp]
     L.hgx_result_release.restype = None
     L.hgx_result_release_many.argtypes = [vp, C.c_uint32]
@@ -496,6 +498,22 @@ class Context:
         n = cnt.value
         mat = out[: min(n, cap) * k].reshape(-1, k) if want_edges else None
         return n, mat, ms.value
+
+    def match_noalloc(self, rules, rule_index: int, vertices_ptr: int, arity_ptr: int, n_edges: int,
+                      stride: int, n_vertices: int, out_ptr: int = 0, cap: int = 0) -> int:
+        """hgx_match_count_noalloc: the large-state join over a state already in device memory
+        (vertices [n_edges * stride] uint32, arity [n_edges] uint8, ids below n_vertices), writing
+        up to `cap` matches to the device buffer at out_ptr, with no device allocation (its working
+        memory is the library's fixed scratch). Returns the match count."""
+        prog = np.frombuffer(encode_rules(rules), np.uint8).copy()
+        cnt = C.c_uint64()
+        rc = self._lib.hgx_match_count_noalloc(self._h, prog.ctypes.data, len(prog), len(rules), rule_index,
+                                               C.c_void_p(vertices_ptr), C.c_void_p(arity_ptr), n_edges,
+                                               stride, n_vertices, C.byref(cnt),
+                                               C.c_void_p(out_ptr) if out_ptr else None, cap)
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return cnt.value
 
     def run_wxf(self, job: bytes) -> bytes:
         """The paclet's job envelope: a WXF job in, the WXF result out (hgx_run_wxf)."""

@@ -56,6 +56,13 @@ __global__ void k_index_keys(const uint32_t* ev, const uint8_t* ar, uint32_t m, 
     val[s] = e;
 }
 
+// Zero fill: a kernel, so the join's stream holds kernels and copies only.
+template <class T>
+__global__ void k_bj_zero(T* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void k_u64_to_u32(const uint64_t* in, uint32_t* out, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = (uint32_t)in[i];
@@ -262,7 +269,7 @@ static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_
     uint32_t* cnt = bget<uint32_t>(mem, nkeys + 1);
     uint64_t* off64 = bget<uint64_t>(mem, nkeys + 1);
     uint32_t* off = bget<uint32_t>(mem, nkeys + 1);
-    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (nkeys + 1), st));
+    hipLaunchKernelGGL(k_bj_zero<uint32_t>, dim3(blocks_for(nkeys + 1, 256)), dim3(256), 0, st, cnt, nkeys + 1);
     if (m) hipLaunchKernelGGL(k_index_count, dim3(blocks_for(m, 256)), dim3(256), 0, st, v.ev, v.ar, m, A, cnt);
     bj_scan(mem, cnt, off64, nkeys, st);
     hipLaunchKernelGGL(k_u64_to_u32, dim3(blocks_for(nkeys + 1, 256)), dim3(256), 0, st, off64, off,
@@ -297,7 +304,7 @@ static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
     uint32_t* lo = bget<uint32_t>(mem, P);
     uint32_t* cnt = bget<uint32_t>(mem, P + 1);
     uint64_t* start = bget<uint64_t>(mem, P + 1);
-    HIP_TRY(hipMemsetAsync(cnt + P, 0, sizeof(uint32_t), J.st));
+    hipLaunchKernelGGL(k_bj_zero<uint32_t>, dim3(1), dim3(64), 0, J.st, cnt + P, (uint64_t)1);
     hipLaunchKernelGGL(k_bj_range, dim3(blocks_for(P, 256)), dim3(256), 0, J.st, J.v, d, F, P, lo, cnt);
     bj_scan(mem, cnt, start, P, J.st);
     const uint64_t T = read1(start + P, J.st);
@@ -306,7 +313,7 @@ static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
     const uint64_t ntiles = (T + BJ_TILE - 1) / BJ_TILE;
     uint64_t* tcnt = bget<uint64_t>(mem, ntiles + 1);
     uint64_t* toff = bget<uint64_t>(mem, ntiles + 1);
-    HIP_TRY(hipMemsetAsync(tcnt + ntiles, 0, sizeof(uint64_t), J.st));
+    hipLaunchKernelGGL(k_bj_zero<uint64_t>, dim3(1), dim3(64), 0, J.st, tcnt + ntiles, (uint64_t)1);
     BJLevel L{};
     L.d = d; L.F = F; L.P = P; L.lo = lo; L.start = start; L.T = T; L.tile_cnt = tcnt; L.tile_off = toff;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, J.grid);
@@ -349,6 +356,15 @@ static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
 // All matches of rule R in the state (ev, ar) on the device: the index, the seeds, the depths.
 void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed) {
     BJMem& mem = *J.mem;
+    if (!mem.arena) {
+        // a fixed region: refuse a state whose index alone cannot fit before launching anything
+        const uint64_t nkeys = (uint64_t)nv * J.v.A, slots = (uint64_t)J.v.m * J.v.A;
+        const uint64_t need = (nkeys + 1) * 16ull + slots * 16ull + (uint64_t)J.v.m * 5ull + (1ull << 20);
+        if (need > mem.cap - mem.used)
+            throw Error{HGX_ERR_CAPACITY, "a state of " + std::to_string(J.v.m) + " edges needs " +
+                                              std::to_string(need >> 20) + " MiB for its index, more than the " +
+                                              std::to_string(mem.cap >> 20) + " MiB fixed scratch"};
+    }
     hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;
     if (timed) {
         HIP_TRY(hipEventCreate(&t0)); HIP_TRY(hipEventCreate(&t1)); HIP_TRY(hipEventCreate(&t2));
@@ -367,7 +383,7 @@ void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed
         HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tmp, ids, flag, seeds, nsel, (int)m, J.st));
         HIP_TRY(hipcub::DeviceSelect::Flagged(mem.get(tmp), tmp, ids, flag, seeds, nsel, (int)m, J.st));
     } else {
-        HIP_TRY(hipMemsetAsync(nsel, 0, sizeof(uint32_t), J.st));
+        hipLaunchKernelGGL(k_bj_zero<uint32_t>, dim3(1), dim3(64), 0, J.st, nsel, (uint64_t)1);
     }
     const uint64_t P0 = read1(nsel, J.st);
     J.alg_bytes += (uint64_t)m * (4ull * J.v.A + 1) + P0 * 4ull;

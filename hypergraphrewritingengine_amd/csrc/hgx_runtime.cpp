@@ -454,6 +454,12 @@ extern "C" int hgx_ctx_create(int device, uint64_t budget_bytes, hgx_ctx** out) 
         g_create_error = std::string("kernels are built for gfx950; device is ") + prop.gcnArchName;
         return HGX_ERR_NO_DEVICE;
     }
+    try {
+        noalloc_setup(device);                 // the allocation-free entry point's scratch address
+    } catch (const Error& e) {
+        g_create_error = e.msg;
+        return e.code;
+    }
     auto* c = new hgx_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -142,6 +142,8 @@ struct RunHost {
     uint64_t root_edges = 0;
 };
 RunHost parse_run(const hgx_run_desc* d);
+// Looks up, once per device (current), the address of the allocation-free entry point's scratch.
+void noalloc_setup(int device);
 // The rule program (include/hgx.h) into device rules with the reference's join order.
 void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<DRule>& out);
 uint64_t footprint_bytes(const RunHost& h, uint64_t held = 0);

@@ -325,6 +325,23 @@ int hgx_match_count(hgx_ctx* ctx, const uint8_t* rules, uint32_t rules_len, uint
                     uint32_t n_edges, uint64_t* count, uint32_t* match_edges, uint64_t cap,
                     float* device_ms);
 
+/* The same join for callers that cannot allocate device memory: no counterpart in the reference;
+ * it exists for code that must not call hipMalloc (under HIP graph capture, in tight serving
+ * loops, in harnesses that forbid allocation). The state and the output are the caller's device
+ * memory: vertices [n_edges * stride] (edge e's vertices at e * stride, ids below n_vertices),
+ * arity [n_edges] (1..stride), match_edges [cap * n_lhs] (may be NULL when cap is 0); *count is a
+ * host pointer. Every other byte the join uses comes from a fixed device array of
+ * HGX_NOALLOC_SCRATCH_MIB MiB per device, reserved at the first hgx_ctx_create on the device and
+ * held until the process ends; a state or a frontier that does not fit it gives HGX_ERR_CAPACITY
+ * with the reason, before anything is written. Matches and their order are hgx_match_count's. The
+ * array is shared by every call on a device: calls through one context are serialised, calls
+ * that overlap on one device through different contexts race. */
+#define HGX_NOALLOC_SCRATCH_MIB 64
+int hgx_match_count_noalloc(hgx_ctx* ctx, const uint8_t* rules, uint32_t rules_len, uint32_t n_rules,
+                            uint32_t rule, const uint32_t* vertices, const uint8_t* arity, uint32_t n_edges,
+                            uint32_t stride, uint32_t n_vertices, uint64_t* count, uint32_t* match_edges,
+                            uint64_t cap);
+
 /* ---- kernel-level measurement ----------------------------------------------------------- */
 
 /* With profiling on, every launch of the generation kernels is bracketed by HIP events on the
