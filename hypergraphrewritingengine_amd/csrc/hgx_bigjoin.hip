@@ -35,7 +35,7 @@ namespace hgx {
 constexpr uint32_t BJ_THREADS = 256;
 constexpr uint32_t BJ_IPT = 8;                            // items per thread per tile
 constexpr uint32_t BJ_TILE = BJ_THREADS * BJ_IPT;         // items per workgroup tile
-constexpr uint32_t BJ_LDS_PART = BJ_TILE + 2;             // scan entries a tile stages in LDS
+constexpr uint32_t BJ_LDS_PART = 514;                     // partials a tile stages in LDS (+1)
 
 // ---- index -------------------------------------------------------------------------------
 
@@ -131,33 +131,83 @@ __device__ __forceinline__ bool bj_fits(const BJView& v, uint32_t d, const uint3
     return true;
 }
 
+// bj_fits with the partial's edges and the vertices its bound slots need staged in LDS. The pivot
+// slot needs no test: the candidate came from the pivot vertex's list at that position.
+__device__ __forceinline__ bool bj_fits_staged(const BJView& v, uint32_t d, uint32_t pv, const uint32_t* part,
+                                               const uint32_t* want, uint32_t e) {
+    const DRule& R = v.R;
+    const uint32_t p = R.order[d], a = R.la[p];
+    if (v.ar[e] != a) return false;
+    for (uint32_t t = 0; t < d; ++t)
+        if (part[t] == e) return false;
+    const uint32_t* x = v.ev + (size_t)e * v.A;
+    for (uint32_t i = 0; i < a; ++i) {
+        const uint32_t s = R.jsrc[d][i];
+        if (s == 0xFFu || i == pv) continue;
+        const uint32_t want_i = (s >> 4) < d ? want[i] : x[s & 15u];
+        if (x[i] != want_i) return false;
+    }
+    return true;
+}
+
 struct BJLevel {
     uint32_t d;               // depth being added (1 .. n-1)
     const uint32_t* F;        // [P * d] partials, join order
     uint64_t P;
     const uint32_t* lo;       // [P]
-    const uint64_t* start;    // [P + 1] scan of the list lengths
-    uint64_t T;               // items
+    const uint64_t* start;    // [P + 1] scan of the list lengths; start[P] = items (device)
+    uint64_t ntiles;          // tiles provisioned: an upper bound of the items' tiles
     uint64_t* tile_cnt;       // count pass: survivors per tile
     const uint64_t* tile_off; // emit pass: scan of tile_cnt
     uint32_t* Fn;             // emit, inner depth: [S * (d+1)]
     uint32_t* out;            // emit, last depth: matches, n_lhs edges each in pattern order
-    uint64_t out_base, cap;   //   (matches found before this call; room for cap in all)
+    const uint64_t* found;    //   (device: matches written before this call)
+    uint64_t cap;             //   (room for cap matches in all)
+    uint64_t* slot_q;         // last depth: [ntiles * BJ_SLOT] survivors' partials
+    uint32_t* slot_e;         //   and candidates, in item order
 };
 
-// One pass over the items of a depth, a tile of BJ_TILE items per workgroup iteration.
-template <bool EMIT, bool LAST>
+// Passes over the items of a depth, a tile of BJ_TILE items per workgroup iteration. The number of
+// items is read on the device (the scan's total), so no host round trip sizes the launch; tiles
+// past it count zero. The tile's partials -- their item offsets, candidate-list starts and (up to
+// BJ_STAGE_D edges) their edges -- are staged in LDS, so an item reads only its candidate from
+// global memory.
+//   BJ_COUNT   survivors per tile (an inner depth, sizing the next frontier)
+//   BJ_EMIT    survivors written in item order: the next frontier (inner depth), or, at the last
+//              depth, the matches of the tiles whose survivors overflowed their slots
+//   BJ_SLOTS   the last depth in one pass: survivors per tile, and the first BJ_SLOT of each tile
+//              kept in item order as (partial, candidate), placed into the match list after the
+//              scan by k_bj_gather (matches are rare among the items, so a second pass over them
+//              is the exception)
+enum { BJ_COUNT = 0, BJ_EMIT = 1, BJ_SLOTS = 2 };
+constexpr uint32_t BJ_SLOT = 16;
+constexpr uint32_t BJ_STAGE_D = 3;
+constexpr uint32_t BJ_STAGE_A = 4;
+
+template <int MODE, bool LAST>
 __global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
     __shared__ uint64_t sst[BJ_LDS_PART];
+    __shared__ uint32_t slo[BJ_LDS_PART];
+    __shared__ uint32_t spart[BJ_LDS_PART * BJ_STAGE_D];
+    __shared__ uint32_t swant[BJ_LDS_PART * BJ_STAGE_A];   // per partial: the vertex each slot needs
     __shared__ uint32_t wtot[BJ_THREADS / WAVE];
     __shared__ uint64_t sq[2];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
     const uint32_t d = L.d;
     const uint32_t pv = v.R.piv[d];
-    const uint64_t ntiles = (L.T + BJ_TILE - 1) / BJ_TILE;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t pa = v.R.la[v.R.order[d]];
+    const uint64_t T = L.start[L.P];
+    const uint64_t nreal = (T + BJ_TILE - 1) / BJ_TILE;
+    const uint64_t base = (MODE == BJ_EMIT && LAST) ? *L.found : 0;
+    for (uint64_t tile = blockIdx.x; tile < L.ntiles; tile += gridDim.x) {
+        // the same for the whole workgroup: past the items, or (last depth, emit) a tile whose
+        // survivors all fit its slots
+        if (tile >= nreal || (MODE == BJ_EMIT && LAST && L.tile_cnt[tile] <= BJ_SLOT)) {
+            if (MODE != BJ_EMIT && tid == 0 && tile >= nreal) L.tile_cnt[tile] = 0;
+            continue;
+        }
         const uint64_t t0 = tile * BJ_TILE;
-        const uint64_t t1 = t0 + BJ_TILE < L.T ? t0 + BJ_TILE : L.T;
+        const uint64_t t1 = t0 + BJ_TILE < T ? t0 + BJ_TILE : T;
         if (tid == 0) {
             sq[0] = bj_owner(L.start, L.P, t0);
             sq[1] = bj_owner(L.start, L.P, t1 - 1);
@@ -165,23 +215,50 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
         __syncthreads();
         const uint64_t q0 = sq[0], nq = sq[1] - sq[0] + 1;  // partials this tile touches
         const bool staged = nq + 1 <= BJ_LDS_PART;
-        if (staged)
+        const bool sparts = staged && d <= BJ_STAGE_D && pa <= BJ_STAGE_A;
+        if (staged) {
             for (uint64_t i = tid; i <= nq; i += BJ_THREADS) sst[i] = L.start[q0 + i];
+            for (uint64_t i = tid; i < nq; i += BJ_THREADS) slo[i] = L.lo[q0 + i];
+            if (sparts) {
+                for (uint64_t i = tid; i < nq * d; i += BJ_THREADS) spart[i] = L.F[q0 * d + i];
+                // the vertex every slot bound at an earlier depth must carry, once per partial
+                for (uint64_t i = tid; i < nq * pa; i += BJ_THREADS) {
+                    const uint32_t k = (uint32_t)(i % pa), sk = v.R.jsrc[d][k];
+                    uint32_t want = 0;
+                    if (sk != 0xFFu && (sk >> 4) < d)
+                        want = v.ev[(size_t)L.F[(q0 + i / pa) * d + (sk >> 4)] * v.A + (sk & 15u)];
+                    swant[i] = want;
+                }
+            }
+        }
         __syncthreads();
-        uint64_t rank = EMIT ? L.tile_off[tile] : 0;      // running place of this tile's survivors
+        uint64_t rank = MODE == BJ_EMIT ? L.tile_off[tile] : 0;   // running place of the tile's survivors
         uint32_t found = 0;
         for (uint32_t k = 0; k < BJ_IPT; ++k) {
             const uint64_t t = t0 + (uint64_t)k * BJ_THREADS + tid;
             bool ok = false;
             uint64_t q = 0;
             uint32_t e = 0;
+            const uint32_t* part = nullptr;
             if (t < t1) {
-                q = staged ? q0 + bj_owner(sst, nq, t) : bj_owner(L.start, L.P, t);
-                const uint64_t j = t - (staged ? sst[q - q0] : L.start[q]);
-                e = pv == 0xFFu ? (uint32_t)j : v.idx[L.lo[q] + j];
-                ok = bj_fits(v, d, L.F + q * d, e);
+                uint64_t j;
+                if (staged) {
+                    const uint64_t r = bj_owner(sst, nq, t);
+                    q = q0 + r;
+                    j = t - sst[r];
+                    e = pv == 0xFFu ? (uint32_t)j : v.idx[slo[r] + j];
+                    part = sparts ? spart + r * d : L.F + q * d;
+                    if (sparts) ok = bj_fits_staged(v, d, pv, part, swant + r * pa, e);
+                    else ok = bj_fits(v, d, part, e);
+                } else {
+                    q = bj_owner(L.start, L.P, t);
+                    j = t - L.start[q];
+                    e = pv == 0xFFu ? (uint32_t)j : v.idx[L.lo[q] + j];
+                    part = L.F + q * d;
+                    ok = bj_fits(v, d, part, e);
+                }
             }
-            if (!EMIT) { found += ok ? 1u : 0u; continue; }
+            if (MODE == BJ_COUNT) { found += ok ? 1u : 0u; continue; }
             // place among this round's survivors: within the wave, then across the waves
             const uint64_t bal = __ballot(ok);
             if (lane == 0) wtot[w] = (uint32_t)__popcll(bal);
@@ -194,11 +271,15 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
             __syncthreads();
             if (ok) {
                 const uint64_t o = rank + before + lanes_below(bal);
-                const uint32_t* part = L.F + q * d;
-                if (LAST) {
+                if (MODE == BJ_SLOTS) {
+                    if (o < BJ_SLOT) {
+                        L.slot_q[tile * BJ_SLOT + o] = q;
+                        L.slot_e[tile * BJ_SLOT + o] = e;
+                    }
+                } else if (LAST) {
                     const uint32_t n = v.R.n_lhs;
-                    const uint64_t g = L.out_base + o;
-                    if (L.out && g < L.cap) {
+                    const uint64_t g = base + o;
+                    if (g < L.cap) {
                         for (uint32_t x = 0; x < d; ++x) L.out[g * n + v.R.order[x]] = part[x];
                         L.out[g * n + v.R.order[d]] = e;
                     }
@@ -209,18 +290,54 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
             }
             rank += round;
         }
-        if (!EMIT) {
+        if (MODE == BJ_COUNT) {
             const uint32_t ws = wave_sum_u32(found);
             if (lane == 0) wtot[w] = ws;
             __syncthreads();
             if (tid == 0) {
-                uint64_t s = 0;
-                for (uint32_t x = 0; x < BJ_THREADS / WAVE; ++x) s += wtot[x];
-                L.tile_cnt[tile] = s;
+                uint64_t sm = 0;
+                for (uint32_t x = 0; x < BJ_THREADS / WAVE; ++x) sm += wtot[x];
+                L.tile_cnt[tile] = sm;
             }
+        } else if (MODE == BJ_SLOTS) {
+            if (tid == 0) L.tile_cnt[tile] = rank;
         }
         __syncthreads();
     }
+}
+
+// The slotted survivors of the last depth into the match list: slot i of a tile whose survivors all
+// fit its slots goes to place found + tile_off[tile] + i.
+__global__ void k_bj_gather(BJView v, BJLevel L) {
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t tile = x / BJ_SLOT, i = x % BJ_SLOT;
+    if (tile >= L.ntiles) return;
+    const uint64_t c = L.tile_cnt[tile];
+    if (c > BJ_SLOT || i >= c) return;
+    const uint64_t g = *L.found + L.tile_off[tile] + i;
+    if (g >= L.cap) return;
+    const uint32_t n = v.R.n_lhs, d = L.d;
+    const uint64_t q = L.slot_q[tile * BJ_SLOT + i];
+    const uint32_t* part = L.F + q * d;
+    for (uint32_t t = 0; t < d; ++t) L.out[g * n + v.R.order[t]] = part[t];
+    L.out[g * n + v.R.order[d]] = L.slot_e[tile * BJ_SLOT + i];
+}
+
+// Device-side bookkeeping of a depth: acc[0] matches found, acc[1] algorithmic bytes (the
+// candidate reads of every item, the partials' lists, the frontier or the matches written).
+__global__ void k_bj_tally(uint64_t* acc, const uint64_t* T, const uint64_t* S, uint64_t P, uint32_t d,
+                           uint32_t a, uint32_t n, uint32_t last, uint64_t cap) {
+    const uint64_t s = *S, t = *T;
+    uint64_t bytes = t * 4ull * (1ull + a) + P * (4ull * d + 12ull);
+    if (last) {
+        const uint64_t before = acc[0];
+        const uint64_t room = cap > before ? cap - before : 0;
+        bytes += (s < room ? s : room) * 4ull * n;
+        acc[0] = before + s;
+    } else {
+        bytes += s * 4ull * (d + 1);
+    }
+    acc[1] += bytes;
 }
 
 }  // namespace hgx
@@ -261,7 +378,7 @@ static uint32_t blocks_for(uint64_t n, uint32_t threads) { return (uint32_t)std:
 
 // CSR (vertex, position) -> ascending edges of a state held on the device as ev [m * A], ar [m],
 // vertex ids below nv.
-static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_t* alg_bytes) {
+static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_t* alg_bytes, uint32_t* maxlist) {
     const uint32_t m = v.m, A = v.A;
     const uint64_t nkeys = (uint64_t)nv * A, slots = (uint64_t)m * A;
     if (nkeys >= 0xFFFFFFF0ull || slots >= 0x7FFFFFF0ull)
@@ -272,6 +389,11 @@ static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_
     hipLaunchKernelGGL(k_bj_zero<uint32_t>, dim3(blocks_for(nkeys + 1, 256)), dim3(256), 0, st, cnt, nkeys + 1);
     if (m) hipLaunchKernelGGL(k_index_count, dim3(blocks_for(m, 256)), dim3(256), 0, st, v.ev, v.ar, m, A, cnt);
     bj_scan(mem, cnt, off64, nkeys, st);
+    {   // the longest list, which bounds the candidates of every partial match
+        size_t tmp = 0;
+        HIP_TRY(hipcub::DeviceReduce::Max(nullptr, tmp, cnt, maxlist, (int)(nkeys + 1), st));
+        HIP_TRY(hipcub::DeviceReduce::Max(mem.get(tmp), tmp, cnt, maxlist, (int)(nkeys + 1), st));
+    }
     hipLaunchKernelGGL(k_u64_to_u32, dim3(blocks_for(nkeys + 1, 256)), dim3(256), 0, st, off64, off,
                        (uint32_t)(nkeys + 1));
     uint32_t* kin = bget<uint32_t>(mem, slots);
@@ -295,7 +417,9 @@ static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_
 }
 
 // Extends the partials F (P of them, d edges each) by pattern edge order[d], and on to the last
-// depth; splits F in two, in order, when the next frontier would outgrow the memory at hand.
+// depth; splits F in two, in order, when the next frontier would outgrow the memory at hand. The
+// host learns a depth's survivors only where it must size the next frontier; the last depth
+// writes its matches at the device-side running count and leaves the host nothing to wait for.
 static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
     BJMem& mem = *J.mem;
     const uint32_t n = J.v.R.n_lhs, a = J.v.R.la[J.v.R.order[d]];
@@ -307,36 +431,53 @@ static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
     hipLaunchKernelGGL(k_bj_zero<uint32_t>, dim3(1), dim3(64), 0, J.st, cnt + P, (uint64_t)1);
     hipLaunchKernelGGL(k_bj_range, dim3(blocks_for(P, 256)), dim3(256), 0, J.st, J.v, d, F, P, lo, cnt);
     bj_scan(mem, cnt, start, P, J.st);
-    const uint64_t T = read1(start + P, J.st);
-    J.alg_bytes += P * (4ull * d + 12ull);
-    if (T == 0) { mem.rewind(mk); return; }
-    const uint64_t ntiles = (T + BJ_TILE - 1) / BJ_TILE;
+    // tiles to provision: the longest candidate list bounds every partial's; an exact count (one
+    // round trip) only when that bound is loose beyond use
+    const uint64_t per = J.v.R.piv[d] == 0xFFu ? (uint64_t)J.v.m : std::max<uint64_t>(1, J.maxlist);
+    uint64_t ntiles = (P * per + BJ_TILE - 1) / BJ_TILE;
+    if (P * per / std::max<uint64_t>(1, P) != per || ntiles > ((uint64_t)1 << 20)) {
+        const uint64_t T = read1(start + P, J.st);
+        ntiles = (T + BJ_TILE - 1) / BJ_TILE;
+    }
+    if (ntiles == 0) { mem.rewind(mk); return; }
     uint64_t* tcnt = bget<uint64_t>(mem, ntiles + 1);
     uint64_t* toff = bget<uint64_t>(mem, ntiles + 1);
     hipLaunchKernelGGL(k_bj_zero<uint64_t>, dim3(1), dim3(64), 0, J.st, tcnt + ntiles, (uint64_t)1);
     BJLevel L{};
-    L.d = d; L.F = F; L.P = P; L.lo = lo; L.start = start; L.T = T; L.tile_cnt = tcnt; L.tile_off = toff;
+    L.d = d; L.F = F; L.P = P; L.lo = lo; L.start = start; L.ntiles = ntiles; L.tile_cnt = tcnt; L.tile_off = toff;
+    L.found = J.acc; L.cap = J.cap;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, J.grid);
-    if (last) hipLaunchKernelGGL((k_bj_items<false, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
-    else hipLaunchKernelGGL((k_bj_items<false, false>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+    if (last) {
+        if (J.out) {
+            L.slot_q = bget<uint64_t>(mem, ntiles * BJ_SLOT);
+            L.slot_e = bget<uint32_t>(mem, ntiles * BJ_SLOT);
+            hipLaunchKernelGGL((k_bj_items<BJ_SLOTS, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+        } else {
+            hipLaunchKernelGGL((k_bj_items<BJ_COUNT, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+        }
+    } else {
+        hipLaunchKernelGGL((k_bj_items<BJ_COUNT, false>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+    }
     HIP_TRY(hipGetLastError());
     bj_scan(mem, tcnt, toff, ntiles, J.st);
-    const uint64_t S = read1(toff + ntiles, J.st);
-    // every item reads its candidate id and the candidate's vertices
-    J.alg_bytes += T * 4ull * (1ull + a);
     if (last) {
-        if (J.out && J.found < J.cap && S) {
-            L.out = J.out; L.out_base = J.found; L.cap = J.cap;
-            hipLaunchKernelGGL((k_bj_items<true, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+        if (J.out) {
+            L.out = J.out;
+            hipLaunchKernelGGL(k_bj_gather, dim3(blocks_for(ntiles * BJ_SLOT, 256)), dim3(256), 0, J.st, J.v, L);
+            // tiles whose survivors overflowed their slots: a second pass over those tiles only
+            hipLaunchKernelGGL((k_bj_items<BJ_EMIT, true>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
             HIP_TRY(hipGetLastError());
-            J.alg_bytes += std::min(S, J.cap - J.found) * 4ull * n;
         }
-        J.found += S;
+        hipLaunchKernelGGL(k_bj_tally, dim3(1), dim3(1), 0, J.st, J.acc, start + P, toff + ntiles, P, d, a, n,
+                           1u, J.out ? J.cap : 0ull);
         mem.rewind(mk);
         return;
     }
+    const uint64_t S = read1(toff + ntiles, J.st);
     if (S == 0) { mem.rewind(mk); return; }
-    const uint64_t need = S * (4ull * (d + 1) + 16ull) + ((S * 4ull + BJ_TILE - 1) / BJ_TILE) * 16ull;
+    // the next frontier, the next depth's lists and scan, and its tiles (counts, offsets, slots)
+    const uint64_t need = S * (4ull * (d + 1) + 16ull) +
+                          (S * std::max<uint64_t>(1, J.maxlist) / BJ_TILE + 2) * (16ull + 12ull * BJ_SLOT);
     if (need > mem.room()) {
         if (P == 1) throw Error{HGX_ERR_CAPACITY, "one partial match of the join has more extensions than memory holds"};
         mem.rewind(mk);
@@ -346,9 +487,9 @@ static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
     }
     uint32_t* Fn = bget<uint32_t>(mem, S * (d + 1));
     L.Fn = Fn;
-    hipLaunchKernelGGL((k_bj_items<true, false>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
+    hipLaunchKernelGGL((k_bj_items<BJ_EMIT, false>), dim3(grid), dim3(BJ_THREADS), 0, J.st, J.v, L);
     HIP_TRY(hipGetLastError());
-    J.alg_bytes += S * 4ull * (d + 1);
+    hipLaunchKernelGGL(k_bj_tally, dim3(1), dim3(1), 0, J.st, J.acc, start + P, toff + ntiles, P, d, a, n, 0u, 0ull);
     bj_depth(J, d + 1, Fn, S);
     mem.rewind(mk);
 }
@@ -370,12 +511,15 @@ void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed
         HIP_TRY(hipEventCreate(&t0)); HIP_TRY(hipEventCreate(&t1)); HIP_TRY(hipEventCreate(&t2));
         HIP_TRY(hipEventRecord(t0, J.st));
     }
-    bj_index(mem, J.v, nv, J.st, &J.index_bytes);
+    uint32_t* scal = bget<uint32_t>(mem, 2);          // seeds, longest candidate list
+    uint32_t* nsel = scal;
+    J.acc = bget<uint64_t>(mem, 2);
+    hipLaunchKernelGGL(k_bj_zero<uint64_t>, dim3(1), dim3(64), 0, J.st, J.acc, (uint64_t)2);
+    bj_index(mem, J.v, nv, J.st, &J.index_bytes, scal + 1);
     if (timed) HIP_TRY(hipEventRecord(t1, J.st));
     const uint32_t m = J.v.m;
     uint8_t* flag = bget<uint8_t>(mem, m);
     uint32_t* seeds = bget<uint32_t>(mem, m);
-    uint32_t* nsel = bget<uint32_t>(mem, 1);
     if (m) {
         hipLaunchKernelGGL(k_bj_seed, dim3(blocks_for(m, 256)), dim3(256), 0, J.st, J.v, flag);
         hipcub::CountingInputIterator<uint32_t> ids(0);
@@ -385,7 +529,11 @@ void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed
     } else {
         hipLaunchKernelGGL(k_bj_zero<uint32_t>, dim3(1), dim3(64), 0, J.st, nsel, (uint64_t)1);
     }
-    const uint64_t P0 = read1(nsel, J.st);
+    uint32_t hs[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, J.st));
+    HIP_TRY(hipStreamSynchronize(J.st));
+    const uint64_t P0 = hs[0];
+    J.maxlist = hs[1];
     J.alg_bytes += (uint64_t)m * (4ull * J.v.A + 1) + P0 * 4ull;
     if (J.v.R.n_lhs == 1) {
         if (J.out && P0) {
@@ -396,14 +544,16 @@ void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed
     } else if (P0) {
         bj_depth(J, 1, seeds, P0);
     }
+    if (timed) HIP_TRY(hipEventRecord(t2, J.st));
+    uint64_t acc[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(acc, J.acc, sizeof(acc), hipMemcpyDeviceToHost, J.st));
+    HIP_TRY(hipStreamSynchronize(J.st));
+    if (J.v.R.n_lhs > 1) J.found = acc[0];
+    J.alg_bytes += acc[1];
     if (timed) {
-        HIP_TRY(hipEventRecord(t2, J.st));
-        HIP_TRY(hipStreamSynchronize(J.st));
         HIP_TRY(hipEventElapsedTime(ms_index, t0, t1));
         HIP_TRY(hipEventElapsedTime(ms_join, t1, t2));
         (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(t2);
-    } else {
-        HIP_TRY(hipStreamSynchronize(J.st));
     }
 }
 

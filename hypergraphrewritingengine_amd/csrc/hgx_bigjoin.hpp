@@ -50,6 +50,8 @@ struct BJRun {
     uint32_t* out;            // matches (device), or null to count
     uint64_t cap = 0;
     uint64_t found = 0;
+    uint32_t maxlist = 0;     // the longest candidate list of the index
+    uint64_t* acc = nullptr;  // device: matches found, join bytes (written by the depths)
     uint64_t index_bytes = 0; // algorithmic bytes of the index build
     uint64_t alg_bytes = 0;   // of the join: adjacency reads of every item, frontiers, matches
 };
