@@ -83,8 +83,19 @@ def cpu_baseline(reqs, gpu_counts, budget_s, threads):
         canon += c["canonical_states"]
         raw += c["raw_states"]
         n += 1
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": canon / secs if secs else None, "unit": "canonical states/s",
             "cores": threads if kind == "reference" else 1, "kind": kind,
+            "cpu_model": model, "nproc": os.cpu_count(),
+            "cores_note": "worker threads are held to the CPU share of one GPU on the measuring host (16), "
+                          "whatever nproc reports",
             "sample": f"first {n} of the {len(reqs)} runs, each evolved to the GPU's generation count "
                       f"({raw} raw / {canon} canonical states, {secs:.2f} s engine time)",
             "raw_states_per_s": raw / secs if secs else None, "count_mismatches": mismatch}
@@ -349,6 +360,12 @@ def main():
     ap.add_argument("--config2-steps", type=int, default=6)
     ap.add_argument("--forward", type=int, default=-1, choices=(-1, 0, 1),
                     help="incremental match forwarding: 1 on, 0 off, -1 automatic")
+    ap.add_argument("--no-relations", dest="relations", action="store_false",
+                    help="skip the relations-materialised pass")
+    ap.add_argument("--readback-sample", type=int, default=256)
+    ap.add_argument("--autotune", default="auto", choices=("auto", "on", "off"),
+                    help="tune launch shapes on the batch before the timed passes (untimed): auto = when "
+                         "no tuned shapes were loaded for this device")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a known-size kernel for scaling "
                          "hardware byte counters)")
@@ -369,6 +386,14 @@ def main():
         copy_calibration(args.copy_calibration << 20)
     reqs = [W.batched_run(i) for i in range(args.runs)]
     batch = ctx.prepare(reqs)               # host-side request records, built once
+    tune_s = None
+    if args.autotune == "on" or (args.autotune == "auto" and "autotuned" not in ctx.tuning()["source"]
+                                 and "tuned (" not in ctx.tuning()["source"]):
+        t = time.perf_counter()
+        ctx.autotune(batch, reps=2)
+        tune_s = round(time.perf_counter() - t, 2)
+    tuning = ctx.tuning()
+    print(f"[bench] launch shapes: {tuning} (tuning took {tune_s} s)", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         ctx.evolve_batch(batch)
@@ -414,6 +439,36 @@ def main():
     if tr:
         roofline["traffic"] = tr["bytes"]
         roofline["traffic_source"] = tr["source"]
+
+    # the same batch with the causal and branchial pair lists materialised on the device, and every
+    # array of a sample of its results read back (the reference engine always materialises them)
+    relations = None
+    if args.relations:
+        rreqs = [W.batched_run(i, flags=0x7) for i in range(args.runs)]
+        rbatch = ctx.prepare(rreqs)
+        ctx.evolve_batch(rbatch)
+        t = time.perf_counter()
+        rres = ctx.evolve_batch(rbatch)
+        rel_ms = (time.perf_counter() - t) * 1e3
+        rcanon = sum(int(x) for x in rres.counts()["canonical_states"])
+        k = min(args.readback_sample, len(rres))
+        t = time.perf_counter()
+        nread = 0
+        for i in range(k):
+            r = rres[i]
+            h, _, _ = r.states()
+            ev = r.events()
+            cp, _ = r.causal()
+            br = r.branchial()
+            nread += len(h) + len(ev) + len(cp) + len(br)
+        rb_s = time.perf_counter() - t
+        relations = {"ms_per_step": round(rel_ms, 3), "canonical_states_per_s": round(rcanon / (rel_ms * 1e-3), 1),
+                     "readback": {"runs": k, "records": nread, "s": round(rb_s, 3),
+                                  "records_per_s": round(nread / rb_s, 1) if rb_s else None},
+                     "note": "flags causal|branchial|relations: pair lists built on the device; the readback "
+                             "sample reads states, events, causal and branchial pairs of the first runs"}
+        print(f"[bench] relations: {relations}", file=sys.stderr, flush=True)
+        del rres, rbatch
 
     # plain path: the same requests one at a time, serially
     plain = None
@@ -494,6 +549,8 @@ def main():
         "speedup_vs_plain": round(value / plain["canonical_states_per_s"], 2) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tuning": {"shapes": tuning, "autotune_s": tune_s},
+        "relations": relations,
     }
     print(json.dumps(line), flush=True)
 

@@ -35,7 +35,8 @@ EXPORTED_SYMBOLS = [
     "hgx_ctx_set_option", "hgx_batch_prepare", "hgx_batch_evolve", "hgx_batch_release",
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
     "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
-    "hgx_result_causal_triples", "hgx_match_count_noalloc",
+    "hgx_result_causal_triples", "hgx_match_count_noalloc", "hgx_autotune", "hgx_tune_load",
+    "hgx_tune_get",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -89,6 +90,10 @@ def load_library():
     L.hgx_result_timing.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.hgx_result_status.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.hgx_result_causal_triples.argtypes = [vp, vp, vp, C.c_uint64]
+    L.hgx_autotune.argtypes = [vp, vp, C.c_uint32, C.c_char_p, C.c_char_p, C.c_size_t]
+    L.hgx_tune_load.argtypes = [vp, C.c_char_p]
+    L.hgx_tune_get.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.hgx_tune_get.restype = C.c_size_t
     L.hgx_match_count_noalloc.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_uint32,
                                           C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), vp, C.c_uint64]
     L.hgx_result_release.argtypes = [vp]
@@ -572,6 +577,31 @@ class Context:
         if rc != abi.HGX_OK:
             raise self._err(rc)
         return Result(self._lib, out, self)
+
+    def autotune(self, requests, reps: int = 2, cache_path: str | None = None) -> dict:
+        """Measure launch shapes per size class over `requests` (a list or a PreparedBatch made by
+        this context) and keep the fastest (hgx_autotune); with cache_path, write the JSON cache
+        hgx_ctx_create reads. Returns {"chosen": ..., "measured": ...}."""
+        import json as _json
+        pb = requests if isinstance(requests, PreparedBatch) and requests._h is not None else self.prepare(requests)
+        buf = C.create_string_buffer(1 << 16)
+        rc = self._lib.hgx_autotune(self._h, pb._h, int(reps),
+                                    cache_path.encode() if cache_path else None, buf, len(buf))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+        return _json.loads(buf.value.decode())
+
+    def tune_load(self, path: str):
+        rc = self._lib.hgx_tune_load(self._h, path.encode())
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def tuning(self) -> dict:
+        """The launch shapes in effect and where they came from."""
+        import json as _json
+        buf = C.create_string_buffer(4096)
+        self._lib.hgx_tune_get(self._h, buf, len(buf))
+        return _json.loads(buf.value.decode())
 
     def profile(self, on: bool = True):
         """Event-time every generation kernel launch (see hgx_profile_enable)."""

@@ -183,11 +183,11 @@ struct ExpandClass {
 };
 constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
 // The expansion's size classes are finer at the small end, where a child's workspace is a few KiB
-// and several children share one wave (sub-wave teams of kExpandLanes lanes): a wave hashes four
-// children of up to 32 edges, or two of up to 64, each in its own workspace.
+// and several children share one wave (sub-wave teams, by default: a wave hashes four children of
+// up to 32 edges, or two of up to 64, each in its own workspace; the autotuner may choose others).
 constexpr int kExpandClasses = 6;
 constexpr uint32_t kExpandEdges[kExpandClasses] = {32, 64, 128, 512, 2048, 0xFFFFFFFFu};
-constexpr uint32_t kExpandLanes[kExpandClasses] = {16, 32, 64, 64, 64, 64};
+static_assert(kExpandClasses == kTuneExpandClasses, "one tuned shape per expansion size class");
 }  // namespace
 
 uint32_t Engine::step() {
@@ -384,7 +384,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                         return (match_lds_per_wave(mc.max_ne, mc.max_nextv, F_->v.A, mc.KL, mc.dmax, mc.small != 0,
                                                    t, mc.fwd_n != 0) + 15u) & ~15u;
                     };
-                    const uint32_t team = pick_team(lpw(1), ctx_->join_team);
+                    const int kc = (int)(&x - cls.data());
+                    const uint32_t team = pick_team(lpw(1), ctx_->join_team ? ctx_->join_team : ctx_->tune.join_team[kc]);
                     mc.lds_per_wave = lpw(team);
                     if (x.n != n) {
                         uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
@@ -397,7 +398,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     // up to 16 workgroups per CU (about three times what stays resident): measured
                     // best of 5, 8, 16, 32 and 64 on the batched workload (0.8 ms per pass over 8)
                     timed(ctx_, st, name, (uint64_t)((double)bytes * x.n / n), x.n,
-                          [&] { return launch_match(mc, emit, ctx_->num_cu * 16, team == 1 ? wpb_m : 1, team, st); });
+                          [&] { return launch_match(mc, emit, ctx_->num_cu * ctx_->tune.join_grid[kc], team == 1 ? wpb_m : 1,
+                                                    team, st); }, kc);
                 }
             };
             HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
@@ -710,23 +712,24 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 // Team: one wave per child while a CU still holds several workspaces, else the
                 // workgroup's waves share one child (a CU holds 160 KiB of LDS).
                 const uint32_t team = pick_team(ea.lds_per_wave, ctx_->expand_team);
+                const uint32_t gcap = ctx_->num_cu * ctx_->tune.expand_grid[k];
                 uint32_t wpb_e = 1, lanes = 64;
                 if (team == 1) {
                     // sub-wave teams for small children, when the wave's workspaces fit a workgroup
-                    const uint32_t want = ctx_->expand_lanes ? ctx_->expand_lanes : kExpandLanes[k];
+                    const uint32_t want = ctx_->expand_lanes ? ctx_->expand_lanes : ctx_->tune.expand_lanes[k];
                     if (want < 64 && (uint64_t)ea.lds_per_wave * (64 / want) <= ctx_->max_lds) {
                         lanes = want;
                         wpb_e = waves_per_block(ea.lds_per_wave * (64 / lanes), ctx_->max_lds);
                     } else {
                         wpb_e = place_workspace(tmp, ea.lds_per_wave, ea.n_items, ctx_->max_lds,
-                                                ctx_->num_cu * 8, &ea.scratch);
+                                                gcap, &ea.scratch);
                     }
                 } else if (ea.lds_per_wave > ctx_->max_lds) {
-                    const uint32_t grid = expand_grid(ea.n_items, team, 1, ctx_->num_cu * 8);
+                    const uint32_t grid = expand_grid(ea.n_items, team, 1, gcap);
                     ea.scratch = (uint8_t*)tmp.alloc((size_t)grid * ea.lds_per_wave);
                 }
                 timed(ctx_, st, "k_expand", (uint64_t)((double)bytes * x.n / M), x.n,
-                      [&] { return launch_expand(ea, ctx_->num_cu * 8, wpb_e, team, st, lanes); });
+                      [&] { return launch_expand(ea, gcap, wpb_e, team, st, lanes); }, k);
             }
             timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });
@@ -1166,6 +1169,8 @@ extern "C" int hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* b, hgx_result** r
         return HGX_OK;
     });
 }
+
+uint32_t hgx_batch_size(const hgx_batch* b) { return b ? (uint32_t)b->runs.size() : 0u; }
 
 extern "C" void hgx_batch_release(hgx_batch* b) {
     if (!b) return;

@@ -412,12 +412,16 @@ void prof_resolve(hgx_ctx* ctx) {
     for (auto& p : ctx->prof.pend) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
-        hgx_kernel_stat& s = ctx->prof.acc[p.name];
-        std::snprintf(s.name, sizeof(s.name), "%s", p.name.c_str());
-        s.launches += 1;
-        s.total_ms += ms;
-        s.alg_bytes += p.bytes;
-        s.items += p.items;
+        auto add = [&](const std::string& name) {
+            hgx_kernel_stat& s = ctx->prof.acc[name];
+            std::snprintf(s.name, sizeof(s.name), "%s", name.c_str());
+            s.launches += 1;
+            s.total_ms += ms;
+            s.alg_bytes += p.bytes;
+            s.items += p.items;
+        };
+        add(p.name);
+        if (ctx->prof.by_class && p.cls >= 0) add(p.name + "#" + std::to_string(p.cls));
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
     }
@@ -473,6 +477,7 @@ extern "C" int hgx_ctx_create(int device, uint64_t budget_bytes, hgx_ctx** out) 
     c->pool = std::make_shared<ChunkPool>(budget, (size_t)64 << 20);
     c->num_cu = (uint32_t)prop.multiProcessorCount;
     c->max_lds = (uint32_t)prop.sharedMemPerBlock;
+    (void)tune_load(c, default_tune_cache());   // tuned launch shapes of this device, when cached
     char buf[256];
     std::snprintf(buf, sizeof(buf), "%s, %d CUs, %.1f GiB HBM, LDS/block %u B, budget %.1f GiB",
                   prop.gcnArchName, prop.multiProcessorCount, totalb / 1073741824.0, c->max_lds,

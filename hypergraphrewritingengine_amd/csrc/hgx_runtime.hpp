@@ -170,11 +170,25 @@ struct ProfPending {
     std::string name;
     hipEvent_t a, b;
     uint64_t bytes, items;
+    int cls;                  // size class of the launch, or -1
 };
 struct Profiler {
     bool on = false;
+    bool by_class = false;    // also keep "<kernel>#<class>" totals (the autotuner's view)
     std::vector<ProfPending> pend;
     std::map<std::string, hgx_kernel_stat> acc;
+};
+
+// Launch shapes per size class, chosen by the autotuner (hgx_autotune) or loaded from its cache;
+// the defaults are the hand-fitted rule. None of them changes a result.
+constexpr int kTuneExpandClasses = 6;   // children of up to 32, 64, 128, 512, 2048 edges, more
+constexpr int kTuneJoinClasses = 4;     // parents of up to 128, 512, 2048 edges, more
+struct Tune {
+    uint32_t expand_lanes[kTuneExpandClasses] = {16, 32, 64, 64, 64, 64};   // lanes per child
+    uint32_t expand_grid[kTuneExpandClasses] = {8, 8, 8, 8, 8, 8};          // workgroups per CU
+    uint32_t join_team[kTuneJoinClasses] = {0, 0, 0, 0};                    // waves per state, 0 = by size
+    uint32_t join_grid[kTuneJoinClasses] = {16, 16, 16, 16};                // workgroups per CU
+    std::string source = "built-in defaults";
 };
 
 class Engine;
@@ -198,15 +212,17 @@ struct hgx_ctx {
     uint32_t expand_team = 0;                    // HGX_OPTION_EXPAND_TEAM
     uint32_t join_team = 0;                      // HGX_OPTION_JOIN_TEAM
     uint32_t expand_lanes = 0;                   // HGX_OPTION_EXPAND_LANES
+    hgx::Tune tune;                              // launch shapes per size class
 };
 
 namespace hgx {
 
 // Launch with optional event timing; `bytes` are the launch's algorithmic HBM bytes.
 template <class F>
-void timed(hgx_ctx* ctx, hipStream_t st, const char* name, uint64_t bytes, uint64_t items, F&& launch) {
+void timed(hgx_ctx* ctx, hipStream_t st, const char* name, uint64_t bytes, uint64_t items, F&& launch,
+           int cls = -1) {
     if (!ctx->prof.on) { HIP_TRY(launch()); return; }
-    ProfPending p{name, nullptr, nullptr, bytes, items};
+    ProfPending p{name, nullptr, nullptr, bytes, items, cls};
     HIP_TRY(hipEventCreate(&p.a));
     HIP_TRY(hipEventCreate(&p.b));
     HIP_TRY(hipEventRecord(p.a, st));
@@ -215,6 +231,11 @@ void timed(hgx_ctx* ctx, hipStream_t st, const char* name, uint64_t bytes, uint6
     ctx->prof.pend.push_back(p);
 }
 void prof_resolve(hgx_ctx* ctx);
+// Loads the tuned launch shapes of this device (key "<arch>:<CUs>") from a cache written by
+// hgx_autotune; false when the file or the key is absent.
+bool tune_load(hgx_ctx* ctx, const std::string& path);
+// HGX_TUNE_CACHE, else tune_cache.json beside the library.
+std::string default_tune_cache();
 
 template <class F>
 int guarded(hgx_ctx* ctx, F&& f) {

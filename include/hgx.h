@@ -342,6 +342,25 @@ int hgx_match_count_noalloc(hgx_ctx* ctx, const uint8_t* rules, uint32_t rules_l
                             uint32_t stride, uint32_t n_vertices, uint64_t* count, uint32_t* match_edges,
                             uint64_t cap);
 
+/* ---- autotuning ------------------------------------------------------------------------- */
+
+/* Launch shapes per size class -- for the expansion (children of up to 32, 64, 128, 512, 2048 edges,
+ * more) the lanes per child (16, 32, 64) and workgroups per CU; for the join (parents of up to 128,
+ * 512, 2048 edges, more) the waves per state (by size, 1, 4) and workgroups per CU -- measured on
+ * the device over `batch` (each candidate run reps times after a warm-up pass, launches
+ * event-timed per class) and the fastest kept per class. The choice takes effect in ctx at once
+ * and, with cache_path, is written to a JSON cache under the key "<arch>:<CUs>" (other keys kept);
+ * hgx_ctx_create loads its device's entry from HGX_TUNE_CACHE, else tune_cache.json beside the
+ * library. report (may be NULL) receives the choice and every measurement as JSON. No result
+ * depends on the shapes. No counterpart in the reference, whose GPU launch shapes are fixed by
+ * hand (gpu/src/persistent.cu:917-923). */
+int hgx_autotune(hgx_ctx* ctx, const hgx_batch* batch, uint32_t reps, const char* cache_path,
+                 char* report, size_t report_len);
+/* Load this device's tuned shapes from a cache written by hgx_autotune. */
+int hgx_tune_load(hgx_ctx* ctx, const char* path);
+/* The shapes in effect, as JSON (with their source); returns the full length. */
+size_t hgx_tune_get(const hgx_ctx* ctx, char* buf, size_t len);
+
 /* ---- kernel-level measurement ----------------------------------------------------------- */
 
 /* With profiling on, every launch of the generation kernels is bracketed by HIP events on the
