@@ -360,6 +360,8 @@ def main():
     ap.add_argument("--config2-steps", type=int, default=6)
     ap.add_argument("--forward", type=int, default=-1, choices=(-1, 0, 1),
                     help="incremental match forwarding: 1 on, 0 off, -1 automatic")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="skip the graph-replayed passes")
     ap.add_argument("--no-relations", dest="relations", action="store_false",
                     help="skip the relations-materialised pass")
     ap.add_argument("--readback-sample", type=int, default=256)
@@ -411,6 +413,31 @@ def main():
     value = canon / (ms_per_step / 1000.0)
     print(f"[bench] batched: {ms_per_step:.2f} ms/step, {canon} canonical / {raw} raw states; "
           f"last pass {ctx.last_timing}", file=sys.stderr, flush=True)
+
+    # the same pass recorded into one HIP graph: one graph launch per pass (graph on; the line
+    # above is graph off)
+    graph = None
+    if args.graph:
+        t = time.perf_counter()
+        rp = ctx.capture(batch)
+        cap_s = time.perf_counter() - t
+        rp.launch()
+        t0g = time.perf_counter()
+        for _ in range(args.steps):
+            gres = rp.launch()
+        g_ms = (time.perf_counter() - t0g) * 1000.0 / args.steps
+        gcounts = gres.counts()
+        same = all(int(gcounts["canonical_states"][i]) == counts[i]["canonical_states"] and
+                   int(gcounts["raw_states"][i]) == counts[i]["raw_states"] for i in range(len(counts)))
+        graph = {"ms_per_step": round(g_ms, 3),
+                 "canonical_states_per_s": round(canon / (g_ms / 1000.0), 1),
+                 "capture_s": round(cap_s, 3), "counts_equal_eager": same,
+                 "note": "hgx_batch_capture: the pass recorded once into one HIP graph, replayed with one "
+                         "graph launch per step; every kernel reruns, the per-run totals are checked on the "
+                         "device against the recording"}
+        print(f"[bench] graph: {graph}", file=sys.stderr, flush=True)
+        del gres
+        rp.close()
 
     # the same batch handed over as host buffers (parse + upload inside the call)
     t = time.perf_counter()
@@ -550,6 +577,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "tuning": {"shapes": tuning, "autotune_s": tune_s},
+        "graph": graph,
         "relations": relations,
     }
     print(json.dumps(line), flush=True)

@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = [
     "hgx_result_consumed", "hgx_run_wxf", "hgx_buffer_free", "hgx_wxf_validate",
     "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
     "hgx_result_causal_triples", "hgx_match_count_noalloc", "hgx_autotune", "hgx_tune_load",
-    "hgx_tune_get",
+    "hgx_tune_get", "hgx_batch_capture", "hgx_replay_launch", "hgx_replay_release",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -91,6 +91,10 @@ def load_library():
     L.hgx_result_status.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.hgx_result_causal_triples.argtypes = [vp, vp, vp, C.c_uint64]
     L.hgx_autotune.argtypes = [vp, vp, C.c_uint32, C.c_char_p, C.c_char_p, C.c_size_t]
+    L.hgx_batch_capture.argtypes = [vp, vp, C.POINTER(vp)]
+    L.hgx_replay_launch.argtypes = [vp, vp, C.POINTER(vp)]
+    L.hgx_replay_release.argtypes = [vp]
+    L.hgx_replay_release.restype = None
     L.hgx_tune_load.argtypes = [vp, C.c_char_p]
     L.hgx_tune_get.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.hgx_tune_get.restype = C.c_size_t
@@ -372,6 +376,44 @@ class PreparedBatch:
             pass
 
 
+class Replay:
+    """A prepared batch's pass recorded into one HIP graph (hgx_batch_capture). launch() runs it
+    and returns the results; they are valid until the next launch or close()."""
+
+    def __init__(self, ctx, batch: "PreparedBatch"):
+        self._ctx, self._batch = ctx, batch
+        h = C.c_void_p()
+        rc = ctx._lib.hgx_batch_capture(ctx._h, batch._h, C.byref(h))
+        if rc != abi.HGX_OK:
+            raise ctx._err(rc)
+        self._h = h
+        self.n = batch.n
+
+    def launch(self):
+        ctx = self._ctx
+        outs = (C.c_void_p * max(self.n, 1))()
+        rc = ctx._lib.hgx_replay_launch(ctx._h, self._h, outs)
+        if rc != abi.HGX_OK:
+            raise ctx._err(rc)
+        handles = _BatchHandles(ctx._lib, outs, self.n, ctx)
+        cnts = (abi.Counts * max(self.n, 1))()
+        rc = ctx._lib.hgx_result_counts_many(outs, self.n, cnts)
+        if rc != abi.HGX_OK:
+            raise ctx._err(rc)
+        return ResultList(handles, np.frombuffer(cnts, dtype=_COUNTS_DTYPE, count=self.n))
+
+    def close(self):
+        if self._h:
+            self._ctx._lib.hgx_replay_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """An hgx_ctx on one device: the HBM budget, the stream, the served-request queue."""
 
@@ -435,6 +477,10 @@ class Context:
         """Parse requests and stage their rules and initial states in device memory once
         (hgx_batch_prepare), for repeated evolve_batch calls from device-resident inputs."""
         return PreparedBatch(requests, self)
+
+    def capture(self, batch: "PreparedBatch") -> "Replay":
+        """Record a prepared batch's pass into one HIP graph (hgx_batch_capture)."""
+        return Replay(self, batch)
 
     def evolve_batch(self, requests):
         """Evolve requests (a list or a PreparedBatch) together on the device: a ResultList,

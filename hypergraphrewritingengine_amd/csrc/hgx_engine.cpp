@@ -29,6 +29,52 @@ Engine::Engine(hgx_ctx* ctx) : ctx_(ctx), st_(ctx->stream), pool_(ctx->pool), sl
 
 Engine::~Engine() {
     (void)hipStreamSynchronize(st_);
+    for (void* p : pinned_) (void)hipHostFree(p);
+}
+
+void Engine::set_mode(EngineMode m, std::shared_ptr<PassPlan> plan, uint32_t* check) {
+    mode_ = m;
+    plan_ = std::move(plan);
+    plan_rb_ = plan_poff_ = 0;
+    d_check_ = check;
+}
+
+void Engine::put(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (!bytes) return;
+    if (mode_ != EngineMode::Capture) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return;
+    }
+    void* h = nullptr;
+    HIP_TRY(hipHostMalloc(&h, bytes, hipHostMallocDefault));
+    pinned_.push_back(h);
+    std::memcpy(h, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, st));
+}
+
+void Engine::fill_counts(RunRec& r, const uint64_t* x) {
+    hgx_counts& c = r.counts;
+    std::memset(&c, 0, sizeof(c));
+    c.raw_states = r.states;
+    c.events = r.events;
+    c.canonical_states = x[CTR_CANON];
+    c.fingerprint = x[CTR_FINGERPRINT];
+    const bool rec_causal = r.hp->flags & HGX_REC_CAUSAL;
+    c.causal_triples = rec_causal ? x[CTR_TRIPLES] : 0;
+    c.causal_pairs = rec_causal ? x[CTR_PAIRS] : 0;
+    const bool tr = r.hp->flags & HGX_OPT_TR;
+    c.causal_pairs_tr = tr ? x[CTR_PAIRS_TR] : c.causal_pairs;
+    c.causal_triples_tr = tr ? x[CTR_TRIPLES_TR] : c.causal_triples;
+    c.branchial_pairs = (r.hp->flags & HGX_REC_BRANCHIAL) ? x[CTR_BRANCHIAL] : 0;
+    c.steps_done = r.steps_done;
+    c.truncated = r.status == HGX_ERR_CAPACITY ? 2u : (r.truncated ? 1u : 0u);
+}
+
+void Engine::resolve_counts() {
+    std::vector<uint64_t> ctr((size_t)hw_slots_ * CTR_N);
+    d2h(ctr.data(), d_ctr_, ctr.size(), st_);
+    HIP_TRY(hipStreamSynchronize(st_));
+    for (auto& kv : finished) fill_counts(*kv.second, &ctr[(size_t)kv.second->slot * CTR_N]);
 }
 
 void engine_destroy(std::unique_ptr<Engine>& e) { e.reset(); }
@@ -139,27 +185,23 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
 
 void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
     if (runs.empty()) return;
+    const auto now = Clock::now();
+    if (mode_ == EngineMode::Capture) {
+        // counters are read after each replay (resolve_counts); the run's slot, rules and map
+        // stay with it, since the graph writes them again on every replay
+        for (auto& r : runs) {
+            r->t_done = now;
+            finished[r->id] = r;
+            live_.erase(r->id);
+            live_footprint_ -= r->footprint;
+        }
+        return;
+    }
     std::vector<uint64_t> ctr((size_t)hw_slots_ * CTR_N);       // slots ever taken
     d2h(ctr.data(), d_ctr_, ctr.size(), st_);
     HIP_TRY(hipStreamSynchronize(st_));
-    const auto now = Clock::now();
     for (auto& r : runs) {
-        const uint64_t* x = &ctr[(size_t)r->slot * CTR_N];
-        hgx_counts& c = r->counts;
-        std::memset(&c, 0, sizeof(c));
-        c.raw_states = r->states;
-        c.events = r->events;
-        c.canonical_states = x[CTR_CANON];
-        c.fingerprint = x[CTR_FINGERPRINT];
-        const bool rec_causal = r->hp->flags & HGX_REC_CAUSAL;
-        c.causal_triples = rec_causal ? x[CTR_TRIPLES] : 0;
-        c.causal_pairs = rec_causal ? x[CTR_PAIRS] : 0;
-        const bool tr = r->hp->flags & HGX_OPT_TR;
-        c.causal_pairs_tr = tr ? x[CTR_PAIRS_TR] : c.causal_pairs;
-        c.causal_triples_tr = tr ? x[CTR_TRIPLES_TR] : c.causal_triples;
-        c.branchial_pairs = (r->hp->flags & HGX_REC_BRANCHIAL) ? x[CTR_BRANCHIAL] : 0;
-        c.steps_done = r->steps_done;
-        c.truncated = r->status == HGX_ERR_CAPACITY ? 2u : (r->truncated ? 1u : 0u);
+        fill_counts(*r, &ctr[(size_t)r->slot * CTR_N]);
         r->t_done = now;
         r->device_ms = std::chrono::duration<double, std::milli>(now - r->t_admit).count();
         finished[r->id] = r;
@@ -194,6 +236,10 @@ uint32_t Engine::step() {
     // runs of the frontier still evolving, and the counters before this generation touches them
     bool frontier_live = false;
     for (auto& r : F_runs_) frontier_live |= !r->done && live_.count(r->id) != 0;
+    if (mode_ == EngineMode::Capture) {             // no recovery inside a graph being recorded
+        std::vector<std::shared_ptr<RunRec>> admitted;
+        return step_impl(admitted);
+    }
     if (frontier_live && hw_slots_)
         HIP_TRY(hipMemcpyAsync(d_ctr_snap_, d_ctr_, sizeof(uint64_t) * hw_slots_ * CTR_N,
                                hipMemcpyDeviceToDevice, st_));
@@ -328,8 +374,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
         }
         if (any) {
             std::memcpy(h_tab_.data() + 12ull * cap_slots_, h_active.data(), cap_slots_);
-            h2d((uint8_t*)d_budget_, h_tab_.data(), h_tab_.size(), st);    // budget, sbeg, send, active
-            HIP_TRY(hipMemsetAsync(d_run_acc_, 0, sizeof(uint64_t) * (4ull * cap_slots_ + 1), st));
+            up((uint8_t*)d_budget_, h_tab_.data(), h_tab_.size(), st);    // budget, sbeg, send, active
+            HIP_TRY(launch_fill(d_run_acc_, 0, sizeof(uint64_t) * (4ull * cap_slots_ + 1), st));
             cnt = dalloc<uint32_t>(tmp, n + 1);
             moff = dalloc<uint64_t>(*keep, n + 1);
             if (F_->max_ne > 65535)
@@ -389,8 +435,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     mc.lds_per_wave = lpw(team);
                     if (x.n != n) {
                         uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
-                        h2d(d_r, x.rb.data(), x.rb.size(), st);
-                        h2d(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
+                        up(d_r, x.rb.data(), x.rb.size(), st);
+                        up(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
                         mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
                         mc.n_items = (uint32_t)x.n;
                     }
@@ -402,7 +448,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                                                     team, st); }, kc);
                 }
             };
-            HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
+            HIP_TRY(launch_fill(cnt, 0, sizeof(uint32_t) * (n + 1), st));
             launch_classes(false, mcls_count, "k_match_count", match_bytes + n * 4ull);
             scan_u32_to_u64(tmp, cnt, moff, n, st);
             HIP_TRY(launch_run_totals(moff, ma.state_edges, cnt, F_->v.ne, d_sbeg_, d_send_, cap_slots_, d_tot_,
@@ -410,9 +456,20 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             // per-slot child edges, totals and parent edges are adjacent
             // ([edges cap][totals cap + 1][parent edges cap]): one read
             std::vector<uint64_t> rb(3ull * cap_slots_ + 1);
-            d2h(rb.data(), d_run_edges_, rb.size(), st);
             t_launch = ms_since(t_step);
-            HIP_TRY(hipStreamSynchronize(st));
+            if (mode_ == EngineMode::Capture) {
+                // the recorded decision; each replay checks the device's totals against it
+                if (plan_rb_ >= plan_->rb.size() || plan_->rb[plan_rb_].size() != rb.size())
+                    throw Error{HGX_ERR_INVALID, "the pass being captured left its recording"};
+                rb = plan_->rb[plan_rb_++];
+                uint64_t* want = dalloc<uint64_t>(tmp, cap_slots_);
+                up(want, rb.data() + cap_slots_, cap_slots_, st);
+                HIP_TRY(launch_plan_check(d_tot_, want, d_budget_, cap_slots_, d_check_, st));
+            } else {
+                d2h(rb.data(), d_run_edges_, rb.size(), st);
+                HIP_TRY(hipStreamSynchronize(st));
+                if (mode_ == EngineMode::Record) plan_->rb.push_back(rb);
+            }
             t_count = ms_since(t_step);
             const uint64_t* htot = rb.data() + cap_slots_;
             const uint64_t* hpe = htot + cap_slots_ + 1;
@@ -446,7 +503,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             // status HGX_ERR_CAPACITY (the reference's capacity warning with a partial result,
             // gpu/src/evolve.cu:573-700) -- and the others build this generation unchanged, so
             // their results are what they would be alone.
-            {
+            if (mode_ != EngineMode::Capture) {
                 // per run: (bytes of its children's records in new arenas, frontier index); the
                 // match list and scratch grow arenas that already hold a chunk
                 std::vector<std::pair<uint64_t, uint32_t>> need;
@@ -494,7 +551,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 }
             }
             if (masked) {
-                h2d(d_active_, h_active.data(), cap_slots_, st);
+                up(d_active_, h_active.data(), cap_slots_, st);
                 HIP_TRY(launch_mask_counts(F_->v.run, d_active_, n, cnt, st));
                 scan_u32_to_u64(tmp, cnt, moff, n, st);
             }
@@ -512,7 +569,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     if (c > 0x80000000ull) throw Error{HGX_ERR_UNSUPPORTED, "state map above 2^31 slots"};
                     uint64_t* keys = (uint64_t*)slab_.alloc(8 * c);
                     uint32_t* vals = (uint32_t*)slab_.alloc(4 * c);
-                    HIP_TRY(hipMemsetAsync(keys, 0, 8 * c, st));
+                    HIP_TRY(launch_fill(keys, 0, 8 * c, st));
                     HIP_TRY(launch_rehash(r.map_keys, r.map_vals, r.map_cap, keys, vals, (uint32_t)(c - 1), st));
                     slab_.free(r.map_keys, 8ull * r.map_cap);
                     slab_.free(r.map_vals, 4ull * r.map_cap);
@@ -525,7 +582,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 mv.rule = dalloc<uint32_t>(*keep, M);
                 mv.q = dalloc<uint16_t>(*keep, M * K);
                 mv.cne = dalloc<uint32_t>(tmp, M + 1);
-                HIP_TRY(hipMemsetAsync(mv.cne + M, 0, sizeof(uint32_t), st));
+                HIP_TRY(launch_fill(mv.cne + M, 0, sizeof(uint32_t), st));
                 ma.M = mv;
                 launch_classes(true, mcls_emit, "k_match_emit", match_bytes + M * (12ull + 2ull * K));
             }
@@ -573,8 +630,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             a_items.push_back(it);
         }
         AdmitItem* d_items = dalloc<AdmitItem>(tmp, a_items.size());
-        if (nh) h2d(d_stage, a_rules.data(), a_rules.size(), st);
-        h2d(d_items, a_items.data(), a_items.size(), st);
+        if (nh) up(d_stage, a_rules.data(), a_rules.size(), st);
+        up(d_items, a_items.data(), a_items.size(), st);
         HIP_TRY(launch_admit(d_items, (uint32_t)a_items.size(), d_ctr_, st));
     }
 
@@ -668,7 +725,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             h_runs_[r.slot].ev_base = (uint32_t)r.events;
             h_runs_[r.slot].child_begin = cb[i];
         }
-        h2d(d_runs_, h_runs_.data(), hw_slots_, st);              // slots ever taken
+        up(d_runs_, h_runs_.data(), hw_slots_, st);              // slots ever taken
 
         if (M) {
             // Size classes: children grouped by their request's bound on child edges, each class
@@ -705,8 +762,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 ea.n_items = (uint32_t)x.n;
                 if (x.n != M) {
                     uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
-                    h2d(d_r, x.rb.data(), x.rb.size(), st);
-                    h2d(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
+                    up(d_r, x.rb.data(), x.rb.size(), st);
+                    up(d_r + x.rb.size(), x.rpre.data(), x.rpre.size(), st);
                     ea.rb = d_r; ea.rpre = d_r + x.rb.size(); ea.nrange = (uint32_t)x.rb.size();
                 }
                 // Team: one wave per child while a CU still holds several workspaces, else the
@@ -739,7 +796,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     rev.push_back(RunEvents{F_runs_[i]->slot, (uint32_t)b,
                                             (uint32_t)std::min<uint64_t>(ce[i], b + 4096)});
             RunEvents* d_rev = dalloc<RunEvents>(tmp, rev.size());
-            h2d(d_rev, rev.data(), rev.size(), st);
+            up(d_rev, rev.data(), rev.size(), st);
             HIP_TRY(launch_run_events(d_rev, (uint32_t)rev.size(), C->ev, tcnt, d_ctr_, st));
         }
 
@@ -768,16 +825,16 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     }
                 }
                 r_eoff[NRh] = EC + e;
-                h2d(v.run + M, r_run.data(), NRh, st);
-                h2d(v.ne + M, r_ne.data(), NRh, st);
-                h2d(v.nextv + M, r_nextv.data(), NRh, st);
-                h2d(v.sid + M, r_sid.data(), NRh, st);
-                h2d(v.depth + M, r_depth.data(), NRh, st);
-                h2d(v.eoff + M, r_eoff.data(), NRh + 1, st);
+                up(v.run + M, r_run.data(), NRh, st);
+                up(v.ne + M, r_ne.data(), NRh, st);
+                up(v.nextv + M, r_nextv.data(), NRh, st);
+                up(v.sid + M, r_sid.data(), NRh, st);
+                up(v.depth + M, r_depth.data(), NRh, st);
+                up(v.eoff + M, r_eoff.data(), NRh + 1, st);
                 if (NERh) {
-                    h2d(v.ar + EC, r_ar.data(), NERh, st);
-                    h2d(v.ev + EC * A, r_ev.data(), NERh * A, st);
-                    HIP_TRY(hipMemsetAsync(v.ps + EC, 0, NERh, st));
+                    up(v.ar + EC, r_ar.data(), NERh, st);
+                    up(v.ev + EC * A, r_ev.data(), NERh * A, st);
+                    HIP_TRY(launch_fill(v.ps + EC, 0, NERh, st));
                 }
             }
             if (NR > NRh) {
@@ -792,7 +849,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     e += r->hp->root_edges;
                 }
                 RootCopy* d_copy = dalloc<RootCopy>(tmp, r_copy.size());
-                h2d(d_copy, r_copy.data(), r_copy.size(), st);
+                up(d_copy, r_copy.data(), r_copy.size(), st);
                 HIP_TRY(launch_copy_roots(d_copy, (uint32_t)r_copy.size(), v, st));
             }
             uint32_t rmax_ne = 1, rmax_nv = 1;
@@ -832,9 +889,9 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     bcnt = dalloc<uint32_t>(tmp, n + 1);
                     HIP_TRY(hipMemcpyAsync(bcnt, cnt, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
                     uint8_t* d_bf = dalloc<uint8_t>(tmp, cap_slots_);
-                    h2d(d_bf, bflag.data(), cap_slots_, st);
+                    up(d_bf, bflag.data(), cap_slots_, st);
                     HIP_TRY(launch_mask_counts(F_->v.run, d_bf, n, bcnt, st));
-                    HIP_TRY(hipStreamSynchronize(st));
+                    if (mode_ != EngineMode::Capture) HIP_TRY(hipStreamSynchronize(st));
                 }
                 BranchialArgs ba{};
                 ba.P = F_->v; ba.M = mv; ba.runs = d_runs_; ba.cnt = bcnt; ba.moff = moff; ba.ctr = d_ctr_;
@@ -842,7 +899,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 uint32_t* pcnt = nullptr;
                 if (any_rel) {
                     pcnt = dalloc<uint32_t>(tmp, n + 1);
-                    HIP_TRY(hipMemsetAsync(pcnt, 0, sizeof(uint32_t) * (n + 1), st));
+                    HIP_TRY(launch_fill(pcnt, 0, sizeof(uint32_t) * (n + 1), st));
                 }
                 ba.pcnt = pcnt;
                 timed(ctx_, st, "k_branchial", M * (6ull + 2ull * K) + n * 12ull, n,
@@ -864,7 +921,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     bl.key = key; bl.val = val;
                     bl.pos = dalloc<uint32_t>(tmp, std::max<uint64_t>(1, NK));
                     bl.bc = dalloc<uint32_t>(tmp, M + 1);
-                    HIP_TRY(hipMemsetAsync(bl.bc, 0, sizeof(uint32_t) * (M + 1), st));
+                    HIP_TRY(launch_fill(bl.bc, 0, sizeof(uint32_t) * (M + 1), st));
                     int end_bit = 32;
                     while (end_bit < 64 && (E >> (end_bit - 32)) != 0) ++end_bit;   // keys <= E << 32
                     const uint64_t lbytes = NK * (8ull + 4ull) * 4 + NK * 4ull + M * (12ull + 2ull * K);
@@ -883,8 +940,15 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     uint64_t* poff = dalloc<uint64_t>(tmp, n + 1);
                     scan_u32_to_u64(tmp, pcnt, poff, n, st);
                     h_poff.resize(n + 1);
-                    d2h(h_poff.data(), poff, n + 1, st);
-                    HIP_TRY(hipStreamSynchronize(st));
+                    if (mode_ == EngineMode::Capture) {
+                        if (plan_poff_ >= plan_->poff.size() || plan_->poff[plan_poff_].size() != h_poff.size())
+                            throw Error{HGX_ERR_INVALID, "the pass being captured left its recording"};
+                        h_poff = plan_->poff[plan_poff_++];
+                    } else {
+                        d2h(h_poff.data(), poff, n + 1, st);
+                        HIP_TRY(hipStreamSynchronize(st));
+                        if (mode_ == EngineMode::Record) plan_->poff.push_back(h_poff);
+                    }
                     const uint64_t NB = h_poff[n];
                     if (NB) {
                         C->br_pairs = dalloc<uint32_t>(C->meta, 2 * NB);
@@ -942,8 +1006,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             }
     std::vector<uint32_t> releasable;
     releasable.swap(release_next_);
-    finalize(done_now);                        // synchronises the stream
-    if (done_now.empty()) HIP_TRY(hipStreamSynchronize(st));
+    finalize(done_now);                        // synchronises the stream (not when capturing)
+    if (done_now.empty() && mode_ != EngineMode::Capture) HIP_TRY(hipStreamSynchronize(st));
     t_fin = ms_since(t_step);
     prof_resolve(ctx_);
     for (uint32_t s : releasable) free_slots_.push_back(s);
@@ -1171,6 +1235,124 @@ extern "C" int hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* b, hgx_result** r
 }
 
 uint32_t hgx_batch_size(const hgx_batch* b) { return b ? (uint32_t)b->runs.size() : 0u; }
+
+// ---- a prepared batch's pass recorded into one graph --------------------------------------
+
+struct hgx_replay {
+    hgx_ctx* ctx = nullptr;
+    const hgx_batch* batch = nullptr;
+    std::unique_ptr<Arena> mem;               // the replay check word
+    uint32_t* d_check = nullptr;
+    std::unique_ptr<Engine> E;                // the captured engine: its runs, generations, maps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<Chunk> held;                  // every buffer the captured pass touched
+    uint32_t n = 0;
+    ~hgx_replay() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        E.reset();
+        for (auto& c : held) ctx->pool->put(c);
+        mem.reset();
+    }
+};
+
+extern "C" int hgx_batch_capture(hgx_ctx* ctx, const hgx_batch* b, hgx_replay** out) {
+    if (!ctx || !b || b->ctx != ctx || !out) return HGX_ERR_INVALID;
+    *out = nullptr;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const uint32_t n = (uint32_t)b->runs.size();
+        if (!n) throw Error{HGX_ERR_INVALID, "the batch is empty"};
+        hipStream_t st = ctx->stream;
+        auto plan = std::make_shared<PassPlan>();
+        {   // 1. an eager pass keeps every host decision
+            Engine E(ctx);
+            E.set_mode(EngineMode::Record, plan, nullptr);
+            E.soft_budget_frac = 0.8;
+            E.reserve_slots(n + 1);
+            for (uint32_t i = 0; i < n; ++i) E.submit(b->runs[i], i + 1);
+            while (E.live_or_queued()) E.step();
+            for (uint32_t i = 0; i < n; ++i) {
+                auto r = E.take(i + 1);
+                if (!r || r->status != HGX_OK)
+                    throw Error{HGX_ERR_CAPACITY, "the batch does not fit the budget in one pass; it cannot be captured"};
+            }
+        }
+        // 2. the same pass recorded into a graph, its decisions taken from the plan
+        auto rp = std::make_unique<hgx_replay>();
+        rp->ctx = ctx;
+        rp->batch = b;
+        rp->n = n;
+        rp->mem = std::make_unique<Arena>(ctx->pool.get());
+        rp->d_check = dalloc<uint32_t>(*rp->mem, 64);
+        rp->E = std::make_unique<Engine>(ctx);
+        rp->E->set_mode(EngineMode::Capture, plan, rp->d_check);
+        rp->E->soft_budget_frac = 0.8;
+        rp->E->reserve_slots(n + 1);
+        for (uint32_t i = 0; i < n; ++i) rp->E->submit(b->runs[i], i + 1);
+        HIP_TRY(hipStreamSynchronize(st));
+        struct ProfOff {                       // no event timing inside the graph
+            hgx_ctx* c; bool was;
+            ~ProfOff() { c->prof.on = was; }
+        } prof_off{ctx, ctx->prof.on};
+        ctx->prof.on = false;
+        ctx->pool->hold(true);
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        try {
+            HIP_TRY(launch_fill(rp->d_check, 0, sizeof(uint32_t), st));
+            while (rp->E->live_or_queued()) rp->E->step();
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(st, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            ctx->pool->hold(false);
+            rp->held = ctx->pool->take_held();
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(st, &rp->graph);
+        ctx->pool->hold(false);
+        rp->held = ctx->pool->take_held();
+        if (e != hipSuccess) throw Error{HGX_ERR_DEVICE, std::string("stream capture: ") + hipGetErrorString(e)};
+        HIP_TRY(hipGraphInstantiate(&rp->exec, rp->graph, nullptr, nullptr, 0));
+        if (rp->E->live_or_queued() || rp->E->finished.size() != n)
+            throw Error{HGX_ERR_DEVICE, "the captured pass did not finish every request"};
+        *out = rp.release();
+        return HGX_OK;
+    });
+}
+
+extern "C" int hgx_replay_launch(hgx_ctx* ctx, hgx_replay* rp, hgx_result** results) {
+    if (!ctx || !rp || rp->ctx != ctx || !results) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const auto t0 = Clock::now();
+        HIP_TRY(hipGraphLaunch(rp->exec, ctx->stream));
+        uint32_t check = 0;
+        HIP_TRY(hipMemcpyAsync(&check, rp->d_check, sizeof(check), hipMemcpyDeviceToHost, ctx->stream));
+        rp->E->resolve_counts();                      // synchronises the stream
+        if (check) throw Error{HGX_ERR_DEVICE, "a replay's totals left its recording"};
+        const auto t1 = Clock::now();
+        for (uint32_t i = 0; i < rp->n; ++i) {
+            auto r = rp->E->finished.at(i + 1);
+            r->t_submit = t0;
+            r->t_admit = t0;
+            r->t_done = t1;
+            r->device_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            results[i] = make_result(r);
+        }
+        return HGX_OK;
+    });
+}
+
+extern "C" void hgx_replay_release(hgx_replay* rp) {
+    if (!rp) return;
+    hgx_ctx* ctx = rp->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    delete rp;
+}
 
 extern "C" void hgx_batch_release(hgx_batch* b) {
     if (!b) return;

@@ -72,6 +72,22 @@ struct RunRec {
     double device_ms = 0;
 };
 
+// The host decisions of one eager pass over a prepared batch, per generation: the per-slot child
+// edges, child totals and parent edges read after the count pass, and the branchial pair offsets
+// read before the pairs are written. A pass recorded into a graph takes them from here instead of
+// reading them from the device, and every replay checks the totals on the device.
+struct PassPlan {
+    std::vector<std::vector<uint64_t>> rb;
+    std::vector<std::vector<uint64_t>> poff;
+};
+
+enum class EngineMode {
+    Normal,     // every decision from the device, synchronising where a size is needed
+    Record,     // the same, keeping the decisions in a PassPlan
+    Capture     // under stream capture: decisions from the PassPlan, no synchronisation, every
+                // host-to-device copy from pinned staging the engine keeps, no buffer reused
+};
+
 class Engine {
 public:
     explicit Engine(hgx_ctx* ctx);
@@ -93,6 +109,12 @@ public:
     bool known(uint64_t id) const;
     std::unordered_map<uint64_t, std::shared_ptr<RunRec>> finished;
     double soft_budget_frac = 0.6;
+
+    // Record / capture of a batched pass (hgx_batch_capture). check: a device word the replay
+    // check sets when the device's totals leave the plan.
+    void set_mode(EngineMode m, std::shared_ptr<PassPlan> plan, uint32_t* check);
+    // After a replay of the captured pass: every finished run's counters read from the device.
+    void resolve_counts();
 
 private:
     hgx_ctx* ctx_;
@@ -134,6 +156,19 @@ private:
         uint16_t* q = nullptr;
         uint64_t* moff = nullptr;
     } prev_;
+
+    EngineMode mode_ = EngineMode::Normal;
+    std::shared_ptr<PassPlan> plan_;
+    size_t plan_rb_ = 0, plan_poff_ = 0;      // next plan entries (capture)
+    uint32_t* d_check_ = nullptr;
+    std::vector<void*> pinned_;               // pinned staging of captured copies
+
+    // Host-to-device copy of n elements: straight from src, or (capture) through pinned staging
+    // that lives as long as the engine, since a replay copies from it again.
+    void put(void* dst, const void* src, size_t bytes, hipStream_t st);
+    template <class T>
+    void up(T* dst, const T* src, size_t n, hipStream_t st) { put(dst, src, sizeof(T) * n, st); }
+    void fill_counts(RunRec& r, const uint64_t* x);
 
     uint32_t step_impl(std::vector<std::shared_ptr<RunRec>>& admitted);
     void recover(const std::string& why, std::vector<std::shared_ptr<RunRec>>& admitted, bool frontier_live);

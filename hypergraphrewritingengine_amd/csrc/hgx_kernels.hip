@@ -16,6 +16,7 @@
 // MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <type_traits>
 
@@ -907,6 +908,51 @@ __global__ void k_run_events(const RunEvents* items, uint32_t n, const uint64_t*
         if (pairs_tr) atomicAdd(&row[CTR_PAIRS_TR], (unsigned long long)pairs_tr);
         if (triples_tr) atomicAdd(&row[CTR_TRIPLES_TR], (unsigned long long)triples_tr);
     }
+}
+
+__global__ void k_fill(uint8_t* p, uint32_t word, uint64_t words, uint64_t bytes) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t w = i; w < words; w += stride) ((uint32_t*)p)[w] = word;
+    if (i < bytes - words * 4) p[words * 4 + i] = (uint8_t)word;          // the tail bytes
+}
+
+hipError_t launch_fill(void* p, uint8_t value, uint64_t bytes, hipStream_t st) {
+    if (!bytes) return hipSuccess;
+    const uint32_t word = 0x01010101u * value;
+    // whole 32-bit words from a 4-byte aligned start; the rest bytewise
+    uint8_t* b = (uint8_t*)p;
+    const uint64_t lead = (4 - ((uintptr_t)b & 3)) & 3;
+    if (lead) {
+        const uint64_t k = lead < bytes ? lead : bytes;
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, b, word, (uint64_t)0, k);
+        if (k == bytes) return hipGetLastError();
+        b += k;
+        bytes -= k;
+    }
+    const uint64_t words = bytes / 4;
+    uint64_t blocks = (std::max<uint64_t>(words, 4) + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill, dim3((uint32_t)blocks), dim3(256), 0, st, b, word, words, bytes);
+    return hipGetLastError();
+}
+
+// A run over its budget stops being counted once the count pass is sure of it, so its total is
+// any value above the budget: the decision is what must agree, and below the budget the total.
+__global__ void k_plan_check(const uint64_t* tot, const uint64_t* want, const uint32_t* budget, uint32_t n,
+                             uint32_t* flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t b = budget[i];
+    const bool over = tot[i] > b, want_over = want[i] > b;
+    if (over != want_over || (!over && tot[i] != want[i])) atomicOr(flag, 1u);
+}
+
+hipError_t launch_plan_check(const uint64_t* tot, const uint64_t* want, const uint32_t* budget, uint32_t n,
+                             uint32_t* flag, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_check, dim3((n + 255) / 256), dim3(256), 0, st, tot, want, budget, n, flag);
+    return hipGetLastError();
 }
 
 hipError_t launch_run_events(const RunEvents* items, uint32_t n, const EventView& E, const uint16_t* tcnt,

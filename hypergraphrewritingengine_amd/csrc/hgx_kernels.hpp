@@ -198,6 +198,14 @@ hipError_t launch_run_totals(const uint64_t* moff, const uint64_t* state_edges, 
                              uint64_t* edges, uint64_t* parent_edges, hipStream_t st);
 hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32_t n, uint32_t* cnt,
                               hipStream_t st);
+// Byte fill of [p, p + bytes) by a kernel: the engine clears its buffers with it (never
+// hipMemsetAsync), so a pass it records into a graph holds kernels and copies only.
+hipError_t launch_fill(void* p, uint8_t value, uint64_t bytes, hipStream_t st);
+// Replay check of a recorded pass: flag[0] |= 1 when a run's child total (tot[slot], slots < n)
+// leaves the recording's -- a different cap decision against budget[slot], or, within the budget,
+// a different total.
+hipError_t launch_plan_check(const uint64_t* tot, const uint64_t* want, const uint32_t* budget, uint32_t n,
+                             uint32_t* flag, hipStream_t st);
 hipError_t launch_rehash(const uint64_t* ok, const uint32_t* ov, uint32_t old_cap, uint64_t* keys,
                          uint32_t* vals, uint32_t mask, hipStream_t st);
 

@@ -62,8 +62,21 @@ Chunk ChunkPool::get(size_t min_bytes) {
 
 void ChunkPool::put(const Chunk& c) {
     std::lock_guard<std::mutex> g(mu_);
+    if (holding_) { held_.push_back(c); return; }      // stays in use, owned by the recording
     in_use_ -= c.size;
     free_.push_back(c);
+}
+
+void ChunkPool::hold(bool on) {
+    std::lock_guard<std::mutex> g(mu_);
+    holding_ = on;
+}
+
+std::vector<Chunk> ChunkPool::take_held() {
+    std::lock_guard<std::mutex> g(mu_);
+    std::vector<Chunk> h;
+    h.swap(held_);
+    return h;
 }
 
 Arena::~Arena() { release(); }

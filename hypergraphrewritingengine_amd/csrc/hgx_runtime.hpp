@@ -45,9 +45,16 @@ public:
     uint64_t budget() const { return budget_; }
     uint64_t allocated() const { return total_; }
     uint64_t in_use() const { return in_use_; }
+    // While holding, chunks given back are kept aside instead of reused (a pass being recorded into
+    // a graph keeps every buffer it touched); take_held() hands them to their new owner, who gives
+    // them back with put() when the graph goes.
+    void hold(bool on);
+    std::vector<Chunk> take_held();
 
 private:
     std::mutex mu_;
+    bool holding_ = false;
+    std::vector<Chunk> held_;
     std::vector<Chunk> free_;
     uint64_t budget_;
     size_t chunk_bytes_;

@@ -194,6 +194,23 @@ int  hgx_batch_prepare(hgx_ctx* ctx, const hgx_run_desc* runs, uint32_t n, hgx_b
 int  hgx_batch_evolve(hgx_ctx* ctx, const hgx_batch* batch, hgx_result** results);
 void hgx_batch_release(hgx_batch* batch);
 
+/* A prepared batch's whole pass as one HIP graph. hgx_batch_capture runs the batch once eagerly,
+ * keeping every host decision a generation takes (the per-run child totals and sizes read after
+ * the count pass), then runs it again under stream capture with those decisions, no host
+ * synchronisation, every buffer kept and every host copy staged in pinned memory: one graph of
+ * all generations' launches. hgx_replay_launch launches it (one graph launch per pass), reads the
+ * counters and returns a result per request, identical to hgx_batch_evolve's; the graph also
+ * checks on the device that every generation's per-run totals equal the recording's, and the
+ * launch fails with HGX_ERR_DEVICE if they do not. A replay's results read the replay's buffers:
+ * they are valid until the next launch or the release of that replay, which must come before the
+ * batch's and the context's. A batch whose pass does not fit the budget is refused
+ * (HGX_ERR_CAPACITY). No counterpart in the reference; it is the north star's hipGraph-captured
+ * batched step, for batches evolved repeatedly from device-resident inputs. */
+typedef struct hgx_replay hgx_replay;
+int  hgx_batch_capture(hgx_ctx* ctx, const hgx_batch* batch, hgx_replay** out);
+int  hgx_replay_launch(hgx_ctx* ctx, hgx_replay* replay, hgx_result** results);
+void hgx_replay_release(hgx_replay* replay);
+
 /* ---- the paclet's job envelope (WXF in, WXF out) -------------------------------------- */
 
 /* run_rewriting_core (paclet_source/hg_core.hpp:21-22) on this library: job is a WXF
