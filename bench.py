@@ -357,9 +357,13 @@ def main():
     ap.add_argument("--serve-plain-seconds", type=float, default=8.0)
     ap.add_argument("--no-serve-plain", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the config 2 / 3 lines")
+    ap.add_argument("--others", default="config2,config3a,config3b,ir_audit",
+                    help="which of the other workloads to measure (comma list)")
     ap.add_argument("--config2-steps", type=int, default=6)
     ap.add_argument("--forward", type=int, default=-1, choices=(-1, 0, 1),
                     help="incremental match forwarding: 1 on, 0 off, -1 automatic")
+    ap.add_argument("--expand-lanes", type=int, default=0, choices=(0, 16, 32, 64),
+                    help="lanes per child in the expansion (HGX_OPTION_EXPAND_LANES; 0 = tuned/automatic)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip the graph-replayed passes")
     ap.add_argument("--no-relations", dest="relations", action="store_false",
@@ -383,9 +387,13 @@ def main():
 
     ctx = hgx.Context(0)
     ctx.set_forwarding(args.forward)
+    ctx.set_expand_lanes(args.expand_lanes)
     print(f"[bench] {ctx.info}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
+        # a second calibration of known traffic: a fill kernel writing N bytes, then a read-only
+        # kernel reading them (k_fill, k_read_calibration)
+        ctx.calibration_read(args.copy_calibration << 20)
     reqs = [W.batched_run(i) for i in range(args.runs)]
     batch = ctx.prepare(reqs)               # host-side request records, built once
     tune_s = None
@@ -528,14 +536,17 @@ def main():
 
     # the other measured workloads: one large evolution, and the large-state join
     others = {}
-    if not args.no_others:
-        threads = min(os.cpu_count() or 1, 16)
+    want = set() if args.no_others else {x.strip() for x in args.others.split(",") if x.strip()}
+    threads = min(os.cpu_count() or 1, 16)
+    if "config2" in want:
         others["config2"] = config2_line(ctx, args.config2_steps, 4, threads, args.no_cpu)
         print(f"[bench] config2: {others['config2']}", file=sys.stderr, flush=True)
-        for name, mk in (("config3a", W.config3a), ("config3b", W.config3b)):
+    for name, mk in (("config3a", W.config3a), ("config3b", W.config3b)):
+        if name in want:
             rule, edges = mk()
             others[name] = config3_line(ctx, name, rule, edges, args.no_cpu)
             print(f"[bench] {name}: {others[name]}", file=sys.stderr, flush=True)
+    if "ir_audit" in want:
         others["ir_audit"] = ir_audit_line(ctx, 256, args.no_cpu)
         print(f"[bench] ir_audit: {others['ir_audit']}", file=sys.stderr, flush=True)
 

@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = [
     "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
     "hgx_result_causal_triples", "hgx_match_count_noalloc", "hgx_autotune", "hgx_tune_load",
     "hgx_tune_get", "hgx_batch_capture", "hgx_replay_launch", "hgx_replay_release",
+    "hgx_calibration_read",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -92,6 +93,7 @@ def load_library():
     L.hgx_result_causal_triples.argtypes = [vp, vp, vp, C.c_uint64]
     L.hgx_autotune.argtypes = [vp, vp, C.c_uint32, C.c_char_p, C.c_char_p, C.c_size_t]
     L.hgx_batch_capture.argtypes = [vp, vp, C.POINTER(vp)]
+    L.hgx_calibration_read.argtypes = [vp, C.c_uint64]
     L.hgx_replay_launch.argtypes = [vp, vp, C.POINTER(vp)]
     L.hgx_replay_release.argtypes = [vp]
     L.hgx_replay_release.restype = None
@@ -648,6 +650,13 @@ class Context:
         buf = C.create_string_buffer(4096)
         self._lib.hgx_tune_get(self._h, buf, len(buf))
         return _json.loads(buf.value.decode())
+
+    def calibration_read(self, nbytes: int):
+        """Launches of known traffic for counter calibration (hgx_calibration_read): a fill kernel
+        writing nbytes, then a read-only kernel reading them."""
+        rc = self._lib.hgx_calibration_read(self._h, int(nbytes))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
 
     def profile(self, on: bool = True):
         """Event-time every generation kernel launch (see hgx_profile_enable)."""

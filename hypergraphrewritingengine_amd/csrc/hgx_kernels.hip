@@ -937,6 +937,25 @@ hipError_t launch_fill(void* p, uint8_t value, uint64_t bytes, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Calibration read: every 16-byte word of [p, p + words * 16) read once, one sum written, so the
+// launch's HBM traffic is known (the bytes read, and 8 written).
+__global__ void k_read_calibration(const uint4* p, uint64_t words, unsigned long long* sink) {
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 v = p[i];
+        acc += (uint64_t)v.x + v.y + v.z + v.w;
+    }
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0) atomicAdd(sink, (unsigned long long)acc);
+}
+
+hipError_t launch_read_calibration(const void* p, uint64_t bytes, uint64_t* sink, hipStream_t st) {
+    const uint64_t words = bytes / 16;
+    hipLaunchKernelGGL(k_read_calibration, dim3(2048), dim3(256), 0, st, (const uint4*)p, words,
+                       (unsigned long long*)sink);
+    return hipGetLastError();
+}
+
 // A run over its budget stops being counted once the count pass is sure of it, so its total is
 // any value above the budget: the decision is what must agree, and below the budget the total.
 __global__ void k_plan_check(const uint64_t* tot, const uint64_t* want, const uint32_t* budget, uint32_t n,

@@ -201,6 +201,8 @@ hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32
 // Byte fill of [p, p + bytes) by a kernel: the engine clears its buffers with it (never
 // hipMemsetAsync), so a pass it records into a graph holds kernels and copies only.
 hipError_t launch_fill(void* p, uint8_t value, uint64_t bytes, hipStream_t st);
+// A read-only launch of known traffic (bytes read, 8 written) for counter calibration.
+hipError_t launch_read_calibration(const void* p, uint64_t bytes, uint64_t* sink, hipStream_t st);
 // Replay check of a recorded pass: flag[0] |= 1 when a run's child total (tot[slot], slots < n)
 // leaves the recording's -- a different cap decision against budget[slot], or, within the budget,
 // a different total.

@@ -605,6 +605,23 @@ extern "C" int hgx_wl_hash_batch(hgx_ctx* ctx, const uint64_t* state_offsets, ui
     });
 }
 
+extern "C" int hgx_calibration_read(hgx_ctx* ctx, uint64_t bytes) {
+    if (!ctx || bytes < 16) return HGX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        Arena ar(ctx->pool.get());
+        uint8_t* buf = dalloc<uint8_t>(ar, bytes);
+        uint64_t* sink = dalloc<uint64_t>(ar, 1);
+        HIP_TRY(launch_fill(buf, 1, bytes, ctx->stream));
+        HIP_TRY(launch_fill(sink, 0, 8, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(launch_read_calibration(buf, bytes, sink, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return HGX_OK;
+    });
+}
+
 extern "C" int hgx_profile_enable(hgx_ctx* ctx, int on) {
     if (!ctx) return HGX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);

@@ -392,6 +392,10 @@ typedef struct hgx_kernel_stat {
     uint64_t items;           /* states / children processed                              */
 } hgx_kernel_stat;
 int hgx_profile_enable(hgx_ctx* ctx, int on);
+/* Counter calibration: one fill kernel writes `bytes` (a multiple of 16), then one read-only
+ * kernel (k_read_calibration) reads them once and writes 8 bytes -- launches of known traffic for
+ * scaling FETCH_SIZE / WRITE_SIZE readings. No counterpart in the reference. */
+int hgx_calibration_read(hgx_ctx* ctx, uint64_t bytes);
 /* Copies up to cap entries, sets *n, and clears the accumulators. */
 int hgx_profile_read(hgx_ctx* ctx, hgx_kernel_stat* out, uint32_t cap, uint32_t* n);
 
