@@ -279,9 +279,13 @@ def serve_window(ctx, rate, seconds, first, seed):
     from hypergraphrewritingengine_amd import workloads as W
     from hypergraphrewritingengine_amd import abi
     arr = poisson_arrivals(rate, seconds, seed)
-    reqs = [W.served_run(first + k) for k in range(len(arr))]
-    for r in reqs:                          # requests arrive as the C ABI's arrays
-        abi.make_run_desc(r, [])
+    reqs = []
+    for k in range(len(arr)):
+        r = W.served_run(first + k)
+        abi.make_run_desc(r, [])            # requests arrive as the C ABI's arrays
+        reqs.append(r)
+        if (k + 1) % 50000 == 0:
+            print(f"[bench] serve {rate:g} req/s: {k + 1} of {len(arr)} requests built", file=sys.stderr, flush=True)
     n = len(reqs)
     idx_of, lat, done_at = {}, [0.0] * n, [0.0] * n
     canon = raw = 0
