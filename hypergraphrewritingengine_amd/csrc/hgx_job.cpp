@@ -6,25 +6,30 @@
 // BranchialEdges, Num*). It is written over the library's own C ABI (hgx_evolve_batch and the
 // hgx_result_* readbacks).
 //
-// Served: full multiway evolution with CanonicalizeStates -> None or Full (the GPU backend's
-// grouping, hg_gpu_backend.cpp:269-310: None gives every state its own class, Full groups by the
-// exact IR hash computed on the host and emits each class's representative relabelled to its
-// canonical form; CanonicalHash is the IR hash in both), CanonicalizeEvents -> None, Full,
+// Served: full multiway evolution with CanonicalizeStates -> None, Automatic or Full (the GPU
+// backend's grouping, hg_gpu_backend.cpp:269-310: None gives every state its own class, Automatic
+// groups states of equal content, Full groups by the exact IR hash computed on the host and emits
+// each class's representative relabelled to its canonical form; CanonicalHash is the IR hash in
+// every mode), CanonicalizeEvents -> None, Full,
 // Automatic, Positional or a list of signature keys (events identified by the signature of
 // common/include/hgcommon/event_core.hpp:60-84 over exact state hashes and canonical edge
 // ranks, as Hypergraph::create_event does, hypergraph/src/hypergraph.cpp:448-499), any rule
 // set, one or more initial states, causal edges with or without transitive reduction,
-// branchial edges, RequestedData selection. Options that change what is explored (sampling,
-// caps, canonical exploration, sessions, genesis events, graph properties) are refused with an
-// error rather than ignored, so a caller never receives a different evolution than it asked
-// for; options with no effect on the result (ShowProgress, RandomSeed, ...) are accepted and
-// reported under "Warnings" as skipped, with the reference's Kind/Count/Context schema.
+// branchial edges, RequestedData selection, and the sampling and capping options
+// (TransitionRate, RuleWeights, MatchesPerStateRule, ExplorationProbability, MaxStatesPerStep,
+// MaxSuccessorStatesPerParent, RandomSeed; apply_sampling below). Options that change what is
+// explored and are not served (canonical-state exploration, sessions, genesis events, graph
+// properties) are refused with an error rather than ignored, so a caller never receives a
+// different evolution than it asked for; options with no effect on the result (ShowProgress,
+// ...) are accepted and reported under "Warnings" as skipped, with the reference's
+// Kind/Count/Context schema.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <random>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -38,6 +43,23 @@ namespace job {
 
 using wxf::Value;
 
+// The sampling and capping options (ffi_job.hpp:48-72, as the GPU backend takes them,
+// hg_gpu_backend.cpp:152-160): per-transition thinning, per-rule weights, a per-(state, rule) cap,
+// a per-state exploration coin and the two arrival-order bounds on the states expanded.
+struct Sampling {
+    double explore_p = 1.0;                 // ExplorationProbability
+    double rate = 1.0;                      // TransitionRate
+    std::vector<double> weights;            // RuleWeights
+    uint64_t seed = 0;                      // RandomSeed
+    uint64_t per_step = 0;                  // MaxStatesPerStep (or MatchesPerStep with UniformRandom)
+    uint64_t per_parent = 0;                // MaxSuccessorStatesPerParent
+    uint64_t per_state_rule = 0;            // MatchesPerStateRule
+    // the transition draw is taken when the rate is below 1 or weights are given (rewrite.cu:270)
+    bool draws() const { return rate < 1.0 || !weights.empty(); }
+    bool transitions() const { return draws() || per_state_rule != 0; }
+    bool active() const { return transitions() || explore_p < 1.0 || per_step || per_parent; }
+};
+
 struct Job {
     std::vector<std::vector<std::vector<int64_t>>> states;          // initial states as given
     std::vector<std::pair<std::vector<std::vector<int64_t>>, std::vector<std::vector<int64_t>>>> rules;
@@ -47,7 +69,9 @@ struct Job {
          inc_branchial = true, inc_nstates = true, inc_nevents = true, inc_ncausal = true,
          inc_nbranchial = true, inc_hashes = false;
     bool full = false;                       // CanonicalizeStates -> Full
+    bool automatic = false;                  // CanonicalizeStates -> Automatic (content identity)
     uint32_t ev_keys = 0;                    // CanonicalizeEvents: event signature keys
+    Sampling smp;
     std::vector<std::pair<std::string, std::string>> warnings;   // (kind, context)
 };
 
@@ -58,6 +82,18 @@ struct JobError : std::runtime_error {
 static int64_t as_int(const Value& v, const char* what) {
     if (v.kind != Value::Int) throw JobError(std::string(what) + " must be an integer");
     return v.i;
+}
+
+static double as_real(const Value& v, const char* what) {
+    if (v.kind == Value::Real) return v.r;
+    if (v.kind == Value::Int) return (double)v.i;
+    throw JobError(std::string(what) + " must be a number");
+}
+
+static uint64_t as_count(const Value& v, const char* what) {
+    const int64_t x = as_int(v, what);
+    if (x < 0) throw JobError(std::string(what) + " must be non-negative");
+    return (uint64_t)x;
 }
 
 static bool as_bool(const Value& v, const char* what) {
@@ -93,6 +129,8 @@ static Job parse(const Value& root) {
     if (root.kind != Value::Assoc) throw JobError("the job must be an Association");
     Job j;
     bool have_rules = false;
+    uint64_t max_per_step = 0, matches_per_step = 0;
+    bool uniform = false;
     for (const auto& kv : root.rules) {
         const std::string key = as_name(kv.first);
         const Value& val = kv.second;
@@ -122,9 +160,10 @@ static Job parse(const Value& root) {
                 };
                 if (ok == "CanonicalizeStates") {
                     const std::string m = as_name(ov);
-                    if (m == "Full" || m == "True") j.full = true;
-                    else if (m == "None" || m == "False") j.full = false;
-                    else refuse("only None and Full are served");
+                    j.full = m == "Full" || m == "True";
+                    j.automatic = m == "Automatic";
+                    if (!j.full && !j.automatic && m != "None" && m != "False")
+                        refuse("unknown state canonicalization " + m);
                 } else if (ok == "CanonicalizeEvents") {
                     // event_core.hpp:26-45; the FFI's names (hypergraph_ffi.cpp:236-262)
                     constexpr uint32_t In = 1, Out = 2, Step = 4, Rule = 8, Cons = 16, Prod = 32;
@@ -153,14 +192,26 @@ static Job parse(const Value& root) {
                 } else if (ok == "ExploreFromCanonicalStatesOnly" || ok == "ShowGenesisEvents" ||
                            ok == "QuotientInitialStates") {
                     if (as_bool(ov, ok.c_str())) refuse("True is not served");
-                } else if (ok == "MaxSuccessorStatesPerParent" || ok == "MaxStatesPerStep" ||
-                           ok == "MatchesPerStep" || ok == "MatchesPerStateRule") {
-                    if (as_int(ov, ok.c_str()) != 0) refuse("caps are not served");
-                } else if (ok == "ExplorationProbability" || ok == "TransitionRate") {
-                    const double x = ov.kind == Value::Real ? ov.r : (double)as_int(ov, ok.c_str());
-                    if (x != 1.0) refuse("sampling is not served");
+                } else if (ok == "MaxSuccessorStatesPerParent") {
+                    j.smp.per_parent = as_count(ov, ok.c_str());
+                } else if (ok == "MaxStatesPerStep") {
+                    max_per_step = as_count(ov, ok.c_str());
+                } else if (ok == "MatchesPerStep") {
+                    matches_per_step = as_count(ov, ok.c_str());
+                } else if (ok == "UniformRandom") {
+                    uniform = as_bool(ov, ok.c_str());
+                } else if (ok == "MatchesPerStateRule") {
+                    j.smp.per_state_rule = as_count(ov, ok.c_str());
+                } else if (ok == "RandomSeed") {
+                    j.smp.seed = (uint64_t)as_int(ov, ok.c_str());
+                } else if (ok == "ExplorationProbability") {
+                    j.smp.explore_p = as_real(ov, ok.c_str());
+                } else if (ok == "TransitionRate") {
+                    j.smp.rate = as_real(ov, ok.c_str());
                 } else if (ok == "RuleWeights") {
-                    if (!ov.is_list() || ov.nargs() != 0) refuse("sampling is not served");
+                    need_list(ov, "RuleWeights");
+                    j.smp.weights.clear();
+                    for (size_t k = 0; k < ov.nargs(); ++k) j.smp.weights.push_back(as_real(ov.arg(k), "RuleWeights"));
                 } else if (ok == "GraphProperties") {
                     if (!ov.is_list() || ov.nargs() != 0) refuse("graph properties are not served");
                 } else if (ok == "BranchialStep") {
@@ -183,8 +234,7 @@ static Job parse(const Value& root) {
                         else refuse("component " + c + " is not served");
                     }
                 } else {
-                    // ShowProgress, RandomSeed, UniformRandom, EdgeDeduplication and unknown keys:
-                    // nothing here depends on them
+                    // ShowProgress, EdgeDeduplication and unknown keys: nothing here depends on them
                     j.warnings.emplace_back("OptionSkipped", ok);
                 }
             }
@@ -198,6 +248,9 @@ static Job parse(const Value& root) {
         }
     }
     if (!have_rules || j.rules.empty()) throw JobError("the job has no Rules");
+    // MatchesPerStep is MaxStatesPerStep under another name when UniformRandom selects it
+    // (hypergraph_ffi.cpp:378-385)
+    j.smp.per_step = uniform && matches_per_step ? matches_per_step : max_per_step;
     return j;
 }
 
@@ -247,20 +300,25 @@ static std::vector<uint8_t> program(const Job& j) {
 // dropped (hypergraph_ffi.cpp:598-636). Each state's int64 labels are renumbered densely by
 // first appearance here, as the FFI does, so any non-negative int64 label is taken; the library
 // renumbers the same way, so the dense labels reach it unchanged.
+// The device numbers each initial state's vertices 0.. by first appearance; `orig` (when given)
+// receives, per kept initial state, the job's label of each device label.
 static void roots(const Job& j, std::vector<uint32_t>& eoff, std::vector<uint32_t>& verts,
-                  std::vector<uint32_t>& roff) {
+                  std::vector<uint32_t>& roff, std::vector<std::vector<int64_t>>* orig = nullptr) {
     eoff.assign(1, 0);
     roff.assign(1, 0);
     std::unordered_map<int64_t, uint32_t> label;
+    std::vector<int64_t> names;
     for (const auto& st : j.states) {
         uint32_t kept = 0;
         label.clear();
+        names.clear();
         for (const auto& e : st) {
             uint32_t n = 0;
             for (int64_t v : e) {
                 if (v < 0) continue;
-                const auto it = label.emplace(v, (uint32_t)label.size()).first;
-                verts.push_back(it->second);
+                const auto it = label.emplace(v, (uint32_t)label.size());
+                if (it.second) names.push_back(v);
+                verts.push_back(it.first->second);
                 ++n;
             }
             if (!n) continue;
@@ -268,7 +326,10 @@ static void roots(const Job& j, std::vector<uint32_t>& eoff, std::vector<uint32_
             eoff.push_back((uint32_t)verts.size());
             ++kept;
         }
-        if (kept) roff.push_back((uint32_t)(eoff.size() - 1));
+        if (kept) {
+            roff.push_back((uint32_t)(eoff.size() - 1));
+            if (orig) orig->push_back(names);
+        }
     }
 }
 
@@ -304,18 +365,271 @@ static void check(int rc, hgx_ctx* ctx, const char* what) {
     throw JobError(std::string(what) + " failed (status " + std::to_string(rc) + "): " + buf);
 }
 
+// CanonicalizeStates -> Automatic: states are the same when their edges, in order, hold the
+// same vertices (hg_gpu_backend.cpp:269-310 groups by the content key the evolution deduplicated
+// with, common/include/hgcommon/wl_core.hpp:20-48; reference/oracle_corpus.hpp:85-96 states the
+// same identity as a string). Content compares vertex NAMES, so it depends on how fresh vertices
+// are named: the reference takes every fresh vertex from one global counter, so a vertex a rule
+// creates is a name no other event ever creates. The device names fresh vertices per lineage
+// (siblings may reuse a number), so the comparison here runs on global names instead: an initial
+// vertex is named by the job's own label, and a vertex that first appears in state s (created by
+// the event that produced s) gets a name of its own; a kept vertex keeps its parent's name.
+// Any naming with those equalities gives the reference's classes. The representative of a class
+// is its first state in id order.
+static void content_classes(const Run& R, const std::vector<std::vector<int64_t>>& orig, uint32_t n_roots,
+                            std::vector<int64_t>& rep, std::vector<uint64_t>& reps) {
+    const uint64_t ns = R.soff.size() - 1;
+    std::vector<uint64_t> name(R.verts.size());           // global name of every vertex slot
+    uint64_t fresh = 0;
+    std::unordered_map<uint32_t, uint64_t> of_parent;
+    for (uint64_t s = 0; s < ns; ++s) {
+        const uint64_t x0 = R.eoff[R.soff[s]], x1 = R.eoff[R.soff[s + 1]];
+        if (s < n_roots) {
+            for (uint64_t x = x0; x < x1; ++x) name[x] = (uint64_t)orig[s][R.verts[x]] << 1;
+            continue;
+        }
+        const uint32_t e = R.creator[s];
+        if (e >= R.events.size()) throw JobError("a state without its creating event");
+        const uint32_t p = R.events[e].input_state;
+        of_parent.clear();
+        for (uint64_t x = R.eoff[R.soff[p]]; x < R.eoff[R.soff[p + 1]]; ++x) of_parent.emplace(R.verts[x], name[x]);
+        std::unordered_map<uint32_t, uint64_t> made;
+        for (uint64_t x = x0; x < x1; ++x) {
+            const auto it = of_parent.find(R.verts[x]);
+            if (it != of_parent.end()) { name[x] = it->second; continue; }
+            const auto m = made.emplace(R.verts[x], (fresh << 1) | 1u);
+            if (m.second) ++fresh;
+            name[x] = m.first->second;
+        }
+    }
+    // class key: edge count, then per edge its arity and names; hashed, then compared exactly
+    auto key_of = [&](uint64_t s, std::vector<uint64_t>& k) {
+        k.clear();
+        k.push_back(R.soff[s + 1] - R.soff[s]);
+        for (uint64_t ed = R.soff[s]; ed < R.soff[s + 1]; ++ed) {
+            k.push_back(R.eoff[ed + 1] - R.eoff[ed]);
+            for (uint64_t x = R.eoff[ed]; x < R.eoff[ed + 1]; ++x) k.push_back(name[x]);
+        }
+    };
+    auto hash_of = [](const std::vector<uint64_t>& k) {
+        uint64_t h = 0x9ae16a3b2f90404fULL;
+        for (uint64_t v : k) {
+            h ^= v + 0x632be59bd9b4e019ULL + (h << 6) + (h >> 2);
+            h *= 0xff51afd7ed558ccdULL;
+        }
+        return h;
+    };
+    std::unordered_multimap<uint64_t, uint32_t> seen;
+    std::vector<uint64_t> k, k2;
+    for (uint64_t s = 0; s < ns; ++s) {
+        key_of(s, k);
+        const uint64_t h = hash_of(k);
+        int64_t found = -1;
+        for (auto [it, end] = seen.equal_range(h); it != end; ++it) {
+            key_of(it->second, k2);
+            if (k2 == k) { found = it->second; break; }
+        }
+        if (found < 0) {
+            seen.emplace(h, (uint32_t)s);
+            rep[s] = (int64_t)s;
+            reps.push_back(s);
+        } else {
+            rep[s] = found;
+        }
+    }
+}
+
+// ---- sampling (common/include/hgcommon/sampling_core.hpp, gpu/include/hg_gpu/exploration.hpp) ----
+
+// The rate a rule's transitions survive at: the run's rate times the rule's weight (rules past
+// the weight list weigh 1), clamped to [0, 1] (sampling_core.hpp:28-34).
+static double rate_for_rule(const Sampling& S, uint32_t rule) {
+    const double r = S.rate * (rule < S.weights.size() ? S.weights[rule] : 1.0);
+    return r < 0.0 ? 0.0 : (r > 1.0 ? 1.0 : r);
+}
+
+// Whether a transition survives the draw: splitmix64 of the transition key against the seed
+// (mixed by multiplication, so seed 0 still draws), compared in [0, 1) through its top 53 bits
+// (sampling_core.hpp:42-57).
+static bool transition_survives(uint64_t key, uint64_t seed, double rate) {
+    if (rate >= 1.0) return true;
+    if (rate <= 0.0) return false;
+    const uint64_t x = splitmix(key ^ (seed * 0x9E3779B97F4A7C15ULL));
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0) < rate;
+}
+
+// The order a per-(state, rule) cap keeps its k in: a second mix of the same key, so the rank
+// is not correlated with the rate draw (sampling_core.hpp:66-71).
+static uint64_t transition_rank(uint64_t key, uint64_t seed) {
+    uint64_t x = key ^ (seed * 0x9E3779B97F4A7C15ULL) ^ 0xA5A5A5A5A5A5A5A5ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// The evolution the options describe, cut out of the full multiway evolution the device built.
+//
+// Every decision is the GPU engine's (gpu/src/match.cu:229-256 drain cap, gpu/src/rewrite.cu:
+// 255-274 transition draw, gpu/include/hg_gpu/exploration.hpp:66-122 state_survives_dedup), taken
+// generation by generation in this library's event order:
+//   - a transition (input state, rule, consumed edges) is keyed by the signature of
+//     event_core.hpp:60-84 over the input state's exact hash, the rule and the consumed edges'
+//     canonical ranks in pattern order; MatchesPerStateRule keeps the k matches of each
+//     (state, rule) with the smallest transition_rank (ties by event order), then the rate draw;
+//   - a transition that does not survive leaves no trace: no event, no output state, no
+//     relation;
+//   - a surviving child below the last step is expanded when the exploration coin (splitmix64 of
+//     the seed, the child's step and its state id) passes and then, counted in event order, when
+//     its step has expanded fewer than MaxStatesPerStep children and its parent fewer than
+//     MaxSuccessorStatesPerParent.
+// The two bounds and the coin depend on which states arrive first; the reference says the same
+// of its own (exploration.hpp:102-107). A transition's fate is a function of its isomorphism
+// class and the seed, so the kept subgraph is the same however the run is laid out.
+// Relations are the full evolution's restricted to the surviving events: a causal path to a
+// surviving event runs through its lineage, which survives with it, so the transitive reduction
+// is unchanged. The result is renumbered: states and events keep their order.
+static void apply_sampling(const Job& j, Run& R, std::vector<uint64_t>& ir, std::vector<uint32_t>& rank,
+                           uint32_t n_roots) {
+    const Sampling& S = j.smp;
+    const uint64_t ns = R.hash.size(), ne = R.events.size();
+    // per-event transition key (only when a draw or the cap reads it)
+    std::vector<uint64_t> key(S.transitions() ? ne : 0);
+    for (uint64_t e = 0; e < key.size(); ++e) {
+        const hgx_event& x = R.events[e];
+        uint64_t sig = fnv(FNV_OFFSET, ir[x.input_state]);
+        sig = fnv(sig, (uint64_t)(uint16_t)x.rule);
+        for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k) sig = fnv(sig, rank[R.soff[x.input_state] + R.cpos[k]]);
+        if (sig == 0 || sig == FNV_OFFSET) sig = 1;
+        key[e] = sig;
+    }
+    // the per-(state, rule) cap: each group's k smallest ranks
+    std::vector<uint8_t> capped_in(ne, 1);
+    if (S.per_state_rule) {
+        std::map<std::pair<uint32_t, uint32_t>, std::vector<std::pair<uint64_t, uint32_t>>> groups;
+        for (uint64_t e = 0; e < ne; ++e)
+            groups[{R.events[e].input_state, R.events[e].rule}].push_back({transition_rank(key[e], S.seed), (uint32_t)e});
+        for (auto& g : groups) {
+            auto& v = g.second;
+            std::sort(v.begin(), v.end());
+            for (size_t i = S.per_state_rule; i < v.size(); ++i) capped_in[v[i].second] = 0;
+        }
+    }
+    // the exploration coin's threshold, as the engine encodes the probability (evolve.cu:271-285,
+    // a float); seed 0 with a coin to flip is replaced by a random one (evolve.cu:351-356)
+    float p = (float)S.explore_p;
+    if (!(p > 0.0f)) p = 0.0f;
+    if (p > 1.0f) p = 1.0f;
+    const uint32_t thr = p >= 1.0f ? 0xFFFFFFFFu : p <= 0.0f ? 0u : (uint32_t)((double)p * 4294967296.0);
+    uint64_t coin_seed = S.seed;
+    if (coin_seed == 0 && thr != 0xFFFFFFFFu) {
+        std::random_device rd;
+        coin_seed = ((uint64_t)rd() << 32) | rd();
+        if (coin_seed == 0) coin_seed = 0xA5A5A5A5A5A5A5A5ull;
+    }
+
+    std::vector<uint8_t> exists(ns, 0), expanded(ns, 0), alive(ne, 0);
+    std::vector<uint32_t> new_sid(ns, 0xFFFFFFFFu), new_eid(ne, 0xFFFFFFFFu);
+    uint32_t next_sid = 0, next_eid = 0;
+    for (uint32_t s = 0; s < n_roots && s < ns; ++s) { exists[s] = 1; expanded[s] = 1; new_sid[s] = next_sid++; }
+    std::unordered_map<uint32_t, uint64_t> per_step;
+    std::vector<uint64_t> per_parent(ns, 0);
+    for (uint64_t e = 0; e < ne; ++e) {
+        const hgx_event& x = R.events[e];
+        bool ok = expanded[x.input_state] && capped_in[e];
+        if (ok && S.draws()) ok = transition_survives(key[e], S.seed, rate_for_rule(S, x.rule));
+        if (!ok) continue;
+        alive[e] = 1;
+        new_eid[e] = next_eid++;
+        const uint32_t c = x.output_state;
+        exists[c] = 1;
+        new_sid[c] = next_sid++;
+        const uint32_t cstep = R.step[c];
+        if ((int64_t)cstep >= j.steps) continue;             // not expanded: the last step
+        bool go = true;
+        if (thr != 0xFFFFFFFFu) {
+            go = thr != 0u && (uint32_t)splitmix(coin_seed ^ ((uint64_t)cstep << 32) ^ (uint64_t)new_sid[c]) < thr;
+        }
+        if (go && S.per_step) go = per_step[cstep]++ < S.per_step;
+        if (go && S.per_parent) go = per_parent[x.input_state]++ < S.per_parent;
+        expanded[c] = go;
+    }
+
+    // renumber and restrict
+    Run T;
+    T.c = R.c;
+    const uint64_t ns2 = next_sid, ne2 = next_eid;
+    T.hash.resize(ns2); T.step.resize(ns2); T.creator.assign(ns2, 0xFFFFFFFFu);
+    T.soff.assign(ns2 + 1, 0);
+    T.eoff.assign(1, 0);
+    std::vector<uint64_t> ir2(ir.empty() ? 0 : ns2);
+    std::vector<uint32_t> rank2;
+    for (uint64_t s = 0; s < ns; ++s) {
+        if (!exists[s]) continue;
+        const uint32_t t = new_sid[s];
+        T.hash[t] = R.hash[s];
+        T.step[t] = R.step[s];
+        if (R.creator[s] < ne) T.creator[t] = new_eid[R.creator[s]];
+        if (!ir.empty()) ir2[t] = ir[s];
+        for (uint64_t ed = R.soff[s]; ed < R.soff[s + 1]; ++ed) {
+            for (uint32_t x = R.eoff[ed]; x < R.eoff[ed + 1]; ++x) T.verts.push_back(R.verts[x]);
+            T.eoff.push_back((uint32_t)T.verts.size());
+            if (!rank.empty()) rank2.push_back(rank[ed]);
+        }
+        T.soff[t + 1] = T.eoff.size() - 1;
+    }
+    T.events.resize(ne2);
+    T.coff.assign(1, 0);
+    for (uint64_t e = 0; e < ne; ++e) {
+        if (!alive[e]) continue;
+        hgx_event x = R.events[e];
+        x.input_state = new_sid[x.input_state];
+        x.output_state = new_sid[x.output_state];
+        T.events[new_eid[e]] = x;
+        for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k) T.cpos.push_back(R.cpos[k]);
+        T.coff.push_back((uint32_t)T.cpos.size());
+    }
+    for (size_t k = 0; k < R.causal.size(); ++k) {
+        const hgx_pair& q = R.causal[k];
+        if (q.a >= ne || q.b >= ne || !alive[q.a] || !alive[q.b]) continue;
+        T.causal.push_back(hgx_pair{new_eid[q.a], new_eid[q.b]});
+        T.causal_tr.push_back(R.causal_tr[k]);
+    }
+    for (const hgx_pair& q : R.branchial) {
+        if (q.a >= ne || q.b >= ne || !alive[q.a] || !alive[q.b]) continue;
+        T.branchial.push_back(hgx_pair{new_eid[q.a], new_eid[q.b]});
+    }
+    T.c.raw_states = ns2;
+    T.c.events = ne2;
+    T.c.causal_pairs = T.causal.size();
+    T.c.branchial_pairs = T.branchial.size();
+    uint64_t tr = 0;
+    for (uint8_t f : T.causal_tr) tr += f != 0;
+    T.c.causal_pairs_tr = tr;
+    {
+        std::unordered_map<uint64_t, int> d;
+        uint64_t fp = 0;
+        for (uint64_t h : T.hash) if (d.emplace(h, 1).second) fp += mix64(h);
+        T.c.canonical_states = d.size();
+        T.c.fingerprint = fp;
+    }
+    R = std::move(T);
+    ir = std::move(ir2);
+    rank = std::move(rank2);
+}
+
 static Value evolve(hgx_ctx* ctx, const Job& j) {
     const std::vector<uint8_t> prog = program(j);
     std::vector<uint32_t> eoff, verts, roff;
-    roots(j, eoff, verts, roff);
+    std::vector<std::vector<int64_t>> orig;
+    roots(j, eoff, verts, roff, &orig);
     const uint32_t n_roots = (uint32_t)roff.size() - 1;
     if (n_roots == 0) throw JobError("the job has no non-empty initial state");
     if (j.steps > HGX_MAX_STEPS) throw JobError("Steps above " + std::to_string(HGX_MAX_STEPS));
     uint32_t flags = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL | HGX_REC_RELATIONS;
     // state edges: for the output, and for the IR hash (Full's grouping, CanonicalHash)
-    const bool need_ir = j.full || (j.inc_hashes && j.inc_states) || j.ev_keys;
-    const bool need_rank = (j.ev_keys & (16 | 32)) != 0;
-    if (j.inc_states || j.inc_events || need_ir) flags |= HGX_REC_STATES;
+    const bool need_ir = j.full || (j.inc_hashes && j.inc_states) || j.ev_keys || j.smp.transitions();
+    const bool need_rank = (j.ev_keys & (16 | 32)) != 0 || j.smp.transitions();
+    if (j.inc_states || j.inc_events || need_ir || j.automatic || j.smp.active()) flags |= HGX_REC_STATES;
     if (j.tr) flags |= HGX_OPT_TR;
     hgx_run_desc d{};
     d.rules = prog.data(); d.rules_len = (uint32_t)prog.size(); d.n_rules = (uint32_t)j.rules.size();
@@ -334,7 +648,7 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
         char why[512] = {0};
         if (hgx_result_status(res, why, sizeof(why)) == HGX_ERR_CAPACITY) warns.emplace_back("CapacityExceeded", why);
     }
-    const uint64_t ns = R.c.raw_states, ne = R.c.events;
+    uint64_t ns = R.c.raw_states, ne = R.c.events;
     R.hash.resize(ns); R.step.resize(ns); R.creator.resize(ns);
     check(hgx_result_states(res, R.hash.data(), R.step.data(), R.creator.data(), ns), ctx, "states");
     R.events.resize(ne);
@@ -363,9 +677,16 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (need_ir && ns)
         check(hgx_ir_rank_batch(ctx, R.soff.data(), (uint32_t)ns, R.eoff.data(), R.verts.empty() ? nullptr : R.verts.data(),
                                 ir.data(), need_rank ? rank.data() : nullptr), ctx, "canonical hashes");
+    if (j.smp.active()) {
+        apply_sampling(j, R, ir, rank, n_roots);
+        ns = R.c.raw_states;
+        ne = R.c.events;
+    }
     std::vector<int64_t> rep(ns);
     std::vector<uint64_t> reps;                               // class representatives, in id order
-    if (j.full) {
+    if (j.automatic) {
+        content_classes(R, orig, n_roots, rep, reps);
+    } else if (j.full) {
         std::unordered_map<uint64_t, uint32_t> rep_of_hash;
         for (uint64_t s = 0; s < ns; ++s) {
             auto it = rep_of_hash.emplace(ir[s], (uint32_t)s);
@@ -576,9 +897,6 @@ std::vector<uint8_t> run_gpu_evolution(const GpuJob& g, const HostBridge& host) 
     };
     if (g.explore_from_canonical_states_only) refuse("ExploreFromCanonicalStatesOnly");
     if (g.quotient_initial_states) refuse("QuotientInitialStates");
-    if (g.exploration_probability != 1.0 || g.transition_rate != 1.0 || !g.rule_weights.empty())
-        refuse("sampling");
-    if (g.max_states_per_step || g.max_successor_states_per_parent || g.matches_per_state_rule) refuse("a cap");
     if (!g.graph_properties.empty()) refuse("GraphProperties");
     if (g.branchial_step != 0) refuse("BranchialStep");
     if (g.show_genesis_events) refuse("ShowGenesisEvents");
@@ -586,7 +904,6 @@ std::vector<uint8_t> run_gpu_evolution(const GpuJob& g, const HostBridge& host) 
         g.include_global_edges || g.include_state_bitvectors)
         refuse("that RequestedData component");
     if (!g.session_op.empty() && g.session_op != "Evolve") refuse("a session (" + g.session_op + ")");
-    if (g.state_canon_mode == GpuJob::StateCanonCode::kAutomatic) refuse("CanonicalizeStates -> Automatic");
     if (g.steps < 0) throw std::runtime_error("Steps must be non-negative");
 
     Job j;
@@ -599,6 +916,14 @@ std::vector<uint8_t> run_gpu_evolution(const GpuJob& g, const HostBridge& host) 
     j.steps = g.steps;
     j.tr = g.transitive_reduction;
     j.full = g.state_canon_mode == GpuJob::StateCanonCode::kFull;
+    j.automatic = g.state_canon_mode == GpuJob::StateCanonCode::kAutomatic;
+    j.smp.explore_p = g.exploration_probability;
+    j.smp.rate = g.transition_rate;
+    j.smp.weights = g.rule_weights;
+    j.smp.seed = g.random_seed;
+    j.smp.per_step = g.max_states_per_step;
+    j.smp.per_parent = g.max_successor_states_per_parent;
+    j.smp.per_state_rule = g.matches_per_state_rule;
     constexpr uint32_t In = 1, Out = 2, Step = 4, Cons = 16, Prod = 32;
     j.ev_keys = g.event_canon_mode == GpuJob::EventCanonCode::kFull ? (In | Out)
               : g.event_canon_mode == GpuJob::EventCanonCode::kAutomatic ? (In | Out | Step | Cons | Prod) : 0u;

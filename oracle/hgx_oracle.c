@@ -477,6 +477,8 @@ struct hgxo_result {
     VEC(uint32_t) s_creator; VEC(uint64_t) s_hash; VEC(uint32_t) spool;
     /* events */
     VEC(hgx_event) ev;
+    VEC(uint16_t) ev_cq;    /* consumed edges' positions in the input state, pattern order */
+    VEC(uint32_t) ev_cqoff; /* per event, the start of its run in ev_cq */
     VEC(uint64_t) ev_anc;   /* lineage-step mask of causal ancestors, for tests of the GPU's */
     /* relations */
     VEC(hgx_pair) causal; VEC(uint8_t) causal_tr;
@@ -721,6 +723,11 @@ int hgxo_evolve(const hgx_run_desc* d, hgxo_result** out) {
                 VPUSH(R->s_ne, cne); VPUSH(R->s_step, step); VPUSH(R->s_next, nxt);
                 VPUSH(R->s_creator, evid);
                 VPUSH(R->s_hash, state_hash(R, &ws, child));
+                /* consumed edges as positions in the parent, pattern order */
+                VPUSH(R->ev_cqoff, (uint32_t)R->ev_cq.n);
+                for (uint8_t k = 0; k < rr->n_lhs; ++k)
+                    for (uint32_t q = 0; q < pne; ++q)
+                        if (R->spool.p[R->s_eoff.p[parent] + q] == m->edges[k]) { VPUSH(R->ev_cq, (uint16_t)q); break; }
                 hgx_event ev = { parent, child, m->rule, step };
                 VPUSH(R->ev, ev);
                 /* causal: one triple per consumed edge with a producer; pairs deduplicated
@@ -817,6 +824,7 @@ void hgxo_free(hgxo_result* R) {
     free(R->e_ar.p); free(R->e_voff.p); free(R->e_prod.p); free(R->vpool.p);
     free(R->s_eoff.p); free(R->s_ne.p); free(R->s_step.p); free(R->s_next.p);
     free(R->s_creator.p); free(R->s_hash.p); free(R->spool.p); free(R->ev.p); free(R->ev_anc.p);
+    free(R->ev_cq.p); free(R->ev_cqoff.p);
     free(R->causal.p); free(R->causal_tr.p); free(R->branchial.p);
     free(R);
 }
@@ -869,6 +877,17 @@ int hgxo_state_edges(const hgxo_result* R, uint64_t* soff, uint32_t* eoff, uint3
     if (eoff) eoff[ne] = (uint32_t)nv;
     if (n_edges_total) *n_edges_total = ne;
     if (n_vertices_total) *n_vertices_total = nv;
+    return HGX_OK;
+}
+
+int hgxo_consumed(const hgxo_result* R, uint32_t* offsets, uint16_t* positions, uint64_t* total) {
+    const uint64_t ne = R->ev.n;
+    if (offsets) {
+        for (uint64_t e = 0; e < ne; ++e) offsets[e] = R->ev_cqoff.p[e];
+        offsets[ne] = (uint32_t)R->ev_cq.n;
+    }
+    if (positions && R->ev_cq.n) memcpy(positions, R->ev_cq.p, R->ev_cq.n * sizeof(uint16_t));
+    if (total) *total = R->ev_cq.n;
     return HGX_OK;
 }
 

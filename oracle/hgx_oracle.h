@@ -36,6 +36,10 @@ int  hgxo_branchial(const hgxo_result* r, hgx_pair* pairs);
 int  hgxo_state_edges(const hgxo_result* r, uint64_t* state_offsets, uint32_t* edge_offsets,
                       uint32_t* vertices, uint64_t* n_edges_total, uint64_t* n_vertices_total);
 
+/* Consumed edges of every event as positions in its input state, pattern order: offsets
+ * [n_events+1] into positions (either may be NULL; *total gets the position count). */
+int  hgxo_consumed(const hgxo_result* r, uint32_t* offsets, uint16_t* positions, uint64_t* total);
+
 /* WL canonical hash of one state (edge_offsets[n_edges+1] into vertices). */
 uint64_t hgxo_wl_hash(const uint32_t* edge_offsets, const uint32_t* vertices, uint32_t n_edges);
 

@@ -41,6 +41,7 @@ def lib():
         L.hgxo_num_branchial_pairs.restype = C.c_uint64
         L.hgxo_branchial.argtypes = [C.c_void_p, C.c_void_p]
         L.hgxo_state_edges.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.hgxo_consumed.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.hgxo_wl_hash.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
         L.hgxo_wl_hash.restype = C.c_uint64
         L.hgxo_match.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -55,7 +56,7 @@ class OracleResult:
     """Everything the restatement computed for one request, as numpy arrays."""
 
     def __init__(self, counts, hashes, steps, creators, events, causal, causal_tr, branchial,
-                 state_edges=None):
+                 state_edges=None, consumed=None):
         self.counts = counts
         self.state_hash = hashes
         self.state_step = steps
@@ -65,6 +66,7 @@ class OracleResult:
         self.causal_tr = causal_tr    # (m,) uint8
         self.branchial = branchial    # (k, 2) uint32
         self.state_edges = state_edges
+        self.consumed = consumed      # (offsets[n+1], positions): consumed edges, pattern order
 
 
 def evolve(req: Request, with_edges: bool = False) -> OracleResult:
@@ -104,7 +106,14 @@ def evolve(req: Request, with_edges: bool = False) -> OracleResult:
             vv = np.zeros(max(tot_v.value, 1), np.uint32)
             L.hgxo_state_edges(h, so.ctypes.data, eo.ctypes.data, vv.ctypes.data, None, None)
             edges = (so, eo, vv)
-        return OracleResult(cnt.as_dict(), hashes, steps, creators, events, causal, ctr, br, edges)
+            tot = C.c_uint64()
+            L.hgxo_consumed(h, None, None, C.byref(tot))
+            co = np.zeros(ne + 1, np.uint32)
+            cp = np.zeros(max(tot.value, 1), np.uint16)
+            L.hgxo_consumed(h, co.ctypes.data, cp.ctypes.data, None)
+            consumed = (co, cp[:tot.value])
+        return OracleResult(cnt.as_dict(), hashes, steps, creators, events, causal, ctr, br, edges,
+                            consumed if with_edges else None)
     finally:
         L.hgxo_free(h)
 

@@ -31,7 +31,7 @@ int main(int argc, char** argv) {
     int progress = 0;
     host.progress = [&](const std::string&) { ++progress; };
     int rc = 0;
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 6; ++i) {
         hgx::GpuJob job{i < 2 ? rules : rules2, i < 2 ? init : init2};
         job.steps = 3;
         if (i == 1) {
@@ -40,7 +40,16 @@ int main(int argc, char** argv) {
             job.include_canonical_hashes = true;
             job.transitive_reduction = false;
         }
-        if (i == 3) job.exploration_probability = 0.5;
+        if (i == 3) {
+            job.exploration_probability = 0.5;
+            job.transition_rate = 0.75;
+            job.random_seed = 12345;
+        }
+        if (i == 5) job.explore_from_canonical_states_only = true;
+        if (i == 4) {
+            job.state_canon_mode = hgx::GpuJob::StateCanonCode::kAutomatic;
+            job.event_canon_mode = hgx::GpuJob::EventCanonCode::kFull;
+        }
         try {
             const std::vector<uint8_t> b = hgx::run_gpu_evolution(job, host);
             if (!write(out + "/case" + std::to_string(i) + ".wxf", b)) rc = 3;

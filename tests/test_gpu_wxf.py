@@ -151,6 +151,45 @@ def test_wxf_event_canonicalization(ctx, case, mode):
         assert e["CanonicalId"] <= eid and out["Events"][e["CanonicalId"]]["CanonicalId"] == e["CanonicalId"]
 
 
+@pytest.mark.parametrize("case", [c for c in GOLD_NONE if c["name"] in GOLD_FULL], ids=lambda c: c["name"])
+def test_wxf_automatic_state_identity(ctx, case):
+    """CanonicalizeStates -> Automatic: states of equal content (edges in order, vertex names
+    compared as such) are one class. The golden matrix's 'Automatic None' row fixes the class
+    count; its other columns, and the 'Automatic Automatic|Full' rows, equal the None rows'."""
+    base = {"InitialStates": [case["init"]],
+            "Rules": {f"R{k}": Rule(r["lhs"], r["rhs"]) for k, r in enumerate(case["rules"])},
+            "Steps": case["steps"]}
+    want = GOLD_FULL[case["name"]]
+    plain = loads(ctx.run_wxf(dumps({**base, "Options": {}})))
+    out = loads(ctx.run_wxf(dumps({**base, "Options": {"CanonicalizeStates": Symbol("Automatic")}})))
+    assert out["NumStates"] == want["content_states"] == len(out["States"])
+    for k in ("NumEvents", "NumCausalEdges", "NumBranchialEdges"):
+        assert out[k] == plain[k]
+    # a class's members hold its representative's edges (one initial state: equal names are
+    # equal device labels), and a representative is its class's first state
+    for e in out["Events"].values():
+        rep = e["CanonicalOutputState"]
+        assert rep <= e["OutputState"] and rep in out["States"]
+        assert plain["States"][e["OutputState"]]["Edges"] == plain["States"][rep]["Edges"]
+    for mode in ("Automatic", "Full"):
+        both = loads(ctx.run_wxf(dumps({**base, "Options": {"CanonicalizeStates": Symbol("Automatic"),
+                                                            "CanonicalizeEvents": Symbol(mode)}})))
+        assert both["NumStates"] == want["content_states"]
+        assert both["NumEvents"] == want["events_" + mode.lower()]
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+def test_wxf_automatic_fresh_vertices_are_distinct_names(ctx, k):
+    """{{x}} -> {{x, y}} over k unary edges on one vertex, two steps. Every event creates a vertex
+    no other event creates (the reference draws fresh vertices from one counter), so no two
+    states have equal content: 1 + k + k(k - 1) classes. Up to isomorphism there are 3."""
+    base = {"InitialStates": [[[0]] * k], "Rules": {"R": Rule([[0]], [[0, 1]])}, "Steps": 2}
+    out = loads(ctx.run_wxf(dumps({**base, "Options": {"CanonicalizeStates": Symbol("Automatic")}})))
+    assert out["NumStates"] == 1 + k + k * (k - 1)
+    full = loads(ctx.run_wxf(dumps({**base, "Options": {"CanonicalizeStates": Symbol("Full")}})))
+    assert full["NumStates"] == 3
+
+
 def test_wxf_event_signature_keys(ctx):
     """A key list: {"InputState", "OutputState"} is Full's signature."""
     c = next(c for c in GOLD_NONE if c["name"] == "cycle4-automorphic")
@@ -165,9 +204,9 @@ def test_wxf_requested_data_and_errors(ctx):
     assert set(out) == {"Events", "NumStates", "Warnings"}
     e0 = out["Events"][0]
     assert set(e0) == {"Id", "InputState", "OutputState"}
-    with pytest.raises(hgx.HGXError, match="sampling"):
+    with pytest.raises(hgx.HGXError, match="ExploreFromCanonicalStatesOnly"):
         ctx.run_wxf(dumps({"InitialEdges": [[1, 2]], "Rules": {"R": Rule([[0, 1]], [[0, 1], [1, 2]])},
-                           "Steps": 2, "Options": {"TransitionRate": 0.5}}))
+                           "Steps": 2, "Options": {"ExploreFromCanonicalStatesOnly": Symbol("True")}}))
 
 
 def _frame(b):

@@ -53,9 +53,11 @@ def test_encoder_roundtrip_and_valid_job():
     (_job(Rules={"R": Rule([[0, -1]], [[0]])}), "negative pattern variable"),
     (_job(Rules={"R": Rule([[]], [[0]])}), "empty left-hand side"),
     (_job(Rules={"R": [[0, 1], [1, 2]]}), "Rule[lhs, rhs]"),
-    (_job(Options={"CanonicalizeStates": Symbol("Automatic")}), "only None and Full"),
-    (_job(Options={"MaxStatesPerStep": 10}), "caps are not served"),
-    (_job(Options={"ExplorationProbability": 0.5}), "sampling"),
+    (_job(Options={"CanonicalizeStates": Symbol("Sometimes")}), "unknown state canonicalization"),
+    (_job(Options={"MaxStatesPerStep": -1}), "must be non-negative"),
+    (_job(Options={"ExplorationProbability": Symbol("High")}), "must be a number"),
+    (_job(Options={"RuleWeights": 0.5}), "RuleWeights"),
+    (_job(Options={"ExploreFromCanonicalStatesOnly": Symbol("True")}), "True is not served"),
     (_job(Op="Open"), "sessions"),
     ({"InitialEdges": [[1, 2]], "Steps": 2}, "no Rules"),
     (_job(InitialEdges=[[-1, -2]]), "no non-empty initial state"),
