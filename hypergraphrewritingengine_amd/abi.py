@@ -33,7 +33,44 @@ class RunDesc(C.Structure):
         ("steps", C.c_uint32),
         ("state_cap", C.c_uint32),
         ("flags", C.c_uint32),
+        ("sampling", C.c_void_p),
     ]
+
+
+class Sampling(C.Structure):
+    """hgx_sampling."""
+    _fields_ = [
+        ("transition_rate", C.c_double),
+        ("rule_weights", C.POINTER(C.c_double)),
+        ("n_rule_weights", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("exploration_probability", C.c_double),
+        ("max_states_per_step", C.c_uint64),
+        ("max_successor_states_per_parent", C.c_uint64),
+        ("matches_per_state_rule", C.c_uint64),
+    ]
+
+
+def make_sampling(spec: dict, keep: list):
+    """An hgx_sampling from a dict with any of: rate, weights, seed, explore_p, per_step,
+    per_parent, per_state_rule."""
+    import numpy as np
+    s = Sampling()
+    s.transition_rate = float(spec.get("rate", 1.0))
+    w = spec.get("weights")
+    if w is not None:
+        arr = np.asarray(w, dtype=np.float64)
+        keep.append(arr)
+        s.rule_weights = arr.ctypes.data_as(C.POINTER(C.c_double))
+        s.n_rule_weights = len(arr)
+    s.seed = int(spec.get("seed", 0))
+    s.exploration_probability = float(spec.get("explore_p", 1.0))
+    s.max_states_per_step = int(spec.get("per_step", 0))
+    s.max_successor_states_per_parent = int(spec.get("per_parent", 0))
+    s.matches_per_state_rule = int(spec.get("per_state_rule", 0))
+    keep.append(s)
+    return s
 
 
 class Counts(C.Structure):
@@ -80,6 +117,7 @@ def make_run_desc(req, keep: list) -> RunDesc:
         keep.append(enc["desc_arrays"])
         d = enc["desc"]
         d.steps, d.state_cap, d.flags = int(req.steps), int(req.state_cap), int(req.flags)
+        d.sampling = _sampling_ptr(req, keep)
         return d
     prog = np.frombuffer(enc["prog"], dtype=np.uint8).copy()
     offs, verts, roots = enc["offs"], enc["verts"], enc["roots"]
@@ -99,8 +137,17 @@ def make_run_desc(req, keep: list) -> RunDesc:
     d.steps = int(req.steps)
     d.state_cap = int(req.state_cap)
     d.flags = int(req.flags)
+    d.sampling = _sampling_ptr(req, keep)
     enc["desc"], enc["desc_arrays"] = d, arrays
     return d
+
+
+def _sampling_ptr(req, keep):
+    spec = getattr(req, "sampling", None)
+    if not spec:
+        return None
+    s = make_sampling(spec, keep)
+    return C.cast(C.pointer(s), C.c_void_p)
 
 
 class KernelStat(C.Structure):

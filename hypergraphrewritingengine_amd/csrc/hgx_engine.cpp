@@ -326,10 +326,13 @@ void Engine::recover(const std::string& why, std::vector<std::shared_ptr<RunRec>
     if (F_ && !F_->keep_edges) F_->edges.release();
     F_.reset();
     prev_ = PrevMatches{};
+    F_expand_.clear();
     F_runs_.clear(); F_b_.clear(); F_e_.clear();
     for (uint32_t s2 : release_next_) free_slots_.push_back(s2);
     release_next_.clear();
 }
+
+void frontier_ir(const DevGen& G, uint32_t b, uint32_t e, hipStream_t st, RunRec& r);
 
 uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
     const auto t_step = Clock::now();
@@ -348,6 +351,9 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
     uint32_t* cnt = nullptr;
     uint64_t* moff = nullptr;
     std::vector<uint8_t> h_active(cap_slots_, 0);
+    std::vector<uint32_t> h_parent;        // sampled generations: each match's parent, on the host
+    std::vector<SampledKids> kids;         // children of requests that sample their frontier
+    thinned_ = false;
 
     // ---- 1. the join over the frontier, with the state-cap decision ----
     if (n) {
@@ -373,6 +379,12 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             any = true;
         }
         if (any) {
+            // sampled requests: the exact hash and edge ranks of their frontier states, for the
+            // transition keys of this generation's matches
+            for (uint32_t i = 0; i < nF; ++i) {
+                RunRec& r = *F_runs_[i];
+                if (h_active[r.slot] && r.hp->smp.transitions()) frontier_ir(*F_, F_b_[i], F_e_[i], st, r);
+            }
             std::memcpy(h_tab_.data() + 12ull * cap_slots_, h_active.data(), cap_slots_);
             up((uint8_t*)d_budget_, h_tab_.data(), h_tab_.size(), st);    // budget, sbeg, send, active
             HIP_TRY(launch_fill(d_run_acc_, 0, sizeof(uint64_t) * (4ull * cap_slots_ + 1), st));
@@ -382,6 +394,12 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 throw Error{HGX_ERR_UNSUPPORTED, "states above 65535 edges need the large-state join"};
             MatchArgs ma{};
             ma.P = F_->v; ma.runs = d_runs_; ma.active = d_active_; ma.cnt = cnt; ma.moff = moff;
+            if (!F_expand_.empty()) {                 // sampled requests' unexpanded states
+                if (F_expand_.size() != n) throw Error{HGX_ERR_DEVICE, "expansion flags out of step"};
+                uint8_t* d_smask = dalloc<uint8_t>(tmp, n);
+                up(d_smask, F_expand_.data(), n, st);
+                ma.smask = d_smask;
+            }
             ma.max_ne = std::max(1u, F_->max_ne);
             ma.max_nextv = std::max(1u, F_->max_nextv);
             // forwarding: on when asked; automatic forwards the states of the size classes above
@@ -585,6 +603,9 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 HIP_TRY(launch_fill(mv.cne + M, 0, sizeof(uint32_t), st));
                 ma.M = mv;
                 launch_classes(true, mcls_emit, "k_match_emit", match_bytes + M * (12ull + 2ull * K));
+                bool thin = false;
+                for (uint32_t i = 0; i < nF; ++i) thin |= tot[i] && F_runs_[i]->hp->smp.transitions();
+                if (thin) sample_transitions(n, K, mv, cnt, moff, tot, h_edges, M, h_parent, st);
             }
         }
     }
@@ -969,6 +990,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
         for (uint32_t i = 0; i < nF; ++i) {
             if (!tot[i]) continue;
             auto& r = F_runs_[i];
+            if (r->hp->smp.frontier()) kids.push_back(SampledKids{r.get(), cb[i], ce[i], r->events});
             Segment sg{C, cb[i], ce[i], 0, 0};
             if (!h_poff.empty()) { sg.br_b = h_poff[F_b_[i]]; sg.br_e = h_poff[F_e_[i]]; }
             r->segs.push_back(sg);
@@ -1013,7 +1035,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
     for (uint32_t s : releasable) free_slots_.push_back(s);
     if (F_ && !F_->keep_edges) F_->edges.release();
     F_ = C;
-    if (M) {
+    sample_frontier(kids, C ? C->v.n : 0, mv, M, h_parent, st);
+    if (M && !thinned_) {
         prev_.ar = std::move(keep);
         prev_.n = (uint32_t)M; prev_.K = K;
         prev_.parent = mv.parent; prev_.rule = mv.rule; prev_.q = mv.q; prev_.moff = moff;
@@ -1030,6 +1053,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
         if (F_ && !F_->keep_edges) F_->edges.release();
         F_.reset();
         prev_ = PrevMatches{};
+        F_expand_.clear();
         F_runs_.clear(); F_b_.clear(); F_e_.clear();
         for (uint32_t s2 : release_next_) free_slots_.push_back(s2);
         release_next_.clear();
@@ -1041,6 +1065,190 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                      n, (unsigned long long)M, (unsigned)admitted.size(), done_now.size(), t_launch, t_count,
                      t_emit, t_child, t_fin, ms_since(t_step));
     return live_or_queued();
+}
+
+// ------------------------------------------------------------------------------------------
+// sampling (hgx_sampling): the GPU engine's decisions, generation by generation
+// ------------------------------------------------------------------------------------------
+
+// The exact (IR) hash and canonical edge ranks of a request's frontier states [b, e), computed
+// on the host from the generation's edges (the reference's device computes the same with its IR
+// canonicalizer when sampling, gpu/src/ir_canon.cu:152-280).
+void frontier_ir(const DevGen& G, uint32_t b, uint32_t e, hipStream_t st, RunRec& r) {
+    const uint32_t n = e - b, A = G.v.A;
+    std::vector<uint64_t> eo(n);
+    std::vector<uint32_t> ne(n);
+    d2h(eo.data(), G.v.eoff + b, n, st);
+    d2h(ne.data(), G.v.ne + b, n, st);
+    HIP_TRY(hipStreamSynchronize(st));
+    uint64_t E = 0;
+    for (uint32_t k = 0; k < n; ++k) E += ne[k];
+    std::vector<uint8_t> ar(E);
+    std::vector<uint32_t> ev(E * A);
+    // states of one request are contiguous in the generation, their edges too
+    if (E) {
+        const uint64_t e0 = eo[0];
+        d2h(ar.data(), G.v.ar + e0, E, st);
+        d2h(ev.data(), G.v.ev + e0 * A, E * A, st);
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    std::vector<uint64_t> soff(n + 1, 0);
+    std::vector<uint32_t> eoff(E + 1, 0), verts;
+    verts.reserve(E * 2);
+    r.smp_roff.assign(n + 1, 0);
+    uint64_t x = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ne[k] && eo[k] != eo[0] + x) throw Error{HGX_ERR_DEVICE, "a request's states are not contiguous"};
+        soff[k] = x;
+        r.smp_roff[k] = (uint32_t)x;
+        for (uint32_t j = 0; j < ne[k]; ++j, ++x) {
+            for (uint32_t t = 0; t < ar[x]; ++t) verts.push_back(ev[x * A + t]);
+            eoff[x + 1] = (uint32_t)verts.size();
+        }
+    }
+    soff[n] = x;
+    r.smp_roff[n] = (uint32_t)x;
+    r.smp_ir.assign(n, 0);
+    r.smp_rank.assign(std::max<uint64_t>(E, 1), 0);
+    if (n && hgx_ir_rank_batch(nullptr, soff.data(), n, eoff.data(), verts.empty() ? nullptr : verts.data(),
+                               r.smp_ir.data(), r.smp_rank.data()) != HGX_OK)
+        throw Error{HGX_ERR_DEVICE, "exact hashes of a sampled frontier failed"};
+}
+
+void Engine::sample_transitions(uint32_t n, uint32_t K, MatchView& mv, uint32_t* cnt, uint64_t* moff,
+                                std::vector<uint64_t>& tot, std::vector<uint64_t>& h_edges, uint64_t& M,
+                                std::vector<uint32_t>& h_parent, hipStream_t st) {
+    const uint64_t M0 = M;
+    h_parent.resize(M0);
+    std::vector<uint32_t> rule(M0), cne(M0);
+    std::vector<uint16_t> q(M0 * K);
+    std::vector<uint64_t> off(n + 1);
+    d2h(h_parent.data(), mv.parent, M0, st);
+    d2h(rule.data(), mv.rule, M0, st);
+    d2h(q.data(), mv.q, M0 * K, st);
+    d2h(cne.data(), mv.cne, M0, st);
+    d2h(off.data(), moff, n + 1, st);
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<uint8_t> keep(M0, 1);
+    bool dropped = false;
+    const uint32_t nF = (uint32_t)F_runs_.size();
+    std::vector<std::pair<uint64_t, uint64_t>> grp;
+    for (uint32_t i = 0; i < nF; ++i) {
+        if (!tot[i]) continue;
+        RunRec& r = *F_runs_[i];
+        const SampleHost& S = r.hp->smp;
+        if (!S.transitions()) continue;
+        const uint64_t c0 = off[F_b_[i]], c1 = off[F_e_[i]];
+        std::vector<uint64_t> key(c1 - c0);
+        uint32_t ranks[HGX_MAX_PATTERN_EDGES];
+        for (uint64_t c = c0; c < c1; ++c) {
+            const uint32_t k = h_parent[c] - F_b_[i];
+            if (k >= r.smp_ir.size() || rule[c] >= r.hp->rules.size())
+                throw Error{HGX_ERR_DEVICE, "a sampled match outside its request"};
+            const uint32_t nl = r.hp->rules[rule[c]].n_lhs;
+            for (uint32_t t = 0; t < nl; ++t) ranks[t] = r.smp_rank[r.smp_roff[k] + q[c * K + t]];
+            key[c - c0] = transition_key(r.smp_ir[k], rule[c], ranks, nl);
+        }
+        if (S.per_state_rule) {
+            // a state's matches are contiguous: per (state, rule), the k smallest ranks survive
+            // (ties in event order; gpu/src/match.cu:229-256)
+            for (uint64_t a = c0; a < c1;) {
+                uint64_t z = a;
+                while (z < c1 && h_parent[z] == h_parent[a]) ++z;
+                for (uint32_t ru = 0; ru < r.hp->rules.size(); ++ru) {
+                    grp.clear();
+                    for (uint64_t c = a; c < z; ++c)
+                        if (rule[c] == ru) grp.push_back({transition_rank(key[c - c0], S.seed), c});
+                    if (grp.size() <= S.per_state_rule) continue;
+                    std::sort(grp.begin(), grp.end());
+                    for (size_t x = S.per_state_rule; x < grp.size(); ++x) keep[grp[x].second] = 0;
+                }
+                a = z;
+            }
+        }
+        if (S.draws())                          // the rate draw (gpu/src/rewrite.cu:255-274)
+            for (uint64_t c = c0; c < c1; ++c)
+                if (keep[c] && !transition_survives(key[c - c0], S.seed, S.rate_for(rule[c]))) keep[c] = 0;
+        for (uint64_t c = c0; c < c1; ++c) dropped |= !keep[c];
+    }
+    if (!dropped) return;
+    // the survivors, in order: match list, per-state counts and offsets, per-run totals
+    std::vector<uint32_t> p2, r2, cne2, cnt2(n + 1, 0);
+    std::vector<uint16_t> q2;
+    std::vector<uint64_t> off2(n + 1, 0);
+    p2.reserve(M0); r2.reserve(M0); cne2.reserve(M0 + 1); q2.reserve(M0 * K);
+    for (uint32_t st_ = 0; st_ < n; ++st_) {
+        off2[st_] = p2.size();
+        for (uint64_t c = off[st_]; c < off[st_ + 1]; ++c) {
+            if (!keep[c]) continue;
+            p2.push_back(h_parent[c]);
+            r2.push_back(rule[c]);
+            cne2.push_back(cne[c]);
+            q2.insert(q2.end(), q.begin() + c * K, q.begin() + (c + 1) * K);
+            ++cnt2[st_];
+        }
+    }
+    off2[n] = p2.size();
+    const uint64_t M1 = p2.size();
+    for (uint32_t i = 0; i < nF; ++i) {
+        if (!tot[i]) continue;
+        RunRec& r = *F_runs_[i];
+        const uint64_t a = off2[F_b_[i]], z = off2[F_e_[i]];
+        tot[i] = z - a;
+        uint64_t edges = 0;
+        for (uint64_t c = a; c < z; ++c) edges += cne2[c];
+        h_edges[r.slot] = edges;
+        if (!tot[i]) r.done = true;             // no transition of the generation survived
+    }
+    cne2.push_back(0);
+    if (M1) {
+        up(mv.parent, p2.data(), M1, st);
+        up(mv.rule, r2.data(), M1, st);
+        up(mv.q, q2.data(), M1 * K, st);
+    }
+    up(mv.cne, cne2.data(), M1 + 1, st);
+    up(cnt, cnt2.data(), n + 1, st);
+    up(moff, off2.data(), n + 1, st);
+    M = M1;
+    mv.n = (uint32_t)M1;
+    h_parent.swap(p2);
+    thinned_ = true;
+}
+
+void Engine::sample_frontier(const std::vector<SampledKids>& kids, uint32_t nC, const MatchView& mv, uint64_t M,
+                             std::vector<uint32_t>& h_parent, hipStream_t st) {
+    F_expand_.clear();
+    bool any = false;
+    for (const SampledKids& k : kids) any |= k.r->depth < k.r->hp->steps;
+    if (!any || !M) return;
+    if (h_parent.size() != M) {
+        h_parent.resize(M);
+        d2h(h_parent.data(), mv.parent, M, st);
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    F_expand_.assign(nC, 1);
+    bool masked = false;
+    for (const SampledKids& k : kids) {
+        const RunRec& r = *k.r;
+        const SampleHost& S = r.hp->smp;
+        const uint32_t step = r.depth;              // the children's step
+        if (step >= r.hp->steps) continue;          // the last step: nothing expands them
+        uint64_t at_step = 0;
+        std::unordered_map<uint32_t, uint64_t> at_parent;
+        const uint64_t n_roots = r.hp->n_roots();
+        for (uint32_t c = k.cb; c < k.ce; ++c) {
+            const uint64_t sid = n_roots + k.ev_base + (c - k.cb);
+            // exploration.hpp:93-121: the coin, then the step's bound, then the parent's
+            bool go = true;
+            if (S.thr != 0xFFFFFFFFu)
+                go = S.thr != 0u && (uint32_t)splitmix(S.coin_seed ^ ((uint64_t)step << 32) ^ sid) < S.thr;
+            if (go && S.per_step) go = at_step++ < S.per_step;
+            if (go && S.per_parent) go = at_parent[h_parent[c]]++ < S.per_parent;
+            F_expand_[c] = go ? 1 : 0;
+            masked |= !go;
+        }
+    }
+    if (!masked) F_expand_.clear();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1265,6 +1473,10 @@ extern "C" int hgx_batch_capture(hgx_ctx* ctx, const hgx_batch* b, hgx_replay** 
         HIP_TRY(hipSetDevice(ctx->device));
         const uint32_t n = (uint32_t)b->runs.size();
         if (!n) throw Error{HGX_ERR_INVALID, "the batch is empty"};
+        for (const auto& h : b->runs)
+            if (h->smp.on)
+                throw Error{HGX_ERR_UNSUPPORTED, "sampled requests take host decisions every generation; "
+                                                 "they are not recorded into graphs"};
         hipStream_t st = ctx->stream;
         auto plan = std::make_shared<PassPlan>();
         {   // 1. an eager pass keeps every host decision

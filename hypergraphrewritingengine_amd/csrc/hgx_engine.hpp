@@ -66,10 +66,21 @@ struct RunRec {
     bool finishing = false;          // listed for finalize in the current step
     uint64_t footprint = 0;
     std::vector<Segment> segs;       // every state of the run, in state-id order
+    // A sampled request's frontier states: exact hash and canonical edge ranks (edge position q
+    // of frontier state k has rank smp_rank[smp_roff[k] + q]), for its transition keys.
+    std::vector<uint64_t> smp_ir;
+    std::vector<uint32_t> smp_roff, smp_rank;
     hgx_counts counts{};
     hipStream_t st = nullptr;        // the context's stream (contexts outlive results)
     Clock::time_point t_submit, t_admit, t_done;
     double device_ms = 0;
+};
+
+// A sampled request's children in a new generation: [cb, ce), its events before them.
+struct SampledKids {
+    RunRec* r;
+    uint32_t cb, ce;
+    uint64_t ev_base;
 };
 
 // The host decisions of one eager pass over a prepared batch, per generation: the per-slot child
@@ -156,6 +167,22 @@ private:
         uint16_t* q = nullptr;
         uint64_t* moff = nullptr;
     } prev_;
+
+    // Sampled requests: which frontier states are expanded ([F_->v.n]; empty: every one), and
+    // whether the last generation's match list was thinned (then its matches do not forward:
+    // a child's matches come from its parent's full list).
+    std::vector<uint8_t> F_expand_;
+    bool thinned_ = false;
+    // After the emit pass: the transition decisions of the sampled requests' matches (their
+    // frontier's exact hashes and ranks from frontier_ir), and the match list, the per-state
+    // counts and offsets, the per-run totals and child edges compacted to the survivors.
+    void sample_transitions(uint32_t n, uint32_t K, MatchView& mv, uint32_t* cnt, uint64_t* moff,
+                            std::vector<uint64_t>& tot, std::vector<uint64_t>& h_edges, uint64_t& M,
+                            std::vector<uint32_t>& h_parent, hipStream_t st);
+    // After a generation is built: which of the sampled requests' new states are expanded
+    // (exploration coin, per-step and per-parent bounds, in event order), into F_expand_.
+    void sample_frontier(const std::vector<SampledKids>& kids, uint32_t nC, const MatchView& mv, uint64_t M,
+                         std::vector<uint32_t>& h_parent, hipStream_t st);
 
     EngineMode mode_ = EngineMode::Normal;
     std::shared_ptr<PassPlan> plan_;

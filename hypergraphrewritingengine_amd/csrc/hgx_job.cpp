@@ -17,7 +17,7 @@
 // set, one or more initial states, causal edges with or without transitive reduction,
 // branchial edges, RequestedData selection, and the sampling and capping options
 // (TransitionRate, RuleWeights, MatchesPerStateRule, ExplorationProbability, MaxStatesPerStep,
-// MaxSuccessorStatesPerParent, RandomSeed; apply_sampling below). Options that change what is
+// MaxSuccessorStatesPerParent, RandomSeed; decided by the engine, include/hgx.h hgx_sampling). Options that change what is
 // explored and are not served (canonical-state exploration, sessions, genesis events, graph
 // properties) are refused with an error rather than ignored, so a caller never receives a
 // different evolution than it asked for; options with no effect on the result (ShowProgress,
@@ -29,7 +29,6 @@
 #include <cstring>
 #include <map>
 #include <mutex>
-#include <random>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -45,7 +44,8 @@ using wxf::Value;
 
 // The sampling and capping options (ffi_job.hpp:48-72, as the GPU backend takes them,
 // hg_gpu_backend.cpp:152-160): per-transition thinning, per-rule weights, a per-(state, rule) cap,
-// a per-state exploration coin and the two arrival-order bounds on the states expanded.
+// a per-state exploration coin and the two arrival-order bounds on the states expanded. The
+// engine takes the decisions (hgx_sampling).
 struct Sampling {
     double explore_p = 1.0;                 // ExplorationProbability
     double rate = 1.0;                      // TransitionRate
@@ -439,184 +439,6 @@ static void content_classes(const Run& R, const std::vector<std::vector<int64_t>
     }
 }
 
-// ---- sampling (common/include/hgcommon/sampling_core.hpp, gpu/include/hg_gpu/exploration.hpp) ----
-
-// The rate a rule's transitions survive at: the run's rate times the rule's weight (rules past
-// the weight list weigh 1), clamped to [0, 1] (sampling_core.hpp:28-34).
-static double rate_for_rule(const Sampling& S, uint32_t rule) {
-    const double r = S.rate * (rule < S.weights.size() ? S.weights[rule] : 1.0);
-    return r < 0.0 ? 0.0 : (r > 1.0 ? 1.0 : r);
-}
-
-// Whether a transition survives the draw: splitmix64 of the transition key against the seed
-// (mixed by multiplication, so seed 0 still draws), compared in [0, 1) through its top 53 bits
-// (sampling_core.hpp:42-57).
-static bool transition_survives(uint64_t key, uint64_t seed, double rate) {
-    if (rate >= 1.0) return true;
-    if (rate <= 0.0) return false;
-    const uint64_t x = splitmix(key ^ (seed * 0x9E3779B97F4A7C15ULL));
-    return (double)(x >> 11) * (1.0 / 9007199254740992.0) < rate;
-}
-
-// The order a per-(state, rule) cap keeps its k in: a second mix of the same key, so the rank
-// is not correlated with the rate draw (sampling_core.hpp:66-71).
-static uint64_t transition_rank(uint64_t key, uint64_t seed) {
-    uint64_t x = key ^ (seed * 0x9E3779B97F4A7C15ULL) ^ 0xA5A5A5A5A5A5A5A5ULL;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
-    return x ^ (x >> 31);
-}
-
-// The evolution the options describe, cut out of the full multiway evolution the device built.
-//
-// Every decision is the GPU engine's (gpu/src/match.cu:229-256 drain cap, gpu/src/rewrite.cu:
-// 255-274 transition draw, gpu/include/hg_gpu/exploration.hpp:66-122 state_survives_dedup), taken
-// generation by generation in this library's event order:
-//   - a transition (input state, rule, consumed edges) is keyed by the signature of
-//     event_core.hpp:60-84 over the input state's exact hash, the rule and the consumed edges'
-//     canonical ranks in pattern order; MatchesPerStateRule keeps the k matches of each
-//     (state, rule) with the smallest transition_rank (ties by event order), then the rate draw;
-//   - a transition that does not survive leaves no trace: no event, no output state, no
-//     relation;
-//   - a surviving child below the last step is expanded when the exploration coin (splitmix64 of
-//     the seed, the child's step and its state id) passes and then, counted in event order, when
-//     its step has expanded fewer than MaxStatesPerStep children and its parent fewer than
-//     MaxSuccessorStatesPerParent.
-// The two bounds and the coin depend on which states arrive first; the reference says the same
-// of its own (exploration.hpp:102-107). A transition's fate is a function of its isomorphism
-// class and the seed, so the kept subgraph is the same however the run is laid out.
-// Relations are the full evolution's restricted to the surviving events: a causal path to a
-// surviving event runs through its lineage, which survives with it, so the transitive reduction
-// is unchanged. The result is renumbered: states and events keep their order.
-static void apply_sampling(const Job& j, Run& R, std::vector<uint64_t>& ir, std::vector<uint32_t>& rank,
-                           uint32_t n_roots) {
-    const Sampling& S = j.smp;
-    const uint64_t ns = R.hash.size(), ne = R.events.size();
-    // per-event transition key (only when a draw or the cap reads it)
-    std::vector<uint64_t> key(S.transitions() ? ne : 0);
-    for (uint64_t e = 0; e < key.size(); ++e) {
-        const hgx_event& x = R.events[e];
-        uint64_t sig = fnv(FNV_OFFSET, ir[x.input_state]);
-        sig = fnv(sig, (uint64_t)(uint16_t)x.rule);
-        for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k) sig = fnv(sig, rank[R.soff[x.input_state] + R.cpos[k]]);
-        if (sig == 0 || sig == FNV_OFFSET) sig = 1;
-        key[e] = sig;
-    }
-    // the per-(state, rule) cap: each group's k smallest ranks
-    std::vector<uint8_t> capped_in(ne, 1);
-    if (S.per_state_rule) {
-        std::map<std::pair<uint32_t, uint32_t>, std::vector<std::pair<uint64_t, uint32_t>>> groups;
-        for (uint64_t e = 0; e < ne; ++e)
-            groups[{R.events[e].input_state, R.events[e].rule}].push_back({transition_rank(key[e], S.seed), (uint32_t)e});
-        for (auto& g : groups) {
-            auto& v = g.second;
-            std::sort(v.begin(), v.end());
-            for (size_t i = S.per_state_rule; i < v.size(); ++i) capped_in[v[i].second] = 0;
-        }
-    }
-    // the exploration coin's threshold, as the engine encodes the probability (evolve.cu:271-285,
-    // a float); seed 0 with a coin to flip is replaced by a random one (evolve.cu:351-356)
-    float p = (float)S.explore_p;
-    if (!(p > 0.0f)) p = 0.0f;
-    if (p > 1.0f) p = 1.0f;
-    const uint32_t thr = p >= 1.0f ? 0xFFFFFFFFu : p <= 0.0f ? 0u : (uint32_t)((double)p * 4294967296.0);
-    uint64_t coin_seed = S.seed;
-    if (coin_seed == 0 && thr != 0xFFFFFFFFu) {
-        std::random_device rd;
-        coin_seed = ((uint64_t)rd() << 32) | rd();
-        if (coin_seed == 0) coin_seed = 0xA5A5A5A5A5A5A5A5ull;
-    }
-
-    std::vector<uint8_t> exists(ns, 0), expanded(ns, 0), alive(ne, 0);
-    std::vector<uint32_t> new_sid(ns, 0xFFFFFFFFu), new_eid(ne, 0xFFFFFFFFu);
-    uint32_t next_sid = 0, next_eid = 0;
-    for (uint32_t s = 0; s < n_roots && s < ns; ++s) { exists[s] = 1; expanded[s] = 1; new_sid[s] = next_sid++; }
-    std::unordered_map<uint32_t, uint64_t> per_step;
-    std::vector<uint64_t> per_parent(ns, 0);
-    for (uint64_t e = 0; e < ne; ++e) {
-        const hgx_event& x = R.events[e];
-        bool ok = expanded[x.input_state] && capped_in[e];
-        if (ok && S.draws()) ok = transition_survives(key[e], S.seed, rate_for_rule(S, x.rule));
-        if (!ok) continue;
-        alive[e] = 1;
-        new_eid[e] = next_eid++;
-        const uint32_t c = x.output_state;
-        exists[c] = 1;
-        new_sid[c] = next_sid++;
-        const uint32_t cstep = R.step[c];
-        if ((int64_t)cstep >= j.steps) continue;             // not expanded: the last step
-        bool go = true;
-        if (thr != 0xFFFFFFFFu) {
-            go = thr != 0u && (uint32_t)splitmix(coin_seed ^ ((uint64_t)cstep << 32) ^ (uint64_t)new_sid[c]) < thr;
-        }
-        if (go && S.per_step) go = per_step[cstep]++ < S.per_step;
-        if (go && S.per_parent) go = per_parent[x.input_state]++ < S.per_parent;
-        expanded[c] = go;
-    }
-
-    // renumber and restrict
-    Run T;
-    T.c = R.c;
-    const uint64_t ns2 = next_sid, ne2 = next_eid;
-    T.hash.resize(ns2); T.step.resize(ns2); T.creator.assign(ns2, 0xFFFFFFFFu);
-    T.soff.assign(ns2 + 1, 0);
-    T.eoff.assign(1, 0);
-    std::vector<uint64_t> ir2(ir.empty() ? 0 : ns2);
-    std::vector<uint32_t> rank2;
-    for (uint64_t s = 0; s < ns; ++s) {
-        if (!exists[s]) continue;
-        const uint32_t t = new_sid[s];
-        T.hash[t] = R.hash[s];
-        T.step[t] = R.step[s];
-        if (R.creator[s] < ne) T.creator[t] = new_eid[R.creator[s]];
-        if (!ir.empty()) ir2[t] = ir[s];
-        for (uint64_t ed = R.soff[s]; ed < R.soff[s + 1]; ++ed) {
-            for (uint32_t x = R.eoff[ed]; x < R.eoff[ed + 1]; ++x) T.verts.push_back(R.verts[x]);
-            T.eoff.push_back((uint32_t)T.verts.size());
-            if (!rank.empty()) rank2.push_back(rank[ed]);
-        }
-        T.soff[t + 1] = T.eoff.size() - 1;
-    }
-    T.events.resize(ne2);
-    T.coff.assign(1, 0);
-    for (uint64_t e = 0; e < ne; ++e) {
-        if (!alive[e]) continue;
-        hgx_event x = R.events[e];
-        x.input_state = new_sid[x.input_state];
-        x.output_state = new_sid[x.output_state];
-        T.events[new_eid[e]] = x;
-        for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k) T.cpos.push_back(R.cpos[k]);
-        T.coff.push_back((uint32_t)T.cpos.size());
-    }
-    for (size_t k = 0; k < R.causal.size(); ++k) {
-        const hgx_pair& q = R.causal[k];
-        if (q.a >= ne || q.b >= ne || !alive[q.a] || !alive[q.b]) continue;
-        T.causal.push_back(hgx_pair{new_eid[q.a], new_eid[q.b]});
-        T.causal_tr.push_back(R.causal_tr[k]);
-    }
-    for (const hgx_pair& q : R.branchial) {
-        if (q.a >= ne || q.b >= ne || !alive[q.a] || !alive[q.b]) continue;
-        T.branchial.push_back(hgx_pair{new_eid[q.a], new_eid[q.b]});
-    }
-    T.c.raw_states = ns2;
-    T.c.events = ne2;
-    T.c.causal_pairs = T.causal.size();
-    T.c.branchial_pairs = T.branchial.size();
-    uint64_t tr = 0;
-    for (uint8_t f : T.causal_tr) tr += f != 0;
-    T.c.causal_pairs_tr = tr;
-    {
-        std::unordered_map<uint64_t, int> d;
-        uint64_t fp = 0;
-        for (uint64_t h : T.hash) if (d.emplace(h, 1).second) fp += mix64(h);
-        T.c.canonical_states = d.size();
-        T.c.fingerprint = fp;
-    }
-    R = std::move(T);
-    ir = std::move(ir2);
-    rank = std::move(rank2);
-}
-
 static Value evolve(hgx_ctx* ctx, const Job& j) {
     const std::vector<uint8_t> prog = program(j);
     std::vector<uint32_t> eoff, verts, roff;
@@ -627,9 +449,9 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (j.steps > HGX_MAX_STEPS) throw JobError("Steps above " + std::to_string(HGX_MAX_STEPS));
     uint32_t flags = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL | HGX_REC_RELATIONS;
     // state edges: for the output, and for the IR hash (Full's grouping, CanonicalHash)
-    const bool need_ir = j.full || (j.inc_hashes && j.inc_states) || j.ev_keys || j.smp.transitions();
-    const bool need_rank = (j.ev_keys & (16 | 32)) != 0 || j.smp.transitions();
-    if (j.inc_states || j.inc_events || need_ir || j.automatic || j.smp.active()) flags |= HGX_REC_STATES;
+    const bool need_ir = j.full || (j.inc_hashes && j.inc_states) || j.ev_keys;
+    const bool need_rank = (j.ev_keys & (16 | 32)) != 0;
+    if (j.inc_states || j.inc_events || need_ir || j.automatic) flags |= HGX_REC_STATES;
     if (j.tr) flags |= HGX_OPT_TR;
     hgx_run_desc d{};
     d.rules = prog.data(); d.rules_len = (uint32_t)prog.size(); d.n_rules = (uint32_t)j.rules.size();
@@ -637,6 +459,19 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     d.n_edges = (uint32_t)(eoff.size() - 1);
     d.root_offsets = roff.data(); d.n_roots = n_roots;
     d.steps = (uint32_t)j.steps; d.state_cap = 0; d.flags = flags;
+    // sampling and caps: decided by the engine generation by generation (include/hgx.h)
+    hgx_sampling sp{};
+    if (j.smp.active()) {
+        sp.transition_rate = j.smp.rate;
+        sp.rule_weights = j.smp.weights.empty() ? nullptr : j.smp.weights.data();
+        sp.n_rule_weights = (uint32_t)j.smp.weights.size();
+        sp.seed = j.smp.seed;
+        sp.exploration_probability = j.smp.explore_p;
+        sp.max_states_per_step = j.smp.per_step;
+        sp.max_successor_states_per_parent = j.smp.per_parent;
+        sp.matches_per_state_rule = j.smp.per_state_rule;
+        d.sampling = &sp;
+    }
     hgx_result* res = nullptr;
     check(hgx_evolve_batch(ctx, &d, 1, &res), ctx, "evolution");
     Run R;
@@ -648,7 +483,7 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
         char why[512] = {0};
         if (hgx_result_status(res, why, sizeof(why)) == HGX_ERR_CAPACITY) warns.emplace_back("CapacityExceeded", why);
     }
-    uint64_t ns = R.c.raw_states, ne = R.c.events;
+    const uint64_t ns = R.c.raw_states, ne = R.c.events;
     R.hash.resize(ns); R.step.resize(ns); R.creator.resize(ns);
     check(hgx_result_states(res, R.hash.data(), R.step.data(), R.creator.data(), ns), ctx, "states");
     R.events.resize(ne);
@@ -677,11 +512,6 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (need_ir && ns)
         check(hgx_ir_rank_batch(ctx, R.soff.data(), (uint32_t)ns, R.eoff.data(), R.verts.empty() ? nullptr : R.verts.data(),
                                 ir.data(), need_rank ? rank.data() : nullptr), ctx, "canonical hashes");
-    if (j.smp.active()) {
-        apply_sampling(j, R, ir, rank, n_roots);
-        ns = R.c.raw_states;
-        ne = R.c.events;
-    }
     std::vector<int64_t> rep(ns);
     std::vector<uint64_t> reps;                               // class representatives, in id order
     if (j.automatic) {

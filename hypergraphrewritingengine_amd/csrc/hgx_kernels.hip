@@ -396,7 +396,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
         const uint32_t ne = a.P.ne[s];
         const uint64_t eo = a.P.eoff[s];
         const uint32_t nvid = a.P.nextv[s];                     // labels are < nvid
-        const uint8_t active = a.active[run];
+        const uint8_t active = a.active[run] && (!a.smask || a.smask[s]);
         const uint64_t budget = EMIT ? 0 : (uint64_t)a.budget[run];
         // the run's tally moves under other teams: one read, shared, so the team agrees
         const uint64_t tally = EMIT ? 0 : tm.bcast(tm.leader() ? (uint64_t)a.run_acc[run] : 0ull);

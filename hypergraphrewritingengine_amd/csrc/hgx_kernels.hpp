@@ -46,6 +46,9 @@ struct MatchArgs {
     // of rule r at 2 * eoff[s] + r * ne[s]; 0xFFFF means "at least that many, join again".
     // Null: every seed is counted in the emit pass too.
     uint16_t* seedcnt;
+    // Optional [P.n] per-state flag: 0 = the state is not expanded (a sampled request's
+    // exploration decisions), so it has no matches. Null: every state of an active run is.
+    const uint8_t* smask;
 };
 
 struct ExpandArgs {

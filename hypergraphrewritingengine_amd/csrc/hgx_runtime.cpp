@@ -6,6 +6,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <random>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -277,6 +278,29 @@ RunHost parse_run(const hgx_run_desc* d) {
     RunHost h;
     parse_rules(d->rules, d->rules_len, d->n_rules, h.rules);
     h.steps = d->steps; h.cap = d->state_cap; h.flags = d->flags;
+    if (const hgx_sampling* sp = d->sampling) {
+        SampleHost& m = h.smp;
+        if (sp->n_rule_weights && !sp->rule_weights) throw Error{HGX_ERR_INVALID, "rule weights missing"};
+        m.rate = sp->transition_rate;
+        m.weights.assign(sp->rule_weights, sp->rule_weights + sp->n_rule_weights);
+        m.seed = sp->seed;
+        m.per_step = sp->max_states_per_step;
+        m.per_parent = sp->max_successor_states_per_parent;
+        m.per_state_rule = sp->matches_per_state_rule;
+        float p = (float)sp->exploration_probability;
+        if (!(p > 0.0f)) p = 0.0f;
+        if (p > 1.0f) p = 1.0f;
+        m.thr = p >= 1.0f ? 0xFFFFFFFFu : p <= 0.0f ? 0u : (uint32_t)((double)p * 4294967296.0);
+        m.coin_seed = m.seed;
+        if (m.coin_seed == 0 && m.thr != 0xFFFFFFFFu) {
+            std::random_device rd;
+            m.coin_seed = ((uint64_t)rd() << 32) | rd();
+            if (m.coin_seed == 0) m.coin_seed = 0xA5A5A5A5A5A5A5A5ull;
+        }
+        m.on = m.transitions() || m.frontier();
+        if (m.on && h.cap)
+            throw Error{HGX_ERR_UNSUPPORTED, "a state cap together with sampling (the cap counts states before thinning)"};
+    }
     if (d->n_edges && (!d->edge_offsets || !d->vertices)) throw Error{HGX_ERR_INVALID, "null state arrays"};
     const uint32_t n_roots = d->root_offsets ? d->n_roots : 1;
     if (d->root_offsets && (d->n_roots == 0 || d->root_offsets[0] != 0 || d->root_offsets[d->n_roots] != d->n_edges))

@@ -128,8 +128,52 @@ struct StagedSrc {
     uint32_t A = 0;
 };
 
+// A request's sampling and capping (hgx_sampling), resolved: the exploration coin's threshold
+// as the reference encodes it (a float probability scaled to 2^32, gpu/src/evolve.cu:271-285)
+// and its seed (a random one when 0 and a coin is flipped, :351-356).
+struct SampleHost {
+    bool on = false;
+    double rate = 1.0;
+    std::vector<double> weights;
+    uint64_t seed = 0, coin_seed = 0;
+    uint32_t thr = 0xFFFFFFFFu;
+    uint64_t per_step = 0, per_parent = 0, per_state_rule = 0;
+    bool draws() const { return rate < 1.0 || !weights.empty(); }      // rewrite.cu:270
+    bool transitions() const { return draws() || per_state_rule != 0; }
+    bool frontier() const { return thr != 0xFFFFFFFFu || per_step || per_parent; }
+    double rate_for(uint32_t rule) const {                                // sampling_core.hpp:28-34
+        const double r = rate * (rule < weights.size() ? weights[rule] : 1.0);
+        return r < 0.0 ? 0.0 : (r > 1.0 ? 1.0 : r);
+    }
+};
+
+// The transition draw: splitmix64 of the key against the seed (mixed by multiplication, so seed
+// 0 still draws), compared in [0, 1) through its top 53 bits (sampling_core.hpp:42-57).
+inline bool transition_survives(uint64_t key, uint64_t seed, double rate) {
+    if (rate >= 1.0) return true;
+    if (rate <= 0.0) return false;
+    const uint64_t x = splitmix(key ^ (seed * 0x9E3779B97F4A7C15ULL));
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0) < rate;
+}
+// The order a per-(state, rule) cap keeps its k in: a second mix of the key, uncorrelated with
+// the draw (sampling_core.hpp:66-71).
+inline uint64_t transition_rank(uint64_t key, uint64_t seed) {
+    uint64_t x = key ^ (seed * 0x9E3779B97F4A7C15ULL) ^ 0xA5A5A5A5A5A5A5A5ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+// The transition key: the event signature over the input state's exact hash, the rule and the
+// consumed edges' canonical ranks in pattern order (EVENT_SIG_TRANSITION, event_core.hpp:55-84).
+inline uint64_t transition_key(uint64_t input_hash, uint32_t rule, const uint32_t* ranks, uint32_t n) {
+    uint64_t sig = fnv(fnv(FNV_OFFSET, input_hash), (uint64_t)(uint16_t)rule);
+    for (uint32_t i = 0; i < n; ++i) sig = fnv(sig, ranks[i]);
+    return sig == 0 || sig == FNV_OFFSET ? 1 : sig;
+}
+
 struct RunHost {
     StagedSrc staged;
+    SampleHost smp;
     std::vector<DRule> rules;
     // Initial states, flat: root k holds edges [root_eb[k], root_eb[k+1]); edge e holds
     // verts[edge_vb[e] .. edge_vb[e+1]), vertices renumbered per root by first appearance.

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import itertools
 from dataclasses import dataclass, field
-from typing import Iterable, Sequence
+from typing import Optional, Iterable, Sequence
 
 import numpy as np
 
@@ -134,6 +134,8 @@ class Request:
     state_cap: int = 0
     flags: int = 0x3
     multi_root: bool = False
+    # hgx_sampling as a dict: rate, weights, seed, explore_p, per_step, per_parent, per_state_rule
+    sampling: Optional[dict] = None
     _enc: dict = field(default_factory=dict, repr=False)
 
     def encoded(self):

@@ -74,6 +74,38 @@ enum {
  * Limits: 1 <= n_lhs <= 16, 0 <= n_rhs <= 16, 1 <= arity <= 16.
  */
 
+/* Sampling and capping of one request: the options TransitionRate, RuleWeights,
+ * MatchesPerStateRule, ExplorationProbability, MaxStatesPerStep, MaxSuccessorStatesPerParent
+ * and RandomSeed (paclet_source/ffi_job.hpp:48-72, as the GPU backend takes them,
+ * hg_gpu_backend.cpp:152-160), with the GPU engine's decisions, taken generation by generation:
+ *   - a transition (input state, rule, consumed edges) is keyed by the event signature over the
+ *     input state's exact (IR) hash, the rule and the consumed edges' canonical ranks in pattern
+ *     order (event_core.hpp:55-84); matches_per_state_rule keeps the k matches of each
+ *     (state, rule) with the smallest transition rank (ties in event order), then each kept
+ *     match survives with probability transition_rate x rule_weights[rule] (clamped to [0, 1];
+ *     rules past the weights weigh 1), by a draw keyed on the transition and the seed
+ *     (common/include/hgcommon/sampling_core.hpp). A transition that does not survive makes
+ *     no event, no state and no relation;
+ *   - a new state below the last step is expanded when the exploration coin passes
+ *     (exploration_probability, keyed on the seed, its step and its state id; a seed of 0 with
+ *     a coin to flip is replaced by a random one) and then, counted in event order, while its
+ *     step has expanded fewer than max_states_per_step states and its parent fewer than
+ *     max_successor_states_per_parent (0 = no bound) (gpu/include/hg_gpu/exploration.hpp:66-122).
+ * Transition decisions need the exact hash and edge ranks of every expanded state: the host
+ * computes them (the IR canonicalizer) between generations, so a sampled request costs a host
+ * round trip per generation. Sampled requests are not recorded into graphs. */
+typedef struct hgx_sampling {
+    double          transition_rate;          /* 1: every transition                     */
+    const double*   rule_weights;             /* per-rule multipliers, or NULL            */
+    uint32_t        n_rule_weights;
+    uint32_t        reserved;
+    uint64_t        seed;
+    double          exploration_probability;  /* 1: every new state is expanded          */
+    uint64_t        max_states_per_step;      /* 0: no bound                             */
+    uint64_t        max_successor_states_per_parent;
+    uint64_t        matches_per_state_rule;
+} hgx_sampling;
+
 /* One evolution request. Vertex ids of the initial state(s) are arbitrary uint32; each
  * root is renumbered densely by first appearance, as the FFI does
  * (paclet_source/hypergraph_ffi.cpp:599-636). */
@@ -91,6 +123,7 @@ typedef struct hgx_run_desc {
                                       largest step s <= steps whose cumulative raw
                                       state count is <= state_cap.                    */
     uint32_t        flags;         /* HGX_REC_* | HGX_OPT_TR                          */
+    const hgx_sampling* sampling;  /* NULL: the full multiway evolution               */
 } hgx_run_desc;
 
 /* Per-request totals. Field meanings follow the reference's counters:

@@ -28,12 +28,28 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_struct_layouts_match_header():
-    # 3 pointers + 2 x u32, a pointer, u32 + pad, a pointer, 4 x u32 (LP64)
-    assert C.sizeof(abi.RunDesc) == 64
-    assert C.sizeof(abi.Counts) == 9 * 8 + 2 * 4
-    assert C.sizeof(abi.Event) == 16
-    assert C.sizeof(abi.Pair) == 8
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors against the header itself: sizes and field offsets as the C compiler
+    lays them out."""
+    import subprocess
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    structs = {"hgx_run_desc": abi.RunDesc, "hgx_sampling": abi.Sampling, "hgx_counts": abi.Counts,
+               "hgx_event": abi.Event, "hgx_pair": abi.Pair}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hgx.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", inc, str(src), "-o", str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
 
 
 def test_footprint_estimate_is_monotone():

@@ -1,4 +1,6 @@
-"""GPU: the sampling and capping options through the job envelope (hgx_run_wxf).
+"""GPU: the sampling and capping options, through the job envelope (hgx_run_wxf) and through the
+C ABI (hgx_run_desc.sampling) in batches and the served path; the engine takes the decisions
+generation by generation and expands only what survives.
 
 TransitionRate, RuleWeights and MatchesPerStateRule thin transitions by a draw keyed on the
 transition's isomorphism class (sampling_core.hpp, event_core.hpp:55); ExplorationProbability,
@@ -151,3 +153,69 @@ def test_sampling_with_state_canonicalization(ctx):
     states, _, _, _ = _decoded(raw)
     assert full["NumStates"] == len(set(int(h) for h in hgx.ir_hash(states)))
     assert full["NumEvents"] == raw["NumEvents"]
+
+
+# ---- the engine's sampling through the C ABI (hgx_run_desc.sampling) ----
+
+def _request(case, kw):
+    from hypergraphrewritingengine_amd.rules import Request, rule_from_lists
+    rules, init, steps = CASES[case]
+    return Request([rule_from_lists(l, r, k) for k, (l, r) in enumerate(rules)], [init], steps,
+                   flags=0x1 | 0x2 | 0x4 | 0x8, multi_root=True, sampling=dict(kw) if kw else None)
+
+
+def _arrays(res):
+    so, eo, vv = res.state_edges()
+    states = [[[int(x) for x in vv[eo[e]:eo[e + 1]]] for e in range(int(so[s]), int(so[s + 1]))]
+              for s in range(len(so) - 1)]
+    off, pos = res.consumed()
+    events = [(int(a), int(b), int(r), tuple(int(x) for x in pos[off[e]:off[e + 1]]))
+              for e, (a, b, r, _) in enumerate(res.events())]
+    cp, _ = res.causal()
+    return states, events, sorted((int(a), int(b)) for a, b in cp), sorted((int(a), int(b)) for a, b in res.branchial())
+
+
+def _want(case, kw):
+    rules, init, steps = CASES[case]
+    st, ev, ca, br = SM.sampled(rules, init, steps, **kw)
+    return st, ev, sorted((a, b) for a, b, _ in ca), sorted(br)
+
+
+MIX = [("split", dict(rate=0.75, seed=1)), ("split", {}), ("split_flip", dict(per_state_rule=2, seed=0x5EED)),
+       ("path", dict(rate=0.75, explore_p=0.7, per_step=3, per_parent=2, seed=11)), ("path", {}),
+       ("split_flip", dict(weights=[1.0, 0.0], seed=7)), ("split", dict(explore_p=0.5, seed=2)),
+       ("split", dict(per_step=2)), ("split_flip", dict(rate=0.5, seed=2))]
+
+
+def test_engine_sampling_in_a_mixed_batch(ctx):
+    """Sampled and unsampled requests in one batch: the thinned match list is compacted across
+    requests, and every result equals its own restatement (the unsampled ones the full
+    evolution)."""
+    res = ctx.evolve_batch([_request(c, kw) for c, kw in MIX])
+    for i, (c, kw) in enumerate(MIX):
+        got = _arrays(res[i])
+        assert got == _want(c, kw), (c, kw)
+        assert res[i].counts["raw_states"] == len(got[0]) and res[i].counts["events"] == len(got[1])
+
+
+def test_engine_sampling_served(ctx):
+    """The same requests through the served path, arriving a few per generation."""
+    ids, got = {}, {}
+    pending = list(range(len(MIX)))
+    while pending or ids:
+        for _ in range(2):
+            if pending:
+                i = pending.pop(0)
+                ids[ctx.submit(_request(*MIX[i]))] = i
+        ctx.serve_step()
+        for rid, r in ctx.drain().items():
+            got[ids.pop(rid)] = r
+    for i, (c, kw) in enumerate(MIX):
+        assert _arrays(got[i]) == _want(c, kw), (c, kw)
+
+
+def test_sampled_batch_is_not_captured(ctx):
+    batch = ctx.prepare([_request("split", dict(rate=0.75, seed=1)), _request("split", {})])
+    with pytest.raises(hgx.HGXError, match="not recorded into graphs"):
+        ctx.capture(batch)
+    batch.close()
