@@ -449,8 +449,19 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                                                    t, mc.fwd_n != 0) + 15u) & ~15u;
                     };
                     const int kc = (int)(&x - cls.data());
-                    const uint32_t team = pick_team(lpw(1), ctx_->join_team ? ctx_->join_team : ctx_->tune.join_team[kc]);
+                    uint32_t team = pick_team(lpw(1), ctx_->join_team ? ctx_->join_team : ctx_->tune.join_team[kc]);
                     mc.lds_per_wave = lpw(team);
+                    uint32_t gws_grid = 0;
+                    if (lpw(8) > ctx_->max_lds) {
+                        // states whose workspace does not fit LDS: eight-wave teams over
+                        // per-workgroup slices of global memory (at most 2 per CU, 4 GiB in all)
+                        team = 8;
+                        mc.lds_per_wave = lpw(8);
+                        uint32_t cap = std::max<uint32_t>(8u, std::min<uint64_t>(2ull * ctx_->num_cu,
+                                                                                 (4ull << 30) / mc.lds_per_wave));
+                        gws_grid = launch_grid((uint32_t)x.n, 1, cap);
+                        mc.scratch = (uint8_t*)tmp.alloc((size_t)gws_grid * mc.lds_per_wave);
+                    }
                     if (x.n != n) {
                         uint32_t* d_r = dalloc<uint32_t>(tmp, x.rb.size() + x.rpre.size());
                         up(d_r, x.rb.data(), x.rb.size(), st);
@@ -458,12 +469,12 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                         mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
                         mc.n_items = (uint32_t)x.n;
                     }
-                    const uint32_t wpb_m = waves_per_block(mc.lds_per_wave, ctx_->max_lds);   // checks the fit
+                    const uint32_t wpb_m = mc.scratch ? 1u : waves_per_block(mc.lds_per_wave, ctx_->max_lds);
                     // up to 16 workgroups per CU (about three times what stays resident): measured
                     // best of 5, 8, 16, 32 and 64 on the batched workload (0.8 ms per pass over 8)
+                    const uint32_t gcap = mc.scratch ? gws_grid : ctx_->num_cu * ctx_->tune.join_grid[kc];
                     timed(ctx_, st, name, (uint64_t)((double)bytes * x.n / n), x.n,
-                          [&] { return launch_match(mc, emit, ctx_->num_cu * ctx_->tune.join_grid[kc], team == 1 ? wpb_m : 1,
-                                                    team, st); }, kc);
+                          [&] { return launch_match(mc, emit, gcap, team == 1 ? wpb_m : 1, team, st); }, kc);
                 }
             };
             HIP_TRY(launch_fill(cnt, 0, sizeof(uint32_t) * (n + 1), st));

@@ -347,12 +347,16 @@ uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uin
 // whose workspace would otherwise leave a CU one or two waves). The one-wave passes are held to
 // five waves per SIMD (96 registers; the emit pass measured 2.56 -> 2.35 ms per batched pass,
 // where its unbounded 107 allow four); a bound of six spills in the join and measured slower.
-template <bool EMIT, class JoinLane, uint32_t W>
+// GWS: the workspace is a per-workgroup slice of global memory (a.scratch) -- the same code for
+// states too large for LDS, with W = 8.
+template <bool EMIT, class JoinLane, uint32_t W, bool GWS = false>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match(MatchArgs a) {
     extern __shared__ uint8_t lds[];
     constexpr uint32_t N = Team<W>::N;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    uint8_t* p = lds + (W == 1 ? (size_t)w * a.lds_per_wave : 0);
+    uint8_t* p;
+    if constexpr (GWS) p = a.scratch + (size_t)blockIdx.x * a.lds_per_wave;
+    else p = lds + (W == 1 ? (size_t)w * a.lds_per_wave : 0);
     const Team<W> tm((uint64_t*)(p + a.lds_per_wave - 64));      // the workspace's last 64 bytes
     const uint32_t A = a.P.A;
     uint32_t* sv = (uint32_t*)carve(p, a.max_ne * A * 4);      // vertices
@@ -677,13 +681,18 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
 // hypergraph/src/rewriter.cpp:11-205, with hgcommon rewrite semantics rewrite_core.hpp:53-87).
 // ------------------------------------------------------------------------------------------
 
+// A slot is read before it is claimed: most children repeat a hash already in the map, and a
+// plain load that finds it (or another key: keys are only ever written once, 0 -> key) settles
+// the probe without an atomic; only an empty-looking slot is claimed by compare-and-swap, whose
+// result is authoritative if the load was stale.
 __device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
     if (h == 0) return atomicExch(&run->zero_seen, 1u) == 0u;
     const uint32_t mask = run->map_mask;
     uint32_t i = (uint32_t)mix64(h) & mask;
     while (true) {
-        const uint64_t old = atomicCAS((unsigned long long*)&run->map_keys[i], 0ull,
-                                       (unsigned long long)h);
+        uint64_t old = __atomic_load_n(&run->map_keys[i], __ATOMIC_RELAXED);
+        if (old == 0)
+            old = atomicCAS((unsigned long long*)&run->map_keys[i], 0ull, (unsigned long long)h);
         if (old == 0) { run->map_vals[i] = sid; return true; }
         if (old == h) return false;
         i = (i + 1) & mask;
@@ -1404,6 +1413,18 @@ static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
 template <uint32_t W>
 static void launch_match_team(const MatchArgs& a, bool emit, uint32_t grid, uint32_t wpb, hipStream_t st) {
     const uint32_t threads = W == 1 ? 64 * wpb : 64 * W;
+    if constexpr (W == 8) {
+        if (a.scratch) {
+            if (a.small) {
+                if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneLds, 8, true>), dim3(grid), dim3(threads), 0, st, a);
+                else hipLaunchKernelGGL((k_match<false, JoinLaneLds, 8, true>), dim3(grid), dim3(threads), 0, st, a);
+            } else {
+                if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneBig, 8, true>), dim3(grid), dim3(threads), 0, st, a);
+                else hipLaunchKernelGGL((k_match<false, JoinLaneBig, 8, true>), dim3(grid), dim3(threads), 0, st, a);
+            }
+            return;
+        }
+    }
     const size_t lds = (size_t)a.lds_per_wave * (W == 1 ? wpb : 1);
     if (a.small) {
         if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneLds, W>), dim3(grid), dim3(threads), lds, st, a);
@@ -1418,6 +1439,7 @@ hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32
                         hipStream_t st) {
     const uint32_t items = a.nrange ? a.n_items : a.P.n;
     if (!items) return hipSuccess;
+    if (a.scratch && team != 8) return hipErrorInvalidValue;     // global workspaces: 8-wave teams
     const uint32_t grid = grid_for(items, team == 1 ? wpb : 1, grid_cap);
     switch (team) {
         case 1: launch_match_team<1>(a, emit, grid, wpb, st); break;

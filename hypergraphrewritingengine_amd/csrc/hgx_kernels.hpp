@@ -49,6 +49,9 @@ struct MatchArgs {
     // Optional [P.n] per-state flag: 0 = the state is not expanded (a sampled request's
     // exploration decisions), so it has no matches. Null: every state of an active run is.
     const uint8_t* smask;
+    // Global-memory workspaces, lds_per_wave bytes per workgroup (eight-wave teams), for states
+    // whose workspace does not fit LDS. Null: the workspaces are in LDS.
+    uint8_t* scratch;
 };
 
 struct ExpandArgs {

@@ -366,6 +366,16 @@ RunHost parse_run(const hgx_run_desc* d) {
     for (auto& R : h.rules)
         for (int e = 0; e < R.n_rhs; ++e) { amin = std::min<uint32_t>(amin, R.ra[e]); amax = std::max<uint32_t>(amax, R.ra[e]); }
     h.uni_arity = (amin == amax && amax >= 2 && amax <= 4) ? amax : 0u;
+    // edge positions (match list) and local vertex ids (hash workspaces) are 16-bit: refused
+    // before any kernel sees a state that could exceed them (one larger state's matches:
+    // hgx_match_count)
+    if ((uint64_t)h.max_root_nv + (uint64_t)h.steps * h.max_new > 65535)
+        throw Error{HGX_ERR_UNSUPPORTED, "a state of this request can exceed 65535 vertices (16-bit local "
+                                         "vertex ids in the hash workspaces)"};
+    if ((uint64_t)h.max_root_ne + (uint64_t)h.steps * h.max_gain > 65535)
+        throw Error{HGX_ERR_UNSUPPORTED, "a state of this request can exceed 65535 edges (largest initial state " +
+                                             std::to_string(h.max_root_ne) + " edges, " + std::to_string(h.steps) +
+                                             " steps); one such state's matches: hgx_match_count"};
     return h;
 }
 
@@ -434,7 +444,6 @@ uint32_t waves_per_block(uint32_t lds_per_wave, uint32_t max_lds) {
     return w;
 }
 
-// After a stream synchronisation: fold finished launches into the per-kernel totals.
 uint32_t place_workspace(Arena& ar, uint32_t per_wave, uint32_t items, uint32_t max_lds, uint32_t grid_cap,
                          uint8_t** scratch) {
     *scratch = nullptr;
@@ -445,6 +454,7 @@ uint32_t place_workspace(Arena& ar, uint32_t per_wave, uint32_t items, uint32_t 
     return wpb;
 }
 
+// After a stream synchronisation: fold finished launches into the per-kernel totals.
 void prof_resolve(hgx_ctx* ctx) {
     for (auto& p : ctx->prof.pend) {
         float ms = 0;
