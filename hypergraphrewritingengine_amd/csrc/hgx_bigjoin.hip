@@ -56,6 +56,35 @@ __global__ void k_index_keys(const uint32_t* ev, const uint8_t* ar, uint32_t m, 
     val[s] = e;
 }
 
+// The key of a vertex tuple (arity, vertices): equal tuples have equal keys; the join verifies
+// every vertex of a candidate, so a collision costs a rejected candidate, never a wrong match.
+__device__ __forceinline__ uint64_t tuple_key(uint32_t a, const uint32_t* x) {
+    uint64_t h = 0x9E3779B97F4A7C15ULL ^ a;
+    for (uint32_t i = 0; i < a; ++i) h = mix64(h ^ ((uint64_t)x[i] + 0x632BE59BD9B4E019ULL * (i + 1)));
+    return h ? h : 1;
+}
+
+// One entry per run of equal keys in the sorted tuple index: (key -> first position, length).
+__global__ void k_tuple_table(const uint64_t* tkey, uint32_t m, uint64_t* slot, uint2* run, uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t k = tkey[i];
+    if (i && tkey[i - 1] == k) return;                     // not a run start
+    uint32_t n = 1;
+    while (i + n < m && tkey[i + n] == k) ++n;
+    uint32_t h = (uint32_t)mix64(k) & mask;
+    while (atomicCAS((unsigned long long*)&slot[h], 0ull, (unsigned long long)k) != 0ull) h = (h + 1) & mask;
+    run[h] = make_uint2(i, n);
+}
+
+__global__ void k_tuple_keys(const uint32_t* ev, const uint8_t* ar, uint32_t m, uint32_t A, uint64_t* key,
+                             uint32_t* val) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    key[e] = tuple_key(ar[e], ev + (size_t)e * A);
+    val[e] = e;
+}
+
 // Zero fill: a kernel, so the join's stream holds kernels and copies only.
 template <class T>
 __global__ void k_bj_zero(T* p, uint64_t n) {
@@ -92,6 +121,26 @@ __global__ void k_bj_range(BJView v, uint32_t d, const uint32_t* F, uint64_t P, 
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P) return;
     const DRule& R = v.R;
+    if ((v.full >> d) & 1u) {
+        // a closing edge: the run of edges whose tuple key is the bound tuple's
+        const uint32_t a = R.la[R.order[d]];
+        uint32_t x[16];
+        for (uint32_t i = 0; i < a; ++i) {
+            const uint32_t s = R.jsrc[d][i];
+            x[i] = v.ev[(size_t)F[q * d + (s >> 4)] * v.A + (s & 15u)];
+        }
+        const uint64_t k = tuple_key(a, x);
+        uint32_t h = (uint32_t)mix64(k) & v.tmask;
+        uint32_t l0 = 0, n = 0;
+        for (uint32_t probe = 0; probe <= v.tmask; ++probe, h = (h + 1) & v.tmask) {
+            const uint64_t s = v.tslot[h];
+            if (s == k) { const uint2 r = v.trun[h]; l0 = r.x; n = r.y; break; }
+            if (s == 0) break;
+        }
+        lo[q] = l0;
+        cnt[q] = n;
+        return;
+    }
     const uint32_t pv = R.piv[d];
     if (pv == 0xFFu) { lo[q] = 0; cnt[q] = v.m; return; }
     const uint32_t s = R.jsrc[d][pv];                      // bound at an earlier depth
@@ -194,7 +243,8 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
     __shared__ uint64_t sq[2];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
     const uint32_t d = L.d;
-    const uint32_t pv = v.R.piv[d];
+    const bool full = (v.full >> d) & 1u;                 // candidates from the tuple index
+    const uint32_t pv = full ? 0xFFFFFFFEu : v.R.piv[d];  // (a closing edge checks every slot)
     const uint32_t pa = v.R.la[v.R.order[d]];
     const uint64_t T = L.start[L.P];
     const uint64_t nreal = (T + BJ_TILE - 1) / BJ_TILE;
@@ -246,14 +296,14 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_items(BJView v, BJLevel L) {
                     const uint64_t r = bj_owner(sst, nq, t);
                     q = q0 + r;
                     j = t - sst[r];
-                    e = pv == 0xFFu ? (uint32_t)j : v.idx[slo[r] + j];
+                    e = full ? v.tidx[slo[r] + j] : pv == 0xFFu ? (uint32_t)j : v.idx[slo[r] + j];
                     part = sparts ? spart + r * d : L.F + q * d;
                     if (sparts) ok = bj_fits_staged(v, d, pv, part, swant + r * pa, e);
                     else ok = bj_fits(v, d, part, e);
                 } else {
                     q = bj_owner(L.start, L.P, t);
                     j = t - L.start[q];
-                    e = pv == 0xFFu ? (uint32_t)j : v.idx[L.lo[q] + j];
+                    e = full ? v.tidx[L.lo[q] + j] : pv == 0xFFu ? (uint32_t)j : v.idx[L.lo[q] + j];
                     part = L.F + q * d;
                     ok = bj_fits(v, d, part, e);
                 }
@@ -414,6 +464,44 @@ static void bj_index(BJMem& mem, BJView& v, uint32_t nv, hipStream_t st, uint64_
     v.idx = idx;
     // read the state once, write and read the pairs through the sort, write the offsets
     *alg_bytes += (uint64_t)m * (4ull * A + 1) + slots * 8ull * 3 + (nkeys + 1) * 16ull;
+    // closing edges: the depths whose pattern edge is bound in every slot by earlier depths
+    const DRule& R = v.R;
+    v.full = 0;
+    for (uint32_t d = 1; d < R.n_lhs; ++d) {
+        const uint32_t a = R.la[R.order[d]];
+        bool all = true;
+        for (uint32_t i = 0; i < a; ++i) {
+            const uint32_t src = R.jsrc[d][i];
+            all &= src != 0xFFu && (src >> 4) < d;
+        }
+        if (all) v.full |= 1u << d;
+    }
+    v.tkey = nullptr;
+    v.tidx = nullptr;
+    if (v.full && m) {
+        uint64_t* tk_in = bget<uint64_t>(mem, m);
+        uint32_t* tv_in = bget<uint32_t>(mem, m);
+        uint64_t* tk = bget<uint64_t>(mem, m);
+        uint32_t* tv = bget<uint32_t>(mem, m);
+        hipLaunchKernelGGL(k_tuple_keys, dim3(blocks_for(m, 256)), dim3(256), 0, st, v.ev, v.ar, m, A, tk_in, tv_in);
+        size_t tmp = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, tk_in, tk, tv_in, tv, (int)m, 0, 64, st));
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(mem.get(tmp), tmp, tk_in, tk, tv_in, tv, (int)m, 0, 64, st));
+        HIP_TRY(hipGetLastError());
+        v.tkey = tk;
+        v.tidx = tv;
+        uint32_t cap = 1;
+        while (cap < 2 * m) cap <<= 1;                      // load factor <= 1/2
+        v.tslot = bget<uint64_t>(mem, cap);
+        v.trun = bget<uint2>(mem, cap);
+        v.tmask = cap - 1;
+        hipLaunchKernelGGL(k_bj_zero<uint64_t>, dim3(blocks_for(cap, 256)), dim3(256), 0, st, v.tslot, (uint64_t)cap);
+        hipLaunchKernelGGL(k_tuple_table, dim3(blocks_for(m, 256)), dim3(256), 0, st, tk, m, v.tslot, v.trun, v.tmask);
+        HIP_TRY(hipGetLastError());
+        *alg_bytes += (uint64_t)m * (4ull * A + 1) + (uint64_t)m * 12ull * 3 + (uint64_t)cap * 16ull;
+    } else {
+        v.full = 0;
+    }
 }
 
 // Extends the partials F (P of them, d edges each) by pattern edge order[d], and on to the last
@@ -433,9 +521,11 @@ static void bj_depth(BJRun& J, uint32_t d, const uint32_t* F, uint64_t P) {
     bj_scan(mem, cnt, start, P, J.st);
     // tiles to provision: the longest candidate list bounds every partial's; an exact count (one
     // round trip) only when that bound is loose beyond use
+    const bool full = (J.v.full >> d) & 1u;
     const uint64_t per = J.v.R.piv[d] == 0xFFu ? (uint64_t)J.v.m : std::max<uint64_t>(1, J.maxlist);
     uint64_t ntiles = (P * per + BJ_TILE - 1) / BJ_TILE;
-    if (P * per / std::max<uint64_t>(1, P) != per || ntiles > ((uint64_t)1 << 20)) {
+    // a closing edge has about one candidate per partial: the exact count, not the list bound
+    if (full || P * per / std::max<uint64_t>(1, P) != per || ntiles > ((uint64_t)1 << 20)) {
         const uint64_t T = read1(start + P, J.st);
         ntiles = (T + BJ_TILE - 1) / BJ_TILE;
     }
@@ -500,7 +590,7 @@ void bj_match(BJRun& J, uint32_t nv, float* ms_index, float* ms_join, bool timed
     if (!mem.arena) {
         // a fixed region: refuse a state whose index alone cannot fit before launching anything
         const uint64_t nkeys = (uint64_t)nv * J.v.A, slots = (uint64_t)J.v.m * J.v.A;
-        const uint64_t need = (nkeys + 1) * 16ull + slots * 16ull + (uint64_t)J.v.m * 5ull + (1ull << 20);
+        const uint64_t need = (nkeys + 1) * 16ull + slots * 16ull + (uint64_t)J.v.m * 93ull + (1ull << 20);
         if (need > mem.cap - mem.used)
             throw Error{HGX_ERR_CAPACITY, "a state of " + std::to_string(J.v.m) + " edges needs " +
                                               std::to_string(need >> 20) + " MiB for its index, more than the " +
@@ -573,7 +663,7 @@ extern "C" int hgx_match_count(hgx_ctx* ctx, const uint8_t* rules, uint32_t rule
         hgx_run_desc d{};
         d.rules = rules; d.rules_len = rules_len; d.n_rules = n_rules; d.steps = 0;
         d.edge_offsets = edge_offsets; d.vertices = vertices; d.n_edges = n_edges;
-        RunHost h = parse_run(&d);                     // validates rules and arities, renumbers
+        RunHost h = parse_run(&d, false);              // validates rules and arities, renumbers
         const DRule R = h.rules[rule];
         const uint32_t m = h.root_ne(0);
         uint32_t A = 1;

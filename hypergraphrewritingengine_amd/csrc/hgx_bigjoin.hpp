@@ -19,6 +19,18 @@ struct BJView {
     uint32_t m, A;
     const uint32_t* off;    // [nkeys + 1]
     const uint32_t* idx;    // [occurrences]
+    // Closing edges: a depth whose pattern edge has every vertex bound by earlier depths takes
+    // its candidates from the edges with exactly that vertex tuple -- the edges sorted by a
+    // 64-bit key of (arity, vertices), stably, so equal keys stay in edge order. Bit d of
+    // `full` marks such a depth (0: no tuple index).
+    const uint64_t* tkey;   // [m] sorted keys
+    const uint32_t* tidx;   // [m] edges in key order
+    uint32_t full;
+    // Each distinct key's run [lo, lo + cnt) of tidx, in an open-addressing table (key 0 empty;
+    // a zero key is stored as 1, which only widens its run's verification)
+    uint64_t* tslot;        // [tmask + 1] keys
+    uint2* trun;            // [tmask + 1] (lo, cnt)
+    uint32_t tmask;
 };
 
 // Device memory of one join: an Arena, or a fixed region (the allocation-free entry point)

@@ -271,7 +271,7 @@ void parse_rules(const uint8_t* p, uint32_t len, uint32_t n, std::vector<DRule>&
     if (k != len) throw Error{HGX_ERR_INVALID, "rule program has trailing bytes"};
 }
 
-RunHost parse_run(const hgx_run_desc* d) {
+RunHost parse_run(const hgx_run_desc* d, bool evolve) {
     if (!d) throw Error{HGX_ERR_INVALID, "null request"};
     if (!d->rules || d->n_rules == 0) throw Error{HGX_ERR_INVALID, "request has no rules"};
     if (d->steps > HGX_MAX_STEPS) throw Error{HGX_ERR_UNSUPPORTED, "steps above HGX_MAX_STEPS"};
@@ -369,10 +369,10 @@ RunHost parse_run(const hgx_run_desc* d) {
     // edge positions (match list) and local vertex ids (hash workspaces) are 16-bit: refused
     // before any kernel sees a state that could exceed them (one larger state's matches:
     // hgx_match_count)
-    if ((uint64_t)h.max_root_nv + (uint64_t)h.steps * h.max_new > 65535)
+    if (evolve && (uint64_t)h.max_root_nv + (uint64_t)h.steps * h.max_new > 65535)
         throw Error{HGX_ERR_UNSUPPORTED, "a state of this request can exceed 65535 vertices (16-bit local "
                                          "vertex ids in the hash workspaces)"};
-    if ((uint64_t)h.max_root_ne + (uint64_t)h.steps * h.max_gain > 65535)
+    if (evolve && (uint64_t)h.max_root_ne + (uint64_t)h.steps * h.max_gain > 65535)
         throw Error{HGX_ERR_UNSUPPORTED, "a state of this request can exceed 65535 edges (largest initial state " +
                                              std::to_string(h.max_root_ne) + " edges, " + std::to_string(h.steps) +
                                              " steps); one such state's matches: hgx_match_count"};

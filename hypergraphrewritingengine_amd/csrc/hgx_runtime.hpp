@@ -192,7 +192,9 @@ struct RunHost {
     bool root_uni = false;       // every root edge has one arity 2..4 (the roots alone)
     uint64_t root_edges = 0;
 };
-RunHost parse_run(const hgx_run_desc* d);
+// evolve = false: the request is one state for the large-state join, not an evolution (no
+// limit on its size)
+RunHost parse_run(const hgx_run_desc* d, bool evolve = true);
 // Looks up, once per device (current), the address of the allocation-free entry point's scratch.
 void noalloc_setup(int device);
 // The rule program (include/hgx.h) into device rules with the reference's join order.
