@@ -184,6 +184,25 @@ def config2_line(ctx, steps, cpu_steps, threads, no_cpu):
     return out
 
 
+def sampled_line(ctx, steps=8, rate=0.25, seed=1):
+    """The large evolution's rule thinned by a transition draw (TransitionRate, hgx_sampling):
+    eight steps, where the full evolution would hold about 10^10 states. The engine keys each
+    draw on the transition's exact hash and ranks (host IR of every frontier state) and expands
+    only the survivors."""
+    from hypergraphrewritingengine_amd import workloads as W
+    req = W.config2(steps=steps)
+    req.sampling = {"rate": rate, "seed": seed}
+    ctx.evolve(W.config2(steps=4)).close()
+    t = time.perf_counter()
+    r = ctx.evolve(req)
+    wall = time.perf_counter() - t
+    c = r.counts
+    r.close()
+    return {"steps": steps, "transition_rate": rate, "seed": seed, "wall_ms": round(wall * 1e3, 3),
+            "raw_states": c["raw_states"], "canonical_states": c["canonical_states"],
+            "steps_done": c["steps_done"], "raw_states_per_s": round(c["raw_states"] / wall, 1)}
+
+
 def config3_line(ctx, name, rule, edges, no_cpu):
     """All matches of one rule in one large random state (device CSR index + seeded join)."""
     ctx.match([rule], 0, edges)
@@ -361,7 +380,7 @@ def main():
     ap.add_argument("--serve-plain-seconds", type=float, default=8.0)
     ap.add_argument("--no-serve-plain", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the config 2 / 3 lines")
-    ap.add_argument("--others", default="config2,config3a,config3b,ir_audit",
+    ap.add_argument("--others", default="config2,config3a,config3b,sampled,ir_audit",
                     help="which of the other workloads to measure (comma list)")
     ap.add_argument("--config2-steps", type=int, default=6)
     ap.add_argument("--forward", type=int, default=-1, choices=(-1, 0, 1),
@@ -550,6 +569,9 @@ def main():
             rule, edges = mk()
             others[name] = config3_line(ctx, name, rule, edges, args.no_cpu)
             print(f"[bench] {name}: {others[name]}", file=sys.stderr, flush=True)
+    if "sampled" in want:
+        others["sampled"] = sampled_line(ctx)
+        print(f"[bench] sampled: {others['sampled']}", file=sys.stderr, flush=True)
     if "ir_audit" in want:
         others["ir_audit"] = ir_audit_line(ctx, 256, args.no_cpu)
         print(f"[bench] ir_audit: {others['ir_audit']}", file=sys.stderr, flush=True)
