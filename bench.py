@@ -387,6 +387,8 @@ def main():
                     help="incremental match forwarding: 1 on, 0 off, -1 automatic")
     ap.add_argument("--expand-lanes", type=int, default=0, choices=(0, 16, 32, 64),
                     help="lanes per child in the expansion (HGX_OPTION_EXPAND_LANES; 0 = tuned/automatic)")
+    ap.add_argument("--join-lanes", type=int, default=0, choices=(0, 16, 32, 64),
+                    help="lanes per parent state in the join (HGX_OPTION_JOIN_LANES; 0 = tuned/automatic)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip the graph-replayed passes")
     ap.add_argument("--no-relations", dest="relations", action="store_false",
@@ -411,6 +413,7 @@ def main():
     ctx = hgx.Context(0)
     ctx.set_forwarding(args.forward)
     ctx.set_expand_lanes(args.expand_lanes)
+    ctx.set_join_lanes(args.join_lanes)
     print(f"[bench] {ctx.info}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)

@@ -45,6 +45,7 @@ OPTION_DELTA_CAP = 2   # HGX_OPTION_DELTA_CAP
 OPTION_EXPAND_TEAM = 3  # HGX_OPTION_EXPAND_TEAM
 OPTION_JOIN_TEAM = 4    # HGX_OPTION_JOIN_TEAM
 OPTION_EXPAND_LANES = 5 # HGX_OPTION_EXPAND_LANES
+OPTION_JOIN_LANES = 6   # HGX_OPTION_JOIN_LANES
 
 
 class HGXError(RuntimeError):
@@ -465,6 +466,13 @@ class Context:
         """Lanes that rewrite and hash one child when a wave is the team: 64, 32 or 16 (sub-wave
         teams), 0 automatic. Results never depend on it."""
         rc = self._lib.hgx_ctx_set_option(self._h, OPTION_EXPAND_LANES, int(lanes))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def set_join_lanes(self, lanes: int):
+        """Lanes that join one parent state when a wave is the team: 64, 32 or 16 (sub-wave
+        teams), 0 automatic. Results never depend on it."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_JOIN_LANES, int(lanes))
         if rc != abi.HGX_OK:
             raise self._err(rc)
 

@@ -223,13 +223,14 @@ struct ExpandClass {
     uint64_t n = 0;
     std::vector<uint32_t> rb, rpre{0};
 };
-constexpr uint32_t kClassEdges[4] = {128, 512, 2048, 0xFFFFFFFFu};
+constexpr uint32_t kClassEdges[6] = {32, 64, 128, 512, 2048, 0xFFFFFFFFu};
 // The expansion's size classes are finer at the small end, where a child's workspace is a few KiB
 // and several children share one wave (sub-wave teams, by default: a wave hashes four children of
 // up to 32 edges, or two of up to 64, each in its own workspace; the autotuner may choose others).
 constexpr int kExpandClasses = 6;
 constexpr uint32_t kExpandEdges[kExpandClasses] = {32, 64, 128, 512, 2048, 0xFFFFFFFFu};
 static_assert(kExpandClasses == kTuneExpandClasses, "one tuned shape per expansion size class");
+static_assert(sizeof(kClassEdges) / sizeof(kClassEdges[0]) == kTuneJoinClasses, "one tuned shape per join size class");
 }  // namespace
 
 uint32_t Engine::step() {
@@ -423,7 +424,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             // (requests not advancing) keep the zero count set here.
             auto launch_classes = [&](bool emit, std::vector<ExpandClass>& cls, const char* name,
                                       uint64_t bytes) {
-                cls.assign(4, ExpandClass{});
+                cls.assign(6, ExpandClass{});
                 for (uint32_t i = 0; i < nF; ++i) {
                     const RunRec& r = *F_runs_[i];
                     if (emit ? !tot[i] : !h_active[r.slot]) continue;
@@ -440,7 +441,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 for (auto& x : cls) {
                     if (!x.n) continue;
                     MatchArgs mc = ma;
-                    if (ctx_->forward == -1 && x.ne <= kClassEdges[1]) { mc.fwd_n = 0; mc.dmax = 0; }
+                    if (ctx_->forward == -1 && x.ne <= 512) { mc.fwd_n = 0; mc.dmax = 0; }
                     mc.max_ne = std::max(1u, std::min(x.ne, F_->max_ne));
                     mc.max_nextv = std::max(1u, std::min(x.nextv, F_->max_nextv));
                     // Team: as for the expansion, by the one-wave workspace
@@ -469,12 +470,18 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                         mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
                         mc.n_items = (uint32_t)x.n;
                     }
-                    const uint32_t wpb_m = mc.scratch ? 1u : waves_per_block(mc.lds_per_wave, ctx_->max_lds);
+                    // sub-wave teams for small states, when the wave's workspaces fit a workgroup
+                    uint32_t lanes = 64;
+                    if (team == 1 && !mc.scratch) {
+                        const uint32_t want = ctx_->join_lanes ? ctx_->join_lanes : ctx_->tune.join_lanes[kc];
+                        if (want < 64 && (uint64_t)mc.lds_per_wave * (64 / want) <= ctx_->max_lds) lanes = want;
+                    }
+                    const uint32_t wpb_m = mc.scratch ? 1u : waves_per_block(mc.lds_per_wave * (64 / lanes), ctx_->max_lds);
                     // up to 16 workgroups per CU (about three times what stays resident): measured
                     // best of 5, 8, 16, 32 and 64 on the batched workload (0.8 ms per pass over 8)
                     const uint32_t gcap = mc.scratch ? gws_grid : ctx_->num_cu * ctx_->tune.join_grid[kc];
                     timed(ctx_, st, name, (uint64_t)((double)bytes * x.n / n), x.n,
-                          [&] { return launch_match(mc, emit, gcap, team == 1 ? wpb_m : 1, team, st); }, kc);
+                          [&] { return launch_match(mc, emit, gcap, team == 1 ? wpb_m : 1, team, st, lanes); }, kc);
                 }
             };
             HIP_TRY(launch_fill(cnt, 0, sizeof(uint32_t) * (n + 1), st));

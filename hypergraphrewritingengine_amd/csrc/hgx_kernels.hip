@@ -348,16 +348,20 @@ uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uin
 // five waves per SIMD (96 registers; the emit pass measured 2.56 -> 2.35 ms per batched pass,
 // where its unbounded 107 allow four); a bound of six spills in the join and measured slower.
 // GWS: the workspace is a per-workgroup slice of global memory (a.scratch) -- the same code for
-// states too large for LDS, with W = 8.
-template <bool EMIT, class JoinLane, uint32_t W, bool GWS = false>
+// states too large for LDS, with W = 8. G < 64 (with W = 1): sub-wave teams, 64 / G states per
+// wave, each team with its own workspace, for small states whose seeds would leave most of a
+// wave's lanes idle.
+template <bool EMIT, class JoinLane, uint32_t W, bool GWS = false, uint32_t G = 64>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match(MatchArgs a) {
+    using TM = typename TeamSel<W, G>::type;
     extern __shared__ uint8_t lds[];
-    constexpr uint32_t N = Team<W>::N;
+    constexpr uint32_t N = TM::N;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    const uint32_t ti = W == 1 ? threadIdx.x / G : 0;         // team within the workgroup
     uint8_t* p;
     if constexpr (GWS) p = a.scratch + (size_t)blockIdx.x * a.lds_per_wave;
-    else p = lds + (W == 1 ? (size_t)w * a.lds_per_wave : 0);
-    const Team<W> tm((uint64_t*)(p + a.lds_per_wave - 64));      // the workspace's last 64 bytes
+    else p = lds + (W == 1 ? (size_t)ti * a.lds_per_wave : 0);
+    const TM tm((uint64_t*)(p + a.lds_per_wave - 64));         // the workspace's last 64 bytes
     const uint32_t A = a.P.A;
     uint32_t* sv = (uint32_t*)carve(p, a.max_ne * A * 4);      // vertices
     uint8_t* sa = carve(p, a.max_ne);                           // arity
@@ -385,10 +389,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
 
     // States strided over the teams: neighbouring states (often one request's, of similar size)
     // go to different teams, which balances the launch better than contiguous chunks do.
-    const uint32_t tpb = W == 1 ? blockDim.x / WAVE : 1;      // teams per workgroup
+    const uint32_t tpb = W == 1 ? (blockDim.x / WAVE) * (WAVE / G) : 1;   // teams per workgroup
     const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
-    for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? w : 0); ls < n_items; ls += teams) {
+    for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? ti : 0); ls < n_items; ls += teams) {
         uint32_t s = ls;
         if (a.nrange) {                                         // subset: logical -> state
             uint32_t lo = 0, hi = a.nrange;
@@ -1410,6 +1414,19 @@ static uint32_t grid_for(uint32_t items, uint32_t per_block, uint32_t cap) {
     return (g + 7u) & ~7u;
 }
 
+template <uint32_t G>
+static void launch_match_lanes(const MatchArgs& a, bool emit, uint32_t grid, uint32_t wpb, hipStream_t st) {
+    const uint32_t threads = 64 * wpb;
+    const size_t lds = (size_t)a.lds_per_wave * wpb * (64 / G);
+    if (a.small) {
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneLds, 1, false, G>), dim3(grid), dim3(threads), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneLds, 1, false, G>), dim3(grid), dim3(threads), lds, st, a);
+    } else {
+        if (emit) hipLaunchKernelGGL((k_match<true, JoinLaneBig, 1, false, G>), dim3(grid), dim3(threads), lds, st, a);
+        else hipLaunchKernelGGL((k_match<false, JoinLaneBig, 1, false, G>), dim3(grid), dim3(threads), lds, st, a);
+    }
+}
+
 template <uint32_t W>
 static void launch_match_team(const MatchArgs& a, bool emit, uint32_t grid, uint32_t wpb, hipStream_t st) {
     const uint32_t threads = W == 1 ? 64 * wpb : 64 * W;
@@ -1436,10 +1453,17 @@ static void launch_match_team(const MatchArgs& a, bool emit, uint32_t grid, uint
 }
 
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, uint32_t team,
-                        hipStream_t st) {
+                        hipStream_t st, uint32_t lanes) {
     const uint32_t items = a.nrange ? a.n_items : a.P.n;
     if (!items) return hipSuccess;
     if (a.scratch && team != 8) return hipErrorInvalidValue;     // global workspaces: 8-wave teams
+    if (lanes != 64) {                                           // sub-wave teams: one wave per team set
+        if (team != 1 || a.scratch || (lanes != 16 && lanes != 32)) return hipErrorInvalidValue;
+        const uint32_t grid = grid_for(items, wpb * (64 / lanes), grid_cap);
+        if (lanes == 16) launch_match_lanes<16>(a, emit, grid, wpb, st);
+        else launch_match_lanes<32>(a, emit, grid, wpb, st);
+        return hipGetLastError();
+    }
     const uint32_t grid = grid_for(items, team == 1 ? wpb : 1, grid_cap);
     switch (team) {
         case 1: launch_match_team<1>(a, emit, grid, wpb, st); break;

@@ -176,8 +176,10 @@ struct RootCopy {
 hipError_t launch_copy_roots(const RootCopy* items, uint32_t n, const GenView& v, hipStream_t st);
 
 // team: waves per state (1, 4 or 8); with team 1, wpb states per workgroup.
+// team: waves per state (1, 4, 8); lanes (team 1 only): 16 or 32 for sub-wave teams (64 / lanes
+// states per wave, each with its own lds_per_wave workspace), 64 for one wave per state.
 hipError_t launch_match(const MatchArgs& a, bool emit, uint32_t grid_cap, uint32_t wpb, uint32_t team,
-                        hipStream_t st);
+                        hipStream_t st, uint32_t lanes = 64);
 uint32_t expand_lds_per_wave(uint32_t max_child_ne, uint32_t max_nv, uint32_t max_nextv, uint32_t A, bool lean);
 // team: waves per child (1, 2, 4 or 8); with team 1, wpb children per workgroup.
 // team: waves sharing one child (1, 2, 4, 8); lanes: with team 1, the lanes sharing one child

@@ -572,6 +572,10 @@ extern "C" int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value) 
         ctx->expand_lanes = (uint32_t)value;
         return HGX_OK;
     }
+    if (option == HGX_OPTION_JOIN_LANES && (value == 0 || value == 16 || value == 32 || value == 64)) {
+        ctx->join_lanes = (uint32_t)value;
+        return HGX_OK;
+    }
     ctx->err = "unknown option or value";
     return HGX_ERR_INVALID;
 }

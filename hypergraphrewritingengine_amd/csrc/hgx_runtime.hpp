@@ -235,12 +235,13 @@ struct Profiler {
 // Launch shapes per size class, chosen by the autotuner (hgx_autotune) or loaded from its cache;
 // the defaults are the hand-fitted rule. None of them changes a result.
 constexpr int kTuneExpandClasses = 6;   // children of up to 32, 64, 128, 512, 2048 edges, more
-constexpr int kTuneJoinClasses = 4;     // parents of up to 128, 512, 2048 edges, more
+constexpr int kTuneJoinClasses = 6;     // parents of up to 32, 64, 128, 512, 2048 edges, more
 struct Tune {
     uint32_t expand_lanes[kTuneExpandClasses] = {16, 32, 64, 64, 64, 64};   // lanes per child
     uint32_t expand_grid[kTuneExpandClasses] = {8, 8, 8, 8, 8, 8};          // workgroups per CU
-    uint32_t join_team[kTuneJoinClasses] = {0, 0, 0, 0};                    // waves per state, 0 = by size
-    uint32_t join_grid[kTuneJoinClasses] = {16, 16, 16, 16};                // workgroups per CU
+    uint32_t join_team[kTuneJoinClasses] = {0, 0, 0, 0, 0, 0};              // waves per state, 0 = by size
+    uint32_t join_lanes[kTuneJoinClasses] = {32, 32, 64, 64, 64, 64};       // lanes per state (one-wave teams)
+    uint32_t join_grid[kTuneJoinClasses] = {16, 16, 16, 16, 16, 16};        // workgroups per CU
     std::string source = "built-in defaults";
 };
 
@@ -265,6 +266,7 @@ struct hgx_ctx {
     uint32_t expand_team = 0;                    // HGX_OPTION_EXPAND_TEAM
     uint32_t join_team = 0;                      // HGX_OPTION_JOIN_TEAM
     uint32_t expand_lanes = 0;                   // HGX_OPTION_EXPAND_LANES
+    uint32_t join_lanes = 0;                     // HGX_OPTION_JOIN_LANES
     hgx::Tune tune;                              // launch shapes per size class
 };
 

@@ -2,8 +2,9 @@
 //
 // For every size class of the expansion (children of up to 32, 64, 128, 512, 2048 edges, more)
 // it chooses the lanes per child (16 / 32 / 64: sub-wave teams or one wave) and the workgroups per
-// CU; for every size class of the join (parents of up to 128, 512, 2048 edges, more) the waves per
-// state and the workgroups per CU. Each candidate is run over a caller's prepared batch with the
+// CU; for every size class of the join (parents of up to 32, 64, 128, 512, 2048 edges, more) the
+// waves per state, the lanes per state when one wave is the team (16 / 32 / 64), and the
+// workgroups per CU. Each candidate is run over a caller's prepared batch with the
 // launches event-timed per class, and each class keeps the fastest candidate (classes are separate
 // launches, so one sweep serves them all). The choice is kept in the context and written to a JSON
 // cache keyed by "<arch>:<CUs>", which hgx_ctx_create loads for its device (from HGX_TUNE_CACHE,
@@ -48,7 +49,8 @@ static std::string json_array(const uint32_t (&a)[N]) {
 
 static std::string tune_json(const Tune& t) {
     return "{\"expand_lanes\": " + json_array(t.expand_lanes) + ", \"expand_grid\": " + json_array(t.expand_grid) +
-           ", \"join_team\": " + json_array(t.join_team) + ", \"join_grid\": " + json_array(t.join_grid) +
+           ", \"join_team\": " + json_array(t.join_team) + ", \"join_lanes\": " + json_array(t.join_lanes) +
+           ", \"join_grid\": " + json_array(t.join_grid) +
            ", \"source\": \"" + t.source + "\"}";
 }
 
@@ -88,6 +90,7 @@ static bool valid(const Tune& t) {
     for (uint32_t x : t.expand_lanes) if (x != 16 && x != 32 && x != 64) return false;
     for (uint32_t x : t.expand_grid) if (x < 1 || x > 64) return false;
     for (uint32_t x : t.join_team) if (x != 0 && x != 1 && x != 4 && x != 8) return false;
+    for (uint32_t x : t.join_lanes) if (x != 16 && x != 32 && x != 64) return false;
     for (uint32_t x : t.join_grid) if (x < 1 || x > 64) return false;
     return true;
 }
@@ -102,7 +105,8 @@ bool tune_load(hgx_ctx* ctx, const std::string& path) {
     if (!cache_entry(ss.str(), key, entry)) return false;
     Tune t;
     if (!read_array(entry, "expand_lanes", t.expand_lanes) || !read_array(entry, "expand_grid", t.expand_grid) ||
-        !read_array(entry, "join_team", t.join_team) || !read_array(entry, "join_grid", t.join_grid) || !valid(t))
+        !read_array(entry, "join_team", t.join_team) || !read_array(entry, "join_lanes", t.join_lanes) ||
+        !read_array(entry, "join_grid", t.join_grid) || !valid(t))
         return false;
     t.source = "tuned (" + key + ", " + path + ")";
     ctx->tune = t;
@@ -136,10 +140,11 @@ extern "C" int hgx_autotune(hgx_ctx* ctx, const hgx_batch* batch, uint32_t reps,
         if (!n) throw Error{HGX_ERR_INVALID, "the tuning batch is empty"};
         if (reps < 1) reps = 1;
         // forced options would hide what the candidates change
-        const uint32_t saved_lanes = ctx->expand_lanes, saved_jt = ctx->join_team, saved_et = ctx->expand_team;
+        const uint32_t saved_lanes = ctx->expand_lanes, saved_jt = ctx->join_team, saved_et = ctx->expand_team,
+                       saved_jl = ctx->join_lanes;
         const bool saved_on = ctx->prof.on, saved_cls = ctx->prof.by_class;
         const Tune saved = ctx->tune;
-        ctx->expand_lanes = 0; ctx->join_team = 0; ctx->expand_team = 0;
+        ctx->expand_lanes = 0; ctx->join_team = 0; ctx->expand_team = 0; ctx->join_lanes = 0;
         std::vector<hgx_result*> res(n, nullptr);
         // one candidate: every class set alike, the best of `reps` passes per class
         auto measure = [&](const Tune& t) {
@@ -187,14 +192,17 @@ extern "C" int hgx_autotune(hgx_ctx* ctx, const hgx_batch* batch, uint32_t reps,
                 }
             }
         log += "], \"join\": [";
-        // join: waves per state x workgroups per CU (count and emit passes together)
+        // join: (waves per state, lanes per state) x workgroups per CU (count and emit passes
+        // together); lanes below 64 only with one-wave teams
         double jb[kTuneJoinClasses];
         std::fill(jb, jb + kTuneJoinClasses, -1.0);
         first = true;
-        for (uint32_t team : {0u, 1u, 4u})
+        const uint32_t shapes[5][2] = {{1, 16}, {1, 32}, {1, 64}, {0, 64}, {4, 64}};
+        for (const auto& sh : shapes)
             for (uint32_t grid : {8u, 16u, 32u}) {
                 Tune t = out;
-                std::fill(t.join_team, t.join_team + kTuneJoinClasses, team);
+                std::fill(t.join_team, t.join_team + kTuneJoinClasses, sh[0]);
+                std::fill(t.join_lanes, t.join_lanes + kTuneJoinClasses, sh[1]);
                 std::fill(t.join_grid, t.join_grid + kTuneJoinClasses, grid);
                 const auto m = measure(t);
                 for (int k = 0; k < kTuneJoinClasses; ++k) {
@@ -203,10 +211,13 @@ extern "C" int hgx_autotune(hgx_ctx* ctx, const hgx_batch* batch, uint32_t reps,
                     if (c < 0) continue;
                     const double ms = c + (e > 0 ? e : 0.0);
                     log += std::string(first ? "" : ", ") + "{\"class\": " + std::to_string(k) + ", \"team\": " +
-                           std::to_string(team) + ", \"grid\": " + std::to_string(grid) + ", \"ms\": " +
-                           std::to_string(ms) + "}";
+                           std::to_string(sh[0]) + ", \"lanes\": " + std::to_string(sh[1]) + ", \"grid\": " +
+                           std::to_string(grid) + ", \"ms\": " + std::to_string(ms) + "}";
                     first = false;
-                    if (jb[k] < 0 || ms < jb[k]) { jb[k] = ms; out.join_team[k] = team; out.join_grid[k] = grid; }
+                    if (jb[k] < 0 || ms < jb[k]) {
+                        jb[k] = ms;
+                        out.join_team[k] = sh[0]; out.join_lanes[k] = sh[1]; out.join_grid[k] = grid;
+                    }
                 }
             }
         log += "]}";
@@ -214,6 +225,7 @@ extern "C" int hgx_autotune(hgx_ctx* ctx, const hgx_batch* batch, uint32_t reps,
         out.source = "autotuned (" + key + ", " + std::to_string(n) + "-request batch)";
         ctx->tune = out;
         ctx->expand_lanes = saved_lanes; ctx->join_team = saved_jt; ctx->expand_team = saved_et;
+        ctx->join_lanes = saved_jl;
         ctx->prof.on = saved_on; ctx->prof.by_class = saved_cls;
         ctx->prof.acc.clear();
         if (cache_path && *cache_path) {

@@ -202,12 +202,16 @@ size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
  *                       its own workspace), for every size class whose wave workspaces fit a
  *                       workgroup's LDS; 0 automatic (default: 16 lanes for children of up to 32
  *                       edges, 32 up to 64, else 64).
+ *   HGX_OPTION_JOIN_LANES lanes that join one parent state when one wave is the team: 64, or
+ *                       32 / 16 (two or four states per wave, each in its own workspace); 0
+ *                       automatic (the tuned or default shape per size class).
  * Returns HGX_ERR_INVALID for an unknown option or value. */
 #define HGX_OPTION_FORWARD 1
 #define HGX_OPTION_DELTA_CAP 2
 #define HGX_OPTION_EXPAND_TEAM 3
 #define HGX_OPTION_JOIN_TEAM 4
 #define HGX_OPTION_EXPAND_LANES 5
+#define HGX_OPTION_JOIN_LANES 6
 int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value);
 
 /* ---- synchronous batch: the plain and the batched path -------------------------------- */

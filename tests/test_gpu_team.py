@@ -219,3 +219,57 @@ def test_expand_lanes_option_rejects_bad_values(ctx):
         with pytest.raises(hgx.HGXError):
             ctx.set_expand_lanes(bad)
     ctx.set_expand_lanes(0)
+
+
+def _by_join_lanes(ctx, reqs):
+    """Join lanes 64, 32, 16 and automatic (one-wave teams), each with forwarding on and off."""
+    out = {}
+    try:
+        ctx.set_join_team(1)
+        for fwd in (1, 0):
+            ctx.set_forwarding(fwd)
+            for lanes in (0, 64, 32, 16):
+                ctx.set_join_lanes(lanes)
+                out[(fwd, lanes)] = ctx.evolve_batch(reqs)
+    finally:
+        ctx.set_join_lanes(0)
+        ctx.set_join_team(0)
+        ctx.set_forwarding(-1)
+    return out
+
+
+def test_subwave_join_random_rules_exact(ctx):
+    """Sub-wave join teams: 16 and 32 lanes per parent state (four or two states per wave, each
+    with its own workspace), forced for every size class whose workspaces fit, with forwarding
+    (the delta joins and the merge) on and off, against one wave per state and the oracle."""
+    reqs = [_random_request(4000 + s) for s in range(40)]
+    reqs += [W.batched_run(i, cap=2048, flags=REL | TR) for i in range(0, 400, 25)]
+    r = _by_join_lanes(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for key in r:
+            _same(r[key][i], r[(0, 64)][i])
+        assert_exact(r[(1, 16)][i], po.evolve(req))
+
+
+def test_subwave_join_mixed_class_batch(ctx):
+    """Parents in every join size class (up to 32, 64, 128, 512 edges) in one batch, so sub-wave
+    launches and wave launches run side by side in one generation."""
+    rb = rule_from_lists([[0, 1], [1, 2]], [[0, 2], [2, 3], [3, 1]])
+    reqs = []
+    for k, m in enumerate((5, 20, 40, 60, 90, 200, 300)):
+        reqs.append(Request([rb], W._connected_init(W.Stream(90 + k), m), 3, state_cap=1500, flags=REL | TR))
+    reqs.append(Request([rule_from_lists([[0], [0, 1]], [[1], [0, 1], [1, 2]])], _mixed_state(4, 40), 3,
+                        state_cap=1500, flags=REL | TR))
+    r = _by_join_lanes(ctx, reqs)
+    for i, req in enumerate(reqs):
+        for key in r:
+            _same(r[key][i], r[(0, 64)][i])
+        assert_exact(r[(0, 0)][i], po.evolve(req))
+
+
+def test_join_lanes_option_rejects_bad_values(ctx):
+    import hypergraphrewritingengine_amd as hgx
+    for bad in (8, 24, 128, -1):
+        with pytest.raises(hgx.HGXError):
+            ctx.set_join_lanes(bad)
+    ctx.set_join_lanes(0)
