@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = [
     "hgx_ir_hash_batch", "hgx_ir_rank_batch", "hgx_ir_canonical_form", "hgx_result_status",
     "hgx_result_causal_triples", "hgx_match_count_noalloc", "hgx_autotune", "hgx_tune_load",
     "hgx_tune_get", "hgx_batch_capture", "hgx_replay_launch", "hgx_replay_release",
-    "hgx_calibration_read",
+    "hgx_calibration_read", "hgx_result_unfold",
 ]
 
 OPTION_FORWARD = 1     # HGX_OPTION_FORWARD
@@ -122,6 +122,7 @@ def load_library():
     L.hgx_batch_release.argtypes = [vp]
     L.hgx_batch_release.restype = None
     L.hgx_result_consumed.argtypes = [vp, vp, vp, C.c_uint64]
+    L.hgx_result_unfold.argtypes = [vp, vp, C.POINTER(abi.Unfolding)]
     L.hgx_run_wxf.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]
     L.hgx_buffer_free.argtypes = [C.POINTER(C.c_uint8)]
     L.hgx_buffer_free.restype = None
@@ -225,6 +226,15 @@ class Result:
         pos = np.zeros(max(1, int(off[-1])), np.uint16)
         self._check(self._lib.hgx_result_consumed(self._h, off.ctypes.data, pos.ctypes.data, int(off[-1])))
         return off, pos[:int(off[-1])]
+
+    def unfold(self) -> dict:
+        """The raw unfolding beneath this result (hgx_result_unfold): every class expansion
+        replayed on every raw instance, on the device. Needs HGX_REC_STATES."""
+        u = abi.Unfolding()
+        rc = self._lib.hgx_result_unfold(self._ctx._h, self._h, C.byref(u))
+        if rc != abi.HGX_OK:
+            raise self._ctx._err(rc)
+        return u.as_dict()
 
     def timing(self):
         d, l = C.c_uint64(), C.c_uint64()

@@ -16,6 +16,8 @@ HGX_REC_BRANCHIAL = 0x2
 HGX_REC_RELATIONS = 0x4
 HGX_REC_STATES = 0x8
 HGX_OPT_TR = 0x100
+HGX_OPT_QUOTIENT = 0x200
+HGX_OPT_QUOTIENT_ROOTS = 0x400
 HGX_REC_DEFAULT = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL
 HGX_MAX_STEPS = 63
 
@@ -87,6 +89,15 @@ class Counts(C.Structure):
         ("steps_done", C.c_uint32),
         ("truncated", C.c_uint32),
     ]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Unfolding(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("raw_events", "causal_triples", "causal_pairs", "causal_pairs_tr",
+                                          "causal_triples_tr", "branchial_pairs", "instances", "classes",
+                                          "class_matches")]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

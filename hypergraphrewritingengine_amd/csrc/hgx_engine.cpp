@@ -177,6 +177,8 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     d.map_mask = (uint32_t)(mc - 1);
     r->ne_bound = r->hp->max_root_ne;
     r->nv_bound = r->hp->max_root_nv;
+    r->q_seen.clear();
+    r->smp_gen = nullptr;
     r->admitted = true;
     r->t_admit = Clock::now();
     live_[r->id] = r;
@@ -384,7 +386,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             // transition keys of this generation's matches
             for (uint32_t i = 0; i < nF; ++i) {
                 RunRec& r = *F_runs_[i];
-                if (h_active[r.slot] && r.hp->smp.transitions()) frontier_ir(*F_, F_b_[i], F_e_[i], st, r);
+                if (h_active[r.slot] && r.hp->smp.transitions() && r.smp_gen != F_.get())
+                    frontier_ir(*F_, F_b_[i], F_e_[i], st, r);
             }
             std::memcpy(h_tab_.data() + 12ull * cap_slots_, h_active.data(), cap_slots_);
             up((uint8_t*)d_budget_, h_tab_.data(), h_tab_.size(), st);    // budget, sbeg, send, active
@@ -1008,7 +1011,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
         for (uint32_t i = 0; i < nF; ++i) {
             if (!tot[i]) continue;
             auto& r = F_runs_[i];
-            if (r->hp->smp.frontier()) kids.push_back(SampledKids{r.get(), cb[i], ce[i], r->events});
+            if (r->hp->host_frontier()) kids.push_back(SampledKids{r.get(), cb[i], ce[i], r->events});
             Segment sg{C, cb[i], ce[i], 0, 0};
             if (!h_poff.empty()) { sg.br_b = h_poff[F_b_[i]]; sg.br_e = h_poff[F_e_[i]]; }
             r->segs.push_back(sg);
@@ -1026,6 +1029,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
         uint32_t s = (uint32_t)M;
         for (auto& r : admitted) {
             const uint32_t nr = r->hp->n_roots();
+            if (r->hp->quotient()) kids.push_back(SampledKids{r.get(), s, s + nr, 0, true});
             r->segs.push_back(Segment{C, s, s + nr, 0, 0});
             r->states = nr;
             if (r->hp->steps == 0) r->done = true;
@@ -1126,6 +1130,7 @@ void frontier_ir(const DevGen& G, uint32_t b, uint32_t e, hipStream_t st, RunRec
     }
     soff[n] = x;
     r.smp_roff[n] = (uint32_t)x;
+    r.smp_gen = &G;
     r.smp_ir.assign(n, 0);
     r.smp_rank.assign(std::max<uint64_t>(E, 1), 0);
     if (n && hgx_ir_rank_batch(nullptr, soff.data(), n, eoff.data(), verts.empty() ? nullptr : verts.data(),
@@ -1238,30 +1243,41 @@ void Engine::sample_frontier(const std::vector<SampledKids>& kids, uint32_t nC, 
     F_expand_.clear();
     bool any = false;
     for (const SampledKids& k : kids) any |= k.r->depth < k.r->hp->steps;
-    if (!any || !M) return;
-    if (h_parent.size() != M) {
-        h_parent.resize(M);
-        d2h(h_parent.data(), mv.parent, M, st);
-        HIP_TRY(hipStreamSynchronize(st));
-    }
+    if (!any || !F_) return;
     F_expand_.assign(nC, 1);
     bool masked = false;
     for (const SampledKids& k : kids) {
-        const RunRec& r = *k.r;
+        RunRec& r = *k.r;
         const SampleHost& S = r.hp->smp;
-        const uint32_t step = r.depth;              // the children's step
+        const uint32_t step = r.depth;              // the new states' step
         if (step >= r.hp->steps) continue;          // the last step: nothing expands them
+        const bool q = r.hp->quotient();
+        if (q) frontier_ir(*F_, k.cb, k.ce, st, r); // exact hashes (and ranks, for the next join)
+        const bool merge_roots = (r.hp->flags & HGX_OPT_QUOTIENT_ROOTS) != 0;
+        const bool smp = !k.roots && S.frontier();
+        if (smp && S.per_parent && h_parent.size() != M) {
+            h_parent.resize(M);
+            d2h(h_parent.data(), mv.parent, M, st);
+            HIP_TRY(hipStreamSynchronize(st));
+        }
         uint64_t at_step = 0;
         std::unordered_map<uint32_t, uint64_t> at_parent;
         const uint64_t n_roots = r.hp->n_roots();
         for (uint32_t c = k.cb; c < k.ce; ++c) {
-            const uint64_t sid = n_roots + k.ev_base + (c - k.cb);
-            // exploration.hpp:93-121: the coin, then the step's bound, then the parent's
+            const uint64_t sid = k.roots ? (c - k.cb) : n_roots + k.ev_base + (c - k.cb);
             bool go = true;
-            if (S.thr != 0xFFFFFFFFu)
+            // exploration.hpp:66-92: the first state of an exact class is expanded, later ones
+            // are not (a hash of 0 -- the canonicalizer gave up -- is no class: kept); every root
+            // is expanded unless isomorphic roots merge (persistent.cu:122-135)
+            if (q) {
+                const uint64_t h = r.smp_ir[c - k.cb];
+                if (h && !r.q_seen.insert(h).second && (!k.roots || merge_roots)) go = false;
+            }
+            // exploration.hpp:93-121: the coin, then the step's bound, then the parent's
+            if (go && smp && S.thr != 0xFFFFFFFFu)
                 go = S.thr != 0u && (uint32_t)splitmix(S.coin_seed ^ ((uint64_t)step << 32) ^ sid) < S.thr;
-            if (go && S.per_step) go = at_step++ < S.per_step;
-            if (go && S.per_parent) go = at_parent[h_parent[c]]++ < S.per_parent;
+            if (go && smp && S.per_step) go = at_step++ < S.per_step;
+            if (go && smp && S.per_parent) go = at_parent[h_parent[c]]++ < S.per_parent;
             F_expand_[c] = go ? 1 : 0;
             masked |= !go;
         }
@@ -1492,9 +1508,9 @@ extern "C" int hgx_batch_capture(hgx_ctx* ctx, const hgx_batch* b, hgx_replay** 
         const uint32_t n = (uint32_t)b->runs.size();
         if (!n) throw Error{HGX_ERR_INVALID, "the batch is empty"};
         for (const auto& h : b->runs)
-            if (h->smp.on)
-                throw Error{HGX_ERR_UNSUPPORTED, "sampled requests take host decisions every generation; "
-                                                 "they are not recorded into graphs"};
+            if (h->smp.on || h->quotient())
+                throw Error{HGX_ERR_UNSUPPORTED, "sampled requests and quotient exploration take host decisions "
+                                                 "every generation; they are not recorded into graphs"};
         hipStream_t st = ctx->stream;
         auto plan = std::make_shared<PassPlan>();
         {   // 1. an eager pass keeps every host decision

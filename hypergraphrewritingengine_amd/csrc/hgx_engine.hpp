@@ -11,6 +11,7 @@
 #include <chrono>
 #include <deque>
 #include <unordered_map>
+#include <unordered_set>
 #include <memory>
 #include <string>
 #include <vector>
@@ -70,17 +71,22 @@ struct RunRec {
     // of frontier state k has rank smp_rank[smp_roff[k] + q]), for its transition keys.
     std::vector<uint64_t> smp_ir;
     std::vector<uint32_t> smp_roff, smp_rank;
+    const void* smp_gen = nullptr;   // the generation smp_* were computed for
+    // Quotient exploration: the exact classes met so far (hash of each).
+    std::unordered_set<uint64_t> q_seen;
     hgx_counts counts{};
     hipStream_t st = nullptr;        // the context's stream (contexts outlive results)
     Clock::time_point t_submit, t_admit, t_done;
     double device_ms = 0;
 };
 
-// A sampled request's children in a new generation: [cb, ce), its events before them.
+// The new states of a request whose frontier the host decides (sampling, quotient exploration)
+// in a new generation: [cb, ce), its events before them.
 struct SampledKids {
     RunRec* r;
     uint32_t cb, ce;
     uint64_t ev_base;
+    bool roots = false;              // [cb, ce) are the request's roots
 };
 
 // The host decisions of one eager pass over a prepared batch, per generation: the per-slot child

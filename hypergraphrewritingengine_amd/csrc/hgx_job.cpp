@@ -6,23 +6,27 @@
 // BranchialEdges, Num*). It is written over the library's own C ABI (hgx_evolve_batch and the
 // hgx_result_* readbacks).
 //
-// Served: full multiway evolution with CanonicalizeStates -> None, Automatic or Full (the GPU
+// Served: multiway evolution with CanonicalizeStates -> None, Automatic or Full (the GPU
 // backend's grouping, hg_gpu_backend.cpp:269-310: None gives every state its own class, Automatic
 // groups states of equal content, Full groups by the exact IR hash computed on the host and emits
 // each class's representative relabelled to its canonical form; CanonicalHash is the IR hash in
-// every mode), CanonicalizeEvents -> None, Full,
-// Automatic, Positional or a list of signature keys (events identified by the signature of
-// common/include/hgcommon/event_core.hpp:60-84 over exact state hashes and canonical edge
-// ranks, as Hypergraph::create_event does, hypergraph/src/hypergraph.cpp:448-499), any rule
-// set, one or more initial states, causal edges with or without transitive reduction,
-// branchial edges, RequestedData selection, and the sampling and capping options
-// (TransitionRate, RuleWeights, MatchesPerStateRule, ExplorationProbability, MaxStatesPerStep,
-// MaxSuccessorStatesPerParent, RandomSeed; decided by the engine, include/hgx.h hgx_sampling). Options that change what is
-// explored and are not served (canonical-state exploration, sessions, genesis events, graph
-// properties) are refused with an error rather than ignored, so a caller never receives a
-// different evolution than it asked for; options with no effect on the result (ShowProgress,
-// ...) are accepted and reported under "Warnings" as skipped, with the reference's
-// Kind/Count/Context schema.
+// every mode), CanonicalizeEvents -> None, Full, Automatic, Positional or a list of signature
+// keys (events identified by the signature of common/include/hgcommon/event_core.hpp:60-84 over
+// exact state hashes and canonical edge ranks, as Hypergraph::create_event does,
+// hypergraph/src/hypergraph.cpp:448-499), ExploreFromCanonicalStatesOnly and
+// QuotientInitialStates (quotient exploration on the device, HGX_OPT_QUOTIENT), any rule set, one
+// or more initial states, causal edges with or without transitive reduction, branchial edges,
+// RequestedData selection, and the sampling and capping options (TransitionRate, RuleWeights,
+// MatchesPerStateRule, ExplorationProbability, MaxStatesPerStep, MaxSuccessorStatesPerParent,
+// RandomSeed; decided by the engine, include/hgx.h hgx_sampling). With exact states and quotient
+// exploration or Automatic event identity the job takes the reference's reconstruction route:
+// events are identified by the recorded class expansion they replay, and with quotient
+// exploration NumEvents / NumCausalEdges / NumBranchialEdges are the raw unfolding's
+// (hgx_result_unfold). Options that change what is explored and are not served (sessions,
+// genesis events, graph properties, quotient exploration under content identity) are refused
+// with an error rather than ignored, so a caller never receives a different evolution than it
+// asked for; options with no effect on the result (ShowProgress, ...) are accepted and reported
+// under "Warnings" as skipped, with the reference's Kind/Count/Context schema.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -71,6 +75,9 @@ struct Job {
     bool full = false;                       // CanonicalizeStates -> Full
     bool automatic = false;                  // CanonicalizeStates -> Automatic (content identity)
     uint32_t ev_keys = 0;                    // CanonicalizeEvents: event signature keys
+    bool positional = false;                 // CanonicalizeEvents -> Positional (raw labelling)
+    bool quotient = false;                   // ExploreFromCanonicalStatesOnly
+    bool quotient_roots = false;             // QuotientInitialStates
     Sampling smp;
     std::vector<std::pair<std::string, std::string>> warnings;   // (kind, context)
 };
@@ -184,13 +191,17 @@ static Job parse(const Value& root) {
                         else if (m == "Full") j.ev_keys = In | Out;
                         else if (m == "Automatic" || m == "Positional") j.ev_keys = In | Out | Step | Cons | Prod;
                         else refuse("unknown event canonicalization " + m);
+                        j.positional = m == "Positional";
                     }
                 } else if (ok == "CausalTransitiveReduction") {
                     j.tr = as_bool(ov, ok.c_str());
                 } else if (ok == "IncludeCanonicalHashes") {
                     j.inc_hashes = as_bool(ov, ok.c_str());
-                } else if (ok == "ExploreFromCanonicalStatesOnly" || ok == "ShowGenesisEvents" ||
-                           ok == "QuotientInitialStates") {
+                } else if (ok == "ExploreFromCanonicalStatesOnly") {
+                    j.quotient = as_bool(ov, ok.c_str());
+                } else if (ok == "QuotientInitialStates") {
+                    j.quotient_roots = as_bool(ov, ok.c_str());
+                } else if (ok == "ShowGenesisEvents") {
                     if (as_bool(ov, ok.c_str())) refuse("True is not served");
                 } else if (ok == "MaxSuccessorStatesPerParent") {
                     j.smp.per_parent = as_count(ov, ok.c_str());
@@ -251,6 +262,9 @@ static Job parse(const Value& root) {
     // MatchesPerStep is MaxStatesPerStep under another name when UniformRandom selects it
     // (hypergraph_ffi.cpp:378-385)
     j.smp.per_step = uniform && matches_per_step ? matches_per_step : max_per_step;
+    if (j.quotient && j.automatic)
+        throw JobError("option ExploreFromCanonicalStatesOnly: quotient exploration under content identity "
+                       "(CanonicalizeStates -> Automatic) is not served");
     return j;
 }
 
@@ -448,11 +462,30 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     if (n_roots == 0) throw JobError("the job has no non-empty initial state");
     if (j.steps > HGX_MAX_STEPS) throw JobError("Steps above " + std::to_string(HGX_MAX_STEPS));
     uint32_t flags = HGX_REC_CAUSAL | HGX_REC_BRANCHIAL | HGX_REC_RELATIONS;
+    constexpr uint32_t In = 1, Out = 2, Step = 4, Cons = 16, Prod = 32;
+    // The reconstruction route (ParallelEvolutionEngine::configure_identity_and_quotient,
+    // parallel_evolution.cpp:1446-1545; the device's qc_route, gpu/src/evolve.cu:183-198): exact
+    // states, and quotient exploration or Automatic event identity. There an event is a recorded
+    // (class, match) expansion and the relations are the raw unfolding's.
+    const bool auto_keys = !j.positional && j.ev_keys == (In | Out | Step | Cons | Prod);
+    const bool wants_route = j.quotient || auto_keys;
+    const bool route = wants_route && j.full && j.steps > 0;
+    std::vector<std::pair<std::string, std::string>> warns = j.warnings;
+    if (wants_route && !j.full)
+        warns.emplace_back("Engine", "quotient exploration and Automatic event identity are defined over exact "
+                                     "classes, which only CanonicalizeStates -> Full computes: every state is "
+                                     "explored and events are identified by their own states' labelling");
+    // a key set whose signature would depend on how a class's slots are numbered
+    if (route && (((j.ev_keys & Cons) && !(j.ev_keys & In)) || ((j.ev_keys & (Prod | Step)) && !(j.ev_keys & Out))))
+        throw JobError("option CanonicalizeEvents: on the reconstruction route (exact states with quotient "
+                       "exploration or Automatic identity) ConsumedEdges needs InputState, and ProducedEdges "
+                       "and Step need OutputState");
     // state edges: for the output, and for the IR hash (Full's grouping, CanonicalHash)
     const bool need_ir = j.full || (j.inc_hashes && j.inc_states) || j.ev_keys;
-    const bool need_rank = (j.ev_keys & (16 | 32)) != 0;
-    if (j.inc_states || j.inc_events || need_ir || j.automatic) flags |= HGX_REC_STATES;
+    const bool need_rank = (j.ev_keys & (Cons | Prod)) != 0 || route;
+    if (j.inc_states || j.inc_events || need_ir || j.automatic || route) flags |= HGX_REC_STATES;
     if (j.tr) flags |= HGX_OPT_TR;
+    if (j.quotient && j.full) flags |= HGX_OPT_QUOTIENT | (j.quotient_roots ? HGX_OPT_QUOTIENT_ROOTS : 0u);
     hgx_run_desc d{};
     d.rules = prog.data(); d.rules_len = (uint32_t)prog.size(); d.n_rules = (uint32_t)j.rules.size();
     d.edge_offsets = eoff.data(); d.vertices = verts.empty() ? nullptr : verts.data();
@@ -478,7 +511,6 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     struct Release { hgx_result* r; ~Release() { hgx_result_release(r); } } rel{res};
     check(hgx_result_counts(res, &R.c), ctx, "counts");
     // a generation beyond the HBM budget: the result holds the ones before it, and says so
-    std::vector<std::pair<std::string, std::string>> warns = j.warnings;
     {
         char why[512] = {0};
         if (hgx_result_status(res, why, sizeof(why)) == HGX_ERR_CAPACITY) warns.emplace_back("CapacityExceeded", why);
@@ -532,13 +564,42 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     // the output state's class (its first state), the rule, and the canonical ranks of the
     // consumed edges (pattern order) and the produced edges (RHS order); the first event in id
     // order with a signature is its class's id, and NumEvents counts the classes.
+    // On the reconstruction route an event is signed by the recorded expansion it replays
+    // (quotient_replay_core.hpp:133-144): each class's expansion is recorded from the first of
+    // its states that was expanded, and an event of any other state of the class is that
+    // recorded event with the same rule and consumed ranks (canonical ranks are one frame for
+    // the whole class); NumEvents counts the recorded expansions' signatures.
     std::vector<int64_t> ev_rep(ne);
     uint64_t n_ev_classes = ne;
+    std::vector<uint32_t> rec_of;                              // route: the recorded event of each event
+    if (route) {
+        std::unordered_map<uint64_t, uint32_t> rec_state;     // exact hash -> recording state
+        std::unordered_map<uint64_t, uint32_t> rec_by_key;    // (class, rule, consumed ranks) -> event
+        rec_of.assign(ne, 0);
+        auto tkey = [&](uint64_t e) {
+            const hgx_event& x = R.events[e];
+            uint64_t k = fnv(fnv(FNV_OFFSET, ir[x.input_state]), (uint64_t)x.rule);
+            for (uint32_t q = R.coff[e]; q < R.coff[e + 1]; ++q) k = fnv(k, rank[R.soff[x.input_state] + R.cpos[q]]);
+            return k;
+        };
+        for (uint64_t e = 0; e < ne; ++e) {
+            const uint32_t in = R.events[e].input_state;
+            const uint32_t rs = rec_state.emplace(ir[in], in).first->second;
+            if (rs == in) rec_by_key.emplace(tkey(e), (uint32_t)e);
+        }
+        for (uint64_t e = 0; e < ne; ++e) {
+            auto it = rec_by_key.find(tkey(e));
+            if (it == rec_by_key.end())
+                throw JobError("an event has no recorded expansion of its class (an exact hash gave up?)");
+            rec_of[e] = it->second;
+        }
+    }
     if (j.ev_keys) {
         std::unordered_map<uint64_t, uint32_t> first_of_class;     // exact hash -> first state
         for (uint64_t s = 0; s < ns; ++s) first_of_class.emplace(ir[s], (uint32_t)s);
         std::unordered_map<uint64_t, uint32_t> first_of_sig;
-        for (uint64_t e = 0; e < ne; ++e) {
+        std::vector<uint64_t> sig_of(route ? ne : 0);
+        auto sign = [&](uint64_t e) {
             const hgx_event& x = R.events[e];
             uint64_t sig = FNV_OFFSET;
             if (j.ev_keys & 1) sig = fnv(sig, ir[x.input_state]);
@@ -549,17 +610,37 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
                 for (uint32_t k = R.coff[e]; k < R.coff[e + 1]; ++k)
                     sig = fnv(sig, rank[R.soff[x.input_state] + R.cpos[k]]);
             if (j.ev_keys & 32) {
-                const uint64_t o0 = R.soff[x.output_state], o1 = R.soff[x.output_state + 1];
+                const uint64_t o1 = R.soff[x.output_state + 1];
                 const uint64_t nrhs = rhs_edges(j, x.rule);
                 for (uint64_t k = o1 - nrhs; k < o1; ++k) sig = fnv(sig, rank[k]);
-                (void)o0;
             }
             if (sig == 0 || sig == FNV_OFFSET) sig = 1;
+            return sig;
+        };
+        for (uint64_t e = 0; e < ne; ++e) {
+            if (route && rec_of[e] != e) continue;            // signed through its recorded event below
+            const uint64_t sig = sign(e);
+            if (route) sig_of[e] = sig;
             ev_rep[e] = first_of_sig.emplace(sig, (uint32_t)e).first->second;
         }
+        if (route)
+            for (uint64_t e = 0; e < ne; ++e) ev_rep[e] = first_of_sig[sig_of[rec_of[e]]];
         n_ev_classes = first_of_sig.size();
     } else {
         for (uint64_t e = 0; e < ne; ++e) ev_rep[e] = (int64_t)e;
+    }
+
+    // On the reconstruction route with quotient exploration the counts a caller is told are the
+    // raw unfolding's (hypergraph_ffi.cpp:1672-1720): every class expansion replayed on every raw
+    // instance, on the device. Full capture is the raw unfolding already.
+    uint64_t n_branchial = R.c.branchial_pairs;
+    int64_t unfold_causal = -1;
+    if (route && j.quotient) {
+        hgx_unfolding u{};
+        check(hgx_result_unfold(ctx, res, &u), ctx, "raw unfolding");
+        if (!j.ev_keys) n_ev_classes = u.raw_events;
+        unfold_causal = (int64_t)(j.tr ? u.causal_pairs_tr : u.causal_pairs);
+        n_branchial = u.branchial_pairs;
     }
 
     Value out = Value::assoc();
@@ -655,8 +736,8 @@ static Value evolve(hgx_ctx* ctx, const Job& j) {
     }
     if (j.inc_nstates) out.put("NumStates", Value::integer((int64_t)reps.size()));
     if (j.inc_nevents) out.put("NumEvents", Value::integer((int64_t)n_ev_classes));
-    if (j.inc_ncausal) out.put("NumCausalEdges", Value::integer(n_causal));
-    if (j.inc_nbranchial) out.put("NumBranchialEdges", Value::integer((int64_t)R.c.branchial_pairs));
+    if (j.inc_ncausal) out.put("NumCausalEdges", Value::integer(unfold_causal >= 0 ? unfold_causal : n_causal));
+    if (j.inc_nbranchial) out.put("NumBranchialEdges", Value::integer((int64_t)n_branchial));
     if (!warns.empty()) {
         Value wl = Value::list();
         for (auto& w : warns) {
@@ -725,8 +806,6 @@ std::vector<uint8_t> run_gpu_evolution(const GpuJob& g, const HostBridge& host) 
     auto refuse = [](const std::string& what) {
         throw std::runtime_error(what + " is not served (this backend evolves the full multiway system)");
     };
-    if (g.explore_from_canonical_states_only) refuse("ExploreFromCanonicalStatesOnly");
-    if (g.quotient_initial_states) refuse("QuotientInitialStates");
     if (!g.graph_properties.empty()) refuse("GraphProperties");
     if (g.branchial_step != 0) refuse("BranchialStep");
     if (g.show_genesis_events) refuse("ShowGenesisEvents");
@@ -747,6 +826,8 @@ std::vector<uint8_t> run_gpu_evolution(const GpuJob& g, const HostBridge& host) 
     j.tr = g.transitive_reduction;
     j.full = g.state_canon_mode == GpuJob::StateCanonCode::kFull;
     j.automatic = g.state_canon_mode == GpuJob::StateCanonCode::kAutomatic;
+    j.quotient = g.explore_from_canonical_states_only;
+    j.quotient_roots = g.quotient_initial_states;
     j.smp.explore_p = g.exploration_probability;
     j.smp.rate = g.transition_rate;
     j.smp.weights = g.rule_weights;

@@ -301,6 +301,10 @@ RunHost parse_run(const hgx_run_desc* d, bool evolve) {
         if (m.on && h.cap)
             throw Error{HGX_ERR_UNSUPPORTED, "a state cap together with sampling (the cap counts states before thinning)"};
     }
+    if ((h.flags & HGX_OPT_QUOTIENT_ROOTS) && !(h.flags & HGX_OPT_QUOTIENT))
+        throw Error{HGX_ERR_INVALID, "HGX_OPT_QUOTIENT_ROOTS without HGX_OPT_QUOTIENT"};
+    if ((h.flags & HGX_OPT_QUOTIENT) && h.cap)
+        throw Error{HGX_ERR_UNSUPPORTED, "a state cap together with quotient exploration (the cap counts raw states)"};
     if (d->n_edges && (!d->edge_offsets || !d->vertices)) throw Error{HGX_ERR_INVALID, "null state arrays"};
     const uint32_t n_roots = d->root_offsets ? d->n_roots : 1;
     if (d->root_offsets && (d->n_roots == 0 || d->root_offsets[0] != 0 || d->root_offsets[d->n_roots] != d->n_edges))

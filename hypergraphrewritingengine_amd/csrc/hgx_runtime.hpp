@@ -191,6 +191,10 @@ struct RunHost {
                                  // so every state of the run does; else 0
     bool root_uni = false;       // every root edge has one arity 2..4 (the roots alone)
     uint64_t root_edges = 0;
+    // quotient exploration: only the first state of each exact class is expanded
+    bool quotient() const { return (flags & HGX_OPT_QUOTIENT) != 0; }
+    // the host decides each generation which new states are expanded
+    bool host_frontier() const { return smp.frontier() || quotient(); }
 };
 // evolve = false: the request is one state for the large-state join, not an evolution (no
 // limit on its size)

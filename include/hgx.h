@@ -61,6 +61,17 @@ enum {
 #define HGX_REC_RELATIONS   0x4u   /* also materialise the pair lists for readback   */
 #define HGX_REC_STATES      0x8u   /* keep every state's edge list for readback      */
 #define HGX_OPT_TR          0x100u /* causal pairs: keep the transitive-reduction tag */
+/* Quotient exploration (ParallelEvolutionEngine::set_explore_from_canonical_states_only,
+ * parallel_evolution.hpp:921-927; the device's state_survives_dedup with dedup on,
+ * gpu/include/hg_gpu/exploration.hpp:66-122): a new state is expanded only when it is the
+ * first state, in state-id order, of its exact (IR) isomorphism class; later states of the
+ * class are kept as states and events but never matched. The exact hashes are computed on the
+ * host between generations (a host round trip per generation, as for sampling). Every root is
+ * expanded unless HGX_OPT_QUOTIENT_ROOTS is also set, when a root isomorphic to an earlier root
+ * is not (EvolveInput::quotient_initial_states, gpu/include/hg_gpu/evolve.hpp:44-52).
+ * hgx_result_unfold gives the raw unfolding beneath such a result. */
+#define HGX_OPT_QUOTIENT        0x200u
+#define HGX_OPT_QUOTIENT_ROOTS  0x400u
 #define HGX_REC_DEFAULT     (HGX_REC_CAUSAL | HGX_REC_BRANCHIAL)
 
 /*
@@ -255,9 +266,9 @@ void hgx_replay_release(hgx_replay* replay);
  * Rule[lhs, rhs], Steps, Options; hypergraph_ffi.cpp:136-376). On HGX_OK *out holds a WXF
  * Association shaped as the GPU backend's result (States, Events, CausalEdges, BranchialEdges,
  * NumStates, NumEvents, NumCausalEdges, NumBranchialEdges, Warnings; hg_gpu_backend.cpp:362-640)
- * of *out_n bytes, to be freed with hgx_buffer_free. Options that would change what is
- * explored (sampling, caps, canonical exploration, sessions) give HGX_ERR_INVALID with the
- * reason in hgx_last_error. */
+ * of *out_n bytes, to be freed with hgx_buffer_free. Options this backend does not serve
+ * (sessions, genesis events, graph properties, per-step branchial selection, canonical
+ * exploration under content identity) give HGX_ERR_INVALID with the reason in hgx_last_error. */
 int  hgx_run_wxf(hgx_ctx* ctx, const uint8_t* job, size_t n, uint8_t** out, size_t* out_n);
 void hgx_buffer_free(uint8_t* p);
 /* Parse and check a WXF job without a device: HGX_OK, or HGX_ERR_INVALID with the reason. */
@@ -324,6 +335,31 @@ typedef struct hgx_triple {
 int hgx_result_causal_triples(const hgx_result* r, hgx_triple* triples, uint8_t* in_tr, uint64_t cap);
 /* Branchial pairs (needs HGX_REC_BRANCHIAL|HGX_REC_RELATIONS), (smaller, larger) event ids. */
 int hgx_result_branchial(const hgx_result* r, hgx_pair* pairs, uint64_t cap);
+
+/* The raw unfolding beneath an evolution: the multiway system every raw state expanded, counted
+ * without materialising it. The result's expansions are grouped into exact (IR) classes; each
+ * class's expansion is recorded once, from the first state of the class that was expanded, in
+ * slots (its edges' canonical ranks); then the device replays every class expansion against
+ * every raw instance of the class, generation by generation from one instance per root, each
+ * instance carrying the producing generation of every slot and the causal ancestry of the events
+ * on its branch. The reference's reconstruction under quotient exploration
+ * (common/include/hgcommon/quotient_replay_core.hpp:112-226 qr_apply, driven by
+ * hypergraph/src/hypergraph.cpp:977-1195; counts hypergraph.cpp:1413-1445). Needs a complete
+ * result made with HGX_REC_STATES; for a result made without HGX_OPT_QUOTIENT the counts equal the
+ * result's own (hgx_counts). States whose exact hash the canonicalizer gave up on, or above 4,096
+ * edges, give HGX_ERR_UNSUPPORTED. */
+typedef struct hgx_unfolding {
+    uint64_t raw_events;         /* applications: num_reconstructed_raw_events               */
+    uint64_t causal_triples;     /* consumed edges with a producer: num_reconstructed_causal_edges */
+    uint64_t causal_pairs;       /* distinct (producer, consumer) event pairs               */
+    uint64_t causal_pairs_tr;    /* the pairs that survive transitive reduction             */
+    uint64_t causal_triples_tr;  /* triples whose pair survives                             */
+    uint64_t branchial_pairs;    /* sibling applications with overlapping consumed edges     */
+    uint64_t instances;          /* raw states below the last step (roots included)        */
+    uint64_t classes;            /* exact classes whose expansion was recorded              */
+    uint64_t class_matches;      /* recorded (class, match) expansions                      */
+} hgx_unfolding;
+int hgx_result_unfold(hgx_ctx* ctx, const hgx_result* r, hgx_unfolding* out);
 /* Device time spent on this request's generations, and its latency
  * (completion - submission) on the served path, in nanoseconds. */
 int hgx_result_timing(const hgx_result* r, uint64_t* device_ns, uint64_t* latency_ns);

@@ -31,7 +31,7 @@ int main(int argc, char** argv) {
     int progress = 0;
     host.progress = [&](const std::string&) { ++progress; };
     int rc = 0;
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 7; ++i) {
         hgx::GpuJob job{i < 2 ? rules : rules2, i < 2 ? init : init2};
         job.steps = 3;
         if (i == 1) {
@@ -45,7 +45,12 @@ int main(int argc, char** argv) {
             job.transition_rate = 0.75;
             job.random_seed = 12345;
         }
-        if (i == 5) job.explore_from_canonical_states_only = true;
+        if (i == 5) {
+            job.explore_from_canonical_states_only = true;
+            job.quotient_initial_states = true;
+            job.event_canon_mode = hgx::GpuJob::EventCanonCode::kAutomatic;
+        }
+        if (i == 6) job.show_genesis_events = true;
         if (i == 4) {
             job.state_canon_mode = hgx::GpuJob::StateCanonCode::kAutomatic;
             job.event_canon_mode = hgx::GpuJob::EventCanonCode::kFull;

@@ -34,7 +34,7 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
     structs = {"hgx_run_desc": abi.RunDesc, "hgx_sampling": abi.Sampling, "hgx_counts": abi.Counts,
-               "hgx_event": abi.Event, "hgx_pair": abi.Pair}
+               "hgx_event": abi.Event, "hgx_pair": abi.Pair, "hgx_unfolding": abi.Unfolding}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hgx.h"', 'int main(void) {']
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

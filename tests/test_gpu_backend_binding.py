@@ -2,7 +2,8 @@
 paclet_source/hg_gpu_backend.hpp:119) called from a compiled C++ caller (tests/cpp/
 gpu_backend_test): each GpuJob's WXF result is byte-for-byte the result of the same job sent as a
 WXF envelope to hgx_run_wxf (run_rewriting_core), whose content tests/test_gpu_wxf.py checks
-against the oracle; a job asking for canonical-state exploration is refused with its reason."""
+against the oracle (tests/test_gpu_quotient.py for quotient exploration); a job asking for genesis
+events is refused with its reason."""
 import os
 import subprocess
 import tempfile
@@ -35,6 +36,10 @@ def _jobs():
                           "TransitionRate": 0.75, "RandomSeed": 12345}}
     yield 4, {"InitialStates": i2, "Rules": r2, "Steps": 3,
               "Options": {"CanonicalizeStates": Symbol("Automatic"), "CanonicalizeEvents": Symbol("Full")}}
+    yield 5, {"InitialStates": i2, "Rules": r2, "Steps": 3,
+              "Options": {"CanonicalizeStates": Symbol("Full"), "CanonicalizeEvents": Symbol("Automatic"),
+                          "ExploreFromCanonicalStatesOnly": Symbol("True"),
+                          "QuotientInitialStates": Symbol("True")}}
 
 
 def test_run_gpu_evolution_equals_the_envelope(ctx):
@@ -43,8 +48,8 @@ def test_run_gpu_evolution_equals_the_envelope(ctx):
     with tempfile.TemporaryDirectory() as td:
         p = subprocess.run([CALLER, td], capture_output=True, text=True, timeout=120)
         assert p.returncode == 0, p.stderr
-        assert "refused 5:" in p.stdout and "ExploreFromCanonicalStatesOnly" in p.stdout, p.stdout
-        assert "progress 5" in p.stdout, p.stdout
+        assert "refused 6:" in p.stdout and "ShowGenesisEvents" in p.stdout, p.stdout
+        assert "progress 6" in p.stdout, p.stdout
         for i, job in _jobs():
             got = open(os.path.join(td, f"case{i}.wxf"), "rb").read()
             want = ctx.run_wxf(dumps(job))

@@ -204,9 +204,9 @@ def test_wxf_requested_data_and_errors(ctx):
     assert set(out) == {"Events", "NumStates", "Warnings"}
     e0 = out["Events"][0]
     assert set(e0) == {"Id", "InputState", "OutputState"}
-    with pytest.raises(hgx.HGXError, match="ExploreFromCanonicalStatesOnly"):
+    with pytest.raises(hgx.HGXError, match="ShowGenesisEvents"):
         ctx.run_wxf(dumps({"InitialEdges": [[1, 2]], "Rules": {"R": Rule([[0, 1]], [[0, 1], [1, 2]])},
-                           "Steps": 2, "Options": {"ExploreFromCanonicalStatesOnly": Symbol("True")}}))
+                           "Steps": 2, "Options": {"ShowGenesisEvents": Symbol("True")}}))
 
 
 def _frame(b):

@@ -57,7 +57,9 @@ def test_encoder_roundtrip_and_valid_job():
     (_job(Options={"MaxStatesPerStep": -1}), "must be non-negative"),
     (_job(Options={"ExplorationProbability": Symbol("High")}), "must be a number"),
     (_job(Options={"RuleWeights": 0.5}), "RuleWeights"),
-    (_job(Options={"ExploreFromCanonicalStatesOnly": Symbol("True")}), "True is not served"),
+    (_job(Options={"ShowGenesisEvents": Symbol("True")}), "True is not served"),
+    (_job(Options={"ExploreFromCanonicalStatesOnly": Symbol("True"), "CanonicalizeStates": Symbol("Automatic")}),
+     "content identity"),
     (_job(Op="Open"), "sessions"),
     ({"InitialEdges": [[1, 2]], "Steps": 2}, "no Rules"),
     (_job(InitialEdges=[[-1, -2]]), "no non-empty initial state"),
