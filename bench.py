@@ -16,7 +16,9 @@ Reported beside the value:
   cpu       the reference CPU engine (oracle/_ref/hgref, built from the reference sources)
             timed on this host on a bounded sample, each run to the same generation as the GPU
             (so the same states), with its counts cross-checked against the GPU's.
-  others    config 2 (one large evolution), configs 3a/3b (large-state joins), and ir_audit:
+  others    config 2 (one large evolution), configs 3a/3b (large-state joins), a sampled
+            evolution, quotient exploration of the reference's published WPP workload (depths
+            5-7, with the raw unfolding replayed beneath it), and ir_audit:
             the host exact (IR) canonicalizer over every raw state of 256 batched runs -- its
             throughput and the WL audit (distinct WL vs distinct IR hashes per run) -- with the
             reference IRCanonicalizer (oracle/_ref/irref) on 20,000 of them, hashes compared.
@@ -203,6 +205,51 @@ def sampled_line(ctx, steps=8, rate=0.25, seed=1):
             "steps_done": c["steps_done"], "raw_states_per_s": round(c["raw_states"] / wall, 1)}
 
 
+# The reference's published quotient-exploration timings (paper/tables/t7_gpu.tex:8-10): the WPP
+# rule from {{0,1},{0,2}}, exact states, quotient exploration; raw states and events of the
+# explored evolution, and its GPU evolve wall (RTX 4090) in ms.
+T7_PUBLISHED = {5: (401, 400, 6.2), 6: (3867, 3866, 14.1), 7: (45317, 45316, 378.4)}
+
+
+def quotient_line(ctx, depths=(5, 6, 7)):
+    """Quotient exploration (ExploreFromCanonicalStatesOnly, exact states) of the WPP rule, the
+    reference's published GPU workload, and the raw unfolding beneath it replayed on the device
+    (hgx_result_unfold): explored states/events against the published ones, and the walls."""
+    from hypergraphrewritingengine_amd import abi
+    from hypergraphrewritingengine_amd.rules import Request, rule_from_lists
+    rules = [rule_from_lists([[0, 1], [0, 2]], [[0, 1], [0, 3], [1, 3], [2, 3]], 0)]
+    flags = 0x1 | 0x2 | 0x8 | 0x100 | abi.HGX_OPT_QUOTIENT
+    ctx.evolve(Request(rules, [[0, 1], [0, 2]], 4, flags=flags)).close()
+    out = {}
+    for d in depths:
+        t = time.perf_counter()
+        r = ctx.evolve(Request(rules, [[0, 1], [0, 2]], d, flags=flags))
+        t_explore = time.perf_counter() - t
+        ctx.profile(True)
+        t = time.perf_counter()
+        u = r.unfold()
+        t_unfold = time.perf_counter() - t
+        stats = ctx.profile_read()
+        ctx.profile(False)
+        c = r.counts
+        r.close()
+        row = {"explored_states": c["raw_states"], "explored_events": c["events"],
+               "explore_ms": round(t_explore * 1e3, 3), "unfold_ms": round(t_unfold * 1e3, 3),
+               "raw_events": u["raw_events"], "raw_causal_pairs_tr": u["causal_pairs_tr"],
+               "raw_branchial_pairs": u["branchial_pairs"], "classes": u["classes"]}
+        k = stats.get("unfold_apply")
+        if k:
+            row["unfold_kernel_ms"] = round(k["total_ms"], 4)
+            row["unfold_kernel_alg_GBps"] = round(k["alg_bytes"] / max(k["total_ms"], 1e-9) / 1e6, 2)
+        if d in T7_PUBLISHED:
+            s0, e0, ms = T7_PUBLISHED[d]
+            row["published"] = {"raw_states": s0, "events": e0, "gpu_ms": ms, "gpu": "RTX 4090",
+                                 "counts_equal": (c["raw_states"], c["events"]) == (s0, e0),
+                                 "explore_speedup": round(ms / (t_explore * 1e3), 2)}
+        out[f"steps{d}"] = row
+    return out
+
+
 def config3_line(ctx, name, rule, edges, no_cpu):
     """All matches of one rule in one large random state (device CSR index + seeded join)."""
     ctx.match([rule], 0, edges)
@@ -380,7 +427,7 @@ def main():
     ap.add_argument("--serve-plain-seconds", type=float, default=8.0)
     ap.add_argument("--no-serve-plain", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the config 2 / 3 lines")
-    ap.add_argument("--others", default="config2,config3a,config3b,sampled,ir_audit",
+    ap.add_argument("--others", default="config2,config3a,config3b,sampled,quotient,ir_audit",
                     help="which of the other workloads to measure (comma list)")
     ap.add_argument("--config2-steps", type=int, default=6)
     ap.add_argument("--forward", type=int, default=-1, choices=(-1, 0, 1),
@@ -575,6 +622,9 @@ def main():
     if "sampled" in want:
         others["sampled"] = sampled_line(ctx)
         print(f"[bench] sampled: {others['sampled']}", file=sys.stderr, flush=True)
+    if "quotient" in want:
+        others["quotient"] = quotient_line(ctx)
+        print(f"[bench] quotient: {others['quotient']}", file=sys.stderr, flush=True)
     if "ir_audit" in want:
         others["ir_audit"] = ir_audit_line(ctx, 256, args.no_cpu)
         print(f"[bench] ir_audit: {others['ir_audit']}", file=sys.stderr, flush=True)

@@ -218,3 +218,18 @@ def test_wxf_quotient_refusals_and_warning(ctx):
     assert out["NumEvents"] == QROWS[f"{case['name']} None full"][1]
     with pytest.raises(hgx.HGXError, match="ConsumedEdges needs InputState"):
         ctx.run_wxf(_job(case, "None", True, CanonicalizeEvents=[Symbol("ConsumedEdges")]))
+
+
+@pytest.mark.parametrize("steps,states,events", [(5, 401, 400), (6, 3867, 3866), (7, 45317, 45316)])
+def test_published_wpp_quotient_sizes(ctx, steps, states, events):
+    """The reference's published quotient workload (paper/tables/t7_gpu.tex:8-10: the WPP rule
+    from {{0,1},{0,2}}, exact states): the explored evolution has the table's raw states and
+    events at every depth; at depth 5 it equals the oracle's state by state, and its unfolding is
+    the full multiway system's."""
+    R = _rules([([[0, 1], [0, 2]], [[0, 1], [0, 3], [1, 3], [2, 3]])])
+    res = ctx.evolve(Request(R, [[0, 1], [0, 2]], steps, flags=0x1 | 0x2 | 0x8 | TR | QUO))
+    assert (res.counts["raw_states"], res.counts["events"]) == (states, events)
+    if steps == 5:
+        _assert_explored_equal(res, Q.explore(R, [[[0, 1], [0, 2]]], steps), relations=False)
+        u = res.unfold()
+        assert {k: u[k] for k in UNFOLD_KEYS} == _full_counts(R, [[[0, 1], [0, 2]]], steps)
