@@ -444,6 +444,9 @@ def main():
     ap.add_argument("--autotune", default="auto", choices=("auto", "on", "off"),
                     help="tune launch shapes on the batch before the timed passes (untimed): auto = when "
                          "no tuned shapes were loaded for this device")
+    ap.add_argument("--counter-flush", type=int, default=0, metavar="MIB",
+                    help="hardware-counter runs: stream MIB MiB through the L2 before and after every "
+                         "timed kernel (HGX_OPTION_COUNTER_FLUSH), so each kernel's write-backs are its own")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a known-size kernel for scaling "
                          "hardware byte counters)")
@@ -461,6 +464,8 @@ def main():
     ctx.set_forwarding(args.forward)
     ctx.set_expand_lanes(args.expand_lanes)
     ctx.set_join_lanes(args.join_lanes)
+    if args.counter_flush:
+        ctx.set_counter_flush(args.counter_flush)
     print(f"[bench] {ctx.info}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)

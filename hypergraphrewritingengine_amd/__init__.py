@@ -46,6 +46,7 @@ OPTION_EXPAND_TEAM = 3  # HGX_OPTION_EXPAND_TEAM
 OPTION_JOIN_TEAM = 4    # HGX_OPTION_JOIN_TEAM
 OPTION_EXPAND_LANES = 5 # HGX_OPTION_EXPAND_LANES
 OPTION_JOIN_LANES = 6   # HGX_OPTION_JOIN_LANES
+OPTION_COUNTER_FLUSH = 7  # HGX_OPTION_COUNTER_FLUSH
 
 
 class HGXError(RuntimeError):
@@ -483,6 +484,13 @@ class Context:
         """Lanes that join one parent state when a wave is the team: 64, 32 or 16 (sub-wave
         teams), 0 automatic. Results never depend on it."""
         rc = self._lib.hgx_ctx_set_option(self._h, OPTION_JOIN_LANES, int(lanes))
+        if rc != abi.HGX_OK:
+            raise self._err(rc)
+
+    def set_counter_flush(self, mib: int):
+        """Hardware-counter runs only: stream a buffer of `mib` MiB through the L2 before and
+        after every timed kernel, so each kernel's HBM write-backs are attributable (0 off)."""
+        rc = self._lib.hgx_ctx_set_option(self._h, OPTION_COUNTER_FLUSH, int(mib))
         if rc != abi.HGX_OK:
             raise self._err(rc)
 

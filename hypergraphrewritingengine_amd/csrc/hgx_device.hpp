@@ -66,7 +66,6 @@ struct DRun {
     uint32_t n_rules;
     uint32_t n_roots, flags, pad0;
     uint64_t* map_keys;      // open-addressing WL-hash set, first writer wins
-    uint32_t* map_vals;
     uint32_t map_mask;       // capacity - 1 (power of two)
     uint32_t zero_seen;      // the hash 0 is kept beside the table (0 is EMPTY)
     uint32_t ev_base;        // events before the current generation

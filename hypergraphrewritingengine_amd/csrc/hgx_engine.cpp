@@ -165,7 +165,6 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     r->map_cap = (uint32_t)mc;
     r->d_rules = (DRule*)slab_.alloc(sizeof(DRule) * r->hp->rules.size());
     r->map_keys = (uint64_t*)slab_.alloc(8 * mc);
-    r->map_vals = (uint32_t*)slab_.alloc(4 * mc);
     DRun& d = h_runs_[slot];
     std::memset(&d, 0, sizeof(d));
     d.rules = r->d_rules;
@@ -173,7 +172,6 @@ void Engine::admit(std::shared_ptr<RunRec>& r) {
     d.n_roots = r->hp->n_roots();
     d.flags = r->hp->flags;
     d.map_keys = r->map_keys;
-    d.map_vals = r->map_vals;
     d.map_mask = (uint32_t)(mc - 1);
     r->ne_bound = r->hp->max_root_ne;
     r->nv_bound = r->hp->max_root_nv;
@@ -212,8 +210,7 @@ void Engine::finalize(std::vector<std::shared_ptr<RunRec>>& runs) {
         release_next_.push_back(r->slot);
         slab_.free(r->d_rules, sizeof(DRule) * r->hp->rules.size());
         slab_.free(r->map_keys, 8ull * r->map_cap);
-        slab_.free(r->map_vals, 4ull * r->map_cap);
-        r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
+        r->d_rules = nullptr; r->map_keys = nullptr;
     }
 }
 
@@ -279,8 +276,7 @@ void Engine::recover(const std::string& why, std::vector<std::shared_ptr<RunRec>
         if (r->slot != 0xFFFFFFFFu) free_slots_.push_back(r->slot);
         slab_.free(r->d_rules, sizeof(DRule) * r->hp->rules.size());
         slab_.free(r->map_keys, 8ull * r->map_cap);
-        slab_.free(r->map_vals, 4ull * r->map_cap);
-        r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
+        r->d_rules = nullptr; r->map_keys = nullptr;
         r->slot = 0xFFFFFFFFu;
         if (r->admitted) {
             live_.erase(r->id);
@@ -295,8 +291,7 @@ void Engine::recover(const std::string& why, std::vector<std::shared_ptr<RunRec>
         if (r->slot != 0xFFFFFFFFu) free_slots_.push_back(r->slot);
         slab_.free(r->d_rules, sizeof(DRule) * r->hp->rules.size());
         slab_.free(r->map_keys, 8ull * r->map_cap);
-        slab_.free(r->map_vals, 4ull * r->map_cap);
-        r->d_rules = nullptr; r->map_keys = nullptr; r->map_vals = nullptr;
+        r->d_rules = nullptr; r->map_keys = nullptr;
         if (r->admitted) {
             live_.erase(r->id);
             live_footprint_ -= r->footprint;
@@ -607,14 +602,12 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                     while (c < need) c <<= 1;
                     if (c > 0x80000000ull) throw Error{HGX_ERR_UNSUPPORTED, "state map above 2^31 slots"};
                     uint64_t* keys = (uint64_t*)slab_.alloc(8 * c);
-                    uint32_t* vals = (uint32_t*)slab_.alloc(4 * c);
                     HIP_TRY(launch_fill(keys, 0, 8 * c, st));
-                    HIP_TRY(launch_rehash(r.map_keys, r.map_vals, r.map_cap, keys, vals, (uint32_t)(c - 1), st));
+                    HIP_TRY(launch_rehash(r.map_keys, r.map_cap, keys, (uint32_t)(c - 1), st));
                     slab_.free(r.map_keys, 8ull * r.map_cap);
-                    slab_.free(r.map_vals, 4ull * r.map_cap);
-                    r.map_keys = keys; r.map_vals = vals; r.map_cap = (uint32_t)c;
+                    r.map_keys = keys; r.map_cap = (uint32_t)c;
                     DRun& d = h_runs_[r.slot];
-                    d.map_keys = keys; d.map_vals = vals; d.map_mask = (uint32_t)(c - 1);
+                    d.map_keys = keys; d.map_mask = (uint32_t)(c - 1);
                 }
                 mv.n = (uint32_t)M; mv.K = K;
                 mv.parent = dalloc<uint32_t>(*keep, M);
@@ -830,7 +823,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                 timed(ctx_, st, "k_expand", (uint64_t)((double)bytes * x.n / M), x.n,
                       [&] { return launch_expand(ea, gcap, wpb_e, team, st, lanes); }, k);
             }
-            timed(ctx_, st, "k_dedup", M * 16ull + M * 12ull, M,
+            timed(ctx_, st, "k_dedup", M * 12ull + M * 8ull, M,
                   [&] { return launch_dedup(v, (uint32_t)M, d_runs_, d_ctr_, st); });
             // each run's event counters from its children's records: one add per run
             for (uint32_t i = 0; i < nF; ++i)                // items of at most 4,096 children
@@ -901,7 +894,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
             }
             WLStatesArgs w{};
             w.n = NR; w.A = A; w.state_off = v.eoff + M; w.ar = v.ar; w.ev = v.ev; w.nv = v.nextv + M;
-            w.out = v.hash + M; w.run_of = v.run + M; w.sid_of = v.sid + M; w.runs = d_runs_; w.ctr = d_ctr_;
+            w.out = v.hash + M; w.run_of = v.run + M; w.runs = d_runs_; w.ctr = d_ctr_;
             w.max_ne = rmax_ne; w.max_nv = rmax_nv;
             bool roots_uniform = true;
             for (auto& r : admitted) roots_uniform = roots_uniform && r->hp->root_uni;

@@ -53,7 +53,6 @@ struct RunRec {
     uint32_t slot = 0xFFFFFFFFu;
     DRule* d_rules = nullptr;
     uint64_t* map_keys = nullptr;
-    uint32_t* map_vals = nullptr;
     uint32_t map_cap = 0;
     uint64_t states = 0, events = 0;
     uint32_t depth = 0, steps_done = 0;

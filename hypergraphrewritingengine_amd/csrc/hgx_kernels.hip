@@ -345,14 +345,15 @@ uint32_t match_lds_per_wave(uint32_t max_ne, uint32_t max_nextv, uint32_t A, uin
 // W is the team: one wave per state (W = 1, up to four per workgroup, each with its own
 // workspace) or the W waves of a workgroup sharing one state and one workspace (large states,
 // whose workspace would otherwise leave a CU one or two waves). The one-wave passes are held to
-// five waves per SIMD (96 registers; the emit pass measured 2.56 -> 2.35 ms per batched pass,
-// where its unbounded 107 allow four); a bound of six spills in the join and measured slower.
+// four waves per SIMD (128 registers): at five (96) the emit pass spilled 19 registers and 96
+// bytes per lane to scratch -- 25 MB of writes per launch instead of 3 MB -- and config 2 ran 3%
+// slower, while the batched pass times alike under both bounds.
 // GWS: the workspace is a per-workgroup slice of global memory (a.scratch) -- the same code for
 // states too large for LDS, with W = 8. G < 64 (with W = 1): sub-wave teams, 64 / G states per
 // wave, each team with its own workspace, for small states whose seeds would leave most of a
 // wave's lanes idle.
 template <bool EMIT, class JoinLane, uint32_t W, bool GWS = false, uint32_t G = 64>
-__global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match(MatchArgs a) {
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match(MatchArgs a) {
     using TM = typename TeamSel<W, G>::type;
     extern __shared__ uint8_t lds[];
     constexpr uint32_t N = TM::N;
@@ -689,15 +690,17 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 5 : 1) void k_match
 // plain load that finds it (or another key: keys are only ever written once, 0 -> key) settles
 // the probe without an atomic; only an empty-looking slot is claimed by compare-and-swap, whose
 // result is authoritative if the load was stale.
-__device__ inline bool map_insert(DRun* run, uint64_t h, uint32_t sid) {
+__device__ inline bool map_insert(DRun* run, uint64_t h) {
     if (h == 0) return atomicExch(&run->zero_seen, 1u) == 0u;
     const uint32_t mask = run->map_mask;
     uint32_t i = (uint32_t)mix64(h) & mask;
     while (true) {
-        uint64_t old = __atomic_load_n(&run->map_keys[i], __ATOMIC_RELAXED);
+        // device scope: a system-scope load (the default of __atomic_load_n) is issued
+        // system-coherent and makes the L2 write the line back before reading it
+        uint64_t old = __hip_atomic_load(&run->map_keys[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0)
             old = atomicCAS((unsigned long long*)&run->map_keys[i], 0ull, (unsigned long long)h);
-        if (old == 0) { run->map_vals[i] = sid; return true; }
+        if (old == 0) return true;
         if (old == h) return false;
         i = (i + 1) & mask;
     }
@@ -962,6 +965,64 @@ __global__ void k_read_calibration(const uint4* p, uint64_t words, unsigned long
     if (lane_id() == 0) atomicAdd(sink, (unsigned long long)acc);
 }
 
+// Reads 16 bytes at the start of each of `lines` distinct 128-byte lines of the buffer: with n
+// lines in the buffer (a power of two) line (i * P) mod n, P odd, is a bijection, so the lines
+// read are distinct and spread over the whole buffer.
+__global__ void k_read_gather(const uint8_t* p, uint32_t lines, uint32_t mask, unsigned long long* sink) {
+    uint64_t acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lines; i += gridDim.x * blockDim.x) {
+        const uint32_t l = (uint32_t)(((uint64_t)i * 0x9E3779B1u) & mask);
+        const uint4 v = *(const uint4*)(p + (uint64_t)l * 128);
+        acc += (uint64_t)v.x + v.y + v.z + v.w;
+    }
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0) atomicAdd(sink, (unsigned long long)acc);
+}
+
+// One device-scope compare-and-swap on the first word of each of `lines` distinct lines (the
+// buffer holds no zero word, so every one fails and nothing changes), then one device-scope add
+// per thread into 4,096 words of 64-byte stride: the atomics' traffic, for the counters.
+__global__ void k_atomic_calibration(uint8_t* p, uint32_t lines, uint32_t mask, unsigned long long* ctr,
+                                     unsigned long long* sink) {
+    uint64_t acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lines; i += gridDim.x * blockDim.x) {
+        const uint32_t l = (uint32_t)(((uint64_t)i * 0x9E3779B1u) & mask);
+        acc += atomicCAS((unsigned long long*)(p + (uint64_t)l * 128), 0ull, 1ull);
+        atomicAdd(&ctr[(i & 4095u) * 8u], 1ull);
+    }
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0) atomicAdd(sink, (unsigned long long)acc);
+}
+
+hipError_t launch_read_gather(void* p, uint64_t bytes, uint64_t* sink, uint64_t* ctr, hipStream_t st) {
+    uint64_t n = 1;                                   // 128-byte lines in the buffer, a power of two
+    while (n * 2 * 128 <= bytes) n *= 2;
+    if (n < 4) return hipSuccess;
+    hipLaunchKernelGGL(k_read_gather, dim3(2048), dim3(256), 0, st, (const uint8_t*)p, (uint32_t)(n / 4),
+                       (uint32_t)(n - 1), (unsigned long long*)sink);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    hipLaunchKernelGGL(k_atomic_calibration, dim3(2048), dim3(256), 0, st, (uint8_t*)p, (uint32_t)(n / 4),
+                       (uint32_t)(n - 1), (unsigned long long*)ctr, (unsigned long long*)sink);
+    return hipGetLastError();
+}
+
+// HGX_OPTION_COUNTER_FLUSH: one streaming read of the flush buffer, whose lines displace every
+// line the L2 holds (the sum goes nowhere the host reads; it is kept live by a store into the
+// buffer's first word only when it is impossible, i.e. never).
+__global__ void k_flush(const uint4* p, uint64_t words) {
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 v = p[i];
+        acc += (uint64_t)v.x + v.y + v.z + v.w;
+    }
+    if (acc == 0xFFFFFFFFFFFFFFFFull) ((uint4*)p)[0] = make_uint4(1, 1, 1, 1);
+}
+
+hipError_t launch_flush(const void* buf, uint64_t bytes, hipStream_t st) {
+    hipLaunchKernelGGL(k_flush, dim3(2048), dim3(256), 0, st, (const uint4*)buf, bytes / 16);
+    return hipGetLastError();
+}
+
 hipError_t launch_read_calibration(const void* p, uint64_t bytes, uint64_t* sink, hipStream_t st) {
     const uint64_t words = bytes / 16;
     hipLaunchKernelGGL(k_read_calibration, dim3(2048), dim3(256), 0, st, (const uint4*)p, words,
@@ -1093,12 +1154,35 @@ hipError_t launch_brl(const BrListArgs& a, int phase, hipStream_t st) {
 // every child's hash into its request's state map, one lane per child, after k_expand -- so no
 // wave of k_expand waits on a global compare-and-swap chain. A key inserted first counts a
 // canonical state and adds to the fingerprint.
+//
+// Device-scope atomics are performed beyond the L2 and each moves a whole request over the
+// fabric (measured on gfx950: 64 bytes per compare-and-swap, taken or not, 32 per add), and the
+// children of one generation that hash alike all find the slot empty before any of them claims
+// it. So a lane whose (request, hash) equals a lower lane's in the wave leaves the insert to that
+// lane, and when the wave's lanes belong to one request (the usual case: a request's children
+// are contiguous) the canonical count and fingerprint are summed over the wave and added once.
 __global__ void k_dedup(GenView C, uint32_t n, DRun* runs, uint64_t* ctr) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t run = C.run[c];
-    const uint64_t h = C.hash[c];
-    if (map_insert(&runs[run], h, C.sid[c])) {
+    const bool valid = c < n;
+    const uint32_t run = valid ? C.run[c] : 0xFFFFFFFFu;
+    const uint64_t h = valid ? C.hash[c] : 0;
+    const uint32_t lane = lane_id();
+    bool dup = false;
+    for (int k = 0; k < WAVE; ++k) {
+        const uint64_t hk = __shfl(h, k, WAVE);
+        const uint32_t rk = __shfl(run, k, WAVE);
+        dup |= k < (int)lane && hk == h && rk == run;
+    }
+    const bool fresh = valid && !dup && map_insert(&runs[run], h);
+    const uint32_t r0 = __shfl(run, 0, WAVE);
+    if (__ballot(valid && run != r0) == 0) {
+        const uint64_t nfresh = (uint64_t)__popcll(__ballot(fresh));
+        const uint64_t fp = wave_sum_u64(fresh ? mix64(h) : 0ull);
+        if (lane == 0 && nfresh) {
+            atomicAdd((unsigned long long*)&ctr[(size_t)r0 * CTR_N + CTR_CANON], (unsigned long long)nfresh);
+            atomicAdd((unsigned long long*)&ctr[(size_t)r0 * CTR_N + CTR_FINGERPRINT], (unsigned long long)fp);
+        }
+    } else if (fresh) {
         atomicAdd((unsigned long long*)&ctr[(size_t)run * CTR_N + CTR_CANON], 1ull);
         atomicAdd((unsigned long long*)&ctr[(size_t)run * CTR_N + CTR_FINGERPRINT], (unsigned long long)mix64(h));
     }
@@ -1299,7 +1383,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_wl_states(WLStatesArg
             a.out[i] = h;
             if (a.run_of) {
                 DRun* RN = &a.runs[a.run_of[i]];
-                if (map_insert(RN, h, a.sid_of[i])) {
+                if (map_insert(RN, h)) {
                     atomicAdd((unsigned long long*)&a.ctr[(size_t)a.run_of[i] * CTR_N + CTR_CANON], 1ull);
                     atomicAdd((unsigned long long*)&a.ctr[(size_t)a.run_of[i] * CTR_N + CTR_FINGERPRINT],
                               (unsigned long long)mix64(h));
@@ -1386,8 +1470,7 @@ __global__ void k_mask_counts(const uint32_t* run, const uint8_t* active, uint32
 }
 
 // Move one run's state map into a larger table.
-__global__ void k_rehash(const uint64_t* old_keys, const uint32_t* old_vals, uint32_t old_cap,
-                         uint64_t* keys, uint32_t* vals, uint32_t mask) {
+__global__ void k_rehash(const uint64_t* old_keys, uint32_t old_cap, uint64_t* keys, uint32_t mask) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= old_cap) return;
     const uint64_t h = old_keys[i];
@@ -1395,7 +1478,7 @@ __global__ void k_rehash(const uint64_t* old_keys, const uint32_t* old_vals, uin
     uint32_t j = (uint32_t)mix64(h) & mask;
     while (true) {
         const uint64_t old = atomicCAS((unsigned long long*)&keys[j], 0ull, (unsigned long long)h);
-        if (old == 0) { vals[j] = old_vals[i]; return; }
+        if (old == 0) return;
         j = (j + 1) & mask;
     }
 }
@@ -1565,10 +1648,9 @@ hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_rehash(const uint64_t* ok, const uint32_t* ov, uint32_t old_cap, uint64_t* keys,
-                         uint32_t* vals, uint32_t mask, hipStream_t st) {
-    hipLaunchKernelGGL(k_rehash, dim3((old_cap + 255) / 256), dim3(256), 0, st, ok, ov, old_cap, keys,
-                       vals, mask);
+hipError_t launch_rehash(const uint64_t* ok, uint32_t old_cap, uint64_t* keys, uint32_t mask, hipStream_t st) {
+    if (!old_cap) return hipSuccess;
+    hipLaunchKernelGGL(k_rehash, dim3((old_cap + 255) / 256), dim3(256), 0, st, ok, old_cap, keys, mask);
     return hipGetLastError();
 }
 

@@ -140,7 +140,6 @@ struct WLStatesArgs {
     const uint32_t* nv;        // [n] distinct vertices per state
     uint64_t* out;             // [n]
     const uint32_t* run_of;    // optional: insert into run maps
-    const uint32_t* sid_of;
     DRun* runs;
     uint64_t* ctr;
     uint32_t max_ne, max_nv;
@@ -211,12 +210,12 @@ hipError_t launch_mask_counts(const uint32_t* run, const uint8_t* active, uint32
 hipError_t launch_fill(void* p, uint8_t value, uint64_t bytes, hipStream_t st);
 // A read-only launch of known traffic (bytes read, 8 written) for counter calibration.
 hipError_t launch_read_calibration(const void* p, uint64_t bytes, uint64_t* sink, hipStream_t st);
+hipError_t launch_read_gather(void* p, uint64_t bytes, uint64_t* sink, uint64_t* ctr, hipStream_t st);
 // Replay check of a recorded pass: flag[0] |= 1 when a run's child total (tot[slot], slots < n)
 // leaves the recording's -- a different cap decision against budget[slot], or, within the budget,
 // a different total.
 hipError_t launch_plan_check(const uint64_t* tot, const uint64_t* want, const uint32_t* budget, uint32_t n,
                              uint32_t* flag, hipStream_t st);
-hipError_t launch_rehash(const uint64_t* ok, const uint32_t* ov, uint32_t old_cap, uint64_t* keys,
-                         uint32_t* vals, uint32_t mask, hipStream_t st);
+hipError_t launch_rehash(const uint64_t* ok, uint32_t old_cap, uint64_t* keys, uint32_t mask, hipStream_t st);
 
 }  // namespace hgx

@@ -396,7 +396,7 @@ uint64_t footprint_bytes(const RunHost& h, uint64_t held) {
     const uint64_t per_state = 2 * max_ne * (4ull * h.max_arity + 2) + 96 + 8ull * h.steps;
     uint64_t map = 64;
     while (map < 2 * states) map <<= 1;
-    return states * per_state + map * 12 + h.rules.size() * sizeof(DRule) + 4096;
+    return states * per_state + map * 8 + h.rules.size() * sizeof(DRule) + 4096;
 }
 
 struct CastU64 {
@@ -541,6 +541,7 @@ extern "C" int hgx_ctx_create(int device, uint64_t budget_bytes, hgx_ctx** out) 
 extern "C" void hgx_ctx_destroy(hgx_ctx* ctx) {
     if (!ctx) return;
     engine_destroy(ctx->serve);
+    if (ctx->flush_buf) (void)hipFree(ctx->flush_buf);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -579,6 +580,21 @@ extern "C" int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value) 
     if (option == HGX_OPTION_JOIN_LANES && (value == 0 || value == 16 || value == 32 || value == 64)) {
         ctx->join_lanes = (uint32_t)value;
         return HGX_OK;
+    }
+    if (option == HGX_OPTION_COUNTER_FLUSH && (value == 0 || (value >= 16 && value <= 1024))) {
+        return guarded(ctx, [&]() -> int {
+            HIP_TRY(hipSetDevice(ctx->device));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            if (ctx->flush_buf) HIP_TRY(hipFree(ctx->flush_buf));
+            ctx->flush_buf = nullptr;
+            ctx->flush_bytes = 0;
+            if (value) {
+                HIP_TRY(hipMalloc(&ctx->flush_buf, (size_t)value << 20));
+                HIP_TRY(launch_fill(ctx->flush_buf, 0, (size_t)value << 20, ctx->stream));
+                ctx->flush_bytes = (uint64_t)value << 20;
+            }
+            return HGX_OK;
+        });
     }
     ctx->err = "unknown option or value";
     return HGX_ERR_INVALID;
@@ -655,10 +671,14 @@ extern "C" int hgx_calibration_read(hgx_ctx* ctx, uint64_t bytes) {
         Arena ar(ctx->pool.get());
         uint8_t* buf = dalloc<uint8_t>(ar, bytes);
         uint64_t* sink = dalloc<uint64_t>(ar, 1);
+        uint64_t* ctr = dalloc<uint64_t>(ar, 4096 * 8);
         HIP_TRY(launch_fill(buf, 1, bytes, ctx->stream));
         HIP_TRY(launch_fill(sink, 0, 8, ctx->stream));
+        HIP_TRY(launch_fill(ctr, 0, 4096 * 64, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         HIP_TRY(launch_read_calibration(buf, bytes, sink, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(launch_read_gather(buf, bytes, sink, ctr, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         return HGX_OK;
     });

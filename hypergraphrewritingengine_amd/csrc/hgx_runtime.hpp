@@ -271,23 +271,33 @@ struct hgx_ctx {
     uint32_t join_team = 0;                      // HGX_OPTION_JOIN_TEAM
     uint32_t expand_lanes = 0;                   // HGX_OPTION_EXPAND_LANES
     uint32_t join_lanes = 0;                     // HGX_OPTION_JOIN_LANES
+    uint64_t flush_bytes = 0;                    // HGX_OPTION_COUNTER_FLUSH
+    void* flush_buf = nullptr;                   // its buffer (hipMalloc'd, freed with the context)
     hgx::Tune tune;                              // launch shapes per size class
 };
 
 namespace hgx {
 
 // Launch with optional event timing; `bytes` are the launch's algorithmic HBM bytes.
+// HGX_OPTION_COUNTER_FLUSH: streams the flush buffer through the L2 (counter runs only).
+hipError_t launch_flush(const void* buf, uint64_t bytes, hipStream_t st);
+
 template <class F>
 void timed(hgx_ctx* ctx, hipStream_t st, const char* name, uint64_t bytes, uint64_t items, F&& launch,
            int cls = -1) {
-    if (!ctx->prof.on) { HIP_TRY(launch()); return; }
-    ProfPending p{name, nullptr, nullptr, bytes, items, cls};
-    HIP_TRY(hipEventCreate(&p.a));
-    HIP_TRY(hipEventCreate(&p.b));
-    HIP_TRY(hipEventRecord(p.a, st));
-    HIP_TRY(launch());
-    HIP_TRY(hipEventRecord(p.b, st));
-    ctx->prof.pend.push_back(p);
+    if (ctx->flush_bytes) HIP_TRY(launch_flush(ctx->flush_buf, ctx->flush_bytes, st));
+    if (!ctx->prof.on) {
+        HIP_TRY(launch());
+    } else {
+        ProfPending p{name, nullptr, nullptr, bytes, items, cls};
+        HIP_TRY(hipEventCreate(&p.a));
+        HIP_TRY(hipEventCreate(&p.b));
+        HIP_TRY(hipEventRecord(p.a, st));
+        HIP_TRY(launch());
+        HIP_TRY(hipEventRecord(p.b, st));
+        ctx->prof.pend.push_back(p);
+    }
+    if (ctx->flush_bytes) HIP_TRY(launch_flush(ctx->flush_buf, ctx->flush_bytes, st));
 }
 void prof_resolve(hgx_ctx* ctx);
 // Loads the tuned launch shapes of this device (key "<arch>:<CUs>") from a cache written by

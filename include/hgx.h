@@ -216,6 +216,11 @@ size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
  *   HGX_OPTION_JOIN_LANES lanes that join one parent state when one wave is the team: 64, or
  *                       32 / 16 (two or four states per wave, each in its own workspace); 0
  *                       automatic (the tuned or default shape per size class).
+ *   HGX_OPTION_COUNTER_FLUSH for hardware-counter runs only: MiB (16..1024) of a buffer that a
+ *                       read kernel (k_flush) streams through before and after every kernel the
+ *                       engine times, so the L2 holds no dirty line of another kernel while one
+ *                       runs and a kernel's write-backs land in it or in the flush after it;
+ *                       0 off (default). Results are unchanged; passes take longer.
  * Returns HGX_ERR_INVALID for an unknown option or value. */
 #define HGX_OPTION_FORWARD 1
 #define HGX_OPTION_DELTA_CAP 2
@@ -223,6 +228,7 @@ size_t hgx_device_info(const hgx_ctx* ctx, char* buf, size_t len);
 #define HGX_OPTION_JOIN_TEAM 4
 #define HGX_OPTION_EXPAND_LANES 5
 #define HGX_OPTION_JOIN_LANES 6
+#define HGX_OPTION_COUNTER_FLUSH 7
 int hgx_ctx_set_option(hgx_ctx* ctx, uint32_t option, int64_t value);
 
 /* ---- synchronous batch: the plain and the batched path -------------------------------- */
@@ -466,8 +472,12 @@ typedef struct hgx_kernel_stat {
 } hgx_kernel_stat;
 int hgx_profile_enable(hgx_ctx* ctx, int on);
 /* Counter calibration: one fill kernel writes `bytes` (a multiple of 16), then one read-only
- * kernel (k_read_calibration) reads them once and writes 8 bytes -- launches of known traffic for
- * scaling FETCH_SIZE / WRITE_SIZE readings. No counterpart in the reference. */
+ * kernel (k_read_calibration) reads them once and writes 8 bytes, then a gather kernel
+ * (k_read_gather) reads 16 bytes from each of n/4 distinct 128-byte lines scattered over the
+ * buffer (n = the buffer's 128-byte lines rounded down to a power of two), then one kernel
+ * (k_atomic_calibration) makes one failing compare-and-swap on each of those lines and one add
+ * per line into 4,096 counters -- launches of known traffic for scaling FETCH_SIZE / WRITE_SIZE
+ * readings: streamed, scattered, and device-scope atomics. No counterpart in the reference. */
 int hgx_calibration_read(hgx_ctx* ctx, uint64_t bytes);
 /* Copies up to cap entries, sets *n, and clears the accumulators. */
 int hgx_profile_read(hgx_ctx* ctx, hgx_kernel_stat* out, uint32_t cap, uint32_t* n);

@@ -273,3 +273,20 @@ def test_join_lanes_option_rejects_bad_values(ctx):
         with pytest.raises(hgx.HGXError):
             ctx.set_join_lanes(bad)
     ctx.set_join_lanes(0)
+
+
+def test_counter_flush_changes_nothing(ctx):
+    """HGX_OPTION_COUNTER_FLUSH (hardware-counter runs) streams a buffer through the L2 around
+    every timed kernel; every result stays the oracle's, and bad sizes are refused."""
+    import hypergraphrewritingengine_amd as hgx
+    reqs = [W.batched_run(i, cap=1024, flags=REL | TR) for i in range(0, 96, 12)]
+    for bad in (1, 15, 2048, -1):
+        with pytest.raises(hgx.HGXError):
+            ctx.set_counter_flush(bad)
+    try:
+        ctx.set_counter_flush(64)
+        r = ctx.evolve_batch(reqs)
+    finally:
+        ctx.set_counter_flush(0)
+    for i, req in enumerate(reqs):
+        assert_exact(r[i], po.evolve(req))
