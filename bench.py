@@ -211,10 +211,11 @@ def sampled_line(ctx, steps=8, rate=0.25, seed=1):
 T7_PUBLISHED = {5: (401, 400, 6.2), 6: (3867, 3866, 14.1), 7: (45317, 45316, 378.4)}
 
 
-def quotient_line(ctx, depths=(5, 6, 7)):
+def quotient_line(ctx, depths=(5, 6, 7, 8)):
     """Quotient exploration (ExploreFromCanonicalStatesOnly, exact states) of the WPP rule, the
     reference's published GPU workload, and the raw unfolding beneath it replayed on the device
-    (hgx_result_unfold): explored states/events against the published ones, and the walls."""
+    (hgx_result_unfold): explored states/events against the published ones (depths 5-7; depth 8
+    has no published figure), and the walls."""
     from hypergraphrewritingengine_amd import abi
     from hypergraphrewritingengine_amd.rules import Request, rule_from_lists
     rules = [rule_from_lists([[0, 1], [0, 2]], [[0, 1], [0, 3], [1, 3], [2, 3]], 0)]

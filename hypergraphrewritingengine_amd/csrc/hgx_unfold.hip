@@ -148,10 +148,15 @@ void check(int rc, const char* what) {
     if (rc != HGX_OK) throw Error{rc, std::string("unfold: reading the result's ") + what + " failed"};
 }
 
-Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c) {
+// `steps`: the requested generations. The states of generation `steps` are never expanded, and
+// a child of that generation is only ever written for an instance of a class first met there --
+// which is never replayed below it -- so their classes are not needed: unless `all_states`
+// (sampled requests, whose recorded state of a class need not be its first), they are left out
+// of the exact canonicalization and point at an empty sentinel class.
+Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c, uint32_t steps, bool all_states) {
     const uint64_t ns = c.raw_states, ne = c.events;
-    std::vector<uint32_t> creator(ns);
-    check(hgx_result_states(res, nullptr, nullptr, creator.data(), ns), "states");
+    std::vector<uint32_t> creator(ns), step(ns);
+    check(hgx_result_states(res, nullptr, step.data(), creator.data(), ns), "states");
     std::vector<hgx_event> ev(ne);
     if (ne) check(hgx_result_events(res, ev.data(), ne), "events");
     uint64_t te = 0, tv = 0;
@@ -168,15 +173,19 @@ Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c) {
         if (soff[s + 1] - soff[s] > kUnfoldMaxSlots)
             throw Error{HGX_ERR_UNSUPPORTED, "unfold: a state above 4096 edges"};
 
-    // exact hash and canonical rank of every edge of every state
-    std::vector<uint64_t> ir(ns);
-    std::vector<uint32_t> rank(std::max<uint64_t>(te, 1));
-    if (ns) check(hgx_ir_rank_batch(nullptr, soff.data(), (uint32_t)ns, eoff.data(), verts.data(), ir.data(),
+    // exact hash and canonical rank of every edge of the states that can be replayed (states are
+    // numbered generation by generation, so those below generation `steps` are a prefix)
+    uint64_t nr = ns;
+    if (!all_states)
+        while (nr > 0 && step[nr - 1] >= steps) --nr;
+    std::vector<uint64_t> ir(nr);
+    std::vector<uint32_t> rank(std::max<uint64_t>(soff[nr], 1));
+    if (nr) check(hgx_ir_rank_batch(nullptr, soff.data(), (uint32_t)nr, eoff.data(), verts.data(), ir.data(),
                                     rank.data()), "exact hashes");
     std::unordered_map<uint64_t, uint32_t> cls_of;
-    std::vector<uint32_t> cls(ns);
+    std::vector<uint32_t> cls(ns, 0xFFFFFFFFu);
     Skeleton K;
-    for (uint64_t s = 0; s < ns; ++s) {
+    for (uint64_t s = 0; s < nr; ++s) {
         if (!ir[s] && soff[s + 1] > soff[s])
             throw Error{HGX_ERR_UNSUPPORTED, "unfold: the exact canonicalizer gave up on a state"};
         auto it = cls_of.emplace(ir[s], (uint32_t)K.cls_slots.size());
@@ -185,7 +194,8 @@ Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c) {
         K.S = std::max<uint32_t>(K.S, (uint32_t)(soff[s + 1] - soff[s]));
         if (creator[s] == 0xFFFFFFFFu) K.roots.push_back(cls[s]);
     }
-    const uint32_t C = (uint32_t)K.cls_slots.size();
+    const uint32_t C = (uint32_t)K.cls_slots.size();      // the sentinel is class C
+    for (uint64_t s = nr; s < ns; ++s) cls[s] = C;
 
     // each class's expansion, recorded from the first of its states that was expanded (events
     // are numbered by generation, then input state: the first event of a class names that state)
@@ -193,6 +203,7 @@ Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c) {
     std::vector<std::vector<uint32_t>> by_cls(C);
     for (uint64_t e = 0; e < ne; ++e) {
         const uint32_t in = ev[e].input_state, k = cls[in];
+        if (k >= C) throw Error{HGX_ERR_DEVICE, "unfold: an event leaves a state of the last generation"};
         if (rec[k] == 0xFFFFFFFFu) rec[k] = in;
         if (rec[k] == in) by_cls[k].push_back((uint32_t)e);
     }
@@ -203,23 +214,27 @@ Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c) {
         const uint32_t mb = (uint32_t)K.m_to.size();
         for (uint32_t e : by_cls[k]) {
             const hgx_event& x = ev[e];
-            const uint64_t r0 = soff[x.input_state], nr = soff[x.input_state + 1] - r0;
+            const uint64_t r0 = soff[x.input_state], np = soff[x.input_state + 1] - r0;
             const uint64_t x0 = soff[x.output_state], nx = soff[x.output_state + 1] - x0;
             const uint32_t nk = coff[e + 1] - coff[e];
-            if (nr < nk || nx < nr - nk) throw Error{HGX_ERR_DEVICE, "unfold: an event's states are inconsistent"};
-            taken.assign(nr, 0);
+            if (np < nk || nx < np - nk) throw Error{HGX_ERR_DEVICE, "unfold: an event's states are inconsistent"};
+            taken.assign(np, 0);
             for (uint32_t q = coff[e]; q < coff[e + 1]; ++q) {
-                if (cpos[q] >= nr) throw Error{HGX_ERR_DEVICE, "unfold: a consumed position is out of range"};
+                if (cpos[q] >= np) throw Error{HGX_ERR_DEVICE, "unfold: a consumed position is out of range"};
                 taken[cpos[q]] = 1;
                 K.m_cons.push_back((uint16_t)rank[r0 + cpos[q]]);
             }
-            // the child keeps the parent's unconsumed edges in order, then the produced ones
-            src.assign(nx, kUnfoldProduced);
-            uint64_t j = 0;
-            for (uint64_t p = 0; p < nr; ++p)
-                if (!taken[p]) src[rank[x0 + j++]] = (uint16_t)rank[r0 + p];
-            K.m_src.insert(K.m_src.end(), src.begin(), src.end());
-            K.m_to.push_back(cls[x.output_state]);
+            // the child keeps the parent's unconsumed edges in order, then the produced ones (a
+            // child of the last generation has no slots: its class is the sentinel)
+            const uint32_t to = cls[x.output_state];
+            if (to < C) {
+                src.assign(nx, kUnfoldProduced);
+                uint64_t j = 0;
+                for (uint64_t p = 0; p < np; ++p)
+                    if (!taken[p]) src[rank[x0 + j++]] = (uint16_t)rank[r0 + p];
+                K.m_src.insert(K.m_src.end(), src.begin(), src.end());
+            }
+            K.m_to.push_back(to);
             K.m_coff.push_back((uint32_t)K.m_cons.size());
             K.m_soff.push_back((uint32_t)K.m_src.size());
         }
@@ -238,6 +253,8 @@ Skeleton build_skeleton(const hgx_result* res, const hgx_counts& c) {
             }
         K.cls_br[k] = b;
     }
+    K.cls_moff.push_back(K.cls_moff.back());              // the sentinel: no slots, no matches
+    K.cls_br.push_back(0);
     return K;
 }
 
@@ -263,13 +280,13 @@ extern "C" int hgx_result_unfold(hgx_ctx* ctx, const hgx_result* res, hgx_unfold
         if (hgx_result_status(res, nullptr, 0) != HGX_OK || c.truncated)
             throw Error{HGX_ERR_INVALID, "unfold: the result is partial (a state cap or the HBM budget stopped it)"};
         std::memset(out, 0, sizeof(*out));
-        const Skeleton K = build_skeleton(res, c);
-        out->classes = 0;
-        for (size_t k = 0; k + 1 < K.cls_moff.size(); ++k) out->classes += K.cls_moff[k + 1] > K.cls_moff[k];
-        out->class_matches = K.m_to.size();
         // the requested generations: a quotient run whose new states all fell into known classes
         // stops early, while the raw unfolding beneath it goes on
         const uint32_t steps = res->run->hp->steps;
+        const Skeleton K = build_skeleton(res, c, steps, res->run->hp->smp.on || steps == 0);
+        out->classes = 0;
+        for (size_t k = 0; k + 1 < K.cls_moff.size(); ++k) out->classes += K.cls_moff[k + 1] > K.cls_moff[k];
+        out->class_matches = K.m_to.size();
         if (K.roots.empty() || steps == 0) {
             out->instances = K.roots.size();
             return HGX_OK;

@@ -233,3 +233,40 @@ def test_published_wpp_quotient_sizes(ctx, steps, states, events):
         _assert_explored_equal(res, Q.explore(R, [[[0, 1], [0, 2]]], steps), relations=False)
         u = res.unfold()
         assert {k: u[k] for k in UNFOLD_KEYS} == _full_counts(R, [[[0, 1], [0, 2]]], steps)
+
+
+def test_quotient_requests_served_and_batched_in_a_mix(ctx):
+    """Quotient requests beside full-capture and sampled ones, in one batch and on the served
+    path: each result equals the same request evolved alone (and the quotient ones the oracle)."""
+    qreqs = []
+    for k, rs in enumerate(RULESETS):
+        qreqs.append(Request(_rules(rs), [ROOTS[k % len(ROOTS)]], 3, flags=REL | TR | QUO, multi_root=True))
+    others = [W.batched_run(i, cap=1024, flags=REL | TR) for i in range(4)]
+    sampled = W.config2(4, flags=REL | TR)
+    sampled.sampling = {"rate": 0.5, "seed": 3}
+    mix = [qreqs[0], others[0], qreqs[1], sampled, qreqs[2], others[1], qreqs[3], others[2], qreqs[4], others[3]]
+    alone = [ctx.evolve(r) for r in mix]
+    batch = ctx.evolve_batch(mix)
+    ids = [ctx.submit(r) for r in mix]
+    served = [ctx.wait(i) for i in ids]
+    for k, r in enumerate(mix):
+        for got in (batch[k], served[k]):
+            assert got.counts == alone[k].counts, k
+            assert _edges(got) == _edges(alone[k]), k
+            assert (got.events() == alone[k].events()).all(), k
+        if r.flags & QUO:
+            x = Q.explore(r.rules, r.init, r.steps)
+            _assert_explored_equal(batch[k], x)
+            assert batch[k].unfold() == served[k].unfold() == alone[k].unfold()
+
+
+def test_unfold_of_a_thinned_full_capture_is_its_own_counts(ctx):
+    """A transition draw keyed on the transition's exact class keeps the same matches on every
+    state of a class, so the replay of a thinned full evolution is that evolution: its counts."""
+    req = W.config2(5, flags=REL | TR)
+    req.sampling = {"rate": 0.5, "seed": 11}
+    res = ctx.evolve(req)
+    c = res.counts
+    u = res.unfold()
+    assert u["raw_events"] == c["events"] > 0
+    assert {k: u[k] for k in UNFOLD_KEYS[1:]} == {k: c[k] for k in UNFOLD_KEYS[1:]}
