@@ -61,7 +61,7 @@ struct UnfoldArgs {
     uint32_t* ccls;              // [W] children (write)
     uint8_t* cprod;              // [W*S]
     uint64_t* canc;              // [W*(d+1)]
-    unsigned long long* acc;     // [5] triples, pairs, pairs_tr, triples_tr, branchial
+    unsigned long long* part;    // [gridDim.x * 5] this launch's per-workgroup sums
 };
 
 __global__ void k_unfold_count(uint32_t N, const uint32_t* __restrict__ cls, const uint32_t* __restrict__ cls_moff,
@@ -122,13 +122,37 @@ __global__ __launch_bounds__(256) void k_unfold_apply(UnfoldArgs a) {
             ca[a.d] = P | up;
         }
     }
+    // per-workgroup sums written to the workgroup's own slots: thousands of workgroups adding into
+    // the same five words would serialise on them
+    __shared__ unsigned long long ws[4][5];
     trip = wave_sum(trip); pairs = wave_sum(pairs); ptr = wave_sum(ptr); ttr = wave_sum(ttr); br = wave_sum(br);
+    const uint32_t wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (trip) atomicAdd(a.acc + 0, trip);
-        if (pairs) atomicAdd(a.acc + 1, pairs);
-        if (ptr) atomicAdd(a.acc + 2, ptr);
-        if (ttr) atomicAdd(a.acc + 3, ttr);
-        if (br) atomicAdd(a.acc + 4, br);
+        ws[wv][0] = trip; ws[wv][1] = pairs; ws[wv][2] = ptr; ws[wv][3] = ttr; ws[wv][4] = br;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long t = 0;
+        for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) t += ws[k][threadIdx.x];
+        a.part[(uint64_t)blockIdx.x * 5 + threadIdx.x] = t;
+    }
+}
+
+// The launch's per-workgroup sums into the five totals (one workgroup of 256).
+__global__ __launch_bounds__(256) void k_unfold_reduce(const unsigned long long* part, uint32_t nblocks,
+                                                       unsigned long long* acc) {
+    __shared__ unsigned long long ws[4][5];
+    unsigned long long t[5] = {0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 5; ++k) t[k] += part[(uint64_t)b * 5 + k];
+    for (int k = 0; k < 5; ++k) t[k] = wave_sum(t[k]);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; ++k) ws[threadIdx.x >> 6][k] = t[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long v = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) v += ws[w][threadIdx.x];
+        acc[threadIdx.x] += v;
     }
 }
 
@@ -301,6 +325,7 @@ extern "C" int hgx_result_unfold(hgx_ctx* ctx, const hgx_result* res, hgx_unfold
         const uint32_t* d_m_soff = put(tables, K.m_soff, st);
         const uint16_t* d_m_src = put(tables, K.m_src, st);
         unsigned long long* d_acc = dalloc<unsigned long long>(tables, 5);
+        unsigned long long* d_part = dalloc<unsigned long long>(tables, 5ull * ctx->num_cu * 8);
         const std::vector<unsigned long long> zero(5, 0);
         h2d(d_acc, zero.data(), 5, st);
         const uint32_t S = K.S;
@@ -336,7 +361,7 @@ extern "C" int hgx_result_unfold(hgx_ctx* ctx, const hgx_result* res, hgx_unfold
             a.W = W; a.N = N; a.d = d; a.S = S; a.write = write ? 1u : 0u;
             a.wofs = wofs; a.cls = d_cls; a.prod = d_prod; a.anc = d_anc;
             a.cls_moff = d_cls_moff; a.cls_br = d_cls_br; a.m_to = d_m_to; a.m_coff = d_m_coff;
-            a.m_cons = d_m_cons; a.m_soff = d_m_soff; a.m_src = d_m_src; a.acc = d_acc;
+            a.m_cons = d_m_cons; a.m_soff = d_m_soff; a.m_src = d_m_src; a.part = d_part;
             if (write) {
                 a.ccls = dalloc<uint32_t>(*next, W);
                 a.cprod = dalloc<uint8_t>(*next, W * S);
@@ -349,6 +374,9 @@ extern "C" int hgx_result_unfold(hgx_ctx* ctx, const hgx_result* res, hgx_unfold
                 hipLaunchKernelGGL(k_unfold_apply, dim3((uint32_t)blocks), dim3(256), 0, st, a);
                 return hipGetLastError();
             });
+            hipLaunchKernelGGL(k_unfold_reduce, dim3(1), dim3(256), 0, st, (const unsigned long long*)d_part,
+                               (uint32_t)blocks, d_acc);
+            HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(st));
             if (!write) break;
             lvl = std::move(next);
