@@ -230,6 +230,17 @@ constexpr int kExpandClasses = 6;
 constexpr uint32_t kExpandEdges[kExpandClasses] = {32, 64, 128, 512, 2048, 0xFFFFFFFFu};
 static_assert(kExpandClasses == kTuneExpandClasses, "one tuned shape per expansion size class");
 static_assert(sizeof(kClassEdges) / sizeof(kClassEdges[0]) == kTuneJoinClasses, "one tuned shape per join size class");
+
+// A stride near n / golden ratio and coprime to n, so i -> i * stride mod n visits 0..n-1 once
+// each, consecutive i far apart (1 for n <= 2: in order).
+uint32_t coprime_stride(uint32_t n) {
+    if (n <= 2) return 1;
+    uint32_t s = (uint32_t)((double)n * 0.6180339887498949);
+    if (s < 1) s = 1;
+    auto gcd = [](uint32_t a, uint32_t b) { while (b) { const uint32_t t = a % b; a = b; b = t; } return a; };
+    while (gcd(s, n) != 1) ++s;
+    return s;
+}
 }  // namespace
 
 uint32_t Engine::step() {
@@ -468,6 +479,7 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                         mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
                         mc.n_items = (uint32_t)x.n;
                     }
+                    if (!emit) mc.spread = coprime_stride((uint32_t)x.n);
                     // sub-wave teams for small states, when the wave's workspaces fit a workgroup
                     uint32_t lanes = 64;
                     if (team == 1 && !mc.scratch) {

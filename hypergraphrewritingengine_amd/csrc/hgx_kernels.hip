@@ -394,11 +394,16 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
     const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
     for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? ti : 0); ls < n_items; ls += teams) {
-        uint32_t s = ls;
+        // The count pass visits the states scattered over the launch, so each request's states
+        // are counted at an even pace from start to end: a request over its cap passes it with
+        // most of its states still ahead, and those are skipped at the tally check below
+        // (in launch order a request's states are counted at once, and the check stops few).
+        const uint32_t li = (!EMIT && a.spread > 1) ? (uint32_t)((uint64_t)ls * a.spread % n_items) : ls;
+        uint32_t s = li;
         if (a.nrange) {                                         // subset: logical -> state
             uint32_t lo = 0, hi = a.nrange;
-            while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= ls) lo = mid; else hi = mid; }
-            s = a.rb[lo] + (ls - a.rpre[lo]);
+            while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (a.rpre[mid] <= li) lo = mid; else hi = mid; }
+            s = a.rb[lo] + (li - a.rpre[lo]);
         }
         // the state's record and its request's flags in two rounds of independent loads
         const uint32_t run = a.P.run[s];

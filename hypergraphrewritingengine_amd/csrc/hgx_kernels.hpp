@@ -41,6 +41,9 @@ struct MatchArgs {
     const uint32_t* rpre;
     uint32_t nrange;
     uint32_t n_items;
+    // Count pass: logical index ls is visited as (ls * spread) mod the launch's states (spread
+    // coprime to that number; 0 or 1: in order).
+    uint32_t spread;
     // Small rules: the count pass's matches per seed of each fully joined state, kept for the
     // emit pass (which then joins each seed once, not twice): [2 * P's edges], state s's seeds
     // of rule r at 2 * eoff[s] + r * ne[s]; 0xFFFF means "at least that many, join again".
