@@ -433,8 +433,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
             }
         tm.sync();
         const uint32_t nslot = tm.excl_scan_lds(iend, nvid + 1);    // iend[x] = start of x's list
-        for (uint32_t x = tm.r; x < nvid; x += N)
-            for (uint32_t i = iend[x]; i < iend[x + 1]; ++i) iown[i] = x;
+        // The emit pass writes matches in the full join's order, and the delta joins of the
+        // forwarding path (join_anchor) stop a list at the first edge past their limit: both need
+        // each list in ascending edge order. The count pass's full join enumerates every
+        // candidate of a list, whatever its order, and joins over the lists as filled.
+        const bool sorted = EMIT || s < a.fwd_n;
+        if (sorted)
+            for (uint32_t x = tm.r; x < nvid; x += N)
+                for (uint32_t i = iend[x]; i < iend[x + 1]; ++i) iown[i] = x;
         tm.sync();
         for (uint32_t e = tm.r; e < ne; e += N)
             for (uint32_t k = 0; k < sa[e]; ++k) {
@@ -446,16 +452,19 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
         tm.sync();
         // each list into ascending edge order: an entry's place is the number of smaller entries
         // of its list, so a hub vertex's list is placed by as many lanes as it has entries
-        for (uint32_t j = tm.r; j < nslot; j += N) {
-            const uint32_t x = iown[j];
-            const uint32_t b = x ? iend[x - 1] : 0u, n = iend[x] - b;
-            const uint16_t key = ie2[j];
-            uint32_t r = 0;
+        if (sorted) {
+            for (uint32_t j = tm.r; j < nslot; j += N) {
+                const uint32_t x = iown[j];
+                const uint32_t b = x ? iend[x - 1] : 0u, n = iend[x] - b;
+                const uint16_t key = ie2[j];
+                uint32_t r = 0;
 #pragma unroll 4
-            for (uint32_t i = 0; i < n; ++i) r += ie2[b + i] < key ? 1u : 0u;
-            ie[b + r] = key;
+                for (uint32_t i = 0; i < n; ++i) r += ie2[b + i] < key ? 1u : 0u;
+                ie[b + r] = key;
+            }
+            tm.sync();
         }
-        tm.sync();
+        const uint16_t* il = sorted ? ie : ie2;
 
         const DRun& RN = a.runs[run];
         const DRule* rules;
@@ -504,7 +513,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
                 if (i < ncombo) {
                     uint32_t r, pp, j;
                     combo(i, r, pp, j);
-                    c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                    c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, il, kept, kept + j, L, nullptr, 0, 0);
                     de += (uint64_t)c * (ne - rules[r].n_lhs + rules[r].n_rhs);
                 }
                 nd += tm.sum(c);
@@ -518,12 +527,12 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
                         uint32_t c = 0, r = 0, pp = 0, j = 0;
                         if (i < ncombo) {
                             combo(i, r, pp, j);
-                            c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L, nullptr, 0, 0);
+                            c = join_anchor(rules[r], pp, sv, sa, ne, A, iend, il, kept, kept + j, L, nullptr, 0, 0);
                         }
                         uint32_t round;
                         const uint32_t inc = tm.incl_scan(c, round);
                         if (c)
-                            join_anchor(rules[r], pp, sv, sa, ne, A, iend, ie, kept, kept + j, L,
+                            join_anchor(rules[r], pp, sv, sa, ne, A, iend, il, kept, kept + j, L,
                                         dlist + (size_t)(o + inc - c) * dstride, dstride, r);
                         o += round;
                     }
@@ -637,10 +646,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
                     if constexpr (SMALL) {
                         const uint32_t kept = (EMIT && sc) ? sc[i] : 0xFFFFu;
                         c = kept != 0xFFFFu ? kept
-                                            : reg_join_any<false>(R, sv, sa, ne, A, iend, ie, seeds[i], RegJoinOut{});
+                                            : reg_join_any<false>(R, sv, sa, ne, A, iend, il, seeds[i], RegJoinOut{});
                         if (!EMIT && sc) sc[i] = (uint16_t)(c < 0xFFFFu ? c : 0xFFFFu);
                     } else {
-                        c = join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
+                        c = join_seed(R, sv, sa, ne, A, iend, il, seeds[i], L, nullptr, 0, 0, nullptr, nullptr,
                                       nullptr, 0, 0, 0);
                     }
                 }
@@ -651,9 +660,9 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
                         if constexpr (SMALL) {
                             const RegJoinOut ro{a.M.q, a.M.K, out + inc - c, a.M.parent, a.M.rule, a.M.cne,
                                                 s, r, child_ne};
-                            reg_join_any<true>(R, sv, sa, ne, A, iend, ie, seeds[i], ro);
+                            reg_join_any<true>(R, sv, sa, ne, A, iend, il, seeds[i], ro);
                         } else {
-                            join_seed(R, sv, sa, ne, A, iend, ie, seeds[i], L, a.M.q, a.M.K, out + inc - c,
+                            join_seed(R, sv, sa, ne, A, iend, il, seeds[i], L, a.M.q, a.M.K, out + inc - c,
                                       a.M.parent, a.M.rule, a.M.cne, s, r, child_ne);
                         }
                     }
