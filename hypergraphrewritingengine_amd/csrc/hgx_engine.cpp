@@ -479,7 +479,8 @@ uint32_t Engine::step_impl(std::vector<std::shared_ptr<RunRec>>& admitted) {
                         mc.rb = d_r; mc.rpre = d_r + x.rb.size(); mc.nrange = (uint32_t)x.rb.size();
                         mc.n_items = (uint32_t)x.n;
                     }
-                    if (!emit) mc.spread = coprime_stride((uint32_t)x.n);
+                    if (!emit)
+                        mc.spread = coprime_stride((uint32_t)((x.n + kCountSpreadBlock - 1) / kCountSpreadBlock));
                     // sub-wave teams for small states, when the wave's workspaces fit a workgroup
                     uint32_t lanes = 64;
                     if (team == 1 && !mc.scratch) {

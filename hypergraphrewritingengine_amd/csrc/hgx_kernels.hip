@@ -393,12 +393,22 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W, W == 1 ? 4 : 1) void k_match
     const uint32_t tpb = W == 1 ? (blockDim.x / WAVE) * (WAVE / G) : 1;   // teams per workgroup
     const uint32_t teams = gridDim.x * tpb;
     const uint32_t n_items = a.nrange ? a.n_items : a.P.n;
-    for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? ti : 0); ls < n_items; ls += teams) {
-        // The count pass visits the states scattered over the launch, so each request's states
-        // are counted at an even pace from start to end: a request over its cap passes it with
-        // most of its states still ahead, and those are skipped at the tally check below
-        // (in launch order a request's states are counted at once, and the check stops few).
-        const uint32_t li = (!EMIT && a.spread > 1) ? (uint32_t)((uint64_t)ls * a.spread % n_items) : ls;
+    // The count pass visits the states scattered over the launch, so each request's states are
+    // counted at an even pace from start to end: a request over its cap passes it with most of
+    // its states still ahead, and those are skipped at the tally check below (in launch order a
+    // request's states are counted at once, and the check stops few). Blocks of
+    // kCountSpreadBlock consecutive states stay together: neighbouring small states share cache
+    // lines, which one state at a time would fetch again for each.
+    const bool spread = !EMIT && a.spread > 1;
+    const uint32_t nblk = (n_items + kCountSpreadBlock - 1) / kCountSpreadBlock;
+    const uint32_t n_loop = spread ? nblk * kCountSpreadBlock : n_items;
+    for (uint32_t ls = blockIdx.x * tpb + (W == 1 ? ti : 0); ls < n_loop; ls += teams) {
+        uint32_t li = ls;
+        if (spread) {
+            li = (uint32_t)((uint64_t)(ls / kCountSpreadBlock) * a.spread % nblk) * kCountSpreadBlock +
+                 ls % kCountSpreadBlock;
+            if (li >= n_items) continue;                        // the last block's tail
+        }
         uint32_t s = li;
         if (a.nrange) {                                         // subset: logical -> state
             uint32_t lo = 0, hi = a.nrange;

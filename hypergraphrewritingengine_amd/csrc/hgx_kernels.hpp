@@ -9,6 +9,8 @@
 
 namespace hgx {
 
+constexpr uint32_t kCountSpreadBlock = 16;
+
 struct MatchArgs {
     GenView P;
     const DRun* runs;
@@ -41,8 +43,8 @@ struct MatchArgs {
     const uint32_t* rpre;
     uint32_t nrange;
     uint32_t n_items;
-    // Count pass: logical index ls is visited as (ls * spread) mod the launch's states (spread
-    // coprime to that number; 0 or 1: in order).
+    // Count pass: the launch's states in blocks of kCountSpreadBlock consecutive ones, block b
+    // visited as (b * spread) mod the number of blocks (spread coprime to it; 0 or 1: in order).
     uint32_t spread;
     // Small rules: the count pass's matches per seed of each fully joined state, kept for the
     // emit pass (which then joins each seed once, not twice): [2 * P's edges], state s's seeds
