@@ -9,7 +9,7 @@
 
 namespace hgx {
 
-constexpr uint32_t kCountSpreadBlock = 16;
+constexpr uint32_t kCountSpreadBlock = 32;   // measured best of 8, 16, 32, 64 (count pass 4.23, 4.22, 4.08, 4.24 ms)
 
 struct MatchArgs {
     GenView P;
